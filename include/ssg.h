@@ -1,0 +1,284 @@
+/*
+ * ssg.h -- C ABI of the MI355X (gfx950) align+stack hot path.
+ *
+ * The entry points replace SerStacker's registration and frame-accumulation
+ * C++ interfaces for the per-frame hot path.  Each declaration names the
+ * reference interface it stands for (paths relative to the SerStacker tree).
+ *
+ * Conventions
+ *  - Plain pointers and sizes only.  Images are described by ssg_image
+ *    (OpenCV-like type codes, byte stride, and where the memory lives).
+ *  - The caller owns every buffer it passes; the library never frees one.
+ *    Opaque handles own their device state and are released by *_destroy.
+ *  - Every call returns 0 on success or a negative SSG_E_* code; the
+ *    message of the last failure on the calling thread is ssg_last_error().
+ *  - Handles are single-owner and not thread-safe (as the reference classes).
+ *    Each handle runs its work on its own HIP stream and calls return after
+ *    the stream has finished, unless documented otherwise.
+ */
+#ifndef SSG_H
+#define SSG_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SSG_ABI_VERSION 1
+
+/* ---- status codes ---------------------------------------------------------------------------- */
+enum {
+  SSG_OK = 0,
+  SSG_E_INVALID_ARG = -1,
+  SSG_E_HIP = -2,
+  SSG_E_NOT_READY = -3, /* e.g. no reference frame set */
+  SSG_E_UNSUPPORTED = -4,
+  SSG_E_NO_MEMORY = -5
+};
+
+/* per-frame registration outcome (ssg_registration_status.code) */
+enum {
+  SSG_REG_OK = 0,
+  SSG_REG_NOT_RUN = 1,
+  SSG_REG_LEVEL_FAILED = 2,     /* informational: a pyramid level failed its Cholesky, frame kept */
+  SSG_REG_RHO_TRANSLATION = 3,  /* c_frame_registration.cc:842-846  rho < 0.75*min_rho */
+  SSG_REG_RHO = 4               /* c_frame_registration.cc:865-868  rho < min_rho */
+};
+
+/* ---- enums (values = the reference's / OpenCV's) ------------------------------------------- */
+/* IMAGE_MOTION_TYPE, core/proc/image_registration/image_transform.h:15-26 */
+enum { SSG_MOTION_TRANSLATION = 0, SSG_MOTION_EUCLIDEAN = 1, SSG_MOTION_SCALED_EUCLIDEAN = 2,
+       SSG_MOTION_AFFINE = 3 };
+/* ECC_ALIGN_METHOD, ecc2.h:59-64 */
+enum { SSG_ECC_ALIGN_FORWARD_ADDITIVE = 0, SSG_ECC_ALIGN_INVERSE_COMPOSITIONAL = 1,
+       SSG_ECC_ALIGN_LM = 2, SSG_ECC_ALIGN_INVERSE_COMPOSITIONAL_LM = 3 };
+/* ECC_INTERPOLATION_METHOD, ecc2.h:32-43 (cv::InterpolationFlags) */
+enum { SSG_INTER_NEAREST = 0, SSG_INTER_LINEAR = 1, SSG_INTER_CUBIC = 2, SSG_INTER_AREA = 3 };
+/* ECC_BORDER_MODE, ecc2.h:46-56 (cv::BorderTypes) */
+enum { SSG_BORDER_CONSTANT = 0, SSG_BORDER_REPLICATE = 1, SSG_BORDER_REFLECT = 2, SSG_BORDER_WRAP = 3,
+       SSG_BORDER_REFLECT101 = 4, SSG_BORDER_TRANSPARENT = 5 };
+/* image element types (OpenCV type codes) */
+enum { SSG_8UC1 = 0, SSG_16UC1 = 2, SSG_32SC1 = 4, SSG_32FC1 = 5, SSG_32FC2 = 13, SSG_32FC3 = 21 };
+/* frame_accumulation_method, core/pipeline/c_image_stacking_pipeline/c_image_stacking_pipeline.h:26-34 */
+enum { SSG_ACC_AVERAGE = 0, SSG_ACC_WEIGHTED_AVERAGE = 1, SSG_ACC_FOCUS_STACK = 2, SSG_ACC_FFT = 3,
+       SSG_ACC_BAYER_AVERAGE = 4 };
+
+/* ---- image view ---------------------------------------------------------------------------- */
+typedef struct ssg_image {
+  void *data;
+  int32_t width;
+  int32_t height;
+  int64_t step;      /* bytes between rows */
+  int32_t type;      /* SSG_8UC1, SSG_32FC1, ... */
+  int32_t on_device; /* 1: data is device memory of the current HIP device; 0: host memory */
+} ssg_image;
+
+/* ---- options (field names, order and defaults mirror the reference structs) ------------------ */
+/* c_ecc_registration_options, c_frame_registration.h:47-64 */
+typedef struct ssg_ecc_registration_options {
+  double scale;
+  double eps;
+  double min_rho;
+  double input_smooth_sigma;
+  double reference_smooth_sigma;
+  double update_step_scale;
+  int32_t se_radius;
+  int32_t ecc_method;
+  int32_t max_iterations;
+  int32_t ecch_max_level;
+  int32_t ecch_minimum_image_size;
+  double normalization_noise;
+  int32_t normalization_scale;
+  int32_t ecch_estimate_translation_first;
+  int32_t replace_planetary_disk_with_mask;
+} ssg_ecc_registration_options;
+
+/* c_eccflow_registration_options, c_frame_registration.h:88-100 */
+typedef struct ssg_eccflow_registration_options {
+  double update_multiplier;
+  double input_smooth_sigma;
+  double reference_smooth_sigma;
+  double noise_level;
+  double scale_factor;
+  int32_t max_iterations;
+  int32_t support_scale;
+  int32_t min_image_size;
+  int32_t max_pyramid_level;
+  int32_t downscale_method; /* ECCFlowDownscaleMethod, ecc2.h:507-511 */
+} ssg_eccflow_registration_options;
+
+/* c_image_registration_options, c_frame_registration.h:119-136 */
+typedef struct ssg_image_registration_options {
+  int32_t motion_type;
+  int32_t ecc_registration_channel;
+  int32_t interpolation;
+  int32_t border_mode;
+  double border_value[4];
+  ssg_ecc_registration_options ecc;
+  ssg_eccflow_registration_options eccflow;
+  int32_t enable_feature_registration;
+  int32_t enable_ecc_registration;
+  int32_t enable_eccflow_registration;
+  int32_t accumulate_and_compensate_turbulent_flow;
+} ssg_image_registration_options;
+
+/* Fills *o with the reference defaults (c_frame_registration.h:47-136). */
+void ssg_default_image_registration_options(ssg_image_registration_options *o);
+
+/* c_image_registration_status (c_frame_registration.h:185-208) plus the
+ * registered transform.  params[] holds the reference's parameter vector
+ * (translation 2, euclidean 3, scaled euclidean 4, affine 6); matrix[] the
+ * equivalent 2x3 map x' = m0 x + m1 y + m2, y' = m3 x + m4 y + m5 from the
+ * reference frame grid to the current frame, at full resolution. */
+typedef struct ssg_registration_status {
+  int32_t ok;
+  int32_t code;
+  double rho_translation;
+  double rho;
+  double eps;
+  int32_t iterations_translation;
+  int32_t iterations;
+  int32_t levels_failed;
+  int32_t nparams;
+  float params[6];
+  double matrix[6];
+  int32_t level_iterations[8];
+} ssg_registration_status;
+
+/* ---- library ------------------------------------------------------------------------------- */
+int ssg_abi_version(void);
+const char *ssg_last_error(void);
+/* Number of HIP devices and the name of the current one; fails if no GPU. */
+int ssg_device_info(int *num_devices, char *name, int name_len);
+
+/* ---- registration: c_frame_registration (c_frame_registration.h:210-354) ------------------- */
+typedef struct ssg_registration ssg_registration;
+
+/* c_frame_registration::c_frame_registration(options) (c_frame_registration.cc:263-) */
+int ssg_registration_create(const ssg_image_registration_options *opts, ssg_registration **out);
+void ssg_registration_destroy(ssg_registration *r);
+
+/* c_frame_registration::setup_reference_frame (c_frame_registration.cc:565-719).
+ * image: SSG_32FC1 (host or device); mask: SSG_8UC1 or NULL. */
+int ssg_registration_setup_reference_frame(ssg_registration *r, const ssg_image *image,
+                                           const ssg_image *mask);
+
+/* c_frame_registration::register_frame (c_frame_registration.cc:721-964), the
+ * plain per-frame form: the ECC loop of each pyramid level syncs with the host
+ * after every iteration, as the reference's loop does (ecc2.cc:1295). */
+int ssg_registration_register_frame(ssg_registration *r, const ssg_image *image, const ssg_image *mask,
+                                    ssg_registration_status *status);
+
+/* Batched form of register_frame for n frames of the reference size, all
+ * resident in device memory: pyramid build, translation-first ECC, rho gates
+ * and the main ECC run for every frame at once, with a per-frame convergence
+ * mask (no counterpart in the reference, which registers frames one by one;
+ * results per frame are those of register_frame).  masks may be NULL. */
+int ssg_registration_register_batch(ssg_registration *r, int n, const ssg_image *frames,
+                                    const ssg_image *masks, ssg_registration_status *status);
+
+/* c_frame_registration::current_remap / c_image_transform::create_remap
+ * (c_image_transform.h:54): writes the SSG_32FC2 remap field of a status's
+ * transform at the reference frame size. */
+int ssg_registration_create_remap(ssg_registration *r, const ssg_registration_status *status,
+                                  ssg_image *map);
+
+/* c_frame_registration::custom_remap / base_remap (c_frame_registration.cc:1265-1417):
+ * dst = remap(src, map) with the given interpolation/border; dst_mask (optional)
+ * = erode5x5(remap(src_mask or 255, CONSTANT 0) >= 255).  The map is either
+ * a SSG_32FC2 field (map != NULL) or the 2x3 matrix of a registration status
+ * (map == NULL, matrix != NULL).  src: SSG_32FC1 or SSG_32FC3. */
+int ssg_remap(const ssg_image *src, const ssg_image *src_mask, const ssg_image *map, const double *matrix,
+              int32_t motion_type, const float *params, ssg_image *dst, ssg_image *dst_mask,
+              int32_t interpolation, int32_t border_mode, double border_value);
+
+/* ---- accumulation: c_frame_accumulation (core/average/c_frame_accumulation.h:14-37) --------- */
+typedef struct ssg_accumulation ssg_accumulation;
+
+/* create_frame_accumulation (c_image_stacking_pipeline.cc:450-466): method is
+ * SSG_ACC_WEIGHTED_AVERAGE or SSG_ACC_AVERAGE (both c_weigthed_average). */
+int ssg_accumulation_create(int32_t method, int32_t width, int32_t height, int32_t channels,
+                            ssg_accumulation **out);
+void ssg_accumulation_destroy(ssg_accumulation *a);
+
+/* c_frame_accumulation::add(src, weights) (c_frame_accumulation.cc:182-221):
+ * weights may be NULL, a SSG_8UC1 mask (nonzero = 1) or a SSG_32FC1 weight map. */
+int ssg_accumulation_add(ssg_accumulation *a, const ssg_image *src, const ssg_image *weights);
+
+/* Fused remap + base_remap mask + weighted add of n registered frames (the
+ * per-frame sequence custom_remap -> weights*mask/255 -> add of
+ * c_image_stacking_pipeline.cc:1644-1776 for every frame at once).
+ * statuses[i].ok == 0 skips frame i.  frame_weights: per-frame scalar weight
+ * or NULL (1).  Frames: SSG_32FC1/SSG_32FC3 device images. */
+int ssg_accumulation_add_registered_batch(ssg_accumulation *a, int n, const ssg_image *frames,
+                                          const ssg_registration_status *statuses, int32_t motion_type,
+                                          const float *frame_weights, int32_t interpolation,
+                                          int32_t border_mode, double border_value);
+
+/* c_frame_accumulation::compute(avg, mask, dscale, ddepth) (c_frame_accumulation.cc:223-248):
+ * avg = sum / weight (FP64 division), mask = weight > 0 (SSG_8UC1, optional). */
+int ssg_accumulation_compute(ssg_accumulation *a, ssg_image *avg, ssg_image *mask, double dscale);
+
+/* c_frame_accumulation::get_acc_counters (c_frame_accumulation.cc:250-260): the weight plane. */
+int ssg_accumulation_get_acc_counters(ssg_accumulation *a, ssg_image *accw);
+
+/* Integer per-pixel sample counts (frames whose mask covered the pixel), SSG_32SC1. */
+int ssg_accumulation_get_counts(ssg_accumulation *a, ssg_image *counts);
+
+/* c_frame_accumulation::accumulated_frames (c_frame_accumulation.h:31). */
+int ssg_accumulation_accumulated_frames(ssg_accumulation *a, int32_t *n);
+
+/* c_frame_accumulation::clear (c_frame_accumulation.h:30). */
+int ssg_accumulation_clear(ssg_accumulation *a);
+
+/* Per-frame scalar weight of the unaligned frames, computed where the pipeline
+ * computes its weights (c_image_stacking_pipeline.cc:1466-1467): the mean of the
+ * squared 3x3 laplacian [0 1 0; 1 -4 1; 0 1 0] with BORDER_REPLICATE.  frames:
+ * SSG_32FC1 device images of one size; weights: n floats (host). */
+int ssg_frame_sharpness(int n, const ssg_image *frames, float *weights);
+
+/* ---- kernel timing (no reference counterpart; the reference's INSTRUMENT_REGION timers,
+ * core/debug.h:183-188, are its closest analogue).  When enabled, every launch of a hot
+ * kernel is bracketed by HIP events on the handle's stream; algorithmic bytes follow
+ * the per-pixel minimum traffic of each kernel. ------------------------------------------ */
+enum { SSG_PROF_ECC_TRANSLATION = 0, SSG_PROF_ECC_MAIN = 1, SSG_PROF_CORRELATION = 2, SSG_PROF_PYRAMID = 3,
+       SSG_PROF_ACCUMULATE = 4, SSG_PROF_COUNT = 8 };
+typedef struct ssg_profile {
+  int64_t launches[SSG_PROF_COUNT];
+  double ms[SSG_PROF_COUNT];     /* summed kernel durations */
+  double bytes[SSG_PROF_COUNT];  /* summed algorithmic bytes */
+} ssg_profile;
+int ssg_registration_set_profiling(ssg_registration *r, int enable);
+int ssg_registration_get_profile(ssg_registration *r, ssg_profile *out, int reset);
+int ssg_accumulation_set_profiling(ssg_accumulation *a, int enable);
+int ssg_accumulation_get_profile(ssg_accumulation *a, ssg_profile *out, int reset);
+
+/* ---- synthetic planetary-disk frames (benchmark and test inputs; no reference counterpart) -- */
+typedef struct ssg_synth_params {
+  int32_t width, height;
+  double disk_radius;
+  double texture_sigma;     /* gaussian filter sigma of the uniform texture noise */
+  double texture_amplitude; /* 0.1 = 10% */
+  double limb_darkening;    /* I = 1 - u*(1-mu), u = 0.6 */
+  double max_shift;         /* translation U(-max_shift, max_shift) px */
+  double max_rotation_deg;  /* rotation U(-r, r) degrees about the centre */
+  double max_scale_dev;     /* scale 1 + U(-d, d) */
+  double max_shear;         /* shear U(-s, s) */
+  double blur_sigma_min, blur_sigma_max; /* per-frame gaussian blur sigma U(min, max); 0 = none */
+  double peak_electrons;    /* Poisson photon noise at this peak (0 = no noise) */
+  double read_noise;        /* gaussian read noise sigma, DN */
+  double output_scale;      /* stored value = DN * output_scale (1/65536 for the f32 configs) */
+  uint64_t seed;
+} ssg_synth_params;
+
+/* Renders frames [first, first+n) into device images (SSG_32FC1 or SSG_16UC1)
+ * and returns each frame's true map (reference grid -> frame) in truth[6*i].
+ * Frame index -1 renders the un-warped reference (identity map, mean blur). */
+int ssg_synth_render(const ssg_synth_params *p, int first, int n, const ssg_image *frames, double *truth);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,258 @@
+"""serstacker_amd -- MI355X (gfx950) align+stack hot path of SerStacker.
+
+Python mirror of the reference's registration and accumulation interfaces,
+calling the HIP implementation through the C ABI of include/ssg.h:
+
+    c_image_registration_options   c_frame_registration.h:119-136
+    c_frame_registration           c_frame_registration.h:210-354
+    c_weigthed_average             core/average/c_frame_accumulation.h:40-70
+
+Images are numpy arrays (host) or torch tensors on the GPU (device); like the
+reference, operations return False and keep an error message on failure.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import logging
+
+import numpy as np
+
+from . import _lib
+from ._lib import (  # noqa: F401
+    ACC_AVERAGE, ACC_WEIGHTED_AVERAGE, BORDER_CONSTANT, BORDER_REFLECT101, BORDER_REPLICATE,
+    ECC_ALIGN_FORWARD_ADDITIVE, INTER_CUBIC, INTER_LINEAR, MOTION_AFFINE, MOTION_EUCLIDEAN,
+    MOTION_SCALED_EUCLIDEAN, MOTION_TRANSLATION, RegistrationStatus, SsgError,
+)
+
+log = logging.getLogger("serstacker_amd")
+
+__all__ = ["c_image_registration_options", "c_frame_registration", "c_weigthed_average", "image_view",
+           "synth_params", "synth_render", "SsgError"]
+
+
+def _is_torch(x) -> bool:
+    return type(x).__module__.startswith("torch")
+
+
+def image_view(x) -> _lib.Image:
+    """ssg_image view of a numpy array or a torch GPU tensor."""
+    if x is None:
+        return None
+    if _is_torch(x):
+        return _lib.device_image(x)
+    return _lib.host_image(np.ascontiguousarray(x))
+
+
+def c_image_registration_options(**ecc) -> _lib.ImageRegistrationOptions:
+    """The reference defaults (c_frame_registration.h:47-136), ECC enabled with the
+    forward-additive method; keyword arguments override fields of .ecc."""
+    lib = _lib.load()
+    o = _lib.ImageRegistrationOptions()
+    lib.ssg_default_image_registration_options(C.byref(o))
+    o.enable_feature_registration = 0
+    o.enable_ecc_registration = 1
+    o.ecc.ecc_method = _lib.ECC_ALIGN_FORWARD_ADDITIVE
+    for k, v in ecc.items():
+        if hasattr(o.ecc, k):
+            setattr(o.ecc, k, v)
+        else:
+            setattr(o, k, v)
+    return o
+
+
+class c_frame_registration:
+    def __init__(self, options: _lib.ImageRegistrationOptions | None = None):
+        self._lib = _lib.load()
+        self._opts = options if options is not None else c_image_registration_options()
+        h = C.c_void_p()
+        _lib.check(self._lib.ssg_registration_create(C.byref(self._opts), C.byref(h)), "ssg_registration_create")
+        self._h = h
+        self._status = _lib.RegistrationStatus()
+        self._ref_size = None
+        self.last_error = ""
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            self._lib.ssg_registration_destroy(h)
+            self._h = None
+
+    def options(self):
+        return self._opts
+
+    def _fail(self, rc, what):
+        self.last_error = self._lib.ssg_last_error().decode(errors="replace")
+        log.error("%s failed (%d): %s", what, rc, self.last_error)
+        return False
+
+    def setup_reference_frame(self, image, mask=None) -> bool:
+        iv = image_view(image)
+        mv = image_view(mask)
+        rc = self._lib.ssg_registration_setup_reference_frame(self._h, C.byref(iv), C.byref(mv) if mv else None)
+        if rc:
+            return self._fail(rc, "setup_reference_frame")
+        self._ref_size = (iv.width, iv.height)
+        return True
+
+    def register_frame(self, src, srcmask=None) -> bool:
+        """Plain per-frame form (host sync per ECC iteration).  True when the
+        frame passed the rho gates; status() holds the details."""
+        iv = image_view(src)
+        mv = image_view(srcmask)
+        rc = self._lib.ssg_registration_register_frame(self._h, C.byref(iv), C.byref(mv) if mv else None,
+                                                      C.byref(self._status))
+        if rc:
+            return self._fail(rc, "register_frame")
+        return bool(self._status.ok)
+
+    def register_batch(self, frames, masks=None):
+        """Batched registration of device frames; returns a ctypes array of statuses."""
+        views = _lib.images([image_view(f) for f in frames])
+        mviews = _lib.images([image_view(m) for m in masks]) if masks is not None else None
+        st = (_lib.RegistrationStatus * len(frames))()
+        rc = self._lib.ssg_registration_register_batch(self._h, len(frames), views, mviews, st)
+        if rc:
+            self._fail(rc, "register_batch")
+            raise SsgError(self.last_error)
+        return st
+
+    def status(self) -> _lib.RegistrationStatus:
+        return self._status
+
+    def current_remap(self, status=None) -> np.ndarray:
+        st = status if status is not None else self._status
+        w, h = self._ref_size
+        m = np.zeros((h, w, 2), np.float32)
+        mv = _lib.host_image(m)
+        _lib.check(self._lib.ssg_registration_create_remap(self._h, C.byref(st), C.byref(mv)), "create_remap")
+        return m
+
+    def custom_remap(self, src, src_mask=None, interpolation=None, border_mode=None, border_value=0.0,
+                     status=None, rmap=None, want_mask=True):
+        """c_frame_registration::custom_remap (c_frame_registration.cc:1265-1404).
+        Returns (dst, dst_mask) as numpy arrays."""
+        st = status if status is not None else self._status
+        interp = self._opts.interpolation if interpolation is None else interpolation
+        border = self._opts.border_mode if border_mode is None else border_mode
+        src_np = src if not _is_torch(src) else None
+        h, w = (self._ref_size[1], self._ref_size[0])
+        cn = 1 if np.ndim(src) == 2 else src.shape[2]
+        dst = np.zeros((h, w) if cn == 1 else (h, w, cn), np.float32)
+        dm = np.zeros((h, w), np.uint8) if want_mask else None
+        sv = image_view(src if src_np is None else np.ascontiguousarray(src_np, np.float32))
+        smv = image_view(src_mask)
+        dv = _lib.host_image(dst)
+        dmv = _lib.host_image(dm) if want_mask else None
+        mapv = _lib.host_image(np.ascontiguousarray(rmap, np.float32)) if rmap is not None else None
+        params = (C.c_float * 6)(*st.params)
+        rc = self._lib.ssg_remap(C.byref(sv), C.byref(smv) if smv else None, C.byref(mapv) if mapv else None, None,
+                                 self._opts.motion_type, params, C.byref(dv), C.byref(dmv) if dmv else None, interp,
+                                 border, border_value)
+        _lib.check(rc, "ssg_remap")
+        return dst, dm
+
+
+class c_weigthed_average:
+    """c_weigthed_average (core/average/c_frame_accumulation.h:40-70), sum form."""
+
+    def __init__(self, width: int, height: int, channels: int = 1, method: int = _lib.ACC_WEIGHTED_AVERAGE):
+        self._lib = _lib.load()
+        h = C.c_void_p()
+        _lib.check(self._lib.ssg_accumulation_create(method, width, height, channels, C.byref(h)),
+                   "ssg_accumulation_create")
+        self._h = h
+        self.w, self.h, self.cn = width, height, channels
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            self._lib.ssg_accumulation_destroy(h)
+            self._h = None
+
+    def add(self, src, weights=None) -> bool:
+        sv = image_view(src)
+        wv = image_view(weights)
+        rc = self._lib.ssg_accumulation_add(self._h, C.byref(sv), C.byref(wv) if wv else None)
+        if rc:
+            log.error("add: %s", self._lib.ssg_last_error().decode())
+            return False
+        return True
+
+    def add_registered_batch(self, frames, statuses, motion_type, frame_weights=None,
+                             interpolation=_lib.INTER_LINEAR, border_mode=_lib.BORDER_REFLECT101, border_value=0.0):
+        views = _lib.images([image_view(f) for f in frames])
+        wts = (C.c_float * len(frames))(*frame_weights) if frame_weights is not None else None
+        rc = self._lib.ssg_accumulation_add_registered_batch(self._h, len(frames), views, statuses, motion_type, wts,
+                                                             interpolation, border_mode, border_value)
+        _lib.check(rc, "add_registered_batch")
+
+    def compute(self, dscale: float = 1.0):
+        avg = np.zeros((self.h, self.w) if self.cn == 1 else (self.h, self.w, self.cn), np.float32)
+        mask = np.zeros((self.h, self.w), np.uint8)
+        av, mv = _lib.host_image(avg), _lib.host_image(mask)
+        _lib.check(self._lib.ssg_accumulation_compute(self._h, C.byref(av), C.byref(mv), dscale), "compute")
+        return avg, mask
+
+    def get_acc_counters(self) -> np.ndarray:
+        w = np.zeros((self.h, self.w), np.float32)
+        v = _lib.host_image(w)
+        _lib.check(self._lib.ssg_accumulation_get_acc_counters(self._h, C.byref(v)), "get_acc_counters")
+        return w
+
+    def counts(self) -> np.ndarray:
+        c = np.zeros((self.h, self.w), np.int32)
+        v = _lib.host_image(c)
+        _lib.check(self._lib.ssg_accumulation_get_counts(self._h, C.byref(v)), "get_counts")
+        return c
+
+    def accumulated_frames(self) -> int:
+        n = C.c_int32()
+        _lib.check(self._lib.ssg_accumulation_accumulated_frames(self._h, C.byref(n)), "accumulated_frames")
+        return n.value
+
+    def clear(self):
+        _lib.check(self._lib.ssg_accumulation_clear(self._h), "clear")
+
+
+def frame_sharpness(frames) -> np.ndarray:
+    """Per-frame weight (mean squared 3x3 laplacian) of device frames."""
+    lib = _lib.load()
+    views = _lib.images([image_view(f) for f in frames])
+    out = np.zeros(len(frames), np.float32)
+    _lib.check(lib.ssg_frame_sharpness(len(frames), views, out.ctypes.data_as(C.POINTER(C.c_float))),
+               "ssg_frame_sharpness")
+    return out
+
+
+def synth_params(**kw) -> _lib.SynthParams:
+    """Planetary-disk generator parameters (defaults: 1920x1080 affine frames, disk
+    radius 420, texture sigma 3 at 10%, limb darkening 0.6)."""
+    p = _lib.SynthParams()
+    p.width, p.height = 1920, 1080
+    p.disk_radius = 420.0
+    p.texture_sigma = 3.0
+    p.texture_amplitude = 0.1
+    p.limb_darkening = 0.6
+    p.max_shift = 20.0
+    p.max_rotation_deg = 1.0
+    p.max_scale_dev = 0.005
+    p.max_shear = 0.002
+    p.blur_sigma_min, p.blur_sigma_max = 0.8, 2.5
+    p.peak_electrons = 2000.0
+    p.read_noise = 4.0
+    p.output_scale = 1.0 / 65536.0
+    p.seed = 20261202
+    for k, v in kw.items():
+        setattr(p, k, v)
+    return p
+
+
+def synth_render(p: _lib.SynthParams, first: int, frames) -> np.ndarray:
+    """Renders frames[i] = frame first+i (first = -1: the reference) into device
+    tensors; returns the (n, 6) true ref->frame maps."""
+    lib = _lib.load()
+    views = _lib.images([image_view(f) for f in frames])
+    truth = np.zeros((len(frames), 6), np.float64)
+    rc = lib.ssg_synth_render(C.byref(p), first, len(frames), views, truth.ctypes.data_as(C.POINTER(C.c_double)))
+    _lib.check(rc, "ssg_synth_render")
+    return truth

@@ -1,0 +1,1332 @@
+// ssg_api.hip -- host engine and C ABI (include/ssg.h).
+//
+// The registration handle mirrors c_frame_registration (ECC path); the
+// accumulation handle mirrors c_frame_accumulation (c_weigthed_average).
+// Both own one HIP stream (default priority) and their device buffers.
+#include <math.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <algorithm>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/ssg.h"
+#include "ssg_engine.hpp"
+
+using namespace ssg;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const char* fmt, ...) {
+  char buf[1024];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return code;
+}
+
+#define HIPCHK(x)                                                                         \
+  do {                                                                                    \
+    hipError_t e_ = (x);                                                                  \
+    if (e_ != hipSuccess) return fail(SSG_E_HIP, "%s: %s", #x, hipGetErrorString(e_));    \
+  } while (0)
+
+#define CHK(x)             \
+  do {                     \
+    int r_ = (x);          \
+    if (r_ != SSG_OK) return r_; \
+  } while (0)
+
+struct DevBuf {
+  void* p = nullptr;
+  size_t n = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  ~DevBuf() { release(); }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+  int ensure(size_t bytes) {
+    if (bytes <= n) return SSG_OK;
+    release();
+    if (!bytes) return SSG_OK;
+    hipError_t e = hipMalloc(&p, bytes);
+    if (e != hipSuccess) {
+      p = nullptr;
+      return fail(SSG_E_NO_MEMORY, "hipMalloc(%zu bytes): %s", bytes, hipGetErrorString(e));
+    }
+    n = bytes;
+    return SSG_OK;
+  }
+  template <class T>
+  T* as() const {
+    return reinterpret_cast<T*>(p);
+  }
+};
+
+int elem_size(int type) {
+  switch (type) {
+    case SSG_8UC1: return 1;
+    case SSG_16UC1: return 2;
+    case SSG_32SC1: return 4;
+    case SSG_32FC1: return 4;
+    case SSG_32FC2: return 8;
+    case SSG_32FC3: return 12;
+  }
+  return 0;
+}
+
+int channels_of(int type) { return type == SSG_32FC3 ? 3 : (type == SSG_32FC2 ? 2 : 1); }
+
+int check_image(const ssg_image* im, const char* what) {
+  if (!im || !im->data) return fail(SSG_E_INVALID_ARG, "%s: null image", what);
+  const int es = elem_size(im->type);
+  if (!es) return fail(SSG_E_INVALID_ARG, "%s: unsupported type %d", what, im->type);
+  if (im->width <= 0 || im->height <= 0) return fail(SSG_E_INVALID_ARG, "%s: empty image %dx%d", what, im->width, im->height);
+  if (im->step < (int64_t)im->width * es) return fail(SSG_E_INVALID_ARG, "%s: step %lld < row bytes", what, (long long)im->step);
+  return SSG_OK;
+}
+
+// A device view of an image: device images are used in place, host images are
+// copied to tmp.  Returns the device pointer and the row pitch in bytes.
+int device_view(const ssg_image* im, DevBuf& tmp, hipStream_t s, const void** dptr, size_t* pitch_bytes) {
+  if (im->on_device) {
+    *dptr = im->data;
+    *pitch_bytes = (size_t)im->step;
+    return SSG_OK;
+  }
+  const size_t row = (size_t)im->width * elem_size(im->type);
+  const size_t pitch = (row + 255) / 256 * 256;
+  CHK(tmp.ensure(pitch * im->height));
+  HIPCHK(hipMemcpy2DAsync(tmp.p, pitch, im->data, (size_t)im->step, row, im->height, hipMemcpyHostToDevice, s));
+  *dptr = tmp.p;
+  *pitch_bytes = pitch;
+  return SSG_OK;
+}
+
+// Cubic tables (OpenCV interpolateCubic at t = i/32 and BicubicTab_i), per device
+std::mutex g_tab_mu;
+std::vector<int> g_tab_done;
+
+void cubic_coeffs(float x, float* c) {
+  const float A = -0.75f;
+  c[0] = ((A * (x + 1) - 5 * A) * (x + 1) + 8 * A) * (x + 1) - 4 * A;
+  c[1] = ((A + 2) * x - (A + 3)) * x * x + 1;
+  c[2] = ((A + 2) * (1 - x) - (A + 3)) * (1 - x) * (1 - x) + 1;
+  c[3] = 1.f - c[0] - c[1] - c[2];
+}
+
+int ensure_tables(hipStream_t s) {
+  int dev = 0;
+  HIPCHK(hipGetDevice(&dev));
+  std::lock_guard<std::mutex> lk(g_tab_mu);
+  if ((int)g_tab_done.size() > dev && g_tab_done[dev]) return SSG_OK;
+  float tab[32][4];
+  for (int i = 0; i < 32; ++i) cubic_coeffs(i * (1.f / 32), tab[i]);
+  static short itab[32][32][16];
+  for (int fy = 0; fy < 32; ++fy)
+    for (int fx = 0; fx < 32; ++fx) {
+      int w[16], isum = 0;
+      for (int k1 = 0; k1 < 4; ++k1)
+        for (int k2 = 0; k2 < 4; ++k2) {
+          const float v = tab[fy][k1] * tab[fx][k2];
+          long r = lrintf(v * 32768.f);
+          const int iv = r < -32768 ? -32768 : (r > 32767 ? 32767 : (int)r);
+          w[k1 * 4 + k2] = iv;
+          isum += iv;
+        }
+      if (isum != 32768) {
+        int diff = isum - 32768, Mk1 = 2, Mk2 = 2, mk1 = 2, mk2 = 2;
+        for (int k1 = 2; k1 < 4; ++k1)
+          for (int k2 = 2; k2 < 4; ++k2) {
+            if (w[k1 * 4 + k2] < w[mk1 * 4 + mk2]) { mk1 = k1; mk2 = k2; }
+            else if (w[k1 * 4 + k2] > w[Mk1 * 4 + Mk2]) { Mk1 = k1; Mk2 = k2; }
+          }
+        if (diff < 0) w[Mk1 * 4 + Mk2] -= diff;
+        else w[mk1 * 4 + mk2] -= diff;
+      }
+      for (int k = 0; k < 16; ++k) itab[fy][fx][k] = (short)w[k];
+    }
+  set_cubic_table(tab, s);
+  HIPCHK(set_cubic_itab(&itab[0][0][0], s));
+  HIPCHK(hipStreamSynchronize(s));
+  if ((int)g_tab_done.size() <= dev) g_tab_done.resize(dev + 1, 0);
+  g_tab_done[dev] = 1;
+  return SSG_OK;
+}
+
+int motion_nparams(int m) {
+  return m == SSG_MOTION_TRANSLATION ? 2 : (m == SSG_MOTION_EUCLIDEAN ? 3 : (m == SSG_MOTION_SCALED_EUCLIDEAN ? 4 : 6));
+}
+
+// host copy of the device map-coefficient rule (ssg_ecc.hip update_map)
+MapCoef map_coef(int motion, const float* p, const float* C) {
+  MapCoef m{};
+  if (motion == SSG_MOTION_TRANSLATION) {
+    m.c[0] = p[0];
+    m.c[1] = p[1];
+  } else if (motion == SSG_MOTION_AFFINE) {
+    for (int i = 0; i < 6; ++i) m.c[i] = p[i];
+  } else {
+    m.c[0] = (float)sin((double)p[2]);
+    m.c[1] = (float)cos((double)p[2]);
+    m.c[2] = motion == SSG_MOTION_SCALED_EUCLIDEAN ? p[3] : 1.f;
+    m.c[3] = p[0];
+    m.c[4] = p[1];
+    m.c[5] = C ? C[0] : 0.f;
+    m.c[6] = C ? C[1] : 0.f;
+  }
+  return m;
+}
+
+void params_matrix(int motion, const float* p, double* m) {
+  if (motion == SSG_MOTION_TRANSLATION) {
+    m[0] = 1; m[1] = 0; m[2] = p[0];
+    m[3] = 0; m[4] = 1; m[5] = p[1];
+  } else if (motion == SSG_MOTION_AFFINE) {
+    for (int i = 0; i < 6; ++i) m[i] = p[i];
+  } else {
+    const double s = motion == SSG_MOTION_SCALED_EUCLIDEAN ? p[3] : 1.0;
+    const double sa = (float)sin((double)p[2]), ca = (float)cos((double)p[2]);
+    m[0] = s * ca; m[1] = -s * sa; m[2] = p[0];
+    m[3] = s * sa; m[4] = s * ca; m[5] = p[1];
+  }
+}
+
+// c_image_transform::scale_transfrom on the host (float parameters, double factor)
+void scale_params(int motion, float* p, double f) {
+  if (motion == SSG_MOTION_AFFINE) {
+    p[2] = (float)((double)p[2] * f);
+    p[5] = (float)((double)p[5] * f);
+  } else {
+    p[0] = (float)((double)p[0] * f);
+    p[1] = (float)((double)p[1] * f);
+  }
+}
+
+SepKernel gaussian_sep(double sigma) {
+  // c_ecch smoothing kernel: ksize = max(3, 2*int(3*sigma)+1), cv::getGaussianKernel(ksize, sigma)
+  SepKernel k{};
+  int n = 2 * (int)(3 * sigma) + 1;
+  if (n < 3) n = 3;
+  if (n > 31) n = 31;
+  double cd[32], sum = 0;
+  const double scale2X = -0.5 / (sigma * sigma);
+  for (int i = 0; i < n; ++i) {
+    const double x = i - (n - 1) * 0.5;
+    cd[i] = exp(scale2X * x * x);
+    sum += cd[i];
+  }
+  sum = 1. / sum;
+  for (int i = 0; i < n; ++i) k.kx[i] = k.ky[i] = (float)(cd[i] * sum);
+  k.nx = k.ny = n;
+  // cv::getKernelType on the double kernel: a gaussian is symmetric
+  bool sym = true;
+  for (int i = 0; i < n; ++i)
+    if (cd[i] != cd[n - 1 - i]) sym = false;
+  k.tx = k.ty = sym ? 1 : 0;
+  return k;
+}
+
+// Per-launch HIP-event timing of the hot kernels on a handle's stream.
+struct Profiler {
+  bool on = false;
+  std::vector<hipEvent_t> pool;
+  size_t used = 0;
+  struct Pending {
+    int kind;
+    hipEvent_t a, b;
+  };
+  std::vector<Pending> pend;
+  ssg_profile acc{};
+  hipEvent_t a_ = nullptr;
+
+  ~Profiler() {
+    for (hipEvent_t e : pool) (void)hipEventDestroy(e);
+  }
+  hipEvent_t ev() {
+    if (used == pool.size()) {
+      hipEvent_t e = nullptr;
+      if (hipEventCreate(&e) != hipSuccess) return nullptr;
+      pool.push_back(e);
+    }
+    return pool[used++];
+  }
+  void start(hipStream_t s) {
+    if (!on) return;
+    a_ = ev();
+    if (a_) (void)hipEventRecord(a_, s);
+  }
+  void stop(hipStream_t s, int kind) {
+    if (!on || !a_) return;
+    hipEvent_t b = ev();
+    if (!b) return;
+    (void)hipEventRecord(b, s);
+    pend.push_back({kind, a_, b});
+    a_ = nullptr;
+  }
+  // after the stream has finished: fold the pending durations in
+  void collect() {
+    for (const Pending& p : pend) {
+      float ms = 0.f;
+      if (hipEventElapsedTime(&ms, p.a, p.b) == hipSuccess) {
+        acc.ms[p.kind] += ms;
+        acc.launches[p.kind] += 1;
+      }
+    }
+    pend.clear();
+    used = 0;
+  }
+  void add_bytes(int kind, double b) {
+    if (on) acc.bytes[kind] += b;
+  }
+};
+
+}  // namespace
+
+// =====================================================================================================
+// registration
+// =====================================================================================================
+struct ssg_registration {
+  ssg_image_registration_options o{};
+  hipStream_t stream = nullptr;
+  int nlevels = 0;
+  int ref_w = 0, ref_h = 0;  // full-resolution reference size
+  int ew = 0, eh = 0;        // ECC image size (after ecc.scale)
+  int lw[MAX_LEVELS] = {}, lh[MAX_LEVELS] = {}, lpitch[MAX_LEVELS] = {};
+  long long loff[MAX_LEVELS] = {};
+  long long pyr_elems = 0;   // floats per frame pyramid
+  long long mask_elems = 0;  // bytes per frame mask pyramid
+  DevBuf ref_pyr, ref_mask_pyr;
+  bool ref_has_mask[MAX_LEVELS] = {};
+  double ref_mean[MAX_LEVELS] = {};
+  // batch workspace
+  int cap = 0;
+  DevBuf cur_pyr, cur_mask_pyr, half_buf;
+  DevBuf xf_t, xf_m, res, partials, tile_counter, done_counter, active[2], active_count[2];
+  DevBuf frame_ptrs, mask_ptrs, zero_count, frame_list, tmp_in, tmp_mask, sum_tmp;
+  int* pinned_count = nullptr;
+  bool have_ref = false;
+  Profiler prof;
+};
+
+extern "C" void ssg_default_image_registration_options(ssg_image_registration_options* o) {
+  if (!o) return;
+  memset(o, 0, sizeof(*o));
+  o->motion_type = SSG_MOTION_AFFINE;
+  o->ecc_registration_channel = 0;  // color_channel_gray
+  o->interpolation = SSG_INTER_LINEAR;
+  o->border_mode = SSG_BORDER_REFLECT101;
+  o->ecc.scale = 0.5;
+  o->ecc.eps = 0.2;
+  o->ecc.min_rho = 0.8;
+  o->ecc.input_smooth_sigma = 1.0;
+  o->ecc.reference_smooth_sigma = 1.0;
+  o->ecc.update_step_scale = 1.5;
+  o->ecc.se_radius = 5;
+  o->ecc.ecc_method = SSG_ECC_ALIGN_LM;
+  o->ecc.max_iterations = 50;
+  o->ecc.ecch_max_level = 0;
+  o->ecc.ecch_minimum_image_size = 16;
+  o->ecc.normalization_noise = 0.01;
+  o->ecc.normalization_scale = 0;
+  o->ecc.ecch_estimate_translation_first = 1;
+  o->ecc.replace_planetary_disk_with_mask = 0;
+  o->eccflow.update_multiplier = 1.5;
+  o->eccflow.noise_level = -1;
+  o->eccflow.scale_factor = 0.75;
+  o->eccflow.max_iterations = 3;
+  o->eccflow.support_scale = 4;
+  o->eccflow.min_image_size = -1;
+  o->eccflow.max_pyramid_level = -1;
+  o->eccflow.downscale_method = 0;
+  o->enable_feature_registration = 1;
+  o->enable_ecc_registration = 0;
+  o->enable_eccflow_registration = 0;
+}
+
+extern "C" int ssg_abi_version(void) { return SSG_ABI_VERSION; }
+extern "C" const char* ssg_last_error(void) { return g_err.c_str(); }
+
+extern "C" int ssg_device_info(int* num_devices, char* name, int name_len) {
+  int n = 0;
+  HIPCHK(hipGetDeviceCount(&n));
+  if (num_devices) *num_devices = n;
+  if (n <= 0) return fail(SSG_E_HIP, "no HIP device");
+  if (name && name_len > 0) {
+    int dev = 0;
+    HIPCHK(hipGetDevice(&dev));
+    hipDeviceProp_t prop;
+    HIPCHK(hipGetDeviceProperties(&prop, dev));
+    snprintf(name, name_len, "%s (%s)", prop.name, prop.gcnArchName);
+  }
+  return SSG_OK;
+}
+
+extern "C" int ssg_registration_create(const ssg_image_registration_options* opts, ssg_registration** out) {
+  if (!out) return fail(SSG_E_INVALID_ARG, "out is null");
+  *out = nullptr;
+  ssg_image_registration_options o;
+  if (opts) o = *opts;
+  else ssg_default_image_registration_options(&o);
+  if (!o.enable_ecc_registration) return fail(SSG_E_UNSUPPORTED, "only ECC registration is on this path (enable_ecc_registration=0)");
+  if (o.enable_feature_registration) return fail(SSG_E_UNSUPPORTED, "sparse feature registration is not on this path");
+  if (o.ecc.ecc_method != SSG_ECC_ALIGN_FORWARD_ADDITIVE) return fail(SSG_E_UNSUPPORTED, "ecc_method %d: only ECC_ALIGN_FORWARD_ADDITIVE", o.ecc.ecc_method);
+  if (o.motion_type < SSG_MOTION_TRANSLATION || o.motion_type > SSG_MOTION_AFFINE) return fail(SSG_E_UNSUPPORTED, "motion_type %d", o.motion_type);
+  if (!(o.ecc.scale <= 0 || o.ecc.scale == 1 || fabs(o.ecc.scale - 0.5) < 1e-2)) return fail(SSG_E_UNSUPPORTED, "ecc.scale %g: only 1 and 0.5", o.ecc.scale);
+  if (o.ecc.normalization_scale > 0 && o.ecc.normalization_noise > 0) return fail(SSG_E_UNSUPPORTED, "ecc normalization");
+  if (o.ecc.replace_planetary_disk_with_mask) return fail(SSG_E_UNSUPPORTED, "replace_planetary_disk_with_mask");
+  if (o.enable_eccflow_registration) return fail(SSG_E_UNSUPPORTED, "eccflow refinement");
+  ssg_registration* r = new ssg_registration();
+  r->o = o;
+  hipError_t e = hipStreamCreateWithFlags(&r->stream, hipStreamNonBlocking);
+  if (e != hipSuccess) {
+    delete r;
+    return fail(SSG_E_HIP, "hipStreamCreate: %s", hipGetErrorString(e));
+  }
+  e = hipHostMalloc((void**)&r->pinned_count, sizeof(int) * 4, hipHostMallocDefault);
+  if (e != hipSuccess) {
+    (void)hipStreamDestroy(r->stream);
+    delete r;
+    return fail(SSG_E_HIP, "hipHostMalloc: %s", hipGetErrorString(e));
+  }
+  int rc = ensure_tables(r->stream);
+  if (rc != SSG_OK) {
+    (void)hipHostFree(r->pinned_count);
+    (void)hipStreamDestroy(r->stream);
+    delete r;
+    return rc;
+  }
+  *out = r;
+  return SSG_OK;
+}
+
+extern "C" void ssg_registration_destroy(ssg_registration* r) {
+  if (!r) return;
+  if (r->stream) (void)hipStreamSynchronize(r->stream);
+  if (r->pinned_count) (void)hipHostFree(r->pinned_count);
+  if (r->stream) (void)hipStreamDestroy(r->stream);
+  delete r;
+}
+
+namespace {
+
+bool use_half(const ssg_image_registration_options& o) { return o.ecc.scale > 0 && o.ecc.scale != 1; }
+
+// Builds one level-0 image (smoothed or copied) and the pyramid for n frames.
+// src_ptrs: device array of n frame pointers (f32, pitch src_pitch floats).
+int build_pyramids(ssg_registration* r, const float* const* src_ptrs, int src_pitch, int n, double sigma,
+                   float* pyr) {
+  hipStream_t s = r->stream;
+  if (sigma > 0) {
+    const SepKernel k = gaussian_sep(sigma);
+    HIPCHK(launch_sepfilter(src_ptrs, src_pitch, pyr + r->loff[0], r->pyr_elems, r->lpitch[0], r->lw[0], r->lh[0], n, k,
+                            BORDER_REPLICATE, s));
+  } else {
+    HIPCHK(launch_copy_frames(src_ptrs, src_pitch, pyr + r->loff[0], r->pyr_elems, r->lpitch[0], r->lw[0], r->lh[0], n, s));
+  }
+  for (int l = 1; l < r->nlevels; ++l)
+    HIPCHK(launch_pyrdown(pyr + r->loff[l - 1], r->pyr_elems, r->lpitch[l - 1], r->lw[l - 1], r->lh[l - 1],
+                          pyr + r->loff[l], r->pyr_elems, r->lpitch[l], r->lw[l], r->lh[l], n, BORDER_REFLECT101, s));
+  return SSG_OK;
+}
+
+// Mask pyramids: level l = INTER_NEAREST resize of the un-eroded level l-1
+// mask; then each level is eroded 5x5 (BORDER_REPLICATE) when it has a zero
+// (c_ecc_forward_additive::set_current_image / set_reference_image).
+// has_mask[l*n + f] receives whether frame f keeps a mask at level l
+// (reference masks without zeros are dropped).
+int build_mask_pyramids(ssg_registration* r, const uint8_t* const* src_ptrs, int src_pitch, int n, uint8_t* mpyr,
+                        bool is_ref, std::vector<char>& keep) {
+  hipStream_t s = r->stream;
+  // raw (un-eroded) chain into tmp_mask, one level at a time, then erode into mpyr
+  CHK(r->tmp_mask.ensure((size_t)r->mask_elems * n));
+  uint8_t* raw = r->tmp_mask.as<uint8_t>();
+  CHK(r->frame_ptrs.ensure(sizeof(void*) * n * 2));
+  std::vector<const uint8_t*> ptrs(n);
+  CHK(r->zero_count.ensure(sizeof(unsigned) * n));
+  keep.assign((size_t)r->nlevels * n, 0);
+  std::vector<unsigned> zc(n);
+  for (int l = 0; l < r->nlevels; ++l) {
+    const int lp = r->lpitch[l];
+    if (l == 0) {
+      HIPCHK(launch_mask_prepare(src_ptrs, src_pitch, raw + r->loff[0], r->mask_elems, lp, r->lw[0], r->lh[0], r->lw[0],
+                                 r->lh[0], n, s));
+    } else {
+      for (int f = 0; f < n; ++f) ptrs[f] = raw + (size_t)f * r->mask_elems + r->loff[l - 1];
+      const uint8_t** dptrs = r->frame_ptrs.as<const uint8_t*>() + n;
+      HIPCHK(hipMemcpyAsync((void*)dptrs, ptrs.data(), sizeof(void*) * n, hipMemcpyHostToDevice, s));
+      HIPCHK(launch_mask_prepare(dptrs, r->lpitch[l - 1], raw + r->loff[l], r->mask_elems, lp, r->lw[l - 1],
+                                 r->lh[l - 1], r->lw[l], r->lh[l], n, s));
+    }
+    HIPCHK(hipMemsetAsync(r->zero_count.p, 0, sizeof(unsigned) * n, s));
+    HIPCHK(launch_count_zero_u8(raw + r->loff[l], r->mask_elems, lp, r->lw[l], r->lh[l], n, r->zero_count.as<unsigned>(), s));
+    HIPCHK(hipMemcpyAsync(zc.data(), r->zero_count.p, sizeof(unsigned) * n, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    for (int f = 0; f < n; ++f) {
+      uint8_t* src = raw + (size_t)f * r->mask_elems + r->loff[l];
+      uint8_t* dst = mpyr + (size_t)f * r->mask_elems + r->loff[l];
+      if (zc[f] == 0) {
+        keep[(size_t)l * n + f] = is_ref ? 0 : 1;
+        if (!is_ref) HIPCHK(hipMemcpy2DAsync(dst, lp, src, lp, r->lw[l], r->lh[l], hipMemcpyDeviceToDevice, s));
+      } else {
+        keep[(size_t)l * n + f] = 1;
+        HIPCHK(launch_erode5(src, 0, lp, dst, 0, lp, r->lw[l], r->lh[l], 1, BORDER_REPLICATE, 255, s));
+      }
+    }
+  }
+  return SSG_OK;
+}
+
+int ensure_workspace(ssg_registration* r, int n) {
+  if (n <= r->cap) return SSG_OK;
+  const int tx0 = (r->lw[0] + ECC_TW - 1) / ECC_TW, ty0 = (r->lh[0] + ECC_TH - 1) / ECC_TH;
+  CHK(r->cur_pyr.ensure(sizeof(float) * (size_t)r->pyr_elems * n));
+  CHK(r->xf_t.ensure(sizeof(FrameXf) * n));
+  CHK(r->xf_m.ensure(sizeof(FrameXf) * n));
+  CHK(r->res.ensure(sizeof(FrameRes) * n));
+  CHK(r->partials.ensure(sizeof(double) * (size_t)n * tx0 * ty0 * NMOM_FULL));
+  CHK(r->tile_counter.ensure(sizeof(unsigned) * n));
+  CHK(r->done_counter.ensure(sizeof(unsigned) * 4));
+  for (int i = 0; i < 2; ++i) {
+    CHK(r->active[i].ensure(sizeof(int) * n));
+    CHK(r->active_count[i].ensure(sizeof(int) * 4));
+  }
+  CHK(r->frame_ptrs.ensure(sizeof(void*) * n * 2));
+  CHK(r->mask_ptrs.ensure(sizeof(void*) * n));
+  CHK(r->frame_list.ensure(sizeof(int) * n));
+  HIPCHK(hipMemsetAsync(r->tile_counter.p, 0, sizeof(unsigned) * n, r->stream));
+  HIPCHK(hipMemsetAsync(r->done_counter.p, 0, sizeof(unsigned) * 4, r->stream));
+  std::vector<int> list(n);
+  for (int i = 0; i < n; ++i) list[i] = i;
+  HIPCHK(hipMemcpyAsync(r->frame_list.p, list.data(), sizeof(int) * n, hipMemcpyHostToDevice, r->stream));
+  HIPCHK(hipStreamSynchronize(r->stream));
+  r->cap = n;
+  return SSG_OK;
+}
+
+// One ECC pass (translation-first or main) over all levels for n frames.
+// plain: the host waits for each iteration and stops when no frame is active,
+// the per-frame form of the reference loop.
+int run_ecc(ssg_registration* r, int n, int pass, bool has_cmask, bool plain) {
+  hipStream_t s = r->stream;
+  const int motion = pass == 0 ? SSG_MOTION_TRANSLATION : r->o.motion_type;
+  const int L = r->nlevels;
+  const int grid = ecc_persistent_grid();
+  const bool translation_first_ran = pass == 1 && r->o.motion_type != SSG_MOTION_TRANSLATION &&
+                                     r->o.ecc.ecch_estimate_translation_first && r->o.ecc.ecch_max_level != 0;
+  for (int l = L - 1; l >= 0; --l) {
+    // pass 0 works on xf_t; pass 1 on xf_m and seeds its translation from xf_t
+    // when the translation-first pass ran (xf_t == nullptr otherwise)
+    LevelBeginArgs lb{};
+    lb.xf_t = (pass == 0 || translation_first_ran) ? r->xf_t.as<FrameXf>() : nullptr;
+    lb.xf_m = r->xf_m.as<FrameXf>();
+    lb.res = r->res.as<FrameRes>();
+    lb.nframes = n;
+    lb.pass = pass;
+    lb.level = l;
+    lb.nlevels = L;
+    lb.ratio = l == L - 1 ? (double)r->lw[L - 1] / (double)r->lw[0] : (double)r->lw[l] / (double)r->lw[l + 1];
+    lb.main_motion = r->o.motion_type;
+    lb.cf = (float)r->ref_mean[l];
+    lb.active_out = r->active[0].as<int>();
+    lb.active_count_out = r->active_count[0].as<int>();
+    HIPCHK(launch_level_begin(lb, s));
+
+    EccPassArgs a{};
+    a.cur = r->cur_pyr.as<float>() + r->loff[l];
+    a.cur_fstride = r->pyr_elems;
+    a.cur_pitch = r->lpitch[l];
+    a.cw = r->lw[l];
+    a.ch = r->lh[l];
+    a.cmask = has_cmask ? r->cur_mask_pyr.as<uint8_t>() + r->loff[l] : nullptr;
+    a.cmask_fstride = r->mask_elems;
+    a.cmask_pitch = r->lpitch[l];
+    a.ref = r->ref_pyr.as<float>() + r->loff[l];
+    a.ref_pitch = r->lpitch[l];
+    a.w = r->lw[l];
+    a.h = r->lh[l];
+    a.rmask = r->ref_has_mask[l] ? r->ref_mask_pyr.as<uint8_t>() + r->loff[l] : nullptr;
+    a.rmask_pitch = r->lpitch[l];
+    a.cf = (float)r->ref_mean[l];
+    a.tiles_x = (a.w + ECC_TW - 1) / ECC_TW;
+    a.tiles_y = (a.h + ECC_TH - 1) / ECC_TH;
+    a.ntiles = a.tiles_x * a.tiles_y;
+    a.xf = pass == 0 ? r->xf_t.as<FrameXf>() : r->xf_m.as<FrameXf>();
+    a.res = r->res.as<FrameRes>();
+    a.partials = r->partials.as<float>();
+    a.tile_counter = r->tile_counter.as<unsigned>();
+    a.done_counter = r->done_counter.as<unsigned>();
+    a.nframes = n;
+    a.pass = pass;
+    a.level = l;
+    a.max_iterations = r->o.ecc.max_iterations;
+    double max_eps = r->o.ecc.eps * (double)(1 << l);
+    if (max_eps <= 0) max_eps = 1e-3;
+    a.max_eps = (float)max_eps;
+    a.update_step_scale = r->o.ecc.update_step_scale;
+    const int g = std::min(grid, std::max(8, n * a.ntiles));
+    for (int it = 0; it < r->o.ecc.max_iterations; ++it) {
+      a.active_in = r->active[it & 1].as<int>();
+      a.active_count_in = r->active_count[it & 1].as<int>();
+      a.active_out = r->active[(it + 1) & 1].as<int>();
+      a.active_count_out = r->active_count[(it + 1) & 1].as<int>();
+      r->prof.start(s);
+      HIPCHK(launch_ecc_pass(motion, a, g, s));
+      r->prof.stop(s, pass == 0 ? SSG_PROF_ECC_TRANSLATION : SSG_PROF_ECC_MAIN);
+      const bool check = plain || ((it & 3) == 3);
+      if (check && it + 1 < r->o.ecc.max_iterations) {
+        HIPCHK(hipMemcpyAsync(r->pinned_count, a.active_count_out, sizeof(int), hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+        if (*r->pinned_count == 0) break;
+      }
+    }
+  }
+  return SSG_OK;
+}
+
+int run_corr(ssg_registration* r, int n, int pass, bool has_cmask) {
+  CorrPassArgs c{};
+  c.cur = r->cur_pyr.as<float>() + r->loff[0];
+  c.cur_fstride = r->pyr_elems;
+  c.cur_pitch = r->lpitch[0];
+  c.cw = r->lw[0];
+  c.ch = r->lh[0];
+  c.cmask = has_cmask ? r->cur_mask_pyr.as<uint8_t>() + r->loff[0] : nullptr;
+  c.cmask_fstride = r->mask_elems;
+  c.cmask_pitch = r->lpitch[0];
+  c.ref = r->ref_pyr.as<float>() + r->loff[0];
+  c.ref_pitch = r->lpitch[0];
+  c.w = r->lw[0];
+  c.h = r->lh[0];
+  c.rmask = r->ref_has_mask[0] ? r->ref_mask_pyr.as<uint8_t>() + r->loff[0] : nullptr;
+  c.rmask_pitch = r->lpitch[0];
+  c.cf = (float)r->ref_mean[0];
+  c.tiles_x = (c.w + ECC_TW - 1) / ECC_TW;
+  c.tiles_y = (c.h + ECC_TH - 1) / ECC_TH;
+  c.ntiles = c.tiles_x * c.tiles_y;
+  c.xf = pass == 0 ? r->xf_t.as<FrameXf>() : r->xf_m.as<FrameXf>();
+  c.res = r->res.as<FrameRes>();
+  c.partials = r->partials.as<float>();
+  c.tile_counter = r->tile_counter.as<unsigned>();
+  c.frames = r->frame_list.as<int>();
+  c.nlist = n;
+  c.pass = pass;
+  c.min_rho = r->o.ecc.min_rho;
+  const int g = std::min(ecc_persistent_grid(), std::max(8, n * c.ntiles));
+  r->prof.start(r->stream);
+  HIPCHK(launch_corr_pass(pass == 0 ? SSG_MOTION_TRANSLATION : r->o.motion_type, c, g, r->stream));
+  r->prof.stop(r->stream, SSG_PROF_CORRELATION);
+  return SSG_OK;
+}
+
+// prepares level-0 sources (optionally the 0.5 pyrDown of scaleImage) and the pyramids of n frames
+int prepare_batch(ssg_registration* r, int n, const ssg_image* frames, const ssg_image* masks, bool* has_cmask) {
+  hipStream_t s = r->stream;
+  CHK(ensure_workspace(r, n));
+  std::vector<const float*> ptrs(n);
+  int src_pitch = 0;
+  const bool half = use_half(r->o);
+  size_t in_pitch = 0;
+  for (int f = 0; f < n; ++f) {
+    CHK(check_image(&frames[f], "frame"));
+    if (frames[f].type != SSG_32FC1) return fail(SSG_E_UNSUPPORTED, "frame %d: registration takes SSG_32FC1 (gray) images", f);
+    if (frames[f].width != r->ref_w || frames[f].height != r->ref_h)
+      return fail(SSG_E_INVALID_ARG, "frame %d: size %dx%d differs from the reference %dx%d", f, frames[f].width,
+                  frames[f].height, r->ref_w, r->ref_h);
+  }
+  // frames not on the device, or with different steps, go through one staging buffer
+  bool direct = true;
+  for (int f = 0; f < n; ++f)
+    if (!frames[f].on_device || frames[f].step != frames[0].step || (frames[f].step % 4)) direct = false;
+  if (direct) {
+    for (int f = 0; f < n; ++f) ptrs[f] = (const float*)frames[f].data;
+    src_pitch = (int)(frames[0].step / 4);
+  } else {
+    in_pitch = ((size_t)r->ref_w * 4 + 255) / 256 * 256;
+    CHK(r->tmp_in.ensure(in_pitch * r->ref_h * n));
+    for (int f = 0; f < n; ++f) {
+      uint8_t* dst = r->tmp_in.as<uint8_t>() + in_pitch * r->ref_h * f;
+      HIPCHK(hipMemcpy2DAsync(dst, in_pitch, frames[f].data, (size_t)frames[f].step, (size_t)r->ref_w * 4, r->ref_h,
+                              frames[f].on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, s));
+      ptrs[f] = (const float*)dst;
+    }
+    src_pitch = (int)(in_pitch / 4);
+  }
+  if (half) {
+    // scaleImage: pyrDown to the default half size (c_frame_registration.cc:230-250)
+    const size_t hp = ((size_t)r->ew * 4 + 255) / 256 * 256;
+    CHK(r->half_buf.ensure(hp * r->eh * n));
+    // pyrdown takes base + stride: run frame by frame on the pointers
+    for (int f = 0; f < n; ++f) {
+      HIPCHK(launch_pyrdown(ptrs[f], 0, src_pitch, r->ref_w, r->ref_h, r->half_buf.as<float>() + (hp / 4) * r->eh * f, 0,
+                            (int)(hp / 4), r->ew, r->eh, 1, BORDER_REFLECT101, s));
+      ptrs[f] = r->half_buf.as<float>() + (hp / 4) * r->eh * f;
+    }
+    src_pitch = (int)(hp / 4);
+  }
+  HIPCHK(hipMemcpyAsync(r->frame_ptrs.p, ptrs.data(), sizeof(void*) * n, hipMemcpyHostToDevice, s));
+  r->prof.start(s);
+  CHK(build_pyramids(r, r->frame_ptrs.as<const float*>(), src_pitch, n, r->o.ecc.input_smooth_sigma, r->cur_pyr.as<float>()));
+  r->prof.stop(s, SSG_PROF_PYRAMID);
+  {
+    // level 0 reads the frame and writes the smoothed level; each further level reads its parent
+    double b = 8.0 * r->lw[0] * r->lh[0];
+    for (int l = 1; l < r->nlevels; ++l) b += 4.0 * r->lw[l - 1] * r->lh[l - 1] + 4.0 * r->lw[l] * r->lh[l];
+    r->prof.add_bytes(SSG_PROF_PYRAMID, b * n);
+  }
+  *has_cmask = false;
+  if (masks) {
+    if (half) return fail(SSG_E_UNSUPPORTED, "current masks with ecc.scale 0.5");
+    std::vector<const uint8_t*> mp(n);
+    DevBuf tmpm;
+    const size_t mpitch = ((size_t)r->ref_w + 255) / 256 * 256;
+    bool mdirect = true;
+    for (int f = 0; f < n; ++f) {
+      CHK(check_image(&masks[f], "mask"));
+      if (masks[f].type != SSG_8UC1 || masks[f].width != r->ref_w || masks[f].height != r->ref_h)
+        return fail(SSG_E_INVALID_ARG, "mask %d must be SSG_8UC1 of the frame size", f);
+      if (!masks[f].on_device || masks[f].step != masks[0].step) mdirect = false;
+    }
+    int mpitch_el;
+    if (mdirect) {
+      for (int f = 0; f < n; ++f) mp[f] = (const uint8_t*)masks[f].data;
+      mpitch_el = (int)masks[0].step;
+    } else {
+      CHK(tmpm.ensure(mpitch * r->ref_h * n));
+      for (int f = 0; f < n; ++f) {
+        uint8_t* dst = tmpm.as<uint8_t>() + mpitch * r->ref_h * f;
+        HIPCHK(hipMemcpy2DAsync(dst, mpitch, masks[f].data, (size_t)masks[f].step, r->ref_w, r->ref_h,
+                                masks[f].on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, s));
+        mp[f] = dst;
+      }
+      mpitch_el = (int)mpitch;
+    }
+    CHK(r->mask_ptrs.ensure(sizeof(void*) * n));
+    HIPCHK(hipMemcpyAsync(r->mask_ptrs.p, mp.data(), sizeof(void*) * n, hipMemcpyHostToDevice, s));
+    CHK(r->cur_mask_pyr.ensure((size_t)r->mask_elems * n));
+    std::vector<char> keep;
+    CHK(build_mask_pyramids(r, r->mask_ptrs.as<const uint8_t*>(), mpitch_el, n, r->cur_mask_pyr.as<uint8_t>(), false, keep));
+    HIPCHK(hipStreamSynchronize(s));
+    *has_cmask = true;
+  }
+  return SSG_OK;
+}
+
+int register_n(ssg_registration* r, int n, const ssg_image* frames, const ssg_image* masks, ssg_registration_status* st,
+               bool plain) {
+  if (!r->have_ref) return fail(SSG_E_NOT_READY, "setup_reference_frame() was not called");
+  if (n <= 0) return SSG_OK;
+  if (!frames || !st) return fail(SSG_E_INVALID_ARG, "null frames or status");
+  hipStream_t s = r->stream;
+  bool has_cmask = false;
+  CHK(prepare_batch(r, n, frames, masks, &has_cmask));
+  HIPCHK(hipMemsetAsync(r->res.p, 0, sizeof(FrameRes) * n, s));
+  const bool tfirst = r->o.motion_type != SSG_MOTION_TRANSLATION && r->o.ecc.ecch_estimate_translation_first &&
+                      r->o.ecc.ecch_max_level != 0;
+  if (tfirst) {
+    CHK(run_ecc(r, n, 0, has_cmask, plain));
+    CHK(run_corr(r, n, 0, has_cmask));
+  }
+  CHK(run_ecc(r, n, 1, has_cmask, plain));
+  CHK(run_corr(r, n, 1, has_cmask));
+  std::vector<FrameXf> xf(n);
+  std::vector<FrameRes> res(n);
+  HIPCHK(hipMemcpyAsync(xf.data(), r->xf_m.p, sizeof(FrameXf) * n, hipMemcpyDeviceToHost, s));
+  HIPCHK(hipMemcpyAsync(res.data(), r->res.p, sizeof(FrameRes) * n, hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  if (r->prof.on) {
+    // algorithmic bytes of the ECC passes: every iteration of a frame at level l
+    // reads the warped current level and the reference level once (4 + 4 B/px,
+    // +1 B/px per mask); correlation passes read both level-0 images
+    const int maxit = r->o.ecc.max_iterations;
+    for (int f = 0; f < n; ++f) {
+      for (int p = 0; p < 2; ++p) {
+        if (p == 0 && !tfirst) continue;
+        if (p == 1 && res[f].rejected == SSG_REG_RHO_TRANSLATION) continue;
+        for (int l = 0; l < r->nlevels; ++l) {
+          const double bpp = 8.0 + (has_cmask ? 1.0 : 0.0) + (r->ref_has_mask[l] ? 1.0 : 0.0);
+          const int passes = std::min(res[f].level_iters[p][l], maxit);
+          r->prof.add_bytes(p == 0 ? SSG_PROF_ECC_TRANSLATION : SSG_PROF_ECC_MAIN, passes * bpp * r->lw[l] * r->lh[l]);
+        }
+      }
+      const int checks = (tfirst ? 1 : 0) + (res[f].rejected == SSG_REG_RHO_TRANSLATION ? 0 : 1);
+      r->prof.add_bytes(SSG_PROF_CORRELATION, checks * 8.0 * r->lw[0] * r->lh[0]);
+    }
+    r->prof.collect();
+  }
+  const int np = motion_nparams(r->o.motion_type);
+  for (int f = 0; f < n; ++f) {
+    ssg_registration_status& o = st[f];
+    memset(&o, 0, sizeof(o));
+    const FrameRes& R = res[f];
+    o.code = R.rejected ? R.rejected : SSG_REG_OK;
+    o.ok = R.rejected == 0;
+    o.rho_translation = tfirst ? R.rho[0] : 0.0;
+    o.rho = R.rejected == SSG_REG_RHO_TRANSLATION ? 0.0 : R.rho[1];
+    o.eps = R.eps;
+    for (int l = 0; l < r->nlevels; ++l) {
+      o.iterations_translation += tfirst ? R.level_iters[0][l] : 0;
+      o.iterations += R.level_iters[1][l];
+      o.levels_failed += (tfirst ? R.level_failed[0][l] : 0) + R.level_failed[1][l];
+      o.level_iterations[l] = R.level_iters[1][l];
+    }
+    o.nparams = np;
+    float p[6];
+    for (int i = 0; i < 6; ++i) p[i] = xf[f].p[i];
+    if (use_half(r->o)) scale_params(r->o.motion_type, p, 1.0 / r->o.ecc.scale);
+    for (int i = 0; i < 6; ++i) o.params[i] = i < np ? p[i] : 0.f;
+    params_matrix(r->o.motion_type, p, o.matrix);
+  }
+  return SSG_OK;
+}
+
+}  // namespace
+
+extern "C" int ssg_registration_setup_reference_frame(ssg_registration* r, const ssg_image* image, const ssg_image* mask) {
+  if (!r) return fail(SSG_E_INVALID_ARG, "null handle");
+  CHK(check_image(image, "reference image"));
+  if (image->type != SSG_32FC1) return fail(SSG_E_UNSUPPORTED, "reference image: SSG_32FC1 (gray) only");
+  if (mask) {
+    CHK(check_image(mask, "reference mask"));
+    if (mask->type != SSG_8UC1 || mask->width != image->width || mask->height != image->height)
+      return fail(SSG_E_INVALID_ARG, "reference mask must be SSG_8UC1 of the image size");
+  }
+  hipStream_t s = r->stream;
+  r->have_ref = false;
+  r->cap = 0;
+  r->ref_w = image->width;
+  r->ref_h = image->height;
+  const bool half = use_half(r->o);
+  r->ew = half ? (r->ref_w + 1) / 2 : r->ref_w;
+  r->eh = half ? (r->ref_h + 1) / 2 : r->ref_h;
+  if (half && mask) return fail(SSG_E_UNSUPPORTED, "reference mask with ecc.scale 0.5");
+  // level count (ecc2.cc:1009-1026)
+  const int min_size = std::max(4, r->o.ecc.ecch_minimum_image_size);
+  int tw = r->ew, th = r->eh, lv = 1;
+  for (;;) {
+    const int maxlevel = r->o.ecc.ecch_max_level;
+    if (maxlevel >= 0 && lv >= std::max(0, maxlevel)) break;
+    const int nw = ((tw + 1) >> 1) & ~1, nh = ((th + 1) >> 1) & ~1;
+    if (nw < min_size || nh < min_size) break;
+    tw = nw;
+    th = nh;
+    if (++lv >= MAX_LEVELS) break;
+  }
+  r->nlevels = lv;
+  long long off = 0, moff = 0;
+  int w = r->ew, h = r->eh;
+  for (int l = 0; l < lv; ++l) {
+    r->lw[l] = w;
+    r->lh[l] = h;
+    r->lpitch[l] = (w + 63) / 64 * 64;
+    r->loff[l] = off;
+    off += (long long)r->lpitch[l] * h;
+    moff += (long long)r->lpitch[l] * h;
+    w = ((w + 1) >> 1) & ~1;
+    h = ((h + 1) >> 1) & ~1;
+  }
+  r->pyr_elems = off;
+  r->mask_elems = moff;
+  CHK(r->ref_pyr.ensure(sizeof(float) * off));
+  CHK(r->frame_ptrs.ensure(sizeof(void*) * 2));
+  // level-0 source
+  DevBuf tmp;
+  const void* dptr;
+  size_t pitch_b;
+  CHK(device_view(image, tmp, s, &dptr, &pitch_b));
+  const float* src = (const float*)dptr;
+  int src_pitch = (int)(pitch_b / 4);
+  DevBuf halfb;
+  if (half) {
+    const size_t hp = ((size_t)r->ew * 4 + 255) / 256 * 256;
+    CHK(halfb.ensure(hp * r->eh));
+    HIPCHK(launch_pyrdown(src, 0, src_pitch, r->ref_w, r->ref_h, halfb.as<float>(), 0, (int)(hp / 4), r->ew, r->eh, 1,
+                          BORDER_REFLECT101, s));
+    src = halfb.as<const float>();
+    src_pitch = (int)(hp / 4);
+  }
+  HIPCHK(hipMemcpyAsync(r->frame_ptrs.p, &src, sizeof(void*), hipMemcpyHostToDevice, s));
+  CHK(build_pyramids(r, r->frame_ptrs.as<const float*>(), src_pitch, 1, r->o.ecc.reference_smooth_sigma,
+                     r->ref_pyr.as<float>()));
+  for (int l = 0; l < lv; ++l) r->ref_has_mask[l] = false;
+  if (mask) {
+    DevBuf tmpm;
+    const void* mptr;
+    size_t mpitch;
+    CHK(device_view(mask, tmpm, s, &mptr, &mpitch));
+    CHK(r->ref_mask_pyr.ensure((size_t)moff));
+    CHK(r->mask_ptrs.ensure(sizeof(void*)));
+    HIPCHK(hipMemcpyAsync(r->mask_ptrs.p, &mptr, sizeof(void*), hipMemcpyHostToDevice, s));
+    std::vector<char> keep;
+    CHK(build_mask_pyramids(r, r->mask_ptrs.as<const uint8_t*>(), (int)mpitch, 1, r->ref_mask_pyr.as<uint8_t>(), true, keep));
+    for (int l = 0; l < lv; ++l) r->ref_has_mask[l] = keep[l] != 0;
+  }
+  // level means (centring constants)
+  CHK(r->sum_tmp.ensure(sizeof(double) * (r->lh[0] + 1)));
+  for (int l = 0; l < lv; ++l) {
+    HIPCHK(launch_sum_f32(r->ref_pyr.as<float>() + r->loff[l], r->lpitch[l], r->lw[l], r->lh[l], r->sum_tmp.as<double>(), s));
+    double sum = 0;
+    HIPCHK(hipMemcpyAsync(&sum, r->sum_tmp.as<double>() + r->lh[l], sizeof(double), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    r->ref_mean[l] = sum / ((double)r->lw[l] * r->lh[l]);
+  }
+  HIPCHK(hipStreamSynchronize(s));
+  r->have_ref = true;
+  return SSG_OK;
+}
+
+extern "C" int ssg_registration_register_frame(ssg_registration* r, const ssg_image* image, const ssg_image* mask,
+                                               ssg_registration_status* status) {
+  if (!r) return fail(SSG_E_INVALID_ARG, "null handle");
+  return register_n(r, 1, image, mask, status, true);
+}
+
+extern "C" int ssg_registration_register_batch(ssg_registration* r, int n, const ssg_image* frames,
+                                               const ssg_image* masks, ssg_registration_status* status) {
+  if (!r) return fail(SSG_E_INVALID_ARG, "null handle");
+  if (n < 0) return fail(SSG_E_INVALID_ARG, "n < 0");
+  return register_n(r, n, frames, masks, status, false);
+}
+
+extern "C" int ssg_registration_create_remap(ssg_registration* r, const ssg_registration_status* st, ssg_image* map) {
+  if (!r || !st) return fail(SSG_E_INVALID_ARG, "null argument");
+  if (!r->have_ref) return fail(SSG_E_NOT_READY, "no reference frame");
+  CHK(check_image(map, "map"));
+  if (map->type != SSG_32FC2 || map->width != r->ref_w || map->height != r->ref_h)
+    return fail(SSG_E_INVALID_ARG, "map must be SSG_32FC2 of the reference size %dx%d", r->ref_w, r->ref_h);
+  const MapCoef mc = map_coef(r->o.motion_type, st->params, nullptr);
+  hipStream_t s = r->stream;
+  DevBuf tmp;
+  float* dst;
+  size_t pitch;
+  if (map->on_device) {
+    dst = (float*)map->data;
+    pitch = (size_t)map->step;
+  } else {
+    pitch = ((size_t)map->width * 8 + 255) / 256 * 256;
+    CHK(tmp.ensure(pitch * map->height));
+    dst = tmp.as<float>();
+  }
+  HIPCHK(launch_create_map(r->o.motion_type, mc, dst, (int)(pitch / 4), map->width, map->height, s));
+  if (!map->on_device)
+    HIPCHK(hipMemcpy2DAsync(map->data, (size_t)map->step, dst, pitch, (size_t)map->width * 8, map->height,
+                            hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  return SSG_OK;
+}
+
+extern "C" int ssg_remap(const ssg_image* src, const ssg_image* src_mask, const ssg_image* map, const double* matrix,
+                         int32_t motion_type, const float* params, ssg_image* dst, ssg_image* dst_mask,
+                         int32_t interpolation, int32_t border_mode, double border_value) {
+  (void)matrix;
+  if (!dst && !dst_mask) return SSG_OK;
+  if (!map && !params) return fail(SSG_E_INVALID_ARG, "need a map field or transform parameters");
+  if (interpolation != SSG_INTER_LINEAR && interpolation != SSG_INTER_CUBIC && interpolation != SSG_INTER_NEAREST)
+    return fail(SSG_E_UNSUPPORTED, "interpolation %d", interpolation);
+  hipStream_t s = 0;
+  HIPCHK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+  struct StreamGuard {
+    hipStream_t s;
+    ~StreamGuard() { (void)hipStreamSynchronize(s); (void)hipStreamDestroy(s); }
+  } guard{s};
+  CHK(ensure_tables(s));
+  int dw = dst ? dst->width : dst_mask->width, dh = dst ? dst->height : dst_mask->height;
+  int motion = map ? -1 : motion_type;
+  MapCoef mc = map ? MapCoef{} : map_coef(motion_type, params, nullptr);
+  DevBuf tmap;
+  const float* dmap = nullptr;
+  int map_pitch = 0;
+  if (map) {
+    CHK(check_image(map, "map"));
+    if (map->type != SSG_32FC2 || map->width != dw || map->height != dh)
+      return fail(SSG_E_INVALID_ARG, "map must be SSG_32FC2 of the destination size");
+    const void* p;
+    size_t pb;
+    CHK(device_view(map, tmap, s, &p, &pb));
+    dmap = (const float*)p;
+    map_pitch = (int)(pb / 4);
+  }
+  if (dst) {
+    CHK(check_image(src, "src"));
+    CHK(check_image(dst, "dst"));
+    if ((src->type != SSG_32FC1 && src->type != SSG_32FC3) || dst->type != src->type)
+      return fail(SSG_E_UNSUPPORTED, "remap: SSG_32FC1/SSG_32FC3 images of the same type");
+    const int cn = channels_of(src->type);
+    DevBuf tsrc, tdst;
+    const void* sp;
+    size_t spb;
+    CHK(device_view(src, tsrc, s, &sp, &spb));
+    float* dp;
+    size_t dpb;
+    if (dst->on_device) {
+      dp = (float*)dst->data;
+      dpb = (size_t)dst->step;
+    } else {
+      dpb = ((size_t)dw * 4 * cn + 255) / 256 * 256;
+      CHK(tdst.ensure(dpb * dh));
+      dp = tdst.as<float>();
+      // TRANSPARENT keeps the destination: start from its current contents
+      HIPCHK(hipMemcpy2DAsync(dp, dpb, dst->data, (size_t)dst->step, (size_t)dw * 4 * cn, dh, hipMemcpyHostToDevice, s));
+    }
+    HIPCHK(launch_remap(motion, (const float*)sp, (int)(spb / 4), src->width, src->height, cn, mc, dmap, map_pitch, dp,
+                        (int)(dpb / 4), dw, dh, interpolation, border_mode, (float)border_value, s));
+    if (!dst->on_device)
+      HIPCHK(hipMemcpy2DAsync(dst->data, (size_t)dst->step, dp, dpb, (size_t)dw * 4 * cn, dh, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+  }
+  if (dst_mask) {
+    CHK(check_image(dst_mask, "dst_mask"));
+    if (dst_mask->type != SSG_8UC1) return fail(SSG_E_INVALID_ARG, "dst_mask must be SSG_8UC1");
+    int sw = src ? src->width : 0, sh = src ? src->height : 0;
+    const uint8_t* smp = nullptr;
+    int smpitch = 0;
+    DevBuf tsm;
+    if (src_mask) {
+      CHK(check_image(src_mask, "src_mask"));
+      sw = src_mask->width;
+      sh = src_mask->height;
+      const void* p;
+      size_t pb;
+      CHK(device_view(src_mask, tsm, s, &p, &pb));
+      smp = (const uint8_t*)p;
+      smpitch = (int)pb;
+    }
+    if (sw <= 0) return fail(SSG_E_INVALID_ARG, "mask source size unknown");
+    DevBuf m1, m2;
+    const size_t mp = ((size_t)dw + 255) / 256 * 256;
+    CHK(m1.ensure(mp * dh));
+    CHK(m2.ensure(mp * dh));
+    HIPCHK(launch_remap_mask(motion, smp, smpitch, sw, sh, mc, dmap, map_pitch, m1.as<uint8_t>(), (int)mp, dw, dh, s,
+                             interpolation == SSG_INTER_CUBIC ? INTER_CUBIC : INTER_LINEAR));
+    HIPCHK(launch_erode5(m1.as<uint8_t>(), 0, (int)mp, m2.as<uint8_t>(), 0, (int)mp, dw, dh, 1, BORDER_CONSTANT, 255, s));
+    HIPCHK(hipMemcpy2DAsync(dst_mask->data, (size_t)dst_mask->step, m2.p, mp, dw, dh,
+                            dst_mask->on_device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+  }
+  return SSG_OK;
+}
+
+// =====================================================================================================
+// accumulation
+// =====================================================================================================
+struct ssg_accumulation {
+  int method = SSG_ACC_WEIGHTED_AVERAGE;
+  int w = 0, h = 0, cn = 1;
+  long long plane = 0;
+  hipStream_t stream = nullptr;
+  DevBuf acc, cnt, part, part_cnt, maps, weights, valid, frame_ptrs, tmp;
+  int accumulated = 0;
+  Profiler prof;
+};
+
+extern "C" int ssg_accumulation_create(int32_t method, int32_t width, int32_t height, int32_t channels,
+                                       ssg_accumulation** out) {
+  if (!out) return fail(SSG_E_INVALID_ARG, "out is null");
+  *out = nullptr;
+  if (method != SSG_ACC_WEIGHTED_AVERAGE && method != SSG_ACC_AVERAGE)
+    return fail(SSG_E_UNSUPPORTED, "accumulation method %d", method);
+  if (width <= 0 || height <= 0 || (channels != 1 && channels != 3))
+    return fail(SSG_E_INVALID_ARG, "accumulator %dx%dx%d", width, height, channels);
+  ssg_accumulation* a = new ssg_accumulation();
+  a->method = method;
+  a->w = width;
+  a->h = height;
+  a->cn = channels;
+  a->plane = (long long)width * height;
+  hipError_t e = hipStreamCreateWithFlags(&a->stream, hipStreamNonBlocking);
+  if (e != hipSuccess) {
+    delete a;
+    return fail(SSG_E_HIP, "hipStreamCreate: %s", hipGetErrorString(e));
+  }
+  int rc = a->acc.ensure(sizeof(float) * a->plane * (2 * channels + 2));
+  if (rc == SSG_OK) rc = a->cnt.ensure(sizeof(unsigned) * a->plane);
+  if (rc == SSG_OK) rc = ensure_tables(a->stream);
+  if (rc != SSG_OK) {
+    (void)hipStreamDestroy(a->stream);
+    delete a;
+    return rc;
+  }
+  *out = a;
+  return ssg_accumulation_clear(a);
+}
+
+extern "C" void ssg_accumulation_destroy(ssg_accumulation* a) {
+  if (!a) return;
+  if (a->stream) {
+    (void)hipStreamSynchronize(a->stream);
+    (void)hipStreamDestroy(a->stream);
+  }
+  delete a;
+}
+
+extern "C" int ssg_accumulation_clear(ssg_accumulation* a) {
+  if (!a) return fail(SSG_E_INVALID_ARG, "null handle");
+  HIPCHK(hipMemsetAsync(a->acc.p, 0, sizeof(float) * a->plane * (2 * a->cn + 2), a->stream));
+  HIPCHK(hipMemsetAsync(a->cnt.p, 0, sizeof(unsigned) * a->plane, a->stream));
+  HIPCHK(hipStreamSynchronize(a->stream));
+  a->accumulated = 0;
+  return SSG_OK;
+}
+
+extern "C" int ssg_accumulation_add(ssg_accumulation* a, const ssg_image* src, const ssg_image* weights) {
+  if (!a) return fail(SSG_E_INVALID_ARG, "null handle");
+  CHK(check_image(src, "src"));
+  if (src->width != a->w || src->height != a->h)
+    return fail(SSG_E_INVALID_ARG, "current frame (%dx%d) and accumulator (%dx%d) sizes not match", src->width, src->height,
+                a->w, a->h);
+  if (channels_of(src->type) != a->cn || (src->type != SSG_32FC1 && src->type != SSG_32FC3))
+    return fail(SSG_E_INVALID_ARG, "frame type %d does not match the accumulator (%d channels, f32)", src->type, a->cn);
+  hipStream_t s = a->stream;
+  DevBuf ts, tw;
+  const void* sp;
+  size_t spb;
+  CHK(device_view(src, ts, s, &sp, &spb));
+  const uint8_t* mp = nullptr;
+  const float* wp = nullptr;
+  size_t wpb = 0;
+  if (weights) {
+    CHK(check_image(weights, "weights"));
+    if (weights->width != a->w || weights->height != a->h) return fail(SSG_E_INVALID_ARG, "weights size");
+    const void* p;
+    CHK(device_view(weights, tw, s, &p, &wpb));
+    if (weights->type == SSG_8UC1) mp = (const uint8_t*)p;
+    else if (weights->type == SSG_32FC1) wp = (const float*)p;
+    else return fail(SSG_E_INVALID_ARG, "weights must be SSG_8UC1 or SSG_32FC1");
+  }
+  HIPCHK(launch_acc_add((const float*)sp, (int)(spb / 4), a->w, a->h, a->cn, mp, (int)wpb, wp, (int)(wpb / 4),
+                        a->acc.as<float>(), nullptr, a->cnt.as<unsigned>(), a->plane, s));
+  HIPCHK(hipStreamSynchronize(s));
+  ++a->accumulated;
+  return SSG_OK;
+}
+
+extern "C" int ssg_accumulation_add_registered_batch(ssg_accumulation* a, int n, const ssg_image* frames,
+                                                     const ssg_registration_status* statuses, int32_t motion_type,
+                                                     const float* frame_weights, int32_t interpolation,
+                                                     int32_t border_mode, double border_value) {
+  if (!a) return fail(SSG_E_INVALID_ARG, "null handle");
+  if (n <= 0) return SSG_OK;
+  if (!frames || !statuses) return fail(SSG_E_INVALID_ARG, "null frames or statuses");
+  if (interpolation != SSG_INTER_LINEAR && interpolation != SSG_INTER_CUBIC)
+    return fail(SSG_E_UNSUPPORTED, "interpolation %d", interpolation);
+  if (motion_type < SSG_MOTION_TRANSLATION || motion_type > SSG_MOTION_AFFINE)
+    return fail(SSG_E_INVALID_ARG, "motion_type %d", motion_type);
+  const int sw = frames[0].width, sh = frames[0].height;
+  const int64_t step = frames[0].step;
+  for (int f = 0; f < n; ++f) {
+    CHK(check_image(&frames[f], "frame"));
+    if (!frames[f].on_device) return fail(SSG_E_INVALID_ARG, "frame %d: batched frames must be device images", f);
+    if (frames[f].width != sw || frames[f].height != sh || frames[f].step != step || frames[f].type != frames[0].type)
+      return fail(SSG_E_INVALID_ARG, "frame %d: all frames of a batch share size, step and type", f);
+    if (channels_of(frames[f].type) != a->cn) return fail(SSG_E_INVALID_ARG, "frame %d: channel count", f);
+    if (step % 4) return fail(SSG_E_INVALID_ARG, "frame step must be a multiple of 4");
+  }
+  hipStream_t s = a->stream;
+  std::vector<MapCoef> maps(n);
+  std::vector<float> w(n);
+  std::vector<int> valid(n);
+  std::vector<const float*> ptrs(n);
+  int nvalid = 0;
+  for (int f = 0; f < n; ++f) {
+    maps[f] = map_coef(motion_type, statuses[f].params, nullptr);
+    w[f] = frame_weights ? frame_weights[f] : 1.f;
+    // a weight <= 0 contributes nothing (c_frame_accumulation.cc:112: w > 0)
+    valid[f] = statuses[f].ok && w[f] > 0.f;
+    nvalid += valid[f];
+    ptrs[f] = (const float*)frames[f].data;
+  }
+  CHK(a->maps.ensure(sizeof(MapCoef) * n));
+  CHK(a->weights.ensure(sizeof(float) * n));
+  CHK(a->valid.ensure(sizeof(int) * n));
+  CHK(a->frame_ptrs.ensure(sizeof(void*) * n));
+  HIPCHK(hipMemcpyAsync(a->maps.p, maps.data(), sizeof(MapCoef) * n, hipMemcpyHostToDevice, s));
+  HIPCHK(hipMemcpyAsync(a->weights.p, w.data(), sizeof(float) * n, hipMemcpyHostToDevice, s));
+  HIPCHK(hipMemcpyAsync(a->valid.p, valid.data(), sizeof(int) * n, hipMemcpyHostToDevice, s));
+  HIPCHK(hipMemcpyAsync(a->frame_ptrs.p, ptrs.data(), sizeof(void*) * n, hipMemcpyHostToDevice, s));
+  // chunks: enough blocks to fill the chip, each chunk a contiguous frame range
+  const long long tiles = (long long)((a->w + 63) / 64) * ((a->h + 15) / 16);
+  int nch = (int)std::max(1LL, std::min<long long>(16, (4096 + tiles - 1) / tiles));
+  nch = std::min(nch, n);
+  const int per = (n + nch - 1) / nch;
+  nch = (n + per - 1) / per;
+  CHK(a->part.ensure(sizeof(float) * a->plane * (2 * a->cn + 2) * nch));
+  CHK(a->part_cnt.ensure(sizeof(unsigned) * a->plane * nch));
+  AccumArgs g{};
+  g.frames = a->frame_ptrs.as<const float*>();
+  g.src_pitch = (int)(step / 4);
+  g.sw = sw;
+  g.sh = sh;
+  g.cn = a->cn;
+  g.maps = a->maps.as<MapCoef>();
+  g.weights = a->weights.as<float>();
+  g.valid = a->valid.as<int>();
+  g.nframes = n;
+  g.per_chunk = per;
+  g.dw = a->w;
+  g.dh = a->h;
+  g.interp = interpolation == SSG_INTER_CUBIC ? INTER_CUBIC : INTER_LINEAR;
+  g.border = border_mode;
+  g.cval = (float)border_value;
+  g.part = a->part.as<float>();
+  g.part_cnt = a->part_cnt.as<unsigned>();
+  g.plane = a->plane;
+  g.nchunks = nch;
+  a->prof.start(s);
+  HIPCHK(launch_accumulate(motion_type, g, s));
+  a->prof.stop(s, SSG_PROF_ACCUMULATE);
+  HIPCHK(launch_accum_merge(a->part.as<float>(), a->part_cnt.as<unsigned>(), nch, a->plane, a->cn, a->acc.as<float>(),
+                            nullptr, a->cnt.as<unsigned>(), s));
+  HIPCHK(hipStreamSynchronize(s));
+  // algorithmic bytes of the fused kernel: each registered frame is read once
+  // (4 B per channel and destination pixel); each chunk writes its partial planes
+  a->prof.add_bytes(SSG_PROF_ACCUMULATE, (double)nvalid * 4.0 * a->cn * a->plane +
+                                             (double)nch * ((2.0 * a->cn + 2.0) * 4.0 + 4.0) * a->plane);
+  a->prof.collect();
+  a->accumulated += nvalid;
+  return SSG_OK;
+}
+
+extern "C" int ssg_accumulation_compute(ssg_accumulation* a, ssg_image* avg, ssg_image* mask, double dscale) {
+  if (!a) return fail(SSG_E_INVALID_ARG, "null handle");
+  if (a->accumulated < 1) return fail(SSG_E_NOT_READY, "no frames accumulated");
+  hipStream_t s = a->stream;
+  DevBuf ta, tm;
+  float* ap = nullptr;
+  size_t apb = 0;
+  uint8_t* mp = nullptr;
+  size_t mpb = 0;
+  if (avg) {
+    CHK(check_image(avg, "avg"));
+    if (avg->width != a->w || avg->height != a->h || channels_of(avg->type) != a->cn ||
+        (avg->type != SSG_32FC1 && avg->type != SSG_32FC3))
+      return fail(SSG_E_INVALID_ARG, "avg must be f32 with the accumulator's size and channels");
+    if (avg->on_device) {
+      ap = (float*)avg->data;
+      apb = (size_t)avg->step;
+    } else {
+      apb = ((size_t)a->w * 4 * a->cn + 255) / 256 * 256;
+      CHK(ta.ensure(apb * a->h));
+      ap = ta.as<float>();
+    }
+  }
+  if (mask) {
+    CHK(check_image(mask, "mask"));
+    if (mask->type != SSG_8UC1 || mask->width != a->w || mask->height != a->h) return fail(SSG_E_INVALID_ARG, "mask");
+    if (mask->on_device) {
+      mp = (uint8_t*)mask->data;
+      mpb = (size_t)mask->step;
+    } else {
+      mpb = ((size_t)a->w + 255) / 256 * 256;
+      CHK(tm.ensure(mpb * a->h));
+      mp = tm.as<uint8_t>();
+    }
+  }
+  HIPCHK(launch_acc_compute(a->acc.as<float>(), nullptr, a->w, a->h, a->cn, a->plane, ap, (int)(apb / 4), mp, (int)mpb,
+                            dscale, s));
+  if (avg && !avg->on_device)
+    HIPCHK(hipMemcpy2DAsync(avg->data, (size_t)avg->step, ap, apb, (size_t)a->w * 4 * a->cn, a->h, hipMemcpyDeviceToHost, s));
+  if (mask && !mask->on_device)
+    HIPCHK(hipMemcpy2DAsync(mask->data, (size_t)mask->step, mp, mpb, a->w, a->h, hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  return SSG_OK;
+}
+
+extern "C" int ssg_accumulation_get_acc_counters(ssg_accumulation* a, ssg_image* accw) {
+  if (!a) return fail(SSG_E_INVALID_ARG, "null handle");
+  CHK(check_image(accw, "accw"));
+  if (accw->type != SSG_32FC1 || accw->width != a->w || accw->height != a->h) return fail(SSG_E_INVALID_ARG, "accw");
+  // weight plane hi (the lo plane holds the rounding residue)
+  const float* hi = a->acc.as<float>() + (size_t)(2 * a->cn) * a->plane;
+  HIPCHK(hipMemcpy2DAsync(accw->data, (size_t)accw->step, hi, (size_t)a->w * 4, (size_t)a->w * 4, a->h,
+                          accw->on_device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, a->stream));
+  HIPCHK(hipStreamSynchronize(a->stream));
+  return SSG_OK;
+}
+
+extern "C" int ssg_accumulation_get_counts(ssg_accumulation* a, ssg_image* counts) {
+  if (!a) return fail(SSG_E_INVALID_ARG, "null handle");
+  CHK(check_image(counts, "counts"));
+  if (counts->type != SSG_32SC1 || counts->width != a->w || counts->height != a->h) return fail(SSG_E_INVALID_ARG, "counts");
+  HIPCHK(hipMemcpy2DAsync(counts->data, (size_t)counts->step, a->cnt.p, (size_t)a->w * 4, (size_t)a->w * 4, a->h,
+                          counts->on_device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, a->stream));
+  HIPCHK(hipStreamSynchronize(a->stream));
+  return SSG_OK;
+}
+
+extern "C" int ssg_accumulation_accumulated_frames(ssg_accumulation* a, int32_t* n) {
+  if (!a || !n) return fail(SSG_E_INVALID_ARG, "null argument");
+  *n = a->accumulated;
+  return SSG_OK;
+}
+
+extern "C" int ssg_registration_set_profiling(ssg_registration* r, int enable) {
+  if (!r) return fail(SSG_E_INVALID_ARG, "null handle");
+  r->prof.on = enable != 0;
+  return SSG_OK;
+}
+
+extern "C" int ssg_registration_get_profile(ssg_registration* r, ssg_profile* out, int reset) {
+  if (!r || !out) return fail(SSG_E_INVALID_ARG, "null argument");
+  *out = r->prof.acc;
+  if (reset) r->prof.acc = ssg_profile{};
+  return SSG_OK;
+}
+
+extern "C" int ssg_accumulation_set_profiling(ssg_accumulation* a, int enable) {
+  if (!a) return fail(SSG_E_INVALID_ARG, "null handle");
+  a->prof.on = enable != 0;
+  return SSG_OK;
+}
+
+extern "C" int ssg_accumulation_get_profile(ssg_accumulation* a, ssg_profile* out, int reset) {
+  if (!a || !out) return fail(SSG_E_INVALID_ARG, "null argument");
+  *out = a->prof.acc;
+  if (reset) a->prof.acc = ssg_profile{};
+  return SSG_OK;
+}
+
+extern "C" int ssg_frame_sharpness(int n, const ssg_image* frames, float* weights) {
+  if (n <= 0) return SSG_OK;
+  if (!frames || !weights) return fail(SSG_E_INVALID_ARG, "null frames or weights");
+  for (int f = 0; f < n; ++f) {
+    CHK(check_image(&frames[f], "frame"));
+    if (!frames[f].on_device || frames[f].type != SSG_32FC1 || frames[f].width != frames[0].width ||
+        frames[f].height != frames[0].height || frames[f].step != frames[0].step || frames[f].step % 4)
+      return fail(SSG_E_INVALID_ARG, "frame %d: SSG_32FC1 device images of one size and step", f);
+  }
+  const int w = frames[0].width, h = frames[0].height;
+  const int nb = (h + 15) / 16;
+  hipStream_t s = 0;
+  HIPCHK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+  struct StreamGuard {
+    hipStream_t s;
+    ~StreamGuard() { (void)hipStreamSynchronize(s); (void)hipStreamDestroy(s); }
+  } guard{s};
+  DevBuf ptrs, part;
+  CHK(ptrs.ensure(sizeof(void*) * n));
+  CHK(part.ensure(sizeof(double) * (size_t)n * nb));
+  std::vector<const float*> hp(n);
+  for (int f = 0; f < n; ++f) hp[f] = (const float*)frames[f].data;
+  HIPCHK(hipMemcpyAsync(ptrs.p, hp.data(), sizeof(void*) * n, hipMemcpyHostToDevice, s));
+  HIPCHK(launch_sharpness(ptrs.as<const float*>(), (int)(frames[0].step / 4), w, h, n, part.as<double>(), s));
+  std::vector<double> hpart((size_t)n * nb);
+  HIPCHK(hipMemcpyAsync(hpart.data(), part.p, sizeof(double) * hpart.size(), hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  for (int f = 0; f < n; ++f) {
+    double t = 0;
+    for (int b = 0; b < nb; ++b) t += hpart[(size_t)f * nb + b];
+    weights[f] = (float)(t / ((double)w * h));
+  }
+  return SSG_OK;
+}

@@ -1,0 +1,697 @@
+// ssg_ecc.hip -- batched forward-additive ECC for gfx950.
+//
+// One launch = one ECC iteration (ecc2.cc:1295-1362) for every active frame
+// of a batch.  A persistent grid walks (frame, tile) items.  Per 64x64 tile a
+// block:
+//   1. samples the current level image at W(x;p) into LDS with a 2-px halo
+//      (cv::remap INTER_LINEAR, BORDER_REPLICATE, 1/32-px positions),
+//   2. differentiates the warped tile (ecc_differentiate: d5 x s3 separable,
+//      BORDER_REPLICATE of the warped image),
+//   3. evaluates the warped-mask predicate (remap(mask) >= 255, exact u8
+//      fixed point) and the reference mask,
+//   4. accumulates the affine moments of the steepest-descent images:
+//      sum (gx^2, gx gy, gy^2) x {1,x,y,x^2,xy,y^2}, sum (gx,gy) x (g', f', 1)
+//      x {1,x,y}, and the masked first/second moments of f and g, with
+//      f' = f - cf and g' = g - cg centred to limit cancellation,
+//   5. reduces the 41 moments across the wave by recursive halving, across
+//      waves through LDS, and stores the tile's moments in global
+//      coordinates (FP64).
+// The block that finishes a frame's last tile sums the tiles in a fixed order
+// (deterministic), projects the affine moments onto the parameters of the
+// transform (J = M J_affine for translation and euclidean), and solves the
+// normal equations in FP64 (Cholesky; failure rule of cv::invert).  It
+// updates the parameters, eps and the frame's convergence state.  The block
+// that finishes the last frame compacts the active-frame list for the next
+// launch.  No block ever waits on another.
+#include <float.h>
+
+#include "ssg_engine.hpp"
+
+namespace ssg {
+
+constexpr float D5_K1 = 2.f / 3.f;    // d5[3] of ecc_differentiate (ecc2.cc:149)
+constexpr float D5_K2 = -1.f / 12.f;  // d5[4]
+
+__device__ __forceinline__ int motion_nparams(int m) {
+  return m == MOTION_TRANSLATION ? 2 : (m == MOTION_EUCLIDEAN ? 3 : (m == MOTION_SCALED_EUCLIDEAN ? 4 : 6));
+}
+
+// map coefficients of a parameter vector (see warp_xy)
+__device__ void update_map(int motion, FrameXf& X) {
+  if (motion == MOTION_TRANSLATION) {
+    X.mc.c[0] = X.p[0];
+    X.mc.c[1] = X.p[1];
+  } else if (motion == MOTION_AFFINE) {
+    for (int i = 0; i < 6; ++i) X.mc.c[i] = X.p[i];
+  } else {
+    const float ang = X.p[2];
+    const float s = motion == MOTION_SCALED_EUCLIDEAN ? X.p[3] : 1.f;
+    X.mc.c[0] = (float)sin((double)ang);
+    X.mc.c[1] = (float)cos((double)ang);
+    X.mc.c[2] = s;
+    X.mc.c[3] = X.p[0];
+    X.mc.c[4] = X.p[1];
+    X.mc.c[5] = X.C[0];
+    X.mc.c[6] = X.C[1];
+  }
+}
+
+// c_image_transform::scale_transfrom (c_image_transform.cc:130-134, 502-507, 911-915)
+__device__ void scale_xf(int motion, FrameXf& X, double f) {
+  if (motion == MOTION_AFFINE) {
+    X.p[2] = (float)((double)X.p[2] * f);
+    X.p[5] = (float)((double)X.p[5] * f);
+  } else {
+    X.p[0] = (float)((double)X.p[0] * f);
+    X.p[1] = (float)((double)X.p[1] * f);
+    if (motion != MOTION_TRANSLATION) {
+      X.C[0] = (float)((double)X.C[0] * f);
+      X.C[1] = (float)((double)X.C[1] * f);
+    }
+  }
+}
+
+// eps() of the transforms (c_image_transform.cc:136-139, 509-523, 917-924).
+// Euclidean with a fixed scale reads ds from the transform's scale (1), as the
+// reference does, so its eps never drops below max(W,H).
+__device__ float xf_eps(int motion, const float* dp, int w, int h) {
+  if (motion == MOTION_TRANSLATION) return (float)sqrt((double)dp[0] * dp[0] + (double)dp[1] * dp[1]);
+  if (motion == MOTION_AFFINE) {
+    const double a = (double)w * dp[0], b = (double)h * dp[1], c = dp[2];
+    const double d = (double)w * dp[3], e = (double)h * dp[4], f = dp[5];
+    return (float)sqrt(a * a + b * b + c * c + d * d + e * e + f * f);
+  }
+  const double ds = motion == MOTION_SCALED_EUCLIDEAN ? (double)dp[3] : 1.0;
+  const double sa = sin((double)dp[2]);
+  const double m = (double)(w > h ? w : h);
+  const double ws = w * sa, hs = h * sa, ms = m * ds;
+  return (float)sqrt((double)dp[0] * dp[0] + (double)dp[1] * dp[1] + ws * ws + hs * hs + ms * ms);
+}
+
+// global-coordinate moment k of a tile from its tile-local moments
+__device__ double expand_moment(int k, const float* loc, double X0, double Y0) {
+  if (k < 18) {
+    const int b = (k / 6) * 6, i = k - b;
+    const double s0 = loc[b], sx = loc[b + 1], sy = loc[b + 2];
+    switch (i) {
+      case 0: return s0;
+      case 1: return X0 * s0 + sx;
+      case 2: return Y0 * s0 + sy;
+      case 3: return X0 * X0 * s0 + 2.0 * X0 * sx + (double)loc[b + 3];
+      case 4: return X0 * Y0 * s0 + X0 * sy + Y0 * sx + (double)loc[b + 4];
+      default: return Y0 * Y0 * s0 + 2.0 * Y0 * sy + (double)loc[b + 5];
+    }
+  }
+  if (k < 36) {
+    const int b = 18 + ((k - 18) / 3) * 3, i = k - b;
+    const double s0 = loc[b];
+    if (i == 0) return s0;
+    if (i == 1) return X0 * s0 + (double)loc[b + 1];
+    return Y0 * s0 + (double)loc[b + 2];
+  }
+  return loc[k];
+}
+
+// Solve of one ECC iteration (ecc2.cc:1320-1362) from the frame's moments.
+// Runs on one thread; every matrix lives in the LDS scratch sm (the moments G
+// occupy sm[0..63]) so the hot path of the kernel keeps its register budget.
+template <int MOTION, bool FULL>
+__device__ void ecc_solve(double* sm, FrameXf& X, FrameRes& R, const EccPassArgs& a) {
+  const double* G = sm;
+  double* Ha = sm + 64;   // 6x6
+  double* H = sm + 100;   // NP x NP
+  double* L = sm + 136;   // NP x NP
+  double* T = sm + 172;   // NP x 6
+  double* M = sm + 208;   // NP x 6
+  double* ea = sm + 244;  // 6
+  double* e = sm + 250;   // NP
+  double* z = sm + 256;   // NP
+  // ---- masked statistics (cv::meanStdDev, population stdev)
+  double n, sf, sff, sg, sgg;
+  if (FULL) {
+    n = G[36]; sf = G[37]; sff = G[38]; sg = G[39]; sgg = G[40];
+  } else {
+    n = G[9]; sf = G[10]; sff = G[11]; sg = G[12]; sgg = G[13];
+  }
+  const double inv = n > 0 ? 1.0 / n : 0.0;
+  const double mf = sf * inv, mg = sg * inv;  // means of f', g'
+  const double vf = sff * inv - mf * mf, vg = sgg * inv - mg * mg;
+  const double stdf = sqrt(vf > 0 ? vf : 0.0), stdg = sqrt(vg > 0 ? vg : 0.0);
+  const double ratio = stdg / stdf;
+  const double kc = mg - ratio * mf;  // (mu_g - cg) - ratio (mu_f - cf)
+
+  constexpr int NP = MOTION == MOTION_TRANSLATION ? 2
+                     : (MOTION == MOTION_EUCLIDEAN ? 3 : (MOTION == MOTION_SCALED_EUCLIDEAN ? 4 : 6));
+  if (!FULL) {
+    // J = [gx, gy]
+    H[0] = G[0]; H[1] = G[1]; H[2] = G[1]; H[3] = G[2];
+    e[0] = G[3] - ratio * G[5] - kc * G[7];
+    e[1] = G[4] - ratio * G[6] - kc * G[8];
+  } else {
+    // affine moments, J_aff = [gx X, gx Y, gx, gy X, gy Y, gy]; A, B, C = {1, X, Y, XX, XY, YY}
+    const double *A = G, *B = G + 6, *Cc = G + 12;
+    const double up[21] = {A[3], A[4], A[1], B[3], B[4], B[1],
+                                 A[5], A[2], B[4], B[5], B[2],
+                                       A[0], B[1], B[2], B[0],
+                                             Cc[3], Cc[4], Cc[1],
+                                                    Cc[5], Cc[2],
+                                                           Cc[0]};
+    for (int i = 0, q = 0; i < 6; ++i)
+      for (int j = i; j < 6; ++j, ++q) Ha[i * 6 + j] = Ha[j * 6 + i] = up[q];
+    const double *P = G + 18, *Q = G + 21, *Rr = G + 24, *S = G + 27, *GX = G + 30, *GY = G + 33;
+    for (int i = 0; i < 6; ++i) {
+      const int k = (i % 3 == 0) ? 1 : ((i % 3 == 1) ? 2 : 0);  // X, Y, 1
+      const double bg = i < 3 ? P[k] : Q[k];
+      const double bf = i < 3 ? Rr[k] : S[k];
+      const double b1 = i < 3 ? GX[k] : GY[k];
+      ea[i] = bg - ratio * bf - kc * b1;
+    }
+    if (MOTION == MOTION_AFFINE) {
+      for (int i = 0; i < 36; ++i) H[i] = Ha[i];
+      for (int i = 0; i < 6; ++i) e[i] = ea[i];
+    } else {
+      // J = M J_aff (c_image_transform.cc:587-668 rewritten on the affine basis)
+      for (int i = 0; i < NP * 6; ++i) M[i] = 0;
+      M[0 * 6 + 2] = 1;
+      M[1 * 6 + 5] = 1;
+      {
+        const double sa = X.mc.c[0], ca = X.mc.c[1], s = X.mc.c[2], Cx = X.C[0], Cy = X.C[1];
+        M[2 * 6 + 0] = -s * sa; M[2 * 6 + 1] = -s * ca; M[2 * 6 + 2] = s * (sa * Cx + ca * Cy);
+        M[2 * 6 + 3] = s * ca;  M[2 * 6 + 4] = -s * sa; M[2 * 6 + 5] = s * (-ca * Cx + sa * Cy);
+        if (NP > 3) {
+          M[3 * 6 + 0] = ca; M[3 * 6 + 1] = -sa; M[3 * 6 + 2] = -(ca * Cx - sa * Cy);
+          M[3 * 6 + 3] = sa; M[3 * 6 + 4] = ca;  M[3 * 6 + 5] = -(sa * Cx + ca * Cy);
+        }
+      }
+      for (int i = 0; i < NP; ++i)
+        for (int j = 0; j < 6; ++j) {
+          double t = 0;
+          for (int k = 0; k < 6; ++k) t += M[i * 6 + k] * Ha[k * 6 + j];
+          T[i * 6 + j] = t;
+        }
+      for (int i = 0; i < NP; ++i) {
+        double t = 0;
+        for (int k = 0; k < 6; ++k) t += M[i * 6 + k] * ea[k];
+        e[i] = t;
+        for (int j = 0; j < NP; ++j) {
+          double u = 0;
+          for (int k = 0; k < 6; ++k) u += T[i * 6 + k] * M[j * 6 + k];
+          H[i * NP + j] = u;
+        }
+      }
+    }
+  }
+
+  // ---- cv::invert failure rule: n <= 3 -> det == 0; n > 3 -> Cholesky pivot < FLT_EPSILON
+  bool failed = false;
+  if (NP == 2) {
+    failed = (H[0] * H[3] - H[1] * H[2]) == 0.0;
+  } else if (NP == 3) {
+    const double d = H[0] * (H[4] * H[8] - H[5] * H[7]) - H[1] * (H[3] * H[8] - H[5] * H[6]) +
+                     H[2] * (H[3] * H[7] - H[4] * H[6]);
+    failed = d == 0.0;
+  }
+  for (int i = 0; i < NP && !failed; ++i) {
+    for (int j = 0; j < i; ++j) {
+      double s = H[i * NP + j];
+      for (int k = 0; k < j; ++k) s -= L[i * NP + k] * L[j * NP + k];
+      L[i * NP + j] = s / L[j * NP + j];
+    }
+    double s = H[i * NP + i];
+    for (int k = 0; k < i; ++k) s -= L[i * NP + k] * L[i * NP + k];
+    if (NP > 3 ? (s < (double)FLT_EPSILON) : !(s > 0.0)) failed = true;
+    else L[i * NP + i] = sqrt(s);
+  }
+  X.iters += 1;
+  const int k = X.iters;
+  if (failed) {
+    R.level_iters[a.pass][a.level] = k;
+    R.level_failed[a.pass][a.level] = 1;
+    X.flags &= ~XF_ACTIVE;
+    return;
+  }
+  for (int i = 0; i < NP; ++i) {
+    double s = e[i];
+    for (int j = 0; j < i; ++j) s -= L[i * NP + j] * z[j];
+    z[i] = s / L[i * NP + i];
+  }
+  for (int i = NP - 1; i >= 0; --i) {
+    double s = z[i];
+    for (int j = i + 1; j < NP; ++j) s -= L[j * NP + i] * z[j];
+    z[i] = s / L[i * NP + i];
+  }
+  float dp[6] = {0, 0, 0, 0, 0, 0};
+  for (int i = 0; i < NP; ++i) {
+    dp[i] = (float)(-a.update_step_scale * z[i]);
+    X.p[i] = X.p[i] + dp[i];
+  }
+  update_map(MOTION, X);
+  X.cg = (float)((double)X.cg + mg);
+  const float eps = xf_eps(MOTION, dp, a.w, a.h);
+  X.eps = eps;
+  R.eps = eps;
+  if (eps < a.max_eps) {
+    R.level_iters[a.pass][a.level] = k;
+    X.flags &= ~XF_ACTIVE;
+  } else if (k >= a.max_iterations) {
+    // while (_num_iterations++ < _max_iterations): exhaustion leaves max + 1
+    R.level_iters[a.pass][a.level] = k + 1;
+    X.flags &= ~XF_ACTIVE;
+  }
+}
+
+// deterministic block-wide compaction of active frames (ascending order)
+__device__ void compact_active(const FrameXf* xf, int nframes, int* out, int* out_count) {
+  __shared__ int s_wave[16];
+  __shared__ int s_base;
+  if (threadIdx.x == 0) s_base = 0;
+  __syncthreads();
+  const int lane = threadIdx.x & (WAVE - 1), wv = threadIdx.x / WAVE, nw = blockDim.x / WAVE;
+  for (int base = 0; base < nframes; base += blockDim.x) {
+    const int f = base + threadIdx.x;
+    const bool act = f < nframes && (xf[f].flags & XF_ACTIVE);
+    const unsigned long long bal = __ballot(act);
+    const int before = __popcll(bal & ((1ull << lane) - 1ull));
+    if (lane == 0) s_wave[wv] = __popcll(bal);
+    __syncthreads();
+    int off = s_base;
+    for (int i = 0; i < wv; ++i) off += s_wave[i];
+    if (act) out[off + before] = f;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      int t = 0;
+      for (int i = 0; i < nw; ++i) t += s_wave[i];
+      s_base += t;
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *out_count = s_base;
+}
+
+// ------------------------------------------------------------------------------------------------
+template <int MOTION, bool FULL, bool HAS_CMASK, bool HAS_RMASK>
+__global__ void __launch_bounds__(ECC_THREADS) k_ecc_pass(EccPassArgs a) {
+  constexpr int NM = FULL ? NMOM_FULL : NMOM_TRANS;
+  constexpr int LW = ECC_TW + 4, LH = ECC_TH + 4;
+  __shared__ float g[LH][LW];
+  __shared__ float red[ECC_WAVES][64];
+  __shared__ float loc[64];
+  __shared__ double dred[ECC_WAVES][64];
+  __shared__ double s_solve[272];
+  __shared__ int s_flag;
+
+  const int tid = threadIdx.x, lane = tid & (WAVE - 1), wv = tid / WAVE;
+  const int count = *a.active_count_in;
+  if (count == 0) {
+    if (blockIdx.x == 0 && tid == 0) *a.active_count_out = 0;
+    return;
+  }
+  const int items = count * a.ntiles;
+  const int G = gridDim.x, b = blockIdx.x;
+  // consecutive items on one XCD: blocks are dispatched round-robin over 8 XCDs
+  const int vb = (G % 8 == 0) ? (b % 8) * (G / 8) + b / 8 : b;
+  const int slot = ReduceScatter<NM>::slot_of(lane);
+
+  for (int item = vb; item < items; item += G) {
+    const int fi = item / a.ntiles;
+    const int tile = item - fi * a.ntiles;
+    const int f = __builtin_amdgcn_readfirstlane(a.active_in[fi]);
+    const int tx = tile % a.tiles_x, ty = tile / a.tiles_x;
+    const int x0 = tx * ECC_TW, y0 = ty * ECC_TH;
+    const FrameXf& X = a.xf[f];
+    MapCoef mc;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) mc.c[i] = X.mc.c[i];
+    const float cg = X.cg;
+    const float* __restrict__ cur = a.cur + f * a.cur_fstride;
+
+    // ---- 1. warped current image with halo (gw at clamped positions = BORDER_REPLICATE of gw)
+    for (int i = tid; i < LH * LW; i += ECC_THREADS) {
+      const int ly = i / LW, lx = i - ly * LW;
+      const int px = min(max(x0 - 2 + lx, 0), a.w - 1), py = min(max(y0 - 2 + ly, 0), a.h - 1);
+      float sx, sy;
+      warp_xy<MOTION>(mc, px, py, sx, sy);
+      g[ly][lx] = sample_linear_replicate(cur, a.cur_pitch, a.cw, a.ch, quant32(sx), quant32(sy));
+    }
+    // tile fully inside the fast region: every warped mask sample is 255
+    bool interior = false;
+    if (!HAS_CMASK) {
+      const int xe = min(x0 + ECC_TW - 1, a.w - 1), ye = min(y0 + ECC_TH - 1, a.h - 1);
+      float mnx = 1e30f, mxx = -1e30f, mny = 1e30f, mxy = -1e30f;
+      const int cx[4] = {x0, xe, x0, xe}, cy[4] = {y0, y0, ye, ye};
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        float sx, sy;
+        warp_xy<MOTION>(mc, cx[i], cy[i], sx, sy);
+        mnx = fminf(mnx, sx); mxx = fmaxf(mxx, sx); mny = fminf(mny, sy); mxy = fmaxf(mxy, sy);
+      }
+      interior = mnx >= 0.25f && mny >= 0.25f && mxx <= (float)a.cw - 1.25f && mxy <= (float)a.ch - 1.25f;
+    }
+    __syncthreads();
+
+    // ---- 2..4. gradients, masks, moments over a 16-row column strip
+    const int lx = lane, r0 = wv * ECC_RPT, x = x0 + lx;
+    const bool colok = x < a.w;
+    const uint8_t* __restrict__ cmask = HAS_CMASK ? a.cmask + f * a.cmask_fstride : nullptr;
+    auto Rd = [&](int ly) {
+      const float* c = &g[ly + 2][lx + 2];
+      return fmaf(c[2] - c[-2], D5_K2, (c[1] - c[-1]) * D5_K1);
+    };
+    auto Rs = [&](int ly) {
+      const float* c = &g[ly + 2][lx + 2];
+      return fmaf(c[-1] + c[1], 0.25f, c[0] * 0.5f);
+    };
+    float rdm1 = Rd(r0 - 1), rd0 = Rd(r0);
+    float rsm2 = Rs(r0 - 2), rsm1 = Rs(r0 - 1), rs0 = Rs(r0), rsp1 = Rs(r0 + 1);
+
+    float A0 = 0, A1 = 0, A2 = 0, B0 = 0, B1 = 0, B2 = 0, C0 = 0, C1 = 0, C2 = 0;
+    float P0 = 0, P1 = 0, Q0 = 0, Q1 = 0, R0 = 0, R1 = 0, S0 = 0, S1 = 0;
+    float X0a = 0, X1a = 0, Y0a = 0, Y1a = 0;
+    float N = 0, F1 = 0, F2 = 0, G1 = 0, G2 = 0;
+#pragma unroll
+    for (int dy = 0; dy < ECC_RPT; ++dy) {
+      const int ly = r0 + dy, y = y0 + ly;
+      const float rdp1 = Rd(ly + 1);
+      const float rsp2 = Rs(ly + 2);
+      const float gx = fmaf(rdm1 + rdp1, 0.25f, fmaf(rd0, 0.5f, 0.f));
+      const float gy = fmaf(rsp2 - rsm2, D5_K2, fmaf(rsp1 - rsm1, D5_K1, 0.f));
+      const float gw = g[ly + 2][lx + 2];
+      const bool valid = colok && y < a.h;
+      bool m = valid;
+      float fv = 0.f;
+      if (valid) {
+        fv = a.ref[(size_t)y * a.ref_pitch + x];
+        if (!interior) {
+          float sx, sy;
+          warp_xy<MOTION>(mc, x, y, sx, sy);
+          m = remap_mask_u8(cmask, a.cmask_pitch, a.cw, a.ch, quant32(sx), quant32(sy)) >= 255;
+        }
+        if (HAS_RMASK) m = m && a.rmask[(size_t)y * a.rmask_pitch + x] != 0;
+      }
+      const float gxm = m ? gx : 0.f, gym = m ? gy : 0.f;
+      const float fp = m ? fv - a.cf : 0.f, gp = m ? gw - cg : 0.f;
+      const float dyf = (float)dy, dy2 = (float)(dy * dy);
+      if (FULL) {
+        const float Av = gxm * gxm, Bv = gxm * gym, Cv = gym * gym;
+        A0 += Av; A1 = fmaf(Av, dyf, A1); A2 = fmaf(Av, dy2, A2);
+        B0 += Bv; B1 = fmaf(Bv, dyf, B1); B2 = fmaf(Bv, dy2, B2);
+        C0 += Cv; C1 = fmaf(Cv, dyf, C1); C2 = fmaf(Cv, dy2, C2);
+        const float Pv = gxm * gp, Qv = gym * gp, Rv = gxm * fp, Sv = gym * fp;
+        P0 += Pv; P1 = fmaf(Pv, dyf, P1);
+        Q0 += Qv; Q1 = fmaf(Qv, dyf, Q1);
+        R0 += Rv; R1 = fmaf(Rv, dyf, R1);
+        S0 += Sv; S1 = fmaf(Sv, dyf, S1);
+        X0a += gxm; X1a = fmaf(gxm, dyf, X1a);
+        Y0a += gym; Y1a = fmaf(gym, dyf, Y1a);
+      } else {
+        A0 = fmaf(gxm, gxm, A0); B0 = fmaf(gxm, gym, B0); C0 = fmaf(gym, gym, C0);
+        P0 = fmaf(gxm, gp, P0); Q0 = fmaf(gym, gp, Q0);
+        R0 = fmaf(gxm, fp, R0); S0 = fmaf(gym, fp, S0);
+        X0a += gxm; Y0a += gym;
+      }
+      N += m ? 1.f : 0.f;
+      F1 += fp; F2 = fmaf(fp, fp, F2);
+      G1 += gp; G2 = fmaf(gp, gp, G2);
+      rdm1 = rd0; rd0 = rdp1;
+      rsm2 = rsm1; rsm1 = rs0; rs0 = rsp1; rsp1 = rsp2;
+    }
+
+    // ---- 5. tile-local moments (lx, ly from the tile origin), wave reduce-scatter
+    float v[NM];
+    if (FULL) {
+      const float lxf = (float)lx, r0f = (float)r0, r02 = r0f * r0f, lx2 = lxf * lxf;
+      auto six = [&](float* o, float s0, float s1, float s2) {
+        const float sy = fmaf(r0f, s0, s1);
+        o[0] = s0;
+        o[1] = lxf * s0;
+        o[2] = sy;
+        o[3] = lx2 * s0;
+        o[4] = lxf * sy;
+        o[5] = fmaf(r02, s0, fmaf(2.f * r0f, s1, s2));
+      };
+      auto three = [&](float* o, float s0, float s1) {
+        o[0] = s0;
+        o[1] = lxf * s0;
+        o[2] = fmaf(r0f, s0, s1);
+      };
+      six(v + 0, A0, A1, A2);
+      six(v + 6, B0, B1, B2);
+      six(v + 12, C0, C1, C2);
+      three(v + 18, P0, P1);
+      three(v + 21, Q0, Q1);
+      three(v + 24, R0, R1);
+      three(v + 27, S0, S1);
+      three(v + 30, X0a, X1a);
+      three(v + 33, Y0a, Y1a);
+      v[36] = N; v[37] = F1; v[38] = F2; v[39] = G1; v[40] = G2;
+    } else {
+      v[0] = A0; v[1] = B0; v[2] = C0; v[3] = P0; v[4] = Q0; v[5] = R0; v[6] = S0;
+      v[7] = X0a; v[8] = Y0a; v[9] = N; v[10] = F1; v[11] = F2; v[12] = G1; v[13] = G2;
+    }
+    const float mine = ReduceScatter<NM>::run(v, lane);
+    if (slot >= 0) red[wv][slot] = mine;
+    __syncthreads();
+    if (tid < NM) loc[tid] = red[0][tid] + red[1][tid] + red[2][tid] + red[3][tid];
+    __syncthreads();
+    double* part = reinterpret_cast<double*>(a.partials) + ((size_t)f * a.ntiles + tile) * NM;
+    if (tid < NM) part[tid] = FULL ? expand_moment(tid, loc, (double)x0, (double)y0) : (double)loc[tid];
+    __threadfence();
+    __syncthreads();
+    if (tid == 0) s_flag = atomicAdd(&a.tile_counter[f], 1u) == (unsigned)(a.ntiles - 1);
+    __syncthreads();
+    if (!s_flag) continue;
+
+    // ---- last tile of frame f: sum the tiles (fixed order) and solve
+    __threadfence();
+    {
+      const int k = lane, j = wv;
+      double s = 0;
+      if (k < NM) {
+        const double* base = reinterpret_cast<const double*>(a.partials) + (size_t)f * a.ntiles * NM;
+        for (int t = j; t < a.ntiles; t += ECC_WAVES) s += __builtin_nontemporal_load(base + (size_t)t * NM + k);
+      }
+      dred[j][k] = s;
+    }
+    __syncthreads();
+    if (tid < NM) s_solve[tid] = ((dred[0][tid] + dred[1][tid]) + dred[2][tid]) + dred[3][tid];
+    __syncthreads();
+    if (tid == 0) {
+      a.tile_counter[f] = 0;
+      ecc_solve<MOTION, FULL>(s_solve, a.xf[f], a.res[f], a);
+      __threadfence();
+      s_flag = atomicAdd(a.done_counter, 1u) == (unsigned)(count - 1);
+    }
+    __syncthreads();
+    if (s_flag) {
+      // last frame of this iteration: build the next active list
+      __threadfence();
+      compact_active(a.xf, a.nframes, a.active_out, a.active_count_out);
+      if (tid == 0) *a.done_counter = 0;
+    }
+    __syncthreads();
+  }
+}
+
+int ecc_persistent_grid() {
+  static int grid = 0;
+  if (!grid) {
+    int dev = 0, cus = 0;
+    (void)hipGetDevice(&dev);
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0) cus = 256;
+    grid = cus * 4;  // blocks resident per CU at this kernel's LDS/VGPR use
+    grid = (grid + 7) / 8 * 8;
+  }
+  return grid;
+}
+
+template <int MOTION>
+static hipError_t launch_pass_m(const EccPassArgs& a, bool full, int grid, hipStream_t s) {
+  const bool cm = a.cmask != nullptr, rm = a.rmask != nullptr;
+#define SSG_LAUNCH(F, CM, RM) \
+  hipLaunchKernelGGL((k_ecc_pass<MOTION, F, CM, RM>), dim3(grid), dim3(ECC_THREADS), 0, s, a)
+  if (full) {
+    if (cm && rm) SSG_LAUNCH(true, true, true);
+    else if (cm) SSG_LAUNCH(true, true, false);
+    else if (rm) SSG_LAUNCH(true, false, true);
+    else SSG_LAUNCH(true, false, false);
+  } else {
+    if (cm && rm) SSG_LAUNCH(false, true, true);
+    else if (cm) SSG_LAUNCH(false, true, false);
+    else if (rm) SSG_LAUNCH(false, false, true);
+    else SSG_LAUNCH(false, false, false);
+  }
+#undef SSG_LAUNCH
+  return hipGetLastError();
+}
+
+hipError_t launch_ecc_pass(int motion, const EccPassArgs& a, int grid, hipStream_t s) {
+  switch (motion) {
+    case MOTION_TRANSLATION: return launch_pass_m<MOTION_TRANSLATION>(a, false, grid, s);
+    case MOTION_EUCLIDEAN: return launch_pass_m<MOTION_EUCLIDEAN>(a, true, grid, s);
+    case MOTION_SCALED_EUCLIDEAN: return launch_pass_m<MOTION_SCALED_EUCLIDEAN>(a, true, grid, s);
+    case MOTION_AFFINE: return launch_pass_m<MOTION_AFFINE>(a, true, grid, s);
+  }
+  return hipErrorInvalidValue;
+}
+
+// ------------------------------------------------------------------------------------------------
+// Level start (c_ecch::align, ecc2.cc:1133-1176): initial parameters at the
+// coarsest level (scaled by the width ratio), rescale between levels unless
+// the finer level's predecessor failed, reset iteration state, build the
+// active list.  One block.
+__global__ void __launch_bounds__(1024) k_level_begin(LevelBeginArgs a, int use_translation) {
+  const int mmot = a.pass == 0 ? MOTION_TRANSLATION : a.main_motion;
+  for (int f = threadIdx.x; f < a.nframes; f += blockDim.x) {
+    FrameRes& R = a.res[f];
+    FrameXf& X = a.pass == 0 ? a.xf_t[f] : a.xf_m[f];
+    if (R.rejected) {
+      X.flags = 0;
+      continue;
+    }
+    if (a.level == a.nlevels - 1) {
+      for (int i = 0; i < 6; ++i) X.p[i] = 0.f;
+      X.C[0] = X.C[1] = 0.f;
+      if (mmot == MOTION_AFFINE) {
+        X.p[0] = 1.f;
+        X.p[4] = 1.f;
+      } else if (mmot == MOTION_SCALED_EUCLIDEAN) {
+        X.p[3] = 1.f;
+      }
+      if (a.pass == 1 && use_translation) {
+        // set_translation from the translation-first estimate (c_frame_registration.cc:850)
+        const float t0 = a.xf_t[f].p[0], t1 = a.xf_t[f].p[1];
+        if (mmot == MOTION_AFFINE) {
+          X.p[2] = t0;
+          X.p[5] = t1;
+        } else {
+          X.p[0] = t0;
+          X.p[1] = t1;
+        }
+      }
+      if (a.nlevels > 1) scale_xf(mmot, X, a.ratio);
+      for (int l = 0; l < MAX_LEVELS; ++l) {
+        R.level_iters[a.pass][l] = 0;
+        R.level_failed[a.pass][l] = 0;
+      }
+    } else if (!R.level_failed[a.pass][a.level + 1]) {
+      scale_xf(mmot, X, a.ratio);
+    }
+    X.iters = 0;
+    X.flags = XF_ACTIVE;
+    X.cg = a.cf;
+    X.eps = FLT_MAX;
+    update_map(mmot, X);
+  }
+  __syncthreads();
+  compact_active(a.pass == 0 ? a.xf_t : a.xf_m, a.nframes, a.active_out, a.active_count_out);
+}
+
+hipError_t launch_level_begin(const LevelBeginArgs& a, hipStream_t s) {
+  const int use_t = a.pass == 1 && a.xf_t != nullptr;
+  LevelBeginArgs b = a;
+  if (!b.xf_t) b.xf_t = b.xf_m;
+  hipLaunchKernelGGL(k_level_begin, dim3(1), dim3(1024), 0, s, b, use_t);
+  return hipGetLastError();
+}
+
+// ------------------------------------------------------------------------------------------------
+// compute_correlation (ecc2.cc:65-137) for a batch: remap(cur, INTER_LINEAR,
+// BORDER_CONSTANT 0), mask = remap(mask) >= 254 & reference mask, Pearson
+// correlation over the mask; then the rho gate of register_frame.
+template <int MOTION, bool HAS_CMASK, bool HAS_RMASK>
+__global__ void __launch_bounds__(ECC_THREADS) k_corr_pass(CorrPassArgs a) {
+  __shared__ double dred[ECC_WAVES][8];
+  __shared__ int s_flag;
+  const int tid = threadIdx.x, lane = tid & (WAVE - 1), wv = tid / WAVE;
+  const int items = a.nlist * a.ntiles;
+  const int G = gridDim.x, b = blockIdx.x;
+  const int vb = (G % 8 == 0) ? (b % 8) * (G / 8) + b / 8 : b;
+  for (int item = vb; item < items; item += G) {
+    const int fi = item / a.ntiles;
+    const int tile = item - fi * a.ntiles;
+    const int f = __builtin_amdgcn_readfirstlane(a.frames[fi]);
+    if (a.res[f].rejected) continue;
+    const int tx = tile % a.tiles_x, ty = tile / a.tiles_x;
+    const int x0 = tx * ECC_TW, y0 = ty * ECC_TH;
+    MapCoef mc;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) mc.c[i] = a.xf[f].mc.c[i];
+    const float* __restrict__ cur = a.cur + f * a.cur_fstride;
+    const uint8_t* __restrict__ cmask = HAS_CMASK ? a.cmask + f * a.cmask_fstride : nullptr;
+    const int x = x0 + lane;
+    float n = 0, F1 = 0, G1 = 0, F2 = 0, G2 = 0, FG = 0;
+    for (int dy = 0; dy < ECC_RPT; ++dy) {
+      const int y = y0 + wv * ECC_RPT + dy;
+      if (x >= a.w || y >= a.h) continue;
+      float sx, sy;
+      warp_xy<MOTION>(mc, x, y, sx, sy);
+      const int X = quant32(sx), Y = quant32(sy);
+      bool m = remap_mask_u8(cmask, a.cmask_pitch, a.cw, a.ch, X, Y) >= 254;
+      if (HAS_RMASK) m = m && a.rmask[(size_t)y * a.rmask_pitch + x] != 0;
+      if (!m) continue;
+      float gv;
+      sample_linear(cur, a.cur_pitch, a.cw, a.ch, X, Y, BORDER_CONSTANT, 0.f, gv);
+      const float fp = a.ref[(size_t)y * a.ref_pitch + x] - a.cf, gp = gv - a.cf;
+      n += 1.f;
+      F1 += fp; G1 += gp;
+      F2 = fmaf(fp, fp, F2); G2 = fmaf(gp, gp, G2); FG = fmaf(fp, gp, FG);
+    }
+    double v[6] = {n, F1, G1, F2, G2, FG};
+#pragma unroll
+    for (int i = 0; i < 6; ++i) v[i] = wave_sum(v[i]);
+    if (lane == 0)
+      for (int i = 0; i < 6; ++i) dred[wv][i] = v[i];
+    __syncthreads();
+    double* part = reinterpret_cast<double*>(a.partials) + ((size_t)f * a.ntiles + tile) * NMOM_CORR;
+    if (tid < NMOM_CORR) part[tid] = ((dred[0][tid] + dred[1][tid]) + dred[2][tid]) + dred[3][tid];
+    __threadfence();
+    __syncthreads();
+    if (tid == 0) s_flag = atomicAdd(&a.tile_counter[f], 1u) == (unsigned)(a.ntiles - 1);
+    __syncthreads();
+    if (s_flag) {
+      __threadfence();
+      if (tid == 0) {
+        const double* base = reinterpret_cast<const double*>(a.partials) + (size_t)f * a.ntiles * NMOM_CORR;
+        double S[6] = {0, 0, 0, 0, 0, 0};
+        for (int t = 0; t < a.ntiles; ++t)
+          for (int i = 0; i < 6; ++i) S[i] += __builtin_nontemporal_load(base + (size_t)t * NMOM_CORR + i);
+        a.tile_counter[f] = 0;
+        const double inv = S[0] > 0 ? 1.0 / S[0] : 0.0;
+        const double mf = S[1] * inv, mg = S[2] * inv;
+        const double vf = S[3] * inv - mf * mf, vg = S[4] * inv - mg * mg;
+        const double sf = sqrt(vf > 0 ? vf : 0.0), sg = sqrt(vg > 0 ? vg : 0.0);
+        const double cov = S[5] * inv - mf * mg;
+        const double rho = cov / (sf * sg);
+        FrameRes& R = a.res[f];
+        R.rho[a.pass] = rho;
+        const double thr = a.pass == 0 ? 0.75 * a.min_rho : a.min_rho;
+        if (rho < thr) R.rejected = a.pass == 0 ? 3 : 4;
+      }
+    }
+    __syncthreads();
+  }
+}
+
+template <int MOTION>
+static hipError_t launch_corr_m(const CorrPassArgs& a, int grid, hipStream_t s) {
+  const bool cm = a.cmask != nullptr, rm = a.rmask != nullptr;
+#define SSG_LAUNCH(CM, RM) hipLaunchKernelGGL((k_corr_pass<MOTION, CM, RM>), dim3(grid), dim3(ECC_THREADS), 0, s, a)
+  if (cm && rm) SSG_LAUNCH(true, true);
+  else if (cm) SSG_LAUNCH(true, false);
+  else if (rm) SSG_LAUNCH(false, true);
+  else SSG_LAUNCH(false, false);
+#undef SSG_LAUNCH
+  return hipGetLastError();
+}
+
+hipError_t launch_corr_pass(int motion, const CorrPassArgs& a, int grid, hipStream_t s) {
+  switch (motion) {
+    case MOTION_TRANSLATION: return launch_corr_m<MOTION_TRANSLATION>(a, grid, s);
+    case MOTION_EUCLIDEAN: return launch_corr_m<MOTION_EUCLIDEAN>(a, grid, s);
+    case MOTION_SCALED_EUCLIDEAN: return launch_corr_m<MOTION_SCALED_EUCLIDEAN>(a, grid, s);
+    case MOTION_AFFINE: return launch_corr_m<MOTION_AFFINE>(a, grid, s);
+  }
+  return hipErrorInvalidValue;
+}
+
+}  // namespace ssg

@@ -1,0 +1,201 @@
+// ssg_engine.hpp -- structures shared by the host engine and the kernels, and
+// the launcher entry points of each kernel family.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "ssg_device.hpp"
+
+namespace ssg {
+
+// ---- per-frame ECC state (device) ----------------------------------------------------------
+enum : int { XF_ACTIVE = 1 };
+
+struct FrameXf {
+  float p[6];     // parameter vector, reference layout of the pass's transform
+  float C[2];     // euclidean centre
+  MapCoef mc;     // map coefficients derived from p (see warp_xy)
+  float eps;      // eps of the last iteration
+  float cg;       // centring constant for the warped image (previous iteration's mean)
+  int iters;      // iterations done in the current level
+  int flags;      // XF_ACTIVE
+};
+
+constexpr int MAX_LEVELS = 8;
+
+struct FrameRes {
+  int level_iters[2][MAX_LEVELS];  // [pass: 0 translation-first, 1 main][level]
+  int level_failed[2][MAX_LEVELS];
+  int rejected;                    // 0, SSG_REG_RHO_TRANSLATION or SSG_REG_RHO
+  float eps;
+  double rho[2];
+};
+
+// moment counts of the fused ECC pass
+constexpr int NMOM_FULL = 41;   // affine moments (see ssg_ecc.hip)
+constexpr int NMOM_TRANS = 14;  // translation moments
+constexpr int NMOM_CORR = 6;    // correlation moments
+
+constexpr int ECC_TW = 64;                    // tile width: one column per lane
+constexpr int ECC_RPT = 16;                   // rows per thread
+constexpr int ECC_WAVES = 4;                  // waves (row groups) per block
+constexpr int ECC_TH = ECC_RPT * ECC_WAVES;   // tile height
+constexpr int ECC_THREADS = WAVE * ECC_WAVES; // 256
+
+struct LevelGeom {
+  int w, h;          // level size
+  int pitch;         // elements per row (current pyramid)
+  long long offset;  // element offset of the level inside one frame's pyramid
+};
+
+// Arguments of one ECC iteration launch over a batch
+struct EccPassArgs {
+  // current pyramids: frame f, level image = cur + f * cur_fstride (+ level offset already applied)
+  const float* cur;
+  long long cur_fstride;
+  int cur_pitch, cw, ch;
+  const uint8_t* cmask;  // eroded current masks (same layout, u8) or null
+  long long cmask_fstride;
+  int cmask_pitch;
+  // reference level
+  const float* ref;
+  int ref_pitch, w, h;
+  const uint8_t* rmask;  // eroded reference mask or null
+  int rmask_pitch;
+  float cf;              // centring constant of the reference level
+  int tiles_x, tiles_y, ntiles;
+  // state
+  FrameXf* xf;
+  FrameRes* res;
+  float* partials;        // [nframes][ntiles][NMOM]
+  unsigned* tile_counter; // [nframes]
+  unsigned* done_counter; // [1]
+  const int* active_in;
+  const int* active_count_in;
+  int* active_out;
+  int* active_count_out;
+  int nframes;
+  int pass, level;
+  int max_iterations;
+  float max_eps;
+  double update_step_scale;
+};
+
+struct CorrPassArgs {
+  const float* cur;
+  long long cur_fstride;
+  int cur_pitch, cw, ch;
+  const uint8_t* cmask;
+  long long cmask_fstride;
+  int cmask_pitch;
+  const float* ref;
+  int ref_pitch, w, h;
+  const uint8_t* rmask;
+  int rmask_pitch;
+  float cf;
+  int tiles_x, tiles_y, ntiles;
+  const FrameXf* xf;
+  FrameRes* res;
+  float* partials;
+  unsigned* tile_counter;
+  const int* frames;   // frames to check (alive ones)
+  int nlist;
+  int pass;            // 0: translation gate (0.75*min_rho), 1: final gate (min_rho)
+  double min_rho;
+};
+
+struct LevelBeginArgs {
+  FrameXf* xf_t;  // translation-first state
+  FrameXf* xf_m;  // main state
+  FrameRes* res;
+  int nframes;
+  int pass;       // 0 or 1
+  int level, nlevels;
+  double ratio;   // scale factor applied to the transform at this level start
+  int main_motion;
+  float cf;
+  int* active_out;
+  int* active_count_out;
+};
+
+struct AccumArgs {
+  const float* const* frames;  // n device pointers
+  int src_pitch, sw, sh, cn;   // all frames share the geometry
+  const MapCoef* maps;         // per frame
+  const float* weights;        // per frame scalar weight
+  const int* valid;            // per frame 0/1
+  int nframes, per_chunk;      // frames [c*per_chunk, min((c+1)*per_chunk, nframes)) go to chunk c
+  int dw, dh;                  // accumulator size
+  int interp, border;
+  float cval;
+  // chunk partial output planes [chunk][plane][dh*dw]
+  float* part;
+  unsigned* part_cnt;
+  long long plane;             // elements per plane (dw*dh)
+  int nchunks;
+};
+
+// ---- launchers --------------------------------------------------------------------------------
+// pyramids (ssg_pyr.hip)
+struct SepKernel {
+  float kx[32];
+  float ky[32];
+  int nx, ny, tx, ty;  // taps and symmetry type (0 other, 1 symmetric, 2 antisymmetric)
+};
+hipError_t launch_sepfilter(const float* const* src_ptrs, int src_pitch, float* dst, long long dst_fstride,
+                            int dst_pitch, int w, int h, int nframes, const SepKernel& k, int border,
+                            hipStream_t s);
+hipError_t launch_pyrdown(const float* src, long long src_fstride, int src_pitch, int sw, int sh, float* dst,
+                          long long dst_fstride, int dst_pitch, int dw, int dh, int nframes, int border,
+                          hipStream_t s);
+hipError_t launch_copy_frames(const float* const* src_ptrs, int src_pitch, float* dst, long long dst_fstride,
+                              int dst_pitch, int w, int h, int nframes, hipStream_t s);
+hipError_t launch_mask_prepare(const uint8_t* const* src_ptrs, int src_pitch, uint8_t* dst, long long dst_fstride,
+                               int dst_pitch, int sw, int sh, int dw, int dh, int nframes, hipStream_t s);
+hipError_t launch_erode5(const uint8_t* src, long long src_fstride, int src_pitch, uint8_t* dst,
+                         long long dst_fstride, int dst_pitch, int w, int h, int nframes, int border, uint8_t cval,
+                         hipStream_t s);
+hipError_t launch_count_zero_u8(const uint8_t* src, long long fstride, int pitch, int w, int h, int nframes,
+                                unsigned* out, hipStream_t s);
+hipError_t launch_sum_f32(const float* src, int pitch, int w, int h, double* out, hipStream_t s);
+hipError_t launch_sharpness(const float* const* src_ptrs, int pitch, int w, int h, int nframes, double* partial,
+                            hipStream_t s);
+hipError_t launch_u16_to_f32(const uint16_t* src, int src_pitch, float* dst, int dst_pitch, int w, int h,
+                             float scale, hipStream_t s);
+
+// ECC (ssg_ecc.hip)
+hipError_t launch_level_begin(const LevelBeginArgs& a, hipStream_t s);
+hipError_t launch_ecc_pass(int motion, const EccPassArgs& a, int grid, hipStream_t s);
+hipError_t launch_corr_pass(int motion, const CorrPassArgs& a, int grid, hipStream_t s);
+int ecc_persistent_grid();
+
+// accumulation / remap (ssg_accum.hip)
+hipError_t launch_accumulate(int motion, const AccumArgs& a, hipStream_t s);
+hipError_t launch_accum_merge(const float* part, const unsigned* part_cnt, int nchunks, long long plane, int cn,
+                              float* acc, float* accw, unsigned* cnt, hipStream_t s);
+hipError_t launch_acc_add(const float* src, int src_pitch, int w, int h, int cn, const uint8_t* mask, int mask_pitch,
+                          const float* wmap, int wmap_pitch, float* acc, float* accw, unsigned* cnt,
+                          long long plane, hipStream_t s);
+hipError_t launch_acc_compute(const float* acc, const float* accw, int w, int h, int cn, long long plane,
+                              float* avg, int avg_pitch, uint8_t* mask, int mask_pitch, double dscale,
+                              hipStream_t s);
+hipError_t launch_remap(int motion, const float* src, int src_pitch, int sw, int sh, int cn, const MapCoef& mc,
+                        const float* map, int map_pitch, float* dst, int dst_pitch, int dw, int dh, int interp,
+                        int border, float cval, hipStream_t s);
+hipError_t launch_remap_mask(int motion, const uint8_t* src_mask, int src_pitch, int sw, int sh, const MapCoef& mc,
+                             const float* map, int map_pitch, uint8_t* dst, int dst_pitch, int dw, int dh,
+                             hipStream_t s, int interp);
+hipError_t set_cubic_itab(const short* tab, hipStream_t s);
+hipError_t launch_create_map(int motion, const MapCoef& mc, float* map, int map_pitch, int w, int h, hipStream_t s);
+
+// synthetic frames (ssg_synth.hip)
+struct SynthFrame {
+  double A[6];  // scene coordinate = A * (x, y, 1): maps frame pixels to the scene
+  float blur_sigma;
+  uint32_t key;
+};
+
+void set_cubic_table(const float tab[32][4], hipStream_t s);
+
+}  // namespace ssg

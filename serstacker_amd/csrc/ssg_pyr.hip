@@ -1,0 +1,305 @@
+// ssg_pyr.hip -- image pyramid kernels for gfx950: separable smoothing
+// (cv::sepFilter2D), pyrDown, mask preparation, small reductions.
+//
+// All kernels are LDS-tiled: a block loads its tile plus halo once with
+// coalesced reads, filters rows into a second LDS buffer and writes the
+// column-filtered output with coalesced stores.
+#include "ssg_engine.hpp"
+
+namespace ssg {
+
+// ---------------------------------------------------------------------------
+// Separable filter, canonical order of ssg_oracle.c:ora_sep_filter
+// (the OpenCV split SymmRowSmallFilter / RowFilter / SymmColumnFilter,
+// filter.simd.hpp, with fused multiply-adds).
+// ---------------------------------------------------------------------------
+constexpr int SF_TW = 64, SF_TH = 16, SF_MAXR = 15;
+
+__device__ __forceinline__ float row_filter(const float* c, const SepKernel& K) {
+  // c points at the centre tap
+  const int n = K.nx, a = n / 2;
+  const float* k = K.kx;
+  if (K.tx == 1 && n <= 5) {
+    float s = c[0] * k[a];
+    if (n >= 3) s = fmaf(c[-1] + c[1], k[a + 1], s);
+    if (n >= 5) s = fmaf(c[-2] + c[2], k[a + 2], s);
+    return s;
+  }
+  if (K.tx == 2 && n >= 3 && n <= 5) {
+    float s = (c[1] - c[-1]) * k[a + 1];
+    if (n >= 5) s = fmaf(c[2] - c[-2], k[a + 2], s);
+    return s;
+  }
+  float s = 0.f;
+  for (int i = 0; i < n; ++i) s = fmaf(c[i - a], k[i], s);
+  return s;
+}
+
+__device__ __forceinline__ float col_filter(const float* c, int stride, const SepKernel& K) {
+  const int n = K.ny, a = n / 2;
+  const float* k = K.ky;
+  if (K.ty == 1) {
+    float s = fmaf(c[0], k[a], 0.f);
+    for (int i = 1; i <= a; ++i) s = fmaf(c[-i * stride] + c[i * stride], k[a + i], s);
+    return s;
+  }
+  if (K.ty == 2) {
+    float s = 0.f;
+    for (int i = 1; i <= a; ++i) s = fmaf(c[i * stride] - c[-i * stride], k[a + i], s);
+    return s;
+  }
+  float s = 0.f;
+  for (int i = 0; i < n; ++i) s = fmaf(c[(i - a) * stride], k[i], s);
+  return s;
+}
+
+__global__ void __launch_bounds__(256) k_sepfilter(const float* const* __restrict__ src_ptrs, int src_pitch,
+                                                   float* __restrict__ dst, long long dst_fstride, int dst_pitch,
+                                                   int w, int h, SepKernel K, int border) {
+  __shared__ float sm[SF_TH + 2 * SF_MAXR][SF_TW + 2 * SF_MAXR];
+  __shared__ float rb[SF_TH + 2 * SF_MAXR][SF_TW];
+  const int f = blockIdx.z;
+  const float* __restrict__ S = src_ptrs[f];
+  float* __restrict__ D = dst + f * dst_fstride;
+  const int rx = K.nx / 2, ry = K.ny / 2;
+  const int x0 = blockIdx.x * SF_TW, y0 = blockIdx.y * SF_TH;
+  const int TW = SF_TW + 2 * rx, TH = SF_TH + 2 * ry;
+  for (int i = threadIdx.x; i < TW * TH; i += blockDim.x) {
+    const int ly = i / TW, lx = i - ly * TW;
+    const int bx = border_interp(x0 - rx + lx, w, border), by = border_interp(y0 - ry + ly, h, border);
+    sm[ly][lx] = (bx < 0 || by < 0) ? 0.f : S[(size_t)by * src_pitch + bx];
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < TH * SF_TW; i += blockDim.x) {
+    const int ly = i / SF_TW, lx = i - ly * SF_TW;
+    rb[ly][lx] = row_filter(&sm[ly][lx + rx], K);
+  }
+  __syncthreads();
+  const int lx = threadIdx.x & (SF_TW - 1);
+  for (int ly = threadIdx.x / SF_TW; ly < SF_TH; ly += blockDim.x / SF_TW) {
+    const int gx = x0 + lx, gy = y0 + ly;
+    if (gx < w && gy < h) D[(size_t)gy * dst_pitch + gx] = col_filter(&rb[ly + ry][lx], SF_TW, K);
+  }
+}
+
+hipError_t launch_sepfilter(const float* const* src_ptrs, int src_pitch, float* dst, long long dst_fstride,
+                            int dst_pitch, int w, int h, int nframes, const SepKernel& k, int border,
+                            hipStream_t s) {
+  if (k.nx > 2 * SF_MAXR + 1 || k.ny > 2 * SF_MAXR + 1 || nframes <= 0) return hipErrorInvalidValue;
+  dim3 grid((w + SF_TW - 1) / SF_TW, (h + SF_TH - 1) / SF_TH, nframes);
+  hipLaunchKernelGGL(k_sepfilter, grid, dim3(256), 0, s, src_ptrs, src_pitch, dst, dst_fstride, dst_pitch, w, h, k,
+                     border);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// pyrDown (pyramids.cpp pyrDown_): [1 4 6 4 1]/16 at even source pixels.
+// Canonical: r = fma(s-1 + s1, 4, fma(s0, 6, s-2 + s2)); d = fma(r1 + r3, 4,
+// fma(r2, 6, r0 + r4)) * (1/256), the order of ssg_oracle.c:ora_pyr_down.
+// ---------------------------------------------------------------------------
+constexpr int PD_TW = 64, PD_TH = 16;
+constexpr int PD_SW = 2 * PD_TW + 4, PD_SH = 2 * PD_TH + 4;
+
+__global__ void __launch_bounds__(256) k_pyrdown(const float* __restrict__ src, long long src_fstride, int src_pitch,
+                                                 int sw, int sh, float* __restrict__ dst, long long dst_fstride,
+                                                 int dst_pitch, int dw, int dh, int border) {
+  __shared__ float sm[PD_SH][PD_SW + 1];
+  __shared__ float rb[PD_SH][PD_TW];
+  const int f = blockIdx.z;
+  const float* __restrict__ S = src + f * src_fstride;
+  float* __restrict__ D = dst + f * dst_fstride;
+  const int x0 = blockIdx.x * PD_TW, y0 = blockIdx.y * PD_TH;
+  const int sx0 = 2 * x0 - 2, sy0 = 2 * y0 - 2;
+  for (int i = threadIdx.x; i < PD_SW * PD_SH; i += blockDim.x) {
+    const int ly = i / PD_SW, lx = i - ly * PD_SW;
+    const int bx = border_interp(sx0 + lx, sw, border), by = border_interp(sy0 + ly, sh, border);
+    sm[ly][lx] = (bx < 0 || by < 0) ? 0.f : S[(size_t)by * src_pitch + bx];
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < PD_SH * PD_TW; i += blockDim.x) {
+    const int ly = i / PD_TW, lx = i - ly * PD_TW;
+    const float* c = &sm[ly][2 * lx + 2];
+    rb[ly][lx] = fmaf(c[-1] + c[1], 4.f, fmaf(c[0], 6.f, c[-2] + c[2]));
+  }
+  __syncthreads();
+  const int lx = threadIdx.x & (PD_TW - 1);
+  for (int ly = threadIdx.x / PD_TW; ly < PD_TH; ly += blockDim.x / PD_TW) {
+    const int gx = x0 + lx, gy = y0 + ly;
+    if (gx < dw && gy < dh) {
+      const int r = 2 * ly + 2;
+      const float v = fmaf(rb[r - 1][lx] + rb[r + 1][lx], 4.f, fmaf(rb[r][lx], 6.f, rb[r - 2][lx] + rb[r + 2][lx]));
+      D[(size_t)gy * dst_pitch + gx] = v * (1.f / 256.f);
+    }
+  }
+}
+
+hipError_t launch_pyrdown(const float* src, long long src_fstride, int src_pitch, int sw, int sh, float* dst,
+                          long long dst_fstride, int dst_pitch, int dw, int dh, int nframes, int border,
+                          hipStream_t s) {
+  if (nframes <= 0) return hipErrorInvalidValue;
+  dim3 grid((dw + PD_TW - 1) / PD_TW, (dh + PD_TH - 1) / PD_TH, nframes);
+  hipLaunchKernelGGL(k_pyrdown, grid, dim3(256), 0, s, src, src_fstride, src_pitch, sw, sh, dst, dst_fstride,
+                     dst_pitch, dw, dh, border);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+__global__ void k_copy_frames(const float* const* __restrict__ src_ptrs, int src_pitch, float* __restrict__ dst,
+                              long long dst_fstride, int dst_pitch, int w, int h) {
+  const int f = blockIdx.z;
+  const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
+  if (x < w) dst[f * dst_fstride + (size_t)y * dst_pitch + x] = src_ptrs[f][(size_t)y * src_pitch + x];
+}
+
+hipError_t launch_copy_frames(const float* const* src_ptrs, int src_pitch, float* dst, long long dst_fstride,
+                              int dst_pitch, int w, int h, int nframes, hipStream_t s) {
+  dim3 grid((w + 255) / 256, h, nframes);
+  hipLaunchKernelGGL(k_copy_frames, grid, dim3(256), 0, s, src_ptrs, src_pitch, dst, dst_fstride, dst_pitch, w, h);
+  return hipGetLastError();
+}
+
+// cv::resize INTER_NEAREST of a u8 mask: sx = min(floor(x * sw/dw), sw-1)
+// (also used as a plain copy when the sizes are equal)
+__global__ void k_mask_prepare(const uint8_t* const* __restrict__ src_ptrs, int src_pitch, uint8_t* __restrict__ dst,
+                               long long dst_fstride, int dst_pitch, int sw, int sh, int dw, int dh) {
+  const int f = blockIdx.z;
+  const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
+  if (x >= dw) return;
+  const double ifx = (double)sw / dw, ify = (double)sh / dh;
+  const int sx = min((int)floor(x * ifx), sw - 1), sy = min((int)floor(y * ify), sh - 1);
+  dst[f * dst_fstride + (size_t)y * dst_pitch + x] = src_ptrs[f][(size_t)sy * src_pitch + sx];
+}
+
+hipError_t launch_mask_prepare(const uint8_t* const* src_ptrs, int src_pitch, uint8_t* dst, long long dst_fstride,
+                               int dst_pitch, int sw, int sh, int dw, int dh, int nframes, hipStream_t s) {
+  dim3 grid((dw + 255) / 256, dh, nframes);
+  hipLaunchKernelGGL(k_mask_prepare, grid, dim3(256), 0, s, src_ptrs, src_pitch, dst, dst_fstride, dst_pitch, sw, sh,
+                     dw, dh);
+  return hipGetLastError();
+}
+
+// cv::erode with a 5x5 rectangle (min over the window)
+__global__ void k_erode5(const uint8_t* __restrict__ src, long long src_fstride, int src_pitch,
+                         uint8_t* __restrict__ dst, long long dst_fstride, int dst_pitch, int w, int h, int border,
+                         uint8_t cval) {
+  const int f = blockIdx.z;
+  const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
+  if (x >= w) return;
+  const uint8_t* S = src + f * src_fstride;
+  int m = 255;
+  for (int j = -2; j <= 2; ++j) {
+    const int yy = border_interp(y + j, h, border);
+    for (int i = -2; i <= 2; ++i) {
+      const int xx = border_interp(x + i, w, border);
+      const int v = (xx < 0 || yy < 0) ? cval : S[(size_t)yy * src_pitch + xx];
+      m = min(m, v);
+    }
+  }
+  dst[f * dst_fstride + (size_t)y * dst_pitch + x] = (uint8_t)m;
+}
+
+hipError_t launch_erode5(const uint8_t* src, long long src_fstride, int src_pitch, uint8_t* dst,
+                         long long dst_fstride, int dst_pitch, int w, int h, int nframes, int border, uint8_t cval,
+                         hipStream_t s) {
+  dim3 grid((w + 255) / 256, h, nframes);
+  hipLaunchKernelGGL(k_erode5, grid, dim3(256), 0, s, src, src_fstride, src_pitch, dst, dst_fstride, dst_pitch, w, h,
+                     border, cval);
+  return hipGetLastError();
+}
+
+// number of zero pixels per frame (out[f], must be zeroed by the caller)
+__global__ void k_count_zero_u8(const uint8_t* __restrict__ src, long long fstride, int pitch, int w, int h,
+                                unsigned* __restrict__ out) {
+  const int f = blockIdx.z;
+  const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
+  unsigned z = (x < w && src[f * fstride + (size_t)y * pitch + x] == 0) ? 1u : 0u;
+  z = (unsigned)__popcll(__ballot(z));
+  if ((threadIdx.x & (WAVE - 1)) == 0 && z) atomicAdd(&out[f], z);
+}
+
+hipError_t launch_count_zero_u8(const uint8_t* src, long long fstride, int pitch, int w, int h, int nframes,
+                                unsigned* out, hipStream_t s) {
+  dim3 grid((w + 255) / 256, h, nframes);
+  hipLaunchKernelGGL(k_count_zero_u8, grid, dim3(256), 0, s, src, fstride, pitch, w, h, out);
+  return hipGetLastError();
+}
+
+// sum of an image in double (one block per row, fixed order: deterministic)
+__global__ void k_row_sums(const float* __restrict__ src, int pitch, int w, double* __restrict__ rows) {
+  __shared__ double red[4];
+  const int y = blockIdx.x;
+  double s = 0;
+  for (int x = threadIdx.x; x < w; x += blockDim.x) s += (double)src[(size_t)y * pitch + x];
+  s = wave_sum(s);
+  if ((threadIdx.x & (WAVE - 1)) == 0) red[threadIdx.x / WAVE] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) rows[y] = red[0] + red[1] + red[2] + red[3];
+}
+
+__global__ void k_sum_rows(const double* __restrict__ rows, int h, double* __restrict__ out) {
+  if (threadIdx.x == 0) {
+    double s = 0;
+    for (int y = 0; y < h; ++y) s += rows[y];
+    out[0] = s;
+  }
+}
+
+hipError_t launch_sum_f32(const float* src, int pitch, int w, int h, double* out, hipStream_t s) {
+  // out must hold h + 1 doubles: row sums then the total in out[h]
+  hipLaunchKernelGGL(k_row_sums, dim3(h), dim3(256), 0, s, src, pitch, w, out);
+  hipLaunchKernelGGL(k_sum_rows, dim3(1), dim3(64), 0, s, out, h, out + h);
+  return hipGetLastError();
+}
+
+// Per-frame sharpness weight: mean of the squared 3x3 laplacian
+// [0 1 0; 1 -4 1; 0 1 0], BORDER_REPLICATE.  Canonical per pixel
+// L = fma(-4, c, (up + down) + (left + right)).  One block per (row strip,
+// frame) writes a double partial; the host sums the partials in order.
+__global__ void __launch_bounds__(256) k_sharpness(const float* const* __restrict__ src_ptrs, int pitch, int w, int h,
+                                                   int rows_per_block, double* __restrict__ partial) {
+  __shared__ double red[4];
+  const int f = blockIdx.y;
+  const float* __restrict__ S = src_ptrs[f];
+  const int y0 = blockIdx.x * rows_per_block, y1 = min(h, y0 + rows_per_block);
+  double acc = 0;
+  for (int y = y0; y < y1; ++y) {
+    const float* up = S + (size_t)max(y - 1, 0) * pitch;
+    const float* c = S + (size_t)y * pitch;
+    const float* dn = S + (size_t)min(y + 1, h - 1) * pitch;
+    float part = 0.f;
+    for (int x = threadIdx.x; x < w; x += blockDim.x) {
+      const float l = c[max(x - 1, 0)], r = c[min(x + 1, w - 1)];
+      const float L = fmaf(-4.f, c[x], (up[x] + dn[x]) + (l + r));
+      acc += (double)L * (double)L;
+    }
+    (void)part;
+  }
+  acc = wave_sum(acc);
+  if ((threadIdx.x & (WAVE - 1)) == 0) red[threadIdx.x / WAVE] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) partial[(size_t)f * gridDim.x + blockIdx.x] = red[0] + red[1] + red[2] + red[3];
+}
+
+hipError_t launch_sharpness(const float* const* src_ptrs, int pitch, int w, int h, int nframes, double* partial,
+                            hipStream_t s) {
+  // partial must hold nframes * ceil(h / 16) doubles
+  const int rpb = 16;
+  dim3 grid((h + rpb - 1) / rpb, nframes);
+  hipLaunchKernelGGL(k_sharpness, grid, dim3(256), 0, s, src_ptrs, pitch, w, h, rpb, partial);
+  return hipGetLastError();
+}
+
+__global__ void k_u16_to_f32(const uint16_t* __restrict__ src, int src_pitch, float* __restrict__ dst,
+                             int dst_pitch, int w, float scale) {
+  const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
+  if (x < w) dst[(size_t)y * dst_pitch + x] = (float)src[(size_t)y * src_pitch + x] * scale;
+}
+
+hipError_t launch_u16_to_f32(const uint16_t* src, int src_pitch, float* dst, int dst_pitch, int w, int h,
+                             float scale, hipStream_t s) {
+  dim3 grid((w + 255) / 256, h);
+  hipLaunchKernelGGL(k_u16_to_f32, grid, dim3(256), 0, s, src, src_pitch, dst, dst_pitch, w, scale);
+  return hipGetLastError();
+}
+
+}  // namespace ssg

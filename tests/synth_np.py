@@ -1,0 +1,81 @@
+"""Small seeded planetary-disk frames in numpy (test inputs).
+
+A limb-darkened disk (I = 1 - 0.6 (1 - mu)) textured by gaussian-filtered
+uniform noise at 10%, warped by a known affine map, optionally blurred and
+with gaussian noise.  ``frame(x) = scene(T x)``; the registration of a frame
+onto the un-warped scene is ``W = T^-1`` (returned as ``truth``).
+"""
+from __future__ import annotations
+
+import numpy as np
+from scipy import ndimage
+
+
+def scene(w: int, h: int, radius: float, seed: int = 1, tex_sigma: float = 3.0, amp: float = 0.1,
+          margin: int = 32):
+    rng = np.random.default_rng(seed)
+    tw, th = w + 2 * margin, h + 2 * margin
+    tex = ndimage.gaussian_filter(rng.uniform(-1, 1, (th, tw)), tex_sigma)
+    tex /= np.abs(tex).max()
+    cx, cy = 0.5 * (w - 1), 0.5 * (h - 1)
+
+    def render(A):
+        yy, xx = np.mgrid[0:h, 0:w].astype(np.float64)
+        u = A[0, 0] * xx + A[0, 1] * yy + A[0, 2]
+        v = A[1, 0] * xx + A[1, 1] * yy + A[1, 2]
+        r = np.hypot(u - cx, v - cy) / radius
+        mu = np.sqrt(np.clip(1 - r * r, 0, 1))
+        disk = np.where(r < 1, 1 - 0.6 * (1 - mu), 0.0)
+        t = ndimage.map_coordinates(tex, [v + margin, u + margin], order=3, mode="nearest")
+        return disk * (1 + amp * t)
+
+    return render
+
+
+def random_affine(rng, w, h, shift, rot_deg, scale_dev=0.0, shear=0.0):
+    cx, cy = 0.5 * (w - 1), 0.5 * (h - 1)
+    th = np.deg2rad(rng.uniform(-rot_deg, rot_deg))
+    s = 1 + rng.uniform(-scale_dev, scale_dev)
+    hs = rng.uniform(-shear, shear)
+    R = np.array([[np.cos(th), -np.sin(th)], [np.sin(th), np.cos(th)]])
+    M = R @ np.array([[s, hs], [0, s]])
+    t = rng.uniform(-shift, shift, 2)
+    A = np.zeros((2, 3))
+    A[:, :2] = M
+    A[:, 2] = np.array([cx, cy]) + t - M @ np.array([cx, cy])
+    return A
+
+
+def invert(A):
+    M = np.vstack([A, [0, 0, 1]])
+    return np.linalg.inv(M)[:2]
+
+
+def frames(n, w, h, radius, seed=7, shift=4.0, rot_deg=0.5, scale_dev=0.0, shear=0.0, blur=(0.0, 0.0),
+           noise=0.0, scale=1.0, dtype=np.float32):
+    """Returns (reference, frames list, truth list of 2x3 ref->frame maps)."""
+    render = scene(w, h, radius, seed)
+    rng = np.random.default_rng(seed + 1)
+    ref = render(np.array([[1.0, 0, 0], [0, 1.0, 0]]))
+    if blur[1] > 0:
+        ref = ndimage.gaussian_filter(ref, 0.5 * (blur[0] + blur[1]))
+    out, truth = [], []
+    for _ in range(n):
+        A = random_affine(rng, w, h, shift, rot_deg, scale_dev, shear)
+        f = render(A)
+        if blur[1] > 0:
+            f = ndimage.gaussian_filter(f, rng.uniform(*blur))
+        if noise > 0:
+            f = f + rng.normal(0, noise, f.shape)
+        out.append((f * scale).astype(dtype))
+        truth.append(invert(A))
+    return (ref * scale).astype(dtype), out, truth
+
+
+def max_displacement(m1, m2, w, h):
+    """Max over the frame corners (the extremes of an affine difference) of |m1 x - m2 x|."""
+    m1 = np.asarray(m1, np.float64).reshape(2, 3)
+    m2 = np.asarray(m2, np.float64).reshape(2, 3)
+    c = np.array([[0, 0, 1], [w - 1, 0, 1], [0, h - 1, 1], [w - 1, h - 1, 1]], np.float64).T
+    d = (m1 - m2) @ c
+    return float(np.sqrt((d ** 2).sum(0)).max())

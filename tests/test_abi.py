@@ -1,0 +1,110 @@
+"""The C-ABI library loads and exports every entry point include/ssg.h declares
+(no compute calls: this runs without a GPU)."""
+import ctypes as C
+import os
+import re
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "ssg.h")
+LIB = os.path.join(ROOT, "serstacker_amd", "libssg.so")
+
+
+def declared():
+    src = open(HEADER).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    return sorted(set(re.findall(r"\b(ssg_[a-z0-9_]+)\s*\(", src)))
+
+
+@pytest.fixture(scope="module")
+def lib():
+    if not os.path.exists(LIB):
+        pytest.fail("libssg.so is not built (run __graft_entry__.build())")
+    return C.CDLL(LIB)
+
+
+def test_header_declares_entry_points():
+    names = declared()
+    assert "ssg_registration_register_batch" in names
+    assert len(names) >= 20
+
+
+def test_every_declared_symbol_is_exported(lib):
+    missing = [n for n in declared() if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_binding_covers_header():
+    from serstacker_amd import _lib
+
+    assert sorted(_lib.ENTRY_POINTS) == declared()
+
+
+def test_abi_version_and_defaults(lib):
+    from serstacker_amd import _lib
+
+    L = _lib.load()
+    assert L.ssg_abi_version() == 1
+    o = _lib.ImageRegistrationOptions()
+    L.ssg_default_image_registration_options(C.byref(o))
+    # c_frame_registration.h:47-64 and :119-136
+    assert o.motion_type == _lib.MOTION_AFFINE
+    assert o.interpolation == _lib.INTER_LINEAR and o.border_mode == _lib.BORDER_REFLECT101
+    assert (o.ecc.scale, o.ecc.eps, o.ecc.min_rho) == (0.5, 0.2, 0.8)
+    assert (o.ecc.input_smooth_sigma, o.ecc.reference_smooth_sigma, o.ecc.update_step_scale) == (1.0, 1.0, 1.5)
+    assert (o.ecc.max_iterations, o.ecc.ecch_max_level, o.ecc.ecch_minimum_image_size) == (50, 0, 16)
+    assert o.ecc.ecc_method == _lib.ECC_ALIGN_LM and o.ecc.ecch_estimate_translation_first == 1
+    assert (o.eccflow.support_scale, o.eccflow.max_iterations, o.eccflow.scale_factor) == (4, 3, 0.75)
+    assert o.enable_feature_registration == 1 and o.enable_ecc_registration == 0
+
+
+def test_struct_layout_matches_header(tmp_path):
+    """sizeof/offsetof of every ABI struct, compiled from include/ssg.h with gcc,
+    equal the ctypes binding's layout."""
+    import subprocess
+
+    from serstacker_amd import _lib
+
+    structs = {
+        "ssg_image": _lib.Image,
+        "ssg_ecc_registration_options": _lib.EccRegistrationOptions,
+        "ssg_eccflow_registration_options": _lib.EccflowRegistrationOptions,
+        "ssg_image_registration_options": _lib.ImageRegistrationOptions,
+        "ssg_registration_status": _lib.RegistrationStatus,
+        "ssg_profile": _lib.Profile,
+        "ssg_synth_params": _lib.SynthParams,
+    }
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "ssg.h"', "int main(void) {"]
+    for cname, cls in structs.items():
+        lines.append(f'printf("{cname} size %zu\\n", sizeof({cname}));')
+        for fname, _ in cls._fields_:
+            lines.append(f'printf("{cname} {fname} %zu\\n", offsetof({cname}, {fname}));')
+    lines.append("return 0; }")
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split("\n")
+    got = {}
+    for ln in out:
+        if ln:
+            s, f, v = ln.split()
+            got[(s, f)] = int(v)
+    for cname, cls in structs.items():
+        assert got[(cname, "size")] == C.sizeof(cls), cname
+        for fname, _ in cls._fields_:
+            assert got[(cname, fname)] == getattr(cls, fname).offset, (cname, fname)
+
+
+def test_invalid_arguments_fail_without_gpu_work(lib):
+    from serstacker_amd import _lib
+
+    L = _lib.load()
+    out = C.c_void_p()
+    o = _lib.ImageRegistrationOptions()
+    L.ssg_default_image_registration_options(C.byref(o))
+    # the reference defaults select feature registration, which is not on this path
+    assert L.ssg_registration_create(C.byref(o), C.byref(out)) == -4
+    assert b"feature" in L.ssg_last_error() or b"ECC" in L.ssg_last_error()
+    assert L.ssg_accumulation_create(99, 8, 8, 1, C.byref(out)) == -4
