@@ -97,10 +97,10 @@ def main():
     views = _lib.images([S.image_view(f) for f in frames])
 
     def step():
-        wts = S.frame_sharpness(frames)
-        st = reg.register_batch(frames)
+        wts = S.frame_sharpness(views)
+        st = reg.register_batch(views)
         acc.clear()
-        acc.add_registered_batch(frames, st, S.MOTION_AFFINE, frame_weights=wts)
+        acc.add_registered_batch(views, st, S.MOTION_AFFINE, frame_weights=wts)
         avg, mask = acc.compute()
         return st, avg
 
@@ -126,11 +126,13 @@ def main():
     fps = n * a.steps / elapsed
     nok = int(sum(s.ok for s in st))
     its = np.array([s.iterations for s in st])
+    its_t = np.array([s.iterations_translation for s in st])
     print(f"[bench] batched: {fps:.1f} frames/s ({ms_step:.1f} ms/step), {nok}/{n} registered, "
-          f"main iterations mean {its.mean():.1f} min {its.min()} max {its.max()}", file=sys.stderr, flush=True)
+          f"main iterations mean {its.mean():.1f} min {its.min()} max {its.max()}, translation-first "
+          f"mean {its_t.mean():.1f} max {its_t.max()}", file=sys.stderr, flush=True)
 
     kinds = {"ecc_translation": _lib.PROF_ECC_TRANSLATION, "ecc_main": _lib.PROF_ECC_MAIN,
-             "correlation": _lib.PROF_CORRELATION, "pyramid": _lib.PROF_PYRAMID}
+             "ecc_solve": 5, "correlation": _lib.PROF_CORRELATION, "pyramid": _lib.PROF_PYRAMID}
     kernels = {}
     for name, k in kinds.items():
         kernels[name] = {"launches": int(prof_r.launches[k]), "ms": prof_r.ms[k], "bytes": prof_r.bytes[k]}
@@ -147,18 +149,20 @@ def main():
 
     # ---- plain per-frame GPU form: one frame per launch sequence, host sync per ECC iteration
     npl = min(a.plain_frames, n)
-    acc1 = S.c_weigthed_average(W, H, 1)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for i in range(npl):
-        w1 = S.frame_sharpness([frames[i]])
-        reg.register_frame(frames[i])
-        s1 = (_lib.RegistrationStatus * 1)(reg.status())
-        acc1.add_registered_batch([frames[i]], s1, S.MOTION_AFFINE, frame_weights=w1)
-    acc1.compute()
-    torch.cuda.synchronize()
-    plain_fps = npl / (time.perf_counter() - t0)
-    print(f"[bench] plain per-frame GPU: {plain_fps:.1f} frames/s on {npl} frames", file=sys.stderr, flush=True)
+    plain_fps = None
+    if npl > 0:
+        acc1 = S.c_weigthed_average(W, H, 1)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for i in range(npl):
+            w1 = S.frame_sharpness([frames[i]])
+            reg.register_frame(frames[i])
+            s1 = (_lib.RegistrationStatus * 1)(reg.status())
+            acc1.add_registered_batch([frames[i]], s1, S.MOTION_AFFINE, frame_weights=w1)
+        acc1.compute()
+        torch.cuda.synchronize()
+        plain_fps = npl / (time.perf_counter() - t0)
+        print(f"[bench] plain per-frame GPU: {plain_fps:.1f} frames/s on {npl} frames", file=sys.stderr, flush=True)
 
     # ---- CPU baseline and FP64 warp error (oracle; checker only)
     cpu = None
@@ -226,8 +230,8 @@ def main():
                      "unit": "GB/s", "frac": round(dom["frac"], 4), "traffic": None,
                      "avg_launch_us": round(dom["avg_us"], 2), "launches": dom["launches"]},
         "cpu_baseline": cpu,
-        "plain_gpu": {"value": round(plain_fps, 2), "unit": "frames/s", "frames": npl},
-        "speedup_vs_plain": round(fps / plain_fps, 2),
+        "plain_gpu": {"value": round(plain_fps, 2), "unit": "frames/s", "frames": npl} if plain_fps else None,
+        "speedup_vs_plain": round(fps / plain_fps, 2) if plain_fps else None,
         "speedup_vs_cpu": round(fps / cpu["value"], 1) if cpu else None,
         "registered": nok,
         "main_iterations": {"mean": round(float(its.mean()), 2), "min": int(its.min()), "max": int(its.max())},

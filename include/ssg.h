@@ -244,7 +244,7 @@ int ssg_frame_sharpness(int n, const ssg_image *frames, float *weights);
  * kernel is bracketed by HIP events on the handle's stream; algorithmic bytes follow
  * the per-pixel minimum traffic of each kernel. ------------------------------------------ */
 enum { SSG_PROF_ECC_TRANSLATION = 0, SSG_PROF_ECC_MAIN = 1, SSG_PROF_CORRELATION = 2, SSG_PROF_PYRAMID = 3,
-       SSG_PROF_ACCUMULATE = 4, SSG_PROF_COUNT = 8 };
+       SSG_PROF_ACCUMULATE = 4, SSG_PROF_ECC_SOLVE = 5, SSG_PROF_COUNT = 8 };
 typedef struct ssg_profile {
   int64_t launches[SSG_PROF_COUNT];
   double ms[SSG_PROF_COUNT];     /* summed kernel durations */

@@ -43,6 +43,14 @@ def image_view(x) -> _lib.Image:
     return _lib.host_image(np.ascontiguousarray(x))
 
 
+def image_views(frames):
+    """ctypes array of ssg_image views of a list of frames; a ctypes array of
+    views passes through unchanged (build it once for repeated batch calls)."""
+    if isinstance(frames, C.Array):
+        return frames
+    return _lib.images([image_view(f) for f in frames])
+
+
 def c_image_registration_options(**ecc) -> _lib.ImageRegistrationOptions:
     """The reference defaults (c_frame_registration.h:47-136), ECC enabled with the
     forward-additive method; keyword arguments override fields of .ecc."""
@@ -107,7 +115,7 @@ class c_frame_registration:
 
     def register_batch(self, frames, masks=None):
         """Batched registration of device frames; returns a ctypes array of statuses."""
-        views = _lib.images([image_view(f) for f in frames])
+        views = image_views(frames)
         mviews = _lib.images([image_view(m) for m in masks]) if masks is not None else None
         st = (_lib.RegistrationStatus * len(frames))()
         rc = self._lib.ssg_registration_register_batch(self._h, len(frames), views, mviews, st)
@@ -180,7 +188,7 @@ class c_weigthed_average:
 
     def add_registered_batch(self, frames, statuses, motion_type, frame_weights=None,
                              interpolation=_lib.INTER_LINEAR, border_mode=_lib.BORDER_REFLECT101, border_value=0.0):
-        views = _lib.images([image_view(f) for f in frames])
+        views = image_views(frames)
         wts = (C.c_float * len(frames))(*frame_weights) if frame_weights is not None else None
         rc = self._lib.ssg_accumulation_add_registered_batch(self._h, len(frames), views, statuses, motion_type, wts,
                                                              interpolation, border_mode, border_value)
@@ -217,7 +225,7 @@ class c_weigthed_average:
 def frame_sharpness(frames) -> np.ndarray:
     """Per-frame weight (mean squared 3x3 laplacian) of device frames."""
     lib = _lib.load()
-    views = _lib.images([image_view(f) for f in frames])
+    views = image_views(frames)
     out = np.zeros(len(frames), np.float32)
     _lib.check(lib.ssg_frame_sharpness(len(frames), views, out.ctypes.data_as(C.POINTER(C.c_float))),
                "ssg_frame_sharpness")
@@ -251,7 +259,7 @@ def synth_render(p: _lib.SynthParams, first: int, frames) -> np.ndarray:
     """Renders frames[i] = frame first+i (first = -1: the reference) into device
     tensors; returns the (n, 6) true ref->frame maps."""
     lib = _lib.load()
-    views = _lib.images([image_view(f) for f in frames])
+    views = image_views(frames)
     truth = np.zeros((len(frames), 6), np.float64)
     rc = lib.ssg_synth_render(C.byref(p), first, len(frames), views, truth.ctypes.data_as(C.POINTER(C.c_double)))
     _lib.check(rc, "ssg_synth_render")

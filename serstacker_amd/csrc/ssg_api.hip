@@ -313,7 +313,7 @@ struct ssg_registration {
   // batch workspace
   int cap = 0;
   DevBuf cur_pyr, cur_mask_pyr, half_buf;
-  DevBuf xf_t, xf_m, res, partials, tile_counter, done_counter, active[2], active_count[2];
+  DevBuf xf_t, xf_m, res, partials, active[2], active_count[2];
   DevBuf frame_ptrs, mask_ptrs, zero_count, frame_list, tmp_in, tmp_mask, sum_tmp;
   int* pinned_count = nullptr;
   bool have_ref = false;
@@ -496,8 +496,6 @@ int ensure_workspace(ssg_registration* r, int n) {
   CHK(r->xf_m.ensure(sizeof(FrameXf) * n));
   CHK(r->res.ensure(sizeof(FrameRes) * n));
   CHK(r->partials.ensure(sizeof(double) * (size_t)n * tx0 * ty0 * NMOM_FULL));
-  CHK(r->tile_counter.ensure(sizeof(unsigned) * n));
-  CHK(r->done_counter.ensure(sizeof(unsigned) * 4));
   for (int i = 0; i < 2; ++i) {
     CHK(r->active[i].ensure(sizeof(int) * n));
     CHK(r->active_count[i].ensure(sizeof(int) * 4));
@@ -505,8 +503,6 @@ int ensure_workspace(ssg_registration* r, int n) {
   CHK(r->frame_ptrs.ensure(sizeof(void*) * n * 2));
   CHK(r->mask_ptrs.ensure(sizeof(void*) * n));
   CHK(r->frame_list.ensure(sizeof(int) * n));
-  HIPCHK(hipMemsetAsync(r->tile_counter.p, 0, sizeof(unsigned) * n, r->stream));
-  HIPCHK(hipMemsetAsync(r->done_counter.p, 0, sizeof(unsigned) * 4, r->stream));
   std::vector<int> list(n);
   for (int i = 0; i < n; ++i) list[i] = i;
   HIPCHK(hipMemcpyAsync(r->frame_list.p, list.data(), sizeof(int) * n, hipMemcpyHostToDevice, r->stream));
@@ -565,8 +561,6 @@ int run_ecc(ssg_registration* r, int n, int pass, bool has_cmask, bool plain) {
     a.xf = pass == 0 ? r->xf_t.as<FrameXf>() : r->xf_m.as<FrameXf>();
     a.res = r->res.as<FrameRes>();
     a.partials = r->partials.as<float>();
-    a.tile_counter = r->tile_counter.as<unsigned>();
-    a.done_counter = r->done_counter.as<unsigned>();
     a.nframes = n;
     a.pass = pass;
     a.level = l;
@@ -584,6 +578,9 @@ int run_ecc(ssg_registration* r, int n, int pass, bool has_cmask, bool plain) {
       r->prof.start(s);
       HIPCHK(launch_ecc_pass(motion, a, g, s));
       r->prof.stop(s, pass == 0 ? SSG_PROF_ECC_TRANSLATION : SSG_PROF_ECC_MAIN);
+      r->prof.start(s);
+      HIPCHK(launch_ecc_solve(motion, a, s));
+      r->prof.stop(s, SSG_PROF_ECC_SOLVE);
       const bool check = plain || ((it & 3) == 3);
       if (check && it + 1 < r->o.ecc.max_iterations) {
         HIPCHK(hipMemcpyAsync(r->pinned_count, a.active_count_out, sizeof(int), hipMemcpyDeviceToHost, s));
@@ -612,13 +609,12 @@ int run_corr(ssg_registration* r, int n, int pass, bool has_cmask) {
   c.rmask = r->ref_has_mask[0] ? r->ref_mask_pyr.as<uint8_t>() + r->loff[0] : nullptr;
   c.rmask_pitch = r->lpitch[0];
   c.cf = (float)r->ref_mean[0];
-  c.tiles_x = (c.w + ECC_TW - 1) / ECC_TW;
-  c.tiles_y = (c.h + ECC_TH - 1) / ECC_TH;
+  c.tiles_x = (c.w + CORR_TW - 1) / CORR_TW;
+  c.tiles_y = (c.h + CORR_TH - 1) / CORR_TH;
   c.ntiles = c.tiles_x * c.tiles_y;
   c.xf = pass == 0 ? r->xf_t.as<FrameXf>() : r->xf_m.as<FrameXf>();
   c.res = r->res.as<FrameRes>();
   c.partials = r->partials.as<float>();
-  c.tile_counter = r->tile_counter.as<unsigned>();
   c.frames = r->frame_list.as<int>();
   c.nlist = n;
   c.pass = pass;

@@ -19,6 +19,18 @@ enum : int { MOTION_TRANSLATION = 0, MOTION_EUCLIDEAN = 1, MOTION_SCALED_EUCLIDE
 
 constexpr int WAVE = 64;
 
+// Raw buffer access: a 32-bit per-lane byte offset plus a wave-uniform byte
+// offset (SGPR) and a small immediate, instead of a 64-bit address per lane.
+// Callers keep every access inside [0, bytes): the hardware range check does
+// not cover the uniform part, so it is never relied on.
+using rsrc_t = __amdgpu_buffer_rsrc_t;
+__device__ __forceinline__ rsrc_t make_rsrc(const void* base, uint32_t bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), (short)0, (int)bytes, 0x00020000);
+}
+__device__ __forceinline__ float bload_f32(rsrc_t r, uint32_t voff, uint32_t soff) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0));
+}
+
 // cv::borderInterpolate; -1 for CONSTANT/TRANSPARENT
 __device__ __forceinline__ int border_interp(int p, int len, int border) {
   if ((unsigned)p < (unsigned)len) return p;

@@ -16,13 +16,14 @@
 //   5. reduces the 41 moments across the wave by recursive halving, across
 //      waves through LDS, and stores the tile's moments in global
 //      coordinates (FP64).
-// The block that finishes a frame's last tile sums the tiles in a fixed order
-// (deterministic), projects the affine moments onto the parameters of the
-// transform (J = M J_affine for translation and euclidean), and solves the
-// normal equations in FP64 (Cholesky; failure rule of cv::invert).  It
-// updates the parameters, eps and the frame's convergence state.  The block
-// that finishes the last frame compacts the active-frame list for the next
-// launch.  No block ever waits on another.
+// A second launch (one block per frame) sums each frame's tiles in a fixed
+// order (deterministic), projects the affine moments onto the parameters of
+// the transform (J = M J_affine for translation and euclidean), and solves
+// the normal equations in FP64 (Cholesky; failure rule of cv::invert).  It
+// updates the parameters, eps and the frame's convergence state and appends
+// the frames still iterating to the next active list.  The kernel boundary
+// between the two launches makes the partials visible: no block waits on
+// another and no device-scope fence runs in the hot loop.
 #include <float.h>
 
 #include "ssg_engine.hpp"
@@ -289,206 +290,39 @@ __device__ void compact_active(const FrameXf* xf, int nframes, int* out, int* ou
 }
 
 // ------------------------------------------------------------------------------------------------
-template <int MOTION, bool FULL, bool HAS_CMASK, bool HAS_RMASK>
-__global__ void __launch_bounds__(ECC_THREADS) k_ecc_pass(EccPassArgs a) {
+#include "ssg_ecc_pass.inc"
+
+// One block per frame of the batch: sums the frame's tile moments in tile
+// order (deterministic), solves, and appends the frame to the next active
+// list while it keeps iterating.  Runs after k_ecc_pass, so the kernel
+// boundary makes every tile's partial visible without device-scope fences.
+template <int MOTION, bool FULL>
+__global__ void __launch_bounds__(256) k_ecc_solve(EccPassArgs a) {
   constexpr int NM = FULL ? NMOM_FULL : NMOM_TRANS;
-  constexpr int LW = ECC_TW + 4, LH = ECC_TH + 4;
-  __shared__ float g[LH][LW];
-  __shared__ float red[ECC_WAVES][64];
-  __shared__ float loc[64];
   __shared__ double dred[ECC_WAVES][64];
   __shared__ double s_solve[272];
-  __shared__ int s_flag;
-
+  const int f = blockIdx.x;
+  if (!(a.xf[f].flags & XF_ACTIVE)) return;
   const int tid = threadIdx.x, lane = tid & (WAVE - 1), wv = tid / WAVE;
-  const int count = *a.active_count_in;
-  if (count == 0) {
-    if (blockIdx.x == 0 && tid == 0) *a.active_count_out = 0;
-    return;
+  {
+    const int k = lane, j = wv;
+    double s = 0;
+    if (k < NM) {
+      const double* base = reinterpret_cast<const double*>(a.partials) + (size_t)f * a.ntiles * NM;
+      for (int t = j; t < a.ntiles; t += ECC_WAVES) s += base[(size_t)t * NM + k];
+    }
+    dred[j][k] = s;
   }
-  const int items = count * a.ntiles;
-  const int G = gridDim.x, b = blockIdx.x;
-  // consecutive items on one XCD: blocks are dispatched round-robin over 8 XCDs
-  const int vb = (G % 8 == 0) ? (b % 8) * (G / 8) + b / 8 : b;
-  const int slot = ReduceScatter<NM>::slot_of(lane);
-
-  for (int item = vb; item < items; item += G) {
-    const int fi = item / a.ntiles;
-    const int tile = item - fi * a.ntiles;
-    const int f = __builtin_amdgcn_readfirstlane(a.active_in[fi]);
-    const int tx = tile % a.tiles_x, ty = tile / a.tiles_x;
-    const int x0 = tx * ECC_TW, y0 = ty * ECC_TH;
-    const FrameXf& X = a.xf[f];
-    MapCoef mc;
-#pragma unroll
-    for (int i = 0; i < 8; ++i) mc.c[i] = X.mc.c[i];
-    const float cg = X.cg;
-    const float* __restrict__ cur = a.cur + f * a.cur_fstride;
-
-    // ---- 1. warped current image with halo (gw at clamped positions = BORDER_REPLICATE of gw)
-    for (int i = tid; i < LH * LW; i += ECC_THREADS) {
-      const int ly = i / LW, lx = i - ly * LW;
-      const int px = min(max(x0 - 2 + lx, 0), a.w - 1), py = min(max(y0 - 2 + ly, 0), a.h - 1);
-      float sx, sy;
-      warp_xy<MOTION>(mc, px, py, sx, sy);
-      g[ly][lx] = sample_linear_replicate(cur, a.cur_pitch, a.cw, a.ch, quant32(sx), quant32(sy));
+  __syncthreads();
+  if (tid < NM) s_solve[tid] = ((dred[0][tid] + dred[1][tid]) + dred[2][tid]) + dred[3][tid];
+  __syncthreads();
+  if (tid == 0) {
+    ecc_solve<MOTION, FULL>(s_solve, a.xf[f], a.res[f], a);
+    if (a.xf[f].flags & XF_ACTIVE) {
+      // the list order does not affect any frame's result
+      const int at = atomicAdd(a.active_count_out, 1);
+      a.active_out[at] = f;
     }
-    // tile fully inside the fast region: every warped mask sample is 255
-    bool interior = false;
-    if (!HAS_CMASK) {
-      const int xe = min(x0 + ECC_TW - 1, a.w - 1), ye = min(y0 + ECC_TH - 1, a.h - 1);
-      float mnx = 1e30f, mxx = -1e30f, mny = 1e30f, mxy = -1e30f;
-      const int cx[4] = {x0, xe, x0, xe}, cy[4] = {y0, y0, ye, ye};
-#pragma unroll
-      for (int i = 0; i < 4; ++i) {
-        float sx, sy;
-        warp_xy<MOTION>(mc, cx[i], cy[i], sx, sy);
-        mnx = fminf(mnx, sx); mxx = fmaxf(mxx, sx); mny = fminf(mny, sy); mxy = fmaxf(mxy, sy);
-      }
-      interior = mnx >= 0.25f && mny >= 0.25f && mxx <= (float)a.cw - 1.25f && mxy <= (float)a.ch - 1.25f;
-    }
-    __syncthreads();
-
-    // ---- 2..4. gradients, masks, moments over a 16-row column strip
-    const int lx = lane, r0 = wv * ECC_RPT, x = x0 + lx;
-    const bool colok = x < a.w;
-    const uint8_t* __restrict__ cmask = HAS_CMASK ? a.cmask + f * a.cmask_fstride : nullptr;
-    auto Rd = [&](int ly) {
-      const float* c = &g[ly + 2][lx + 2];
-      return fmaf(c[2] - c[-2], D5_K2, (c[1] - c[-1]) * D5_K1);
-    };
-    auto Rs = [&](int ly) {
-      const float* c = &g[ly + 2][lx + 2];
-      return fmaf(c[-1] + c[1], 0.25f, c[0] * 0.5f);
-    };
-    float rdm1 = Rd(r0 - 1), rd0 = Rd(r0);
-    float rsm2 = Rs(r0 - 2), rsm1 = Rs(r0 - 1), rs0 = Rs(r0), rsp1 = Rs(r0 + 1);
-
-    float A0 = 0, A1 = 0, A2 = 0, B0 = 0, B1 = 0, B2 = 0, C0 = 0, C1 = 0, C2 = 0;
-    float P0 = 0, P1 = 0, Q0 = 0, Q1 = 0, R0 = 0, R1 = 0, S0 = 0, S1 = 0;
-    float X0a = 0, X1a = 0, Y0a = 0, Y1a = 0;
-    float N = 0, F1 = 0, F2 = 0, G1 = 0, G2 = 0;
-#pragma unroll
-    for (int dy = 0; dy < ECC_RPT; ++dy) {
-      const int ly = r0 + dy, y = y0 + ly;
-      const float rdp1 = Rd(ly + 1);
-      const float rsp2 = Rs(ly + 2);
-      const float gx = fmaf(rdm1 + rdp1, 0.25f, fmaf(rd0, 0.5f, 0.f));
-      const float gy = fmaf(rsp2 - rsm2, D5_K2, fmaf(rsp1 - rsm1, D5_K1, 0.f));
-      const float gw = g[ly + 2][lx + 2];
-      const bool valid = colok && y < a.h;
-      bool m = valid;
-      float fv = 0.f;
-      if (valid) {
-        fv = a.ref[(size_t)y * a.ref_pitch + x];
-        if (!interior) {
-          float sx, sy;
-          warp_xy<MOTION>(mc, x, y, sx, sy);
-          m = remap_mask_u8(cmask, a.cmask_pitch, a.cw, a.ch, quant32(sx), quant32(sy)) >= 255;
-        }
-        if (HAS_RMASK) m = m && a.rmask[(size_t)y * a.rmask_pitch + x] != 0;
-      }
-      const float gxm = m ? gx : 0.f, gym = m ? gy : 0.f;
-      const float fp = m ? fv - a.cf : 0.f, gp = m ? gw - cg : 0.f;
-      const float dyf = (float)dy, dy2 = (float)(dy * dy);
-      if (FULL) {
-        const float Av = gxm * gxm, Bv = gxm * gym, Cv = gym * gym;
-        A0 += Av; A1 = fmaf(Av, dyf, A1); A2 = fmaf(Av, dy2, A2);
-        B0 += Bv; B1 = fmaf(Bv, dyf, B1); B2 = fmaf(Bv, dy2, B2);
-        C0 += Cv; C1 = fmaf(Cv, dyf, C1); C2 = fmaf(Cv, dy2, C2);
-        const float Pv = gxm * gp, Qv = gym * gp, Rv = gxm * fp, Sv = gym * fp;
-        P0 += Pv; P1 = fmaf(Pv, dyf, P1);
-        Q0 += Qv; Q1 = fmaf(Qv, dyf, Q1);
-        R0 += Rv; R1 = fmaf(Rv, dyf, R1);
-        S0 += Sv; S1 = fmaf(Sv, dyf, S1);
-        X0a += gxm; X1a = fmaf(gxm, dyf, X1a);
-        Y0a += gym; Y1a = fmaf(gym, dyf, Y1a);
-      } else {
-        A0 = fmaf(gxm, gxm, A0); B0 = fmaf(gxm, gym, B0); C0 = fmaf(gym, gym, C0);
-        P0 = fmaf(gxm, gp, P0); Q0 = fmaf(gym, gp, Q0);
-        R0 = fmaf(gxm, fp, R0); S0 = fmaf(gym, fp, S0);
-        X0a += gxm; Y0a += gym;
-      }
-      N += m ? 1.f : 0.f;
-      F1 += fp; F2 = fmaf(fp, fp, F2);
-      G1 += gp; G2 = fmaf(gp, gp, G2);
-      rdm1 = rd0; rd0 = rdp1;
-      rsm2 = rsm1; rsm1 = rs0; rs0 = rsp1; rsp1 = rsp2;
-    }
-
-    // ---- 5. tile-local moments (lx, ly from the tile origin), wave reduce-scatter
-    float v[NM];
-    if (FULL) {
-      const float lxf = (float)lx, r0f = (float)r0, r02 = r0f * r0f, lx2 = lxf * lxf;
-      auto six = [&](float* o, float s0, float s1, float s2) {
-        const float sy = fmaf(r0f, s0, s1);
-        o[0] = s0;
-        o[1] = lxf * s0;
-        o[2] = sy;
-        o[3] = lx2 * s0;
-        o[4] = lxf * sy;
-        o[5] = fmaf(r02, s0, fmaf(2.f * r0f, s1, s2));
-      };
-      auto three = [&](float* o, float s0, float s1) {
-        o[0] = s0;
-        o[1] = lxf * s0;
-        o[2] = fmaf(r0f, s0, s1);
-      };
-      six(v + 0, A0, A1, A2);
-      six(v + 6, B0, B1, B2);
-      six(v + 12, C0, C1, C2);
-      three(v + 18, P0, P1);
-      three(v + 21, Q0, Q1);
-      three(v + 24, R0, R1);
-      three(v + 27, S0, S1);
-      three(v + 30, X0a, X1a);
-      three(v + 33, Y0a, Y1a);
-      v[36] = N; v[37] = F1; v[38] = F2; v[39] = G1; v[40] = G2;
-    } else {
-      v[0] = A0; v[1] = B0; v[2] = C0; v[3] = P0; v[4] = Q0; v[5] = R0; v[6] = S0;
-      v[7] = X0a; v[8] = Y0a; v[9] = N; v[10] = F1; v[11] = F2; v[12] = G1; v[13] = G2;
-    }
-    const float mine = ReduceScatter<NM>::run(v, lane);
-    if (slot >= 0) red[wv][slot] = mine;
-    __syncthreads();
-    if (tid < NM) loc[tid] = red[0][tid] + red[1][tid] + red[2][tid] + red[3][tid];
-    __syncthreads();
-    double* part = reinterpret_cast<double*>(a.partials) + ((size_t)f * a.ntiles + tile) * NM;
-    if (tid < NM) part[tid] = FULL ? expand_moment(tid, loc, (double)x0, (double)y0) : (double)loc[tid];
-    __threadfence();
-    __syncthreads();
-    if (tid == 0) s_flag = atomicAdd(&a.tile_counter[f], 1u) == (unsigned)(a.ntiles - 1);
-    __syncthreads();
-    if (!s_flag) continue;
-
-    // ---- last tile of frame f: sum the tiles (fixed order) and solve
-    __threadfence();
-    {
-      const int k = lane, j = wv;
-      double s = 0;
-      if (k < NM) {
-        const double* base = reinterpret_cast<const double*>(a.partials) + (size_t)f * a.ntiles * NM;
-        for (int t = j; t < a.ntiles; t += ECC_WAVES) s += __builtin_nontemporal_load(base + (size_t)t * NM + k);
-      }
-      dred[j][k] = s;
-    }
-    __syncthreads();
-    if (tid < NM) s_solve[tid] = ((dred[0][tid] + dred[1][tid]) + dred[2][tid]) + dred[3][tid];
-    __syncthreads();
-    if (tid == 0) {
-      a.tile_counter[f] = 0;
-      ecc_solve<MOTION, FULL>(s_solve, a.xf[f], a.res[f], a);
-      __threadfence();
-      s_flag = atomicAdd(a.done_counter, 1u) == (unsigned)(count - 1);
-    }
-    __syncthreads();
-    if (s_flag) {
-      // last frame of this iteration: build the next active list
-      __threadfence();
-      compact_active(a.xf, a.nframes, a.active_out, a.active_count_out);
-      if (tid == 0) *a.done_counter = 0;
-    }
-    __syncthreads();
   }
 }
 
@@ -521,7 +355,26 @@ static hipError_t launch_pass_m(const EccPassArgs& a, bool full, int grid, hipSt
     else SSG_LAUNCH(false, false, false);
   }
 #undef SSG_LAUNCH
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  return hipSuccess;
+}
+
+template <int MOTION>
+static hipError_t launch_solve_m(const EccPassArgs& a, bool full, hipStream_t s) {
+  if (full) hipLaunchKernelGGL((k_ecc_solve<MOTION, true>), dim3(a.nframes), dim3(256), 0, s, a);
+  else hipLaunchKernelGGL((k_ecc_solve<MOTION, false>), dim3(a.nframes), dim3(256), 0, s, a);
   return hipGetLastError();
+}
+
+hipError_t launch_ecc_solve(int motion, const EccPassArgs& a, hipStream_t s) {
+  switch (motion) {
+    case MOTION_TRANSLATION: return launch_solve_m<MOTION_TRANSLATION>(a, false, s);
+    case MOTION_EUCLIDEAN: return launch_solve_m<MOTION_EUCLIDEAN>(a, true, s);
+    case MOTION_SCALED_EUCLIDEAN: return launch_solve_m<MOTION_SCALED_EUCLIDEAN>(a, true, s);
+    case MOTION_AFFINE: return launch_solve_m<MOTION_AFFINE>(a, true, s);
+  }
+  return hipErrorInvalidValue;
 }
 
 hipError_t launch_ecc_pass(int motion, const EccPassArgs& a, int grid, hipStream_t s) {
@@ -601,7 +454,6 @@ hipError_t launch_level_begin(const LevelBeginArgs& a, hipStream_t s) {
 template <int MOTION, bool HAS_CMASK, bool HAS_RMASK>
 __global__ void __launch_bounds__(ECC_THREADS) k_corr_pass(CorrPassArgs a) {
   __shared__ double dred[ECC_WAVES][8];
-  __shared__ int s_flag;
   const int tid = threadIdx.x, lane = tid & (WAVE - 1), wv = tid / WAVE;
   const int items = a.nlist * a.ntiles;
   const int G = gridDim.x, b = blockIdx.x;
@@ -612,7 +464,7 @@ __global__ void __launch_bounds__(ECC_THREADS) k_corr_pass(CorrPassArgs a) {
     const int f = __builtin_amdgcn_readfirstlane(a.frames[fi]);
     if (a.res[f].rejected) continue;
     const int tx = tile % a.tiles_x, ty = tile / a.tiles_x;
-    const int x0 = tx * ECC_TW, y0 = ty * ECC_TH;
+    const int x0 = tx * CORR_TW, y0 = ty * CORR_TH;
     MapCoef mc;
 #pragma unroll
     for (int i = 0; i < 8; ++i) mc.c[i] = a.xf[f].mc.c[i];
@@ -644,31 +496,37 @@ __global__ void __launch_bounds__(ECC_THREADS) k_corr_pass(CorrPassArgs a) {
     __syncthreads();
     double* part = reinterpret_cast<double*>(a.partials) + ((size_t)f * a.ntiles + tile) * NMOM_CORR;
     if (tid < NMOM_CORR) part[tid] = ((dred[0][tid] + dred[1][tid]) + dred[2][tid]) + dred[3][tid];
-    __threadfence();
-    __syncthreads();
-    if (tid == 0) s_flag = atomicAdd(&a.tile_counter[f], 1u) == (unsigned)(a.ntiles - 1);
-    __syncthreads();
-    if (s_flag) {
-      __threadfence();
-      if (tid == 0) {
-        const double* base = reinterpret_cast<const double*>(a.partials) + (size_t)f * a.ntiles * NMOM_CORR;
-        double S[6] = {0, 0, 0, 0, 0, 0};
-        for (int t = 0; t < a.ntiles; ++t)
-          for (int i = 0; i < 6; ++i) S[i] += __builtin_nontemporal_load(base + (size_t)t * NMOM_CORR + i);
-        a.tile_counter[f] = 0;
-        const double inv = S[0] > 0 ? 1.0 / S[0] : 0.0;
-        const double mf = S[1] * inv, mg = S[2] * inv;
-        const double vf = S[3] * inv - mf * mf, vg = S[4] * inv - mg * mg;
-        const double sf = sqrt(vf > 0 ? vf : 0.0), sg = sqrt(vg > 0 ? vg : 0.0);
-        const double cov = S[5] * inv - mf * mg;
-        const double rho = cov / (sf * sg);
-        FrameRes& R = a.res[f];
-        R.rho[a.pass] = rho;
-        const double thr = a.pass == 0 ? 0.75 * a.min_rho : a.min_rho;
-        if (rho < thr) R.rejected = a.pass == 0 ? 3 : 4;
-      }
-    }
-    __syncthreads();
+    __syncthreads();  // dred is rewritten by the next item
+  }
+}
+
+// One block per listed frame: tile sums in tile order, rho, and the gate.
+__global__ void __launch_bounds__(256) k_corr_finalize(CorrPassArgs a) {
+  __shared__ double red[ECC_WAVES][8];
+  const int f = a.frames[blockIdx.x];
+  if (a.res[f].rejected) return;
+  const int tid = threadIdx.x, lane = tid & (WAVE - 1), wv = tid / WAVE;
+  const double* base = reinterpret_cast<const double*>(a.partials) + (size_t)f * a.ntiles * NMOM_CORR;
+  double S[6] = {0, 0, 0, 0, 0, 0};
+  for (int t = tid; t < a.ntiles; t += blockDim.x)
+    for (int i = 0; i < 6; ++i) S[i] += base[(size_t)t * NMOM_CORR + i];
+  for (int i = 0; i < 6; ++i) S[i] = wave_sum(S[i]);
+  if (lane == 0)
+    for (int i = 0; i < 6; ++i) red[wv][i] = S[i];
+  __syncthreads();
+  if (tid == 0) {
+    for (int i = 0; i < 6; ++i) S[i] = ((red[0][i] + red[1][i]) + red[2][i]) + red[3][i];
+    const double inv = S[0] > 0 ? 1.0 / S[0] : 0.0;
+    const double mf = S[1] * inv, mg = S[2] * inv;
+    const double vf = S[3] * inv - mf * mf, vg = S[4] * inv - mg * mg;
+    const double sf = sqrt(vf > 0 ? vf : 0.0), sg = sqrt(vg > 0 ? vg : 0.0);
+    const double cov = S[5] * inv - mf * mg;
+    const double rho = cov / (sf * sg);
+    FrameRes& R = a.res[f];
+    R.rho[a.pass] = rho;
+    // `rho < thr` as in the reference: a NaN rho passes the gate
+    const double thr = a.pass == 0 ? 0.75 * a.min_rho : a.min_rho;
+    if (rho < thr) R.rejected = a.pass == 0 ? 3 : 4;
   }
 }
 
@@ -681,6 +539,9 @@ static hipError_t launch_corr_m(const CorrPassArgs& a, int grid, hipStream_t s) 
   else if (rm) SSG_LAUNCH(false, true);
   else SSG_LAUNCH(false, false);
 #undef SSG_LAUNCH
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_corr_finalize, dim3(a.nlist), dim3(256), 0, s, a);
   return hipGetLastError();
 }
 

@@ -37,11 +37,14 @@ constexpr int NMOM_FULL = 41;   // affine moments (see ssg_ecc.hip)
 constexpr int NMOM_TRANS = 14;  // translation moments
 constexpr int NMOM_CORR = 6;    // correlation moments
 
-constexpr int ECC_TW = 64;                    // tile width: one column per lane
+constexpr int ECC_LW = 64;                    // LDS row width: one column per lane
+constexpr int ECC_TW = ECC_LW - 4;            // output columns per tile (2-px gradient halo each side)
 constexpr int ECC_RPT = 16;                   // rows per thread
 constexpr int ECC_WAVES = 4;                  // waves (row groups) per block
 constexpr int ECC_TH = ECC_RPT * ECC_WAVES;   // tile height
+constexpr int ECC_LH = ECC_TH + 4;            // LDS rows
 constexpr int ECC_THREADS = WAVE * ECC_WAVES; // 256
+constexpr int CORR_TW = 64, CORR_TH = 64;     // correlation-pass tile (one column per lane, no halo)
 
 struct LevelGeom {
   int w, h;          // level size
@@ -69,8 +72,6 @@ struct EccPassArgs {
   FrameXf* xf;
   FrameRes* res;
   float* partials;        // [nframes][ntiles][NMOM]
-  unsigned* tile_counter; // [nframes]
-  unsigned* done_counter; // [1]
   const int* active_in;
   const int* active_count_in;
   int* active_out;
@@ -98,7 +99,6 @@ struct CorrPassArgs {
   const FrameXf* xf;
   FrameRes* res;
   float* partials;
-  unsigned* tile_counter;
   const int* frames;   // frames to check (alive ones)
   int nlist;
   int pass;            // 0: translation gate (0.75*min_rho), 1: final gate (min_rho)
@@ -167,6 +167,7 @@ hipError_t launch_u16_to_f32(const uint16_t* src, int src_pitch, float* dst, int
 // ECC (ssg_ecc.hip)
 hipError_t launch_level_begin(const LevelBeginArgs& a, hipStream_t s);
 hipError_t launch_ecc_pass(int motion, const EccPassArgs& a, int grid, hipStream_t s);
+hipError_t launch_ecc_solve(int motion, const EccPassArgs& a, hipStream_t s);
 hipError_t launch_corr_pass(int motion, const CorrPassArgs& a, int grid, hipStream_t s);
 int ecc_persistent_grid();
 

@@ -82,10 +82,129 @@ __global__ void __launch_bounds__(256) k_sepfilter(const float* const* __restric
   }
 }
 
+// Same filter on 64x64 output tiles for square kernels of radius R <= 7: the
+// source tile is gathered in batches of 4 loads per thread, the column pass
+// slides a (2R+1)-row register window down a 16-row strip per thread.
+constexpr int ST_TW = 64, ST_TH = 64;
+
+template <int R>
+__device__ __forceinline__ float row_filter_r(const float* c, const SepKernel& K) {
+  constexpr int n = 2 * R + 1;
+  const float* k = K.kx;
+  if (K.tx == 1 && n <= 5) {
+    float s = c[0] * k[R];
+    if (n >= 3) s = fmaf(c[-1] + c[1], k[R + 1], s);
+    if (n >= 5) s = fmaf(c[-2] + c[2], k[R + 2], s);
+    return s;
+  }
+  if (K.tx == 2 && n >= 3 && n <= 5) {
+    float s = (c[1] - c[-1]) * k[R + 1];
+    if (n >= 5) s = fmaf(c[2] - c[-2], k[R + 2], s);
+    return s;
+  }
+  float s = 0.f;
+#pragma unroll
+  for (int i = 0; i < n; ++i) s = fmaf(c[i - R], k[i], s);
+  return s;
+}
+
+template <int R>
+__device__ __forceinline__ float col_filter_r(const float* win, const SepKernel& K) {
+  // win[0..2R]: rows -R..R
+  constexpr int n = 2 * R + 1;
+  const float* k = K.ky;
+  if (K.ty == 1) {
+    float s = fmaf(win[R], k[R], 0.f);
+#pragma unroll
+    for (int i = 1; i <= R; ++i) s = fmaf(win[R - i] + win[R + i], k[R + i], s);
+    return s;
+  }
+  if (K.ty == 2) {
+    float s = 0.f;
+#pragma unroll
+    for (int i = 1; i <= R; ++i) s = fmaf(win[R + i] - win[R - i], k[R + i], s);
+    return s;
+  }
+  float s = 0.f;
+#pragma unroll
+  for (int i = 0; i < n; ++i) s = fmaf(win[i], k[i], s);
+  return s;
+}
+
+template <int R>
+__global__ void __launch_bounds__(256) k_sepfilter_t(const float* const* __restrict__ src_ptrs, int src_pitch,
+                                                     float* __restrict__ dst, long long dst_fstride, int dst_pitch,
+                                                     int w, int h, SepKernel K, int border) {
+  constexpr int TW = ST_TW + 2 * R, TH = ST_TH + 2 * R;
+  __shared__ float sm[TH][TW];
+  __shared__ float rb[TH][ST_TW];
+  const int f = blockIdx.z, tid = threadIdx.x;
+  const float* __restrict__ S = src_ptrs[f];
+  float* __restrict__ D = dst + f * dst_fstride;
+  const int x0 = blockIdx.x * ST_TW, y0 = blockIdx.y * ST_TH;
+  {
+    constexpr int NS = TW * TH, PER = (NS + 255) / 256, FB = 4;
+    float* sf = &sm[0][0];
+#pragma nounroll
+    for (int b0 = 0; b0 < PER; b0 += FB) {
+      int off[FB], dsti[FB];
+#pragma unroll
+      for (int j = 0; j < FB; ++j) {
+        const int i0 = tid + (b0 + j) * 256;
+        const bool ok = b0 + j < PER && i0 < NS;
+        const int i = ok ? i0 : 0;
+        const int ly = i / TW, lx = i - ly * TW;
+        const int bx = border_interp(x0 - R + lx, w, border), by = border_interp(y0 - R + ly, h, border);
+        off[j] = (bx < 0 || by < 0) ? -1 : by * src_pitch + bx;
+        dsti[j] = ok ? i : -1;
+      }
+      float v[FB];
+#pragma unroll
+      for (int j = 0; j < FB; ++j) v[j] = off[j] >= 0 ? S[off[j]] : 0.f;
+#pragma unroll
+      for (int j = 0; j < FB; ++j)
+        if (dsti[j] >= 0) sf[dsti[j]] = v[j];
+    }
+  }
+  __syncthreads();
+  for (int i = tid; i < TH * ST_TW; i += 256) {
+    const int ly = i / ST_TW, lx = i - ly * ST_TW;
+    rb[ly][lx] = row_filter_r<R>(&sm[ly][lx + R], K);
+  }
+  __syncthreads();
+  const int lx = tid & 63, r0 = (tid >> 6) * 16, gx = x0 + lx;
+  float win[2 * R + 1];
+#pragma unroll
+  for (int i = 0; i < 2 * R; ++i) win[i + 1] = rb[r0 + i][lx];
+#pragma unroll
+  for (int dy = 0; dy < 16; ++dy) {
+#pragma unroll
+    for (int i = 0; i < 2 * R; ++i) win[i] = win[i + 1];
+    win[2 * R] = rb[r0 + dy + 2 * R][lx];
+    const int gy = y0 + r0 + dy;
+    if (gx < w && gy < h) D[(size_t)gy * dst_pitch + gx] = col_filter_r<R>(win, K);
+  }
+}
+
 hipError_t launch_sepfilter(const float* const* src_ptrs, int src_pitch, float* dst, long long dst_fstride,
                             int dst_pitch, int w, int h, int nframes, const SepKernel& k, int border,
                             hipStream_t s) {
   if (k.nx > 2 * SF_MAXR + 1 || k.ny > 2 * SF_MAXR + 1 || nframes <= 0) return hipErrorInvalidValue;
+  if (k.nx == k.ny && (k.nx & 1) && k.nx <= 15) {
+    dim3 grid((w + ST_TW - 1) / ST_TW, (h + ST_TH - 1) / ST_TH, nframes);
+#define SSG_SF(RR) hipLaunchKernelGGL((k_sepfilter_t<RR>), grid, dim3(256), 0, s, src_ptrs, src_pitch, dst, dst_fstride, dst_pitch, w, h, k, border)
+    switch (k.nx / 2) {
+      case 1: SSG_SF(1); break;
+      case 2: SSG_SF(2); break;
+      case 3: SSG_SF(3); break;
+      case 4: SSG_SF(4); break;
+      case 5: SSG_SF(5); break;
+      case 6: SSG_SF(6); break;
+      default: SSG_SF(7); break;
+    }
+#undef SSG_SF
+    return hipGetLastError();
+  }
   dim3 grid((w + SF_TW - 1) / SF_TW, (h + SF_TH - 1) / SF_TH, nframes);
   hipLaunchKernelGGL(k_sepfilter, grid, dim3(256), 0, s, src_ptrs, src_pitch, dst, dst_fstride, dst_pitch, w, h, k,
                      border);
@@ -97,7 +216,7 @@ hipError_t launch_sepfilter(const float* const* src_ptrs, int src_pitch, float* 
 // Canonical: r = fma(s-1 + s1, 4, fma(s0, 6, s-2 + s2)); d = fma(r1 + r3, 4,
 // fma(r2, 6, r0 + r4)) * (1/256), the order of ssg_oracle.c:ora_pyr_down.
 // ---------------------------------------------------------------------------
-constexpr int PD_TW = 64, PD_TH = 16;
+constexpr int PD_TW = 64, PD_TH = 32;
 constexpr int PD_SW = 2 * PD_TW + 4, PD_SH = 2 * PD_TH + 4;
 
 __global__ void __launch_bounds__(256) k_pyrdown(const float* __restrict__ src, long long src_fstride, int src_pitch,
@@ -105,15 +224,33 @@ __global__ void __launch_bounds__(256) k_pyrdown(const float* __restrict__ src, 
                                                  int dst_pitch, int dw, int dh, int border) {
   __shared__ float sm[PD_SH][PD_SW + 1];
   __shared__ float rb[PD_SH][PD_TW];
-  const int f = blockIdx.z;
+  const int f = blockIdx.z, tid = threadIdx.x;
   const float* __restrict__ S = src + f * src_fstride;
   float* __restrict__ D = dst + f * dst_fstride;
   const int x0 = blockIdx.x * PD_TW, y0 = blockIdx.y * PD_TH;
   const int sx0 = 2 * x0 - 2, sy0 = 2 * y0 - 2;
-  for (int i = threadIdx.x; i < PD_SW * PD_SH; i += blockDim.x) {
-    const int ly = i / PD_SW, lx = i - ly * PD_SW;
-    const int bx = border_interp(sx0 + lx, sw, border), by = border_interp(sy0 + ly, sh, border);
-    sm[ly][lx] = (bx < 0 || by < 0) ? 0.f : S[(size_t)by * src_pitch + bx];
+  {
+    constexpr int NS = PD_SW * PD_SH, PER = (NS + 255) / 256, FB = 4;
+#pragma nounroll
+    for (int b0 = 0; b0 < PER; b0 += FB) {
+      int off[FB], ly[FB], lx[FB];
+#pragma unroll
+      for (int j = 0; j < FB; ++j) {
+        const int i0 = tid + (b0 + j) * 256;
+        const bool ok = b0 + j < PER && i0 < NS;
+        const int i = ok ? i0 : 0;
+        ly[j] = ok ? i / PD_SW : -1;
+        lx[j] = i - (i / PD_SW) * PD_SW;
+        const int bx = border_interp(sx0 + lx[j], sw, border), by = border_interp(sy0 + i / PD_SW, sh, border);
+        off[j] = (bx < 0 || by < 0) ? -1 : by * src_pitch + bx;
+      }
+      float v[FB];
+#pragma unroll
+      for (int j = 0; j < FB; ++j) v[j] = off[j] >= 0 ? S[off[j]] : 0.f;
+#pragma unroll
+      for (int j = 0; j < FB; ++j)
+        if (ly[j] >= 0) sm[ly[j]][lx[j]] = v[j];
+    }
   }
   __syncthreads();
   for (int i = threadIdx.x; i < PD_SH * PD_TW; i += blockDim.x) {
