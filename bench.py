@@ -15,13 +15,18 @@ One step = weights + batched registration + fused warp/accumulate + compute
 of all frames, with the frames resident in HBM before the timed region.
 
 Also reported:
-  * roofline of the dominant kernel (the main-pass ECC iteration kernel):
+  * roofline of the dominant kernel (the kernel family with the largest
+    summed time, the translation-first ECC iteration pass on this workload):
     algorithmic bytes / summed kernel time from HIP events on the library's
-    stream, against the 8 TB/s HBM peak;
+    stream, against the 8 TB/s HBM peak; `traffic` is the measured HBM bytes
+    per launch of that kernel (rocprofv3 FETCH_SIZE + WRITE_SIZE, scaled by
+    the same counters on a copy of known size), read from
+    profiles/pmc_traffic.json when it names the same kernel;
   * plain per-frame GPU form (one frame per launch sequence, host sync per ECC
     iteration), frames/s on a sample of frames;
   * CPU baseline: the FP32 oracle (plain C + OpenMP restatement of the
-    reference algorithm) on a bounded sample, frames/s;
+    reference algorithm) on a bounded sample (frames until about 12 s of CPU
+    work), frames/s;
   * warp error vs the FP64 oracle on a few frames (px).
 """
 from __future__ import annotations
@@ -56,9 +61,13 @@ def main():
     ap.add_argument("--width", type=int, default=1920)
     ap.add_argument("--height", type=int, default=1080)
     ap.add_argument("--plain-frames", type=int, default=24)
-    ap.add_argument("--cpu-frames", type=int, default=3)
+    ap.add_argument("--cpu-frames", type=int, default=64, help="upper bound of the CPU sample")
+    ap.add_argument("--cpu-seconds", type=float, default=12.0, help="the CPU sample stops after this much time")
     ap.add_argument("--oracle-frames", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calib-mib", type=int, default=0,
+                    help="after the timed region, copy a buffer of this size (a known-size reference for "
+                         "rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     a = ap.parse_args()
 
     if a.gpus != 1:
@@ -147,6 +156,23 @@ def main():
     print(f"[bench] kernels: " + ", ".join(f"{k}: {v['ms']:.1f} ms {v['gbs']:.0f} GB/s" for k, v in kernels.items()),
           file=sys.stderr, flush=True)
 
+    if a.copy_calib_mib > 0:
+        nb = a.copy_calib_mib << 20
+        src = torch.ones(nb // 4, dtype=torch.float32, device=dev)
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        print(f"[bench] calibration copy of {a.copy_calib_mib} MiB done", file=sys.stderr, flush=True)
+        del src, dst
+
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "pmc_traffic.json")
+    if os.path.exists(tpath):
+        with open(tpath) as fh:
+            tr = json.load(fh)
+        if tr.get("kernel_family") == dom_name:
+            traffic = tr.get("hbm_bytes_per_launch")
+
     # ---- plain per-frame GPU form: one frame per launch sequence, host sync per ECC iteration
     npl = min(a.plain_frames, n)
     plain_fps = None
@@ -175,13 +201,15 @@ def main():
         o32 = Oracle(32)
         o32.set_threads(threads)
         ref_h = ref.cpu().numpy()
-        sample = [frames[i].cpu().numpy() for i in range(a.cpu_frames)]
         oreg = o32.registration(reg_options(**kw))
         oreg.setup_reference_frame(ref_h)
         A = np.zeros((H, W), np.float32)
         Wt = np.zeros((H, W), np.float32)
-        t0 = time.perf_counter()
-        for f in sample:
+        done = 0
+        cpu_s = 0.0
+        for i in range(min(a.cpu_frames, n)):
+            f = frames[i].cpu().numpy()
+            t0 = time.perf_counter()
             wgt = o32.sharpness(f)
             ost = oreg.register_frame(f)
             t = o32.xform(3, list(ost.params))
@@ -190,9 +218,12 @@ def main():
             wimg = np.full((H, W), np.float32(wgt))
             wr, _ = o32.base_remap(wimg, rmap, interp=1, border=0, want_mask=False)
             o32.weighted_average_update(d, A, Wt, weights=(wr * (m / np.float32(255))).astype(np.float32))
-        cpu_s = time.perf_counter() - t0
-        cpu = {"value": round(len(sample) / cpu_s, 4), "unit": "frames/s", "cores": threads, "kind": "port",
-               "sample": f"{len(sample)} of the {n} frames ({W}x{H}): sharpness, registration (translation-first + "
+            cpu_s += time.perf_counter() - t0
+            done += 1
+            if cpu_s >= a.cpu_seconds:
+                break
+        cpu = {"value": round(done / cpu_s, 4), "unit": "frames/s", "cores": threads, "kind": "port",
+               "sample": f"{done} of the {n} frames ({W}x{H}, {cpu_s:.1f} s): sharpness, registration (translation-first + "
                          f"affine ECC, 4 levels), base_remap and weighted-average update, FP32 oracle, "
                          f"{threads} OpenMP threads"}
         print(f"[bench] cpu oracle: {cpu['value']} frames/s ({threads} threads)", file=sys.stderr, flush=True)
@@ -227,7 +258,7 @@ def main():
                                f"pyramid, translation-first, eps 0.02 px, <=50 iterations",
                    "frames": n, "width": W, "height": H},
         "roofline": {"bound": "hbm", "kernel": dom_name, "achieved": round(dom["gbs"], 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(dom["frac"], 4), "traffic": None,
+                     "unit": "GB/s", "frac": round(dom["frac"], 4), "traffic": traffic,
                      "avg_launch_us": round(dom["avg_us"], 2), "launches": dom["launches"]},
         "cpu_baseline": cpu,
         "plain_gpu": {"value": round(plain_fps, 2), "unit": "frames/s", "frames": npl} if plain_fps else None,

@@ -26,6 +26,8 @@
 // another and no device-scope fence runs in the hot loop.
 #include <float.h>
 
+#include <algorithm>
+
 #include "ssg_engine.hpp"
 
 namespace ssg {
@@ -326,23 +328,38 @@ __global__ void __launch_bounds__(256) k_ecc_solve(EccPassArgs a) {
   }
 }
 
-int ecc_persistent_grid() {
-  static int grid = 0;
-  if (!grid) {
-    int dev = 0, cus = 0;
+static int device_cus() {
+  static const int cus = [] {
+    int dev = 0, n = 0;
     (void)hipGetDevice(&dev);
-    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0) cus = 256;
-    grid = cus * 4;  // blocks resident per CU at this kernel's LDS/VGPR use
-    grid = (grid + 7) / 8 * 8;
-  }
-  return grid;
+    if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || n <= 0) n = 256;
+    return n;
+  }();
+  return cus;
+}
+
+// upper bound of the persistent grids (8 blocks per CU); each pass kernel
+// launches at most the blocks that are resident at once at its own VGPR/LDS use
+int ecc_persistent_grid() { return device_cus() * 8; }
+
+template <int MOTION, bool F, bool CM, bool RM>
+static int pass_resident_grid() {
+  static const int g = [] {
+    int nb = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_ecc_pass<MOTION, F, CM, RM>, ECC_THREADS, 0) !=
+            hipSuccess || nb <= 0)
+      nb = 4;
+    return (device_cus() * nb + 7) / 8 * 8;
+  }();
+  return g;
 }
 
 template <int MOTION>
 static hipError_t launch_pass_m(const EccPassArgs& a, bool full, int grid, hipStream_t s) {
   const bool cm = a.cmask != nullptr, rm = a.rmask != nullptr;
-#define SSG_LAUNCH(F, CM, RM) \
-  hipLaunchKernelGGL((k_ecc_pass<MOTION, F, CM, RM>), dim3(grid), dim3(ECC_THREADS), 0, s, a)
+#define SSG_LAUNCH(F, CM, RM)                                                                       \
+  hipLaunchKernelGGL((k_ecc_pass<MOTION, F, CM, RM>), dim3(std::min(grid, pass_resident_grid<MOTION, F, CM, RM>())), \
+                     dim3(ECC_THREADS), 0, s, a)
   if (full) {
     if (cm && rm) SSG_LAUNCH(true, true, true);
     else if (cm) SSG_LAUNCH(true, true, false);
