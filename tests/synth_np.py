@@ -79,3 +79,24 @@ def max_displacement(m1, m2, w, h):
     c = np.array([[0, 0, 1], [w - 1, 0, 1], [0, h - 1, 1], [w - 1, h - 1, 1]], np.float64).T
     d = (m1 - m2) @ c
     return float(np.sqrt((d ** 2).sum(0)).max())
+
+
+def blob_scene(w: int, h: int, seed: int, n: int = 60, smin: float = 3.0, smax: float = 7.0):
+    """A band-limited analytic scene: a sum of gaussian blobs (sigma smin..smax px).
+    ``render(T)`` evaluates scene(T x) exactly at every pixel, so a frame rendered
+    with T is registered onto render(identity) by W = T^-1 with no sampling bias."""
+    r = np.random.default_rng(seed)
+    xs, ys = r.uniform(-10, w + 10, n), r.uniform(-10, h + 10, n)
+    ss, aa = r.uniform(smin, smax, n), r.uniform(0.2, 1.0, n)
+
+    def render(T):
+        T = np.asarray(T, np.float64).reshape(2, 3)
+        yy, xx = np.mgrid[0:h, 0:w].astype(np.float64)
+        u = T[0, 0] * xx + T[0, 1] * yy + T[0, 2]
+        v = T[1, 0] * xx + T[1, 1] * yy + T[1, 2]
+        out = np.zeros((h, w))
+        for x, y, s, a in zip(xs, ys, ss, aa):
+            out += a * np.exp(-((u - x) ** 2 + (v - y) ** 2) / (2 * s * s))
+        return out
+
+    return render
