@@ -63,7 +63,28 @@ def build(jobs: int = 8, verbose: bool = False) -> str:
     os.replace(tmp, OUT)
     if verbose:
         print(f"built {OUT} from {len(srcs)} sources")
+    build_host_demo(verbose)
     return OUT
+
+
+DEMO_SRC = os.path.join(ROOT, "tests", "cpp", "host_api_demo.cpp")
+DEMO_BIN = os.path.join(ROOT, "tests", "cpp", "host_api_demo")
+
+
+def build_host_demo(verbose: bool = False) -> str:
+    """The plain C++ caller of include/ssg.hpp (host code only: g++, linked to
+    libssg.so and the HIP runtime for its device buffers)."""
+    rocm = os.path.dirname(os.path.dirname(os.path.realpath(hipcc())))
+    cmd = ["g++", "-std=c++17", "-O2", "-Wall", "-D__HIP_PLATFORM_AMD__", f"-I{os.path.join(ROOT, 'include')}",
+           f"-I{os.path.join(rocm, 'include')}", DEMO_SRC, "-o", DEMO_BIN, f"-L{PKG}", "-lssg",
+           f"-Wl,-rpath,$ORIGIN/../../serstacker_amd", f"-L{os.path.join(rocm, 'lib')}", "-lamdhip64",
+           f"-Wl,-rpath,{os.path.join(rocm, 'lib')}"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"host demo build failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+    if verbose:
+        print(f"built {DEMO_BIN}")
+    return DEMO_BIN
 
 
 if __name__ == "__main__":

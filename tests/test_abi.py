@@ -108,3 +108,22 @@ def test_invalid_arguments_fail_without_gpu_work(lib):
     assert L.ssg_registration_create(C.byref(o), C.byref(out)) == -4
     assert b"feature" in L.ssg_last_error() or b"ECC" in L.ssg_last_error()
     assert L.ssg_accumulation_create(99, 8, 8, 1, C.byref(out)) == -4
+
+
+def test_cpp_host_header_compiles():
+    """include/ssg.hpp is plain C++17 over the C ABI (no HIP or OpenCV headers)."""
+    import shutil
+    import subprocess
+    import tempfile
+
+    gxx = shutil.which("g++")
+    if gxx is None:
+        pytest.skip("g++ not available")
+    src = "#include \"ssg.hpp\"\nint main() { ssg::c_frame_registration r; (void)r.options(); return 0; }\n"
+    with tempfile.TemporaryDirectory() as d:
+        f = os.path.join(d, "t.cpp")
+        with open(f, "w") as fh:
+            fh.write(src)
+        r = subprocess.run([gxx, "-std=c++17", "-Wall", "-Werror", "-fsyntax-only", f"-I{os.path.join(ROOT, 'include')}",
+                            f], capture_output=True, text=True)
+        assert r.returncode == 0, r.stderr
