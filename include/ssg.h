@@ -60,7 +60,10 @@ enum { SSG_INTER_NEAREST = 0, SSG_INTER_LINEAR = 1, SSG_INTER_CUBIC = 2, SSG_INT
 enum { SSG_BORDER_CONSTANT = 0, SSG_BORDER_REPLICATE = 1, SSG_BORDER_REFLECT = 2, SSG_BORDER_WRAP = 3,
        SSG_BORDER_REFLECT101 = 4, SSG_BORDER_TRANSPARENT = 5 };
 /* image element types (OpenCV type codes) */
-enum { SSG_8UC1 = 0, SSG_16UC1 = 2, SSG_32SC1 = 4, SSG_32FC1 = 5, SSG_32FC2 = 13, SSG_32FC3 = 21 };
+enum { SSG_8UC1 = 0, SSG_16UC1 = 2, SSG_32SC1 = 4, SSG_32FC1 = 5, SSG_32FC2 = 13, SSG_32SC3 = 20, SSG_32FC3 = 21 };
+/* COLORID of the Bayer patterns, core/io/debayer.h:22-25 */
+enum { SSG_COLORID_BAYER_RGGB = 8, SSG_COLORID_BAYER_GRBG = 9, SSG_COLORID_BAYER_GBRG = 10,
+       SSG_COLORID_BAYER_BGGR = 11 };
 /* frame_accumulation_method, core/pipeline/c_image_stacking_pipeline/c_image_stacking_pipeline.h:26-34 */
 enum { SSG_ACC_AVERAGE = 0, SSG_ACC_WEIGHTED_AVERAGE = 1, SSG_ACC_FOCUS_STACK = 2, SSG_ACC_FFT = 3,
        SSG_ACC_BAYER_AVERAGE = 4 };
@@ -198,33 +201,54 @@ int ssg_remap(const ssg_image *src, const ssg_image *src_mask, const ssg_image *
 typedef struct ssg_accumulation ssg_accumulation;
 
 /* create_frame_accumulation (c_image_stacking_pipeline.cc:450-466): method is
- * SSG_ACC_WEIGHTED_AVERAGE or SSG_ACC_AVERAGE (both c_weigthed_average). */
+ * SSG_ACC_WEIGHTED_AVERAGE or SSG_ACC_AVERAGE (both c_weigthed_average), or
+ * SSG_ACC_BAYER_AVERAGE (c_bayer_average, c_frame_accumulation.h:222-262; channels = 1,
+ * the raw input; the accumulator has the reference's 3 channels, BGR order). */
 int ssg_accumulation_create(int32_t method, int32_t width, int32_t height, int32_t channels,
                             ssg_accumulation **out);
 void ssg_accumulation_destroy(ssg_accumulation *a);
 
-/* c_frame_accumulation::add(src, weights) (c_frame_accumulation.cc:182-221):
- * weights may be NULL, a SSG_8UC1 mask (nonzero = 1) or a SSG_32FC1 weight map. */
+/* c_bayer_average::set_bayer_pattern (c_frame_accumulation.cc:1136-1142): SSG_COLORID_BAYER_*. */
+int ssg_accumulation_set_bayer_pattern(ssg_accumulation *a, int32_t colorid);
+/* c_bayer_average::set_remap (c_frame_accumulation.cc:1149-1152): the SSG_32FC2 field of
+ * source positions for each accumulator pixel, used by the following add() calls; NULL
+ * clears it (clear() also clears it, :1154-1161). */
+int ssg_accumulation_set_remap(ssg_accumulation *a, const ssg_image *map);
+/* SSG_16UC1 raw frames are read as value / 2^bpp, the pipeline's conversion
+ * (c_image_stacking_pipeline_base.cc:269-273); default 16. */
+int ssg_accumulation_set_input_bpp(ssg_accumulation *a, int32_t bpp);
+
+/* c_frame_accumulation::add(src, weights) (c_frame_accumulation.cc:182-221, bayer :1169-1196):
+ * weights may be NULL, a SSG_8UC1 mask (nonzero = 1) or a SSG_32FC1 weight map.  Bayer
+ * sources are SSG_32FC1 or SSG_16UC1 raw frames of the accumulator size. */
 int ssg_accumulation_add(ssg_accumulation *a, const ssg_image *src, const ssg_image *weights);
 
 /* Fused remap + base_remap mask + weighted add of n registered frames (the
  * per-frame sequence custom_remap -> weights*mask/255 -> add of
  * c_image_stacking_pipeline.cc:1644-1776 for every frame at once).
  * statuses[i].ok == 0 skips frame i.  frame_weights: per-frame scalar weight
- * or NULL (1).  Frames: SSG_32FC1/SSG_32FC3 device images. */
+ * or NULL (1).  Frames: SSG_32FC1/SSG_32FC3 device images.
+ * Bayer accumulators: the pipeline's set_remap(current_remap) + add(raw,
+ * current_mask) (:1747-1748) for every frame, frames in order: raw SSG_32FC1 or
+ * SSG_16UC1 device frames, the remap of each status's transform, the base_remap
+ * mask (LINEAR only), frame_weights NULL. */
 int ssg_accumulation_add_registered_batch(ssg_accumulation *a, int n, const ssg_image *frames,
                                           const ssg_registration_status *statuses, int32_t motion_type,
                                           const float *frame_weights, int32_t interpolation,
                                           int32_t border_mode, double border_value);
 
 /* c_frame_accumulation::compute(avg, mask, dscale, ddepth) (c_frame_accumulation.cc:223-248):
- * avg = sum / weight (FP64 division), mask = weight > 0 (SSG_8UC1, optional). */
+ * avg = sum / weight (FP64 division), mask = weight > 0 (SSG_8UC1, optional).
+ * Bayer (:1198-1238): avg SSG_32FC3 = acc / counter per channel (float), mask = any
+ * channel's counter > 0. */
 int ssg_accumulation_compute(ssg_accumulation *a, ssg_image *avg, ssg_image *mask, double dscale);
 
-/* c_frame_accumulation::get_acc_counters (c_frame_accumulation.cc:250-260): the weight plane. */
+/* c_frame_accumulation::get_acc_counters (c_frame_accumulation.cc:250-260): the weight plane
+ * (SSG_32FC1); Bayer (:1240-1250): the counter times (1, 0.5, 1), SSG_32FC3. */
 int ssg_accumulation_get_acc_counters(ssg_accumulation *a, ssg_image *accw);
 
-/* Integer per-pixel sample counts (frames whose mask covered the pixel), SSG_32SC1. */
+/* Integer per-pixel sample counts (frames whose mask covered the pixel), SSG_32SC1; Bayer:
+ * per pixel and channel the taps added with a non-zero weight, SSG_32SC3. */
 int ssg_accumulation_get_counts(ssg_accumulation *a, ssg_image *counts);
 
 /* c_frame_accumulation::accumulated_frames (c_frame_accumulation.h:31). */

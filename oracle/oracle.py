@@ -171,6 +171,11 @@ class Oracle:
         L.ora_weighted_average_update.argtypes = [rp, C.c_int, C.c_int, C.c_int, u8p, rp, rp, rp]
         L.ora_sharpness_mean_sq_laplacian.argtypes = [rp, C.c_int, C.c_int]
         L.ora_sharpness_mean_sq_laplacian.restype = C.c_double
+        L.ora_bayer_channel.argtypes = [C.c_int, C.c_int, C.c_int]
+        L.ora_bayer_channel.restype = C.c_int
+        L.ora_bayer_accumulate.argtypes = [rp, C.c_int, C.c_int, rp, C.c_int, u8p, rp, rp, rp,
+                                           C.POINTER(C.c_int32)]
+        L.ora_bayer_compute.argtypes = [rp, rp, C.c_int, C.c_int, rp, u8p]
 
     # -- helpers -------------------------------------------------------------
     def _r(self, a):
@@ -355,6 +360,34 @@ class Oracle:
     def sharpness(self, src):
         src, sp = self._r(src)
         return self.lib.ora_sharpness_mean_sq_laplacian(sp, src.shape[1], src.shape[0])
+
+    # -- bayer average -------------------------------------------------------------
+    def bayer_channel(self, colorid, x, y):
+        return self.lib.ora_bayer_channel(colorid, x, y)
+
+    def bayer_state(self, w, h):
+        """(acc, cntr, counts) planes of an empty c_bayer_average, (h, w, 3)."""
+        return np.zeros((h, w, 3), self.dt), np.zeros((h, w, 3), self.dt), np.zeros((h, w, 3), np.int32)
+
+    def bayer_accumulate(self, src, state, colorid, rmap=None, mask=None, weights=None):
+        src, sp = self._r(src)
+        h, w = src.shape
+        acc, cntr, counts = state
+        assert acc.dtype == self.dt and cntr.dtype == self.dt and counts.dtype == np.int32
+        rm, rmp = (None, None) if rmap is None else self._r(rmap)
+        m, mp = self._u8(mask)
+        wts, wp = (None, None) if weights is None else self._r(weights)
+        self.lib.ora_bayer_accumulate(sp, w, h, rmp, colorid, mp, wp, acc.ctypes.data_as(self._rp),
+                                      cntr.ctypes.data_as(self._rp), counts.ctypes.data_as(C.POINTER(C.c_int32)))
+
+    def bayer_compute(self, state):
+        acc, cntr, _ = state
+        h, w = acc.shape[:2]
+        avg = np.zeros((h, w, 3), self.dt)
+        mask = np.zeros((h, w), np.uint8)
+        self.lib.ora_bayer_compute(acc.ctypes.data_as(self._rp), cntr.ctypes.data_as(self._rp), w, h,
+                                   avg.ctypes.data_as(self._rp), mask.ctypes.data_as(C.POINTER(C.c_uint8)))
+        return avg, mask
 
     # -- registration ------------------------------------------------------------
     def registration(self, opts: RegOptions):

@@ -1374,3 +1374,67 @@ double ora_sharpness_mean_sq_laplacian(const real *src, int w, int h) {
   free(rs);
   return s / ((double)w * h);
 }
+
+/* ------------------------------------------------------------------------ */
+/* c_bayer_average (c_frame_accumulation.cc:988-1334)                        */
+/* ------------------------------------------------------------------------ */
+int ora_bayer_channel(int colorid, int x, int y) {
+  /* generate_bayer_pattern_mask (:1267-1334): 2x2 cells, B = 0, G = 1, R = 2 */
+  static const int tab[4][4] = {
+      {2, 1, 1, 0}, /* RGGB: R G / G B */
+      {1, 2, 0, 1}, /* GRBG: G R / B G */
+      {1, 0, 2, 1}, /* GBRG: G B / R G */
+      {0, 1, 1, 2}, /* BGGR: B G / G R */
+  };
+  return tab[colorid - 8][((y & 1) << 1) | (x & 1)];
+}
+
+void ora_bayer_accumulate(const real *src, int w, int h, const real *rmap, int colorid,
+                          const uint8_t *mask_u8, const real *weights_f, real *acc, real *cntr,
+                          int32_t *counts) {
+#pragma omp parallel for schedule(static)
+  for (int y = 0; y < h; y++) {
+    for (int x = 0; x < w; x++) {
+      const size_t o = (size_t)y * w + x;
+      if (mask_u8 && !mask_u8[o]) continue;
+      if (!rmap) {
+        /* :996-1039: the pixel's own channel; float arithmetic */
+        const int cc = ora_bayer_channel(colorid, x, y);
+        const real wt = weights_f ? weights_f[o] : (real)1;
+        acc[o * 3 + cc] += weights_f ? src[o] * wt : src[o];
+        cntr[o * 3 + cc] += wt;
+        counts[o * 3 + cc] += wt != 0;
+        continue;
+      }
+      /* :1044-1081: p = rmap(x, y); (int) truncation; side weights in the map's
+       * precision, products in double, float accumulators */
+      const real px = rmap[2 * o], py = rmap[2 * o + 1];
+      const int sx = (int)px, sy = (int)py;
+      if (!(sx >= 0 && sx < w - 1 && sy >= 0 && sy < h - 1)) continue;
+      const double wt = weights_f ? (double)weights_f[o] : 1.0;
+      const double ax = (double)((real)(sx + 1) - px), ay = (double)((real)(sy + 1) - py);
+      const double bx = (double)(px - (real)sx), by = (double)(py - (real)sy);
+      const double s[4] = {ax * ay * wt, bx * ay * wt, ax * by * wt, bx * by * wt};
+      for (int k = 0; k < 4; k++) {
+        const int xx = sx + (k & 1), yy = sy + (k >> 1);
+        const int c = ora_bayer_channel(colorid, xx, yy);
+        const double v = (double)src[(size_t)yy * w + xx];
+        acc[o * 3 + c] = (real)((double)acc[o * 3 + c] + v * s[k]);
+        cntr[o * 3 + c] = (real)((double)cntr[o * 3 + c] + s[k]);
+        counts[o * 3 + c] += s[k] != 0.0;
+      }
+    }
+  }
+}
+
+void ora_bayer_compute(const real *acc, const real *cntr, int w, int h, real *avg, uint8_t *mask) {
+  for (size_t o = 0; o < (size_t)w * h; o++) {
+    int any = 0;
+    for (int c = 0; c < 3; c++) {
+      const real n = cntr[o * 3 + c];
+      avg[o * 3 + c] = n > 0 ? acc[o * 3 + c] / n : (real)0;
+      any |= n > 0;
+    }
+    if (mask) mask[o] = any ? 255 : 0;
+  }
+}

@@ -155,6 +155,22 @@ void ora_weighted_average_update(const ora_real *src, int w, int h, int cn,
                                  ora_real *acc, ora_real *accw);
 double ora_sharpness_mean_sq_laplacian(const ora_real *src, int w, int h);
 
+/* c_bayer_average (c_frame_accumulation.cc:988-1334).  Channels in the
+ * reference's BGR order (BAYER_B = 0, BAYER_G = 1, BAYER_R = 2,
+ * c_frame_accumulation.h:230-234); colorid = COLORID_BAYER_RGGB (8), GRBG (9),
+ * GBRG (10), BGGR (11) (core/io/debayer.h:22-25). */
+int ora_bayer_channel(int colorid, int x, int y);
+/* one add(src, weights) with the remap set by set_remap (rmap may be NULL):
+ * acc, cntr: w*h*3 (the reference's Mat3f accumulator and counter); counts:
+ * w*h*3 taps with a non-zero weight (integer per-cell sample counts). */
+void ora_bayer_accumulate(const ora_real *src, int w, int h, const ora_real *rmap, int colorid,
+                          const uint8_t *mask_u8, const ora_real *weights_f, ora_real *acc,
+                          ora_real *cntr, int32_t *counts);
+/* compute(): avg = acc / cntr per channel (0 where cntr <= 0); mask = any
+ * channel's cntr > 0 (255). */
+void ora_bayer_compute(const ora_real *acc, const ora_real *cntr, int w, int h, ora_real *avg,
+                       uint8_t *mask);
+
 #ifdef __cplusplus
 }
 #endif

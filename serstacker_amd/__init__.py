@@ -222,6 +222,76 @@ class c_weigthed_average:
         _lib.check(self._lib.ssg_accumulation_clear(self._h), "clear")
 
 
+class c_bayer_average:
+    """c_bayer_average (core/average/c_frame_accumulation.h:222-262): raw Bayer frames
+    accumulated per channel (BGR) through an optional remap."""
+
+    def __init__(self, width: int, height: int, colorid: int = _lib.COLORID_BAYER_RGGB, bpp: int = 16):
+        self._lib = _lib.load()
+        h = C.c_void_p()
+        _lib.check(self._lib.ssg_accumulation_create(_lib.ACC_BAYER_AVERAGE, width, height, 1, C.byref(h)),
+                   "ssg_accumulation_create")
+        self._h = h
+        self.w, self.h = width, height
+        self.set_bayer_pattern(colorid)
+        _lib.check(self._lib.ssg_accumulation_set_input_bpp(self._h, bpp), "set_input_bpp")
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            self._lib.ssg_accumulation_destroy(h)
+            self._h = None
+
+    def set_bayer_pattern(self, colorid: int):
+        _lib.check(self._lib.ssg_accumulation_set_bayer_pattern(self._h, colorid), "set_bayer_pattern")
+
+    def set_remap(self, rmap=None):
+        v = image_view(rmap) if rmap is not None else None
+        _lib.check(self._lib.ssg_accumulation_set_remap(self._h, C.byref(v) if v else None), "set_remap")
+
+    def add(self, src, weights=None) -> bool:
+        sv = image_view(src)
+        wv = image_view(weights)
+        rc = self._lib.ssg_accumulation_add(self._h, C.byref(sv), C.byref(wv) if wv else None)
+        if rc:
+            log.error("add: %s", self._lib.ssg_last_error().decode())
+            return False
+        return True
+
+    def add_registered_batch(self, frames, statuses, motion_type):
+        views = image_views(frames)
+        rc = self._lib.ssg_accumulation_add_registered_batch(self._h, len(views), views, statuses, motion_type, None,
+                                                             _lib.INTER_LINEAR, _lib.BORDER_REFLECT101, 0.0)
+        _lib.check(rc, "add_registered_batch")
+
+    def compute(self):
+        avg = np.zeros((self.h, self.w, 3), np.float32)
+        mask = np.zeros((self.h, self.w), np.uint8)
+        av, mv = _lib.host_image(avg), _lib.host_image(mask)
+        _lib.check(self._lib.ssg_accumulation_compute(self._h, C.byref(av), C.byref(mv), 1.0), "compute")
+        return avg, mask
+
+    def get_acc_counters(self) -> np.ndarray:
+        w = np.zeros((self.h, self.w, 3), np.float32)
+        v = _lib.host_image(w)
+        _lib.check(self._lib.ssg_accumulation_get_acc_counters(self._h, C.byref(v)), "get_acc_counters")
+        return w
+
+    def counts(self) -> np.ndarray:
+        c = np.zeros((self.h, self.w, 3), np.int32)
+        v = _lib.host_image(c)
+        _lib.check(self._lib.ssg_accumulation_get_counts(self._h, C.byref(v)), "get_counts")
+        return c
+
+    def accumulated_frames(self) -> int:
+        n = C.c_int32()
+        _lib.check(self._lib.ssg_accumulation_accumulated_frames(self._h, C.byref(n)), "accumulated_frames")
+        return n.value
+
+    def clear(self):
+        _lib.check(self._lib.ssg_accumulation_clear(self._h), "clear")
+
+
 def frame_sharpness(frames) -> np.ndarray:
     """Per-frame weight (mean squared 3x3 laplacian) of device frames."""
     lib = _lib.load()

@@ -15,7 +15,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libssg.so")
 
 SSG_OK = 0
-SSG_8UC1, SSG_16UC1, SSG_32SC1, SSG_32FC1, SSG_32FC2, SSG_32FC3 = 0, 2, 4, 5, 13, 21
+SSG_8UC1, SSG_16UC1, SSG_32SC1, SSG_32FC1, SSG_32FC2, SSG_32SC3, SSG_32FC3 = 0, 2, 4, 5, 13, 20, 21
+COLORID_BAYER_RGGB, COLORID_BAYER_GRBG, COLORID_BAYER_GBRG, COLORID_BAYER_BGGR = 8, 9, 10, 11
 MOTION_TRANSLATION, MOTION_EUCLIDEAN, MOTION_SCALED_EUCLIDEAN, MOTION_AFFINE = 0, 1, 2, 3
 ECC_ALIGN_FORWARD_ADDITIVE, ECC_ALIGN_INVERSE_COMPOSITIONAL, ECC_ALIGN_LM, ECC_ALIGN_INVERSE_COMPOSITIONAL_LM = 0, 1, 2, 3
 INTER_NEAREST, INTER_LINEAR, INTER_CUBIC, INTER_AREA = 0, 1, 2, 3
@@ -160,6 +161,9 @@ _SIGS = [
     ("ssg_accumulation_get_counts", C.c_int, [C.c_void_p, _P(Image)]),
     ("ssg_accumulation_accumulated_frames", C.c_int, [C.c_void_p, _P(C.c_int32)]),
     ("ssg_accumulation_clear", C.c_int, [C.c_void_p]),
+    ("ssg_accumulation_set_bayer_pattern", C.c_int, [C.c_void_p, C.c_int32]),
+    ("ssg_accumulation_set_remap", C.c_int, [C.c_void_p, _P(Image)]),
+    ("ssg_accumulation_set_input_bpp", C.c_int, [C.c_void_p, C.c_int32]),
     ("ssg_frame_sharpness", C.c_int, [C.c_int, _P(Image), _P(C.c_float)]),
     ("ssg_registration_set_profiling", C.c_int, [C.c_void_p, C.c_int]),
     ("ssg_registration_get_profile", C.c_int, [C.c_void_p, _P(Profile), C.c_int]),
@@ -197,7 +201,8 @@ def check(rc: int, what: str = ""):
 
 # ---- image views -------------------------------------------------------------------------------
 _NP_TYPE = {(np.dtype(np.uint8), 1): SSG_8UC1, (np.dtype(np.uint16), 1): SSG_16UC1,
-            (np.dtype(np.int32), 1): SSG_32SC1, (np.dtype(np.float32), 1): SSG_32FC1,
+            (np.dtype(np.int32), 1): SSG_32SC1, (np.dtype(np.int32), 3): SSG_32SC3,
+            (np.dtype(np.float32), 1): SSG_32FC1,
             (np.dtype(np.float32), 2): SSG_32FC2, (np.dtype(np.float32), 3): SSG_32FC3}
 
 
@@ -217,8 +222,10 @@ def device_image(t) -> Image:
     if not t.is_cuda:
         raise ValueError("tensor must be on the GPU")
     cn = 1 if t.dim() == 2 else t.shape[2]
-    dt = {torch.uint8: np.uint8, torch.int16: np.uint16, torch.int32: np.int32,
-          torch.float32: np.float32}[t.dtype]
+    dts = {torch.uint8: np.uint8, torch.int16: np.uint16, torch.int32: np.int32, torch.float32: np.float32}
+    if hasattr(torch, "uint16"):
+        dts[torch.uint16] = np.uint16
+    dt = dts[t.dtype]
     if t.dtype == torch.int16:
         dt = np.uint16
     ty = _NP_TYPE[(np.dtype(dt), cn)]

@@ -82,6 +82,7 @@ int elem_size(int type) {
     case SSG_32FC1: return 4;
     case SSG_32FC2: return 8;
     case SSG_32FC3: return 12;
+    case SSG_32SC3: return 12;
   }
   return 0;
 }
@@ -1024,16 +1025,39 @@ struct ssg_accumulation {
   DevBuf acc, cnt, part, part_cnt, maps, weights, valid, frame_ptrs, tmp;
   int accumulated = 0;
   Profiler prof;
+  // c_bayer_average
+  int colorid = 0;
+  unsigned pattern = 0;
+  int bpp = 16;
+  DevBuf rmap;
+  int rmap_pitch = 0;  // floats per row; 0 = no remap set
+  bool bayer() const { return method == SSG_ACC_BAYER_AVERAGE; }
+  // accumulator planes: weighted average (2*cn + 2) f32 planes; bayer acc[3] + cntr[3]
+  size_t acc_bytes() const { return sizeof(float) * plane * (bayer() ? 6 : 2 * cn + 2); }
+  size_t cnt_bytes() const { return sizeof(unsigned) * plane * (bayer() ? 3 : 1); }
+  BayerState bayer_state() {
+    BayerState st;
+    st.acc = acc.as<float>();
+    st.cntr = acc.as<float>() + 3 * plane;
+    st.cnt = cnt.as<unsigned>();
+    st.plane = plane;
+    st.w = w;
+    st.h = h;
+    st.pattern = pattern;
+    return st;
+  }
 };
 
 extern "C" int ssg_accumulation_create(int32_t method, int32_t width, int32_t height, int32_t channels,
                                        ssg_accumulation** out) {
   if (!out) return fail(SSG_E_INVALID_ARG, "out is null");
   *out = nullptr;
-  if (method != SSG_ACC_WEIGHTED_AVERAGE && method != SSG_ACC_AVERAGE)
+  if (method != SSG_ACC_WEIGHTED_AVERAGE && method != SSG_ACC_AVERAGE && method != SSG_ACC_BAYER_AVERAGE)
     return fail(SSG_E_UNSUPPORTED, "accumulation method %d", method);
   if (width <= 0 || height <= 0 || (channels != 1 && channels != 3))
     return fail(SSG_E_INVALID_ARG, "accumulator %dx%dx%d", width, height, channels);
+  if (method == SSG_ACC_BAYER_AVERAGE && channels != 1)
+    return fail(SSG_E_INVALID_ARG, "a bayer accumulator takes 1-channel raw frames");
   ssg_accumulation* a = new ssg_accumulation();
   a->method = method;
   a->w = width;
@@ -1045,8 +1069,8 @@ extern "C" int ssg_accumulation_create(int32_t method, int32_t width, int32_t he
     delete a;
     return fail(SSG_E_HIP, "hipStreamCreate: %s", hipGetErrorString(e));
   }
-  int rc = a->acc.ensure(sizeof(float) * a->plane * (2 * channels + 2));
-  if (rc == SSG_OK) rc = a->cnt.ensure(sizeof(unsigned) * a->plane);
+  int rc = a->acc.ensure(a->acc_bytes());
+  if (rc == SSG_OK) rc = a->cnt.ensure(a->cnt_bytes());
   if (rc == SSG_OK) rc = ensure_tables(a->stream);
   if (rc != SSG_OK) {
     (void)hipStreamDestroy(a->stream);
@@ -1068,12 +1092,81 @@ extern "C" void ssg_accumulation_destroy(ssg_accumulation* a) {
 
 extern "C" int ssg_accumulation_clear(ssg_accumulation* a) {
   if (!a) return fail(SSG_E_INVALID_ARG, "null handle");
-  HIPCHK(hipMemsetAsync(a->acc.p, 0, sizeof(float) * a->plane * (2 * a->cn + 2), a->stream));
-  HIPCHK(hipMemsetAsync(a->cnt.p, 0, sizeof(unsigned) * a->plane, a->stream));
+  HIPCHK(hipMemsetAsync(a->acc.p, 0, a->acc_bytes(), a->stream));
+  HIPCHK(hipMemsetAsync(a->cnt.p, 0, a->cnt_bytes(), a->stream));
   HIPCHK(hipStreamSynchronize(a->stream));
   a->accumulated = 0;
+  a->rmap_pitch = 0;  // c_bayer_average::clear releases the remap (:1154-1161)
   return SSG_OK;
 }
+
+extern "C" int ssg_accumulation_set_bayer_pattern(ssg_accumulation* a, int32_t colorid) {
+  if (!a) return fail(SSG_E_INVALID_ARG, "null handle");
+  if (!a->bayer()) return fail(SSG_E_INVALID_ARG, "not a bayer accumulator");
+  const unsigned bits = bayer_pattern_bits(colorid);
+  if (!bits) return fail(SSG_E_INVALID_ARG, "colorid %d is not a bayer pattern", colorid);
+  a->colorid = colorid;
+  a->pattern = bits;
+  return SSG_OK;
+}
+
+extern "C" int ssg_accumulation_set_remap(ssg_accumulation* a, const ssg_image* map) {
+  if (!a) return fail(SSG_E_INVALID_ARG, "null handle");
+  if (!a->bayer()) return fail(SSG_E_INVALID_ARG, "not a bayer accumulator");
+  if (!map) {
+    a->rmap_pitch = 0;
+    return SSG_OK;
+  }
+  CHK(check_image(map, "map"));
+  if (map->type != SSG_32FC2 || map->width != a->w || map->height != a->h)
+    return fail(SSG_E_INVALID_ARG, "remap must be SSG_32FC2 of the accumulator size (%dx%d)", a->w, a->h);
+  const size_t row = (size_t)a->w * 8;
+  CHK(a->rmap.ensure(row * a->h));
+  HIPCHK(hipMemcpy2DAsync(a->rmap.p, row, map->data, (size_t)map->step, row, a->h,
+                          map->on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, a->stream));
+  HIPCHK(hipStreamSynchronize(a->stream));
+  a->rmap_pitch = 2 * a->w;
+  return SSG_OK;
+}
+
+extern "C" int ssg_accumulation_set_input_bpp(ssg_accumulation* a, int32_t bpp) {
+  if (!a) return fail(SSG_E_INVALID_ARG, "null handle");
+  if (bpp < 1 || bpp > 16) return fail(SSG_E_INVALID_ARG, "bpp %d", bpp);
+  a->bpp = bpp;
+  return SSG_OK;
+}
+
+namespace {
+int bayer_add(ssg_accumulation* a, const ssg_image* src, const ssg_image* weights) {
+  if (!a->pattern) return fail(SSG_E_NOT_READY, "no bayer pattern set");
+  if (src->type != SSG_32FC1 && src->type != SSG_16UC1)
+    return fail(SSG_E_INVALID_ARG, "bayer frames are SSG_32FC1 or SSG_16UC1 raw images");
+  hipStream_t s = a->stream;
+  DevBuf ts, tw;
+  const void* sp;
+  size_t spb;
+  CHK(device_view(src, ts, s, &sp, &spb));
+  const uint8_t* mp = nullptr;
+  const float* wp = nullptr;
+  size_t wpb = 0;
+  if (weights) {
+    CHK(check_image(weights, "weights"));
+    if (weights->width != a->w || weights->height != a->h) return fail(SSG_E_INVALID_ARG, "weights size");
+    const void* p;
+    CHK(device_view(weights, tw, s, &p, &wpb));
+    if (weights->type == SSG_8UC1) mp = (const uint8_t*)p;
+    else if (weights->type == SSG_32FC1) wp = (const float*)p;
+    else return fail(SSG_E_INVALID_ARG, "weights must be SSG_8UC1 or SSG_32FC1");
+  }
+  const bool u16 = src->type == SSG_16UC1;
+  HIPCHK(launch_bayer_add(a->bayer_state(), sp, (int)(spb / (u16 ? 2 : 4)), u16, 1.f / (float)(1 << a->bpp),
+                          a->rmap_pitch ? a->rmap.as<float>() : nullptr, a->rmap_pitch, mp, (int)wpb, wp,
+                          (int)(wpb / 4), s));
+  HIPCHK(hipStreamSynchronize(s));
+  ++a->accumulated;
+  return SSG_OK;
+}
+}  // namespace
 
 extern "C" int ssg_accumulation_add(ssg_accumulation* a, const ssg_image* src, const ssg_image* weights) {
   if (!a) return fail(SSG_E_INVALID_ARG, "null handle");
@@ -1081,6 +1174,7 @@ extern "C" int ssg_accumulation_add(ssg_accumulation* a, const ssg_image* src, c
   if (src->width != a->w || src->height != a->h)
     return fail(SSG_E_INVALID_ARG, "current frame (%dx%d) and accumulator (%dx%d) sizes not match", src->width, src->height,
                 a->w, a->h);
+  if (a->bayer()) return bayer_add(a, src, weights);
   if (channels_of(src->type) != a->cn || (src->type != SSG_32FC1 && src->type != SSG_32FC3))
     return fail(SSG_E_INVALID_ARG, "frame type %d does not match the accumulator (%d channels, f32)", src->type, a->cn);
   hipStream_t s = a->stream;
@@ -1107,6 +1201,61 @@ extern "C" int ssg_accumulation_add(ssg_accumulation* a, const ssg_image* src, c
   return SSG_OK;
 }
 
+namespace {
+int bayer_batch(ssg_accumulation* a, int n, const ssg_image* frames, const ssg_registration_status* statuses,
+                int motion_type, const float* frame_weights, int interpolation) {
+  if (!a->pattern) return fail(SSG_E_NOT_READY, "no bayer pattern set");
+  if (frame_weights) return fail(SSG_E_INVALID_ARG, "bayer batches take the base_remap mask only (frame_weights NULL)");
+  if (interpolation != SSG_INTER_LINEAR) return fail(SSG_E_UNSUPPORTED, "bayer batch masks are remapped LINEAR");
+  const int type = frames[0].type;
+  if (type != SSG_32FC1 && type != SSG_16UC1)
+    return fail(SSG_E_INVALID_ARG, "bayer frames are SSG_32FC1 or SSG_16UC1 raw images");
+  const int es = type == SSG_16UC1 ? 2 : 4;
+  for (int f = 0; f < n; ++f) {
+    CHK(check_image(&frames[f], "frame"));
+    if (!frames[f].on_device) return fail(SSG_E_INVALID_ARG, "frame %d: batched frames must be device images", f);
+    if (frames[f].width != a->w || frames[f].height != a->h || frames[f].type != type ||
+        frames[f].step != frames[0].step)
+      return fail(SSG_E_INVALID_ARG, "frame %d: bayer batch frames share the accumulator size, type and step", f);
+  }
+  if (frames[0].step % es) return fail(SSG_E_INVALID_ARG, "frame step must be a multiple of the element size");
+  hipStream_t s = a->stream;
+  std::vector<MapCoef> maps(n);
+  std::vector<int> valid(n);
+  std::vector<const void*> ptrs(n);
+  int nvalid = 0;
+  for (int f = 0; f < n; ++f) {
+    maps[f] = map_coef(motion_type, statuses[f].params, nullptr);
+    valid[f] = statuses[f].ok != 0;
+    nvalid += valid[f];
+    ptrs[f] = frames[f].data;
+  }
+  CHK(a->maps.ensure(sizeof(MapCoef) * n));
+  CHK(a->valid.ensure(sizeof(int) * n));
+  CHK(a->frame_ptrs.ensure(sizeof(void*) * n));
+  HIPCHK(hipMemcpyAsync(a->maps.p, maps.data(), sizeof(MapCoef) * n, hipMemcpyHostToDevice, s));
+  HIPCHK(hipMemcpyAsync(a->valid.p, valid.data(), sizeof(int) * n, hipMemcpyHostToDevice, s));
+  HIPCHK(hipMemcpyAsync(a->frame_ptrs.p, ptrs.data(), sizeof(void*) * n, hipMemcpyHostToDevice, s));
+  BayerBatchArgs g{};
+  g.frames = a->frame_ptrs.as<const void*>();
+  g.src_pitch = (int)(frames[0].step / es);
+  g.scale = 1.f / (float)(1 << a->bpp);
+  g.maps = a->maps.as<MapCoef>();
+  g.valid = a->valid.as<int>();
+  g.nframes = n;
+  g.st = a->bayer_state();
+  a->prof.start(s);
+  HIPCHK(launch_bayer_batch(motion_type, type == SSG_16UC1, g, s));
+  a->prof.stop(s, SSG_PROF_ACCUMULATE);
+  HIPCHK(hipStreamSynchronize(s));
+  // algorithmic bytes: every valid raw frame read once; accumulator planes read and written once
+  a->prof.add_bytes(SSG_PROF_ACCUMULATE, (double)nvalid * es * a->plane + 2.0 * (6 * 4 + 3 * 4) * a->plane);
+  a->prof.collect();
+  a->accumulated += nvalid;
+  return SSG_OK;
+}
+}  // namespace
+
 extern "C" int ssg_accumulation_add_registered_batch(ssg_accumulation* a, int n, const ssg_image* frames,
                                                      const ssg_registration_status* statuses, int32_t motion_type,
                                                      const float* frame_weights, int32_t interpolation,
@@ -1118,6 +1267,7 @@ extern "C" int ssg_accumulation_add_registered_batch(ssg_accumulation* a, int n,
     return fail(SSG_E_UNSUPPORTED, "interpolation %d", interpolation);
   if (motion_type < SSG_MOTION_TRANSLATION || motion_type > SSG_MOTION_AFFINE)
     return fail(SSG_E_INVALID_ARG, "motion_type %d", motion_type);
+  if (a->bayer()) return bayer_batch(a, n, frames, statuses, motion_type, frame_weights, interpolation);
   const int sw = frames[0].width, sh = frames[0].height;
   const int64_t step = frames[0].step;
   for (int f = 0; f < n; ++f) {
@@ -1202,16 +1352,17 @@ extern "C" int ssg_accumulation_compute(ssg_accumulation* a, ssg_image* avg, ssg
   size_t apb = 0;
   uint8_t* mp = nullptr;
   size_t mpb = 0;
+  const int ocn = a->bayer() ? 3 : a->cn;
   if (avg) {
     CHK(check_image(avg, "avg"));
-    if (avg->width != a->w || avg->height != a->h || channels_of(avg->type) != a->cn ||
+    if (avg->width != a->w || avg->height != a->h || channels_of(avg->type) != ocn ||
         (avg->type != SSG_32FC1 && avg->type != SSG_32FC3))
       return fail(SSG_E_INVALID_ARG, "avg must be f32 with the accumulator's size and channels");
     if (avg->on_device) {
       ap = (float*)avg->data;
       apb = (size_t)avg->step;
     } else {
-      apb = ((size_t)a->w * 4 * a->cn + 255) / 256 * 256;
+      apb = ((size_t)a->w * 4 * ocn + 255) / 256 * 256;
       CHK(ta.ensure(apb * a->h));
       ap = ta.as<float>();
     }
@@ -1228,10 +1379,14 @@ extern "C" int ssg_accumulation_compute(ssg_accumulation* a, ssg_image* avg, ssg
       mp = tm.as<uint8_t>();
     }
   }
-  HIPCHK(launch_acc_compute(a->acc.as<float>(), nullptr, a->w, a->h, a->cn, a->plane, ap, (int)(apb / 4), mp, (int)mpb,
-                            dscale, s));
+  if (a->bayer()) {
+    HIPCHK(launch_bayer_compute(a->bayer_state(), ap, (int)(apb / 4), mp, (int)mpb, s));
+  } else {
+    HIPCHK(launch_acc_compute(a->acc.as<float>(), nullptr, a->w, a->h, a->cn, a->plane, ap, (int)(apb / 4), mp,
+                              (int)mpb, dscale, s));
+  }
   if (avg && !avg->on_device)
-    HIPCHK(hipMemcpy2DAsync(avg->data, (size_t)avg->step, ap, apb, (size_t)a->w * 4 * a->cn, a->h, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpy2DAsync(avg->data, (size_t)avg->step, ap, apb, (size_t)a->w * 4 * ocn, a->h, hipMemcpyDeviceToHost, s));
   if (mask && !mask->on_device)
     HIPCHK(hipMemcpy2DAsync(mask->data, (size_t)mask->step, mp, mpb, a->w, a->h, hipMemcpyDeviceToHost, s));
   HIPCHK(hipStreamSynchronize(s));
@@ -1241,6 +1396,23 @@ extern "C" int ssg_accumulation_compute(ssg_accumulation* a, ssg_image* avg, ssg
 extern "C" int ssg_accumulation_get_acc_counters(ssg_accumulation* a, ssg_image* accw) {
   if (!a) return fail(SSG_E_INVALID_ARG, "null handle");
   CHK(check_image(accw, "accw"));
+  if (a->bayer()) {
+    if (accw->type != SSG_32FC3 || accw->width != a->w || accw->height != a->h)
+      return fail(SSG_E_INVALID_ARG, "bayer counters are SSG_32FC3 of the accumulator size");
+    DevBuf t;
+    float* dp = (float*)accw->data;
+    size_t pb = (size_t)accw->step;
+    if (!accw->on_device) {
+      pb = (size_t)a->w * 12;
+      CHK(t.ensure(pb * a->h));
+      dp = t.as<float>();
+    }
+    HIPCHK(launch_bayer_counters(a->bayer_state(), dp, (int)(pb / 4), a->stream));
+    if (!accw->on_device)
+      HIPCHK(hipMemcpy2DAsync(accw->data, (size_t)accw->step, dp, pb, pb, a->h, hipMemcpyDeviceToHost, a->stream));
+    HIPCHK(hipStreamSynchronize(a->stream));
+    return SSG_OK;
+  }
   if (accw->type != SSG_32FC1 || accw->width != a->w || accw->height != a->h) return fail(SSG_E_INVALID_ARG, "accw");
   // weight plane hi (the lo plane holds the rounding residue)
   const float* hi = a->acc.as<float>() + (size_t)(2 * a->cn) * a->plane;
@@ -1253,6 +1425,21 @@ extern "C" int ssg_accumulation_get_acc_counters(ssg_accumulation* a, ssg_image*
 extern "C" int ssg_accumulation_get_counts(ssg_accumulation* a, ssg_image* counts) {
   if (!a) return fail(SSG_E_INVALID_ARG, "null handle");
   CHK(check_image(counts, "counts"));
+  if (a->bayer()) {
+    // planes [c][h*w] -> interleaved SSG_32SC3 on the host side of the copy
+    if (counts->type != SSG_32SC3 || counts->width != a->w || counts->height != a->h)
+      return fail(SSG_E_INVALID_ARG, "bayer counts are SSG_32SC3 of the accumulator size");
+    std::vector<unsigned> planes((size_t)3 * a->plane);
+    HIPCHK(hipMemcpyAsync(planes.data(), a->cnt.p, sizeof(unsigned) * 3 * a->plane, hipMemcpyDeviceToHost, a->stream));
+    HIPCHK(hipStreamSynchronize(a->stream));
+    std::vector<unsigned> inter((size_t)3 * a->plane);
+    for (long long o = 0; o < a->plane; ++o)
+      for (int c = 0; c < 3; ++c) inter[(size_t)o * 3 + c] = planes[(size_t)c * a->plane + o];
+    HIPCHK(hipMemcpy2DAsync(counts->data, (size_t)counts->step, inter.data(), (size_t)a->w * 12, (size_t)a->w * 12, a->h,
+                            counts->on_device ? hipMemcpyHostToDevice : hipMemcpyHostToHost, a->stream));
+    HIPCHK(hipStreamSynchronize(a->stream));
+    return SSG_OK;
+  }
   if (counts->type != SSG_32SC1 || counts->width != a->w || counts->height != a->h) return fail(SSG_E_INVALID_ARG, "counts");
   HIPCHK(hipMemcpy2DAsync(counts->data, (size_t)counts->step, a->cnt.p, (size_t)a->w * 4, (size_t)a->w * 4, a->h,
                           counts->on_device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, a->stream));

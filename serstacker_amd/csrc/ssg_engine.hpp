@@ -136,6 +136,26 @@ struct AccumArgs {
   int nchunks;
 };
 
+// c_bayer_average state: planes [c][h*w] for c = B, G, R (the reference's BGR order)
+struct BayerState {
+  float* acc;       // 3 planes: sum of value * weight (float, the reference's Mat3f)
+  float* cntr;      // 3 planes: sum of weights
+  unsigned* cnt;    // 3 planes: taps with a non-zero weight
+  long long plane;  // elements per plane
+  int w, h;
+  unsigned pattern; // 2-bit channel of each 2x2 cell position ((y&1)<<1 | (x&1))
+};
+
+struct BayerBatchArgs {
+  const void* const* frames;  // n device pointers, f32 or u16 raw
+  int src_pitch;              // elements per row
+  float scale;                // u16 -> f32 scale (2^-bpp)
+  const MapCoef* maps;
+  const int* valid;
+  int nframes;
+  BayerState st;
+};
+
 // ---- launchers --------------------------------------------------------------------------------
 // pyramids (ssg_pyr.hip)
 struct SepKernel {
@@ -189,6 +209,16 @@ hipError_t launch_remap_mask(int motion, const uint8_t* src_mask, int src_pitch,
                              hipStream_t s, int interp);
 hipError_t set_cubic_itab(const short* tab, hipStream_t s);
 hipError_t launch_create_map(int motion, const MapCoef& mc, float* map, int map_pitch, int w, int h, hipStream_t s);
+
+// bayer average (ssg_bayer.hip)
+unsigned bayer_pattern_bits(int colorid);  // 0 for a colorid that is not a Bayer pattern
+hipError_t launch_bayer_add(const BayerState& st, const void* src, int src_pitch, bool u16, float scale,
+                            const float* rmap, int rmap_pitch, const uint8_t* mask, int mask_pitch,
+                            const float* wts, int wts_pitch, hipStream_t s);
+hipError_t launch_bayer_batch(int motion, bool u16, const BayerBatchArgs& a, hipStream_t s);
+hipError_t launch_bayer_compute(const BayerState& st, float* avg, int avg_pitch, uint8_t* mask, int mask_pitch,
+                                hipStream_t s);
+hipError_t launch_bayer_counters(const BayerState& st, float* out, int out_pitch, hipStream_t s);
 
 // synthetic frames (ssg_synth.hip)
 struct SynthFrame {

@@ -481,3 +481,99 @@ def test_base_remap_mask_composition(oracle64):
     m = oracle64.remap_u8(smask, rmap, border=CONSTANT, cval=0)
     m = np.where(m >= 255, 255, 0).astype(np.uint8)
     np.testing.assert_array_equal(dmask, ndimage.grey_erosion(m, size=(5, 5), mode="constant", cval=255))
+
+
+# ---- c_bayer_average ------------------------------------------------------------------------------
+B, G, R = 0, 1, 2
+# the 2x2 cell layouts of generate_bayer_pattern_mask (c_frame_accumulation.cc:1267-1334)
+BAYER_CELLS = {8: [[R, G], [G, B]], 9: [[G, R], [B, G]], 10: [[G, B], [R, G]], 11: [[B, G], [G, R]]}
+
+
+@pytest.mark.parametrize("colorid", [8, 9, 10, 11])
+def test_bayer_pattern_tables(oracle32, colorid):
+    for y in range(4):
+        for x in range(6):
+            assert oracle32.bayer_channel(colorid, x, y) == BAYER_CELLS[colorid][y & 1][x & 1]
+
+
+def _bayer_brute(frames, colorid, rmaps=None, masks=None, weights=None):
+    """float64 restatement of _bayer_accumulate (c_frame_accumulation.cc:988-1124), one add per frame."""
+    h, w = frames[0].shape
+    acc = np.zeros((h, w, 3)); cntr = np.zeros((h, w, 3)); cnt = np.zeros((h, w, 3), np.int64)
+    cell = np.array(BAYER_CELLS[colorid])
+    for k, f in enumerate(frames):
+        for y in range(h):
+            for x in range(w):
+                if masks is not None and not masks[k][y, x]:
+                    continue
+                wt = 1.0 if weights is None else float(weights[k][y, x])
+                if rmaps is None:
+                    c = cell[y & 1, x & 1]
+                    acc[y, x, c] += f[y, x] * wt
+                    cntr[y, x, c] += wt
+                    cnt[y, x, c] += wt != 0
+                    continue
+                px, py = rmaps[k][y, x]
+                sx, sy = int(px), int(py)  # truncation toward zero
+                if not (0 <= sx < w - 1 and 0 <= sy < h - 1):
+                    continue
+                ax, ay, bx, by = sx + 1 - px, sy + 1 - py, px - sx, py - sy
+                for (i, j, s) in ((0, 0, ax * ay), (1, 0, bx * ay), (0, 1, ax * by), (1, 1, bx * by)):
+                    c = cell[(sy + j) & 1, (sx + i) & 1]
+                    acc[y, x, c] += f[sy + j, sx + i] * s * wt
+                    cntr[y, x, c] += s * wt
+                    cnt[y, x, c] += s * wt != 0
+    return acc, cntr, cnt
+
+
+@pytest.mark.parametrize("colorid", [8, 11])
+@pytest.mark.parametrize("wmode", ["none", "mask", "weights"])
+def test_bayer_accumulate_without_remap(oracle64, colorid, wmode):
+    r = rng(70)
+    frames = [r.uniform(0, 1, (10, 14)) for _ in range(4)]
+    masks = [(r.uniform(size=(10, 14)) > 0.3).astype(np.uint8) * 255 for _ in range(4)] if wmode == "mask" else None
+    wts = [r.uniform(0, 2, (10, 14)) * (r.uniform(size=(10, 14)) > 0.2) for _ in range(4)] if wmode == "weights" else None
+    st = oracle64.bayer_state(14, 10)
+    for k, f in enumerate(frames):
+        oracle64.bayer_accumulate(f, st, colorid, mask=None if masks is None else masks[k],
+                                  weights=None if wts is None else wts[k])
+    acc, cntr, cnt = _bayer_brute(frames, colorid, masks=masks, weights=wts)
+    np.testing.assert_allclose(st[0], acc, atol=1e-12)
+    np.testing.assert_allclose(st[1], cntr, atol=1e-12)
+    np.testing.assert_array_equal(st[2], cnt)
+
+
+@pytest.mark.parametrize("colorid", [8, 9, 10, 11])
+def test_bayer_accumulate_remap_brute_force(oracle64, colorid):
+    r = rng(71)
+    frames = [r.uniform(0, 1, (12, 16)) for _ in range(3)]
+    yy, xx = np.mgrid[0:12, 0:16].astype(np.float64)
+    rmaps = [np.stack([xx + r.uniform(-2.5, 2.5), yy + r.uniform(-2.5, 2.5)], -1) for _ in range(3)]
+    rmaps[1][..., 0] += 0.01 * yy  # a sheared one
+    st = oracle64.bayer_state(16, 12)
+    for f, m in zip(frames, rmaps):
+        oracle64.bayer_accumulate(f, st, colorid, rmap=m)
+    acc, cntr, cnt = _bayer_brute(frames, colorid, rmaps=rmaps)
+    np.testing.assert_allclose(st[0], acc, atol=1e-12)
+    np.testing.assert_allclose(st[1], cntr, atol=1e-12)
+    np.testing.assert_array_equal(st[2], cnt)
+    # the four side weights are a partition of unity wherever a frame landed
+    landed = cnt.sum(-1) > 0
+    assert np.all(np.abs(st[1].sum(-1)[landed] - np.round(st[1].sum(-1)[landed])) < 1e-12)
+
+
+def test_bayer_even_shift_keeps_the_phase(oracle32):
+    """An even integer shift reads every destination pixel from a source pixel of
+    the same Bayer colour: one channel per pixel, exact copies, integer counts."""
+    r = rng(72)
+    f = r.uniform(0, 1, (12, 16)).astype(np.float32)
+    yy, xx = np.mgrid[0:12, 0:16].astype(np.float32)
+    st = oracle32.bayer_state(16, 12)
+    oracle32.bayer_accumulate(f, st, 8, rmap=np.stack([xx + 2, yy - 2], -1))
+    avg, mask = oracle32.bayer_compute(st)
+    for y in range(2, 12):
+        for x in range(0, 13):
+            c = BAYER_CELLS[8][y & 1][x & 1]
+            assert st[2][y, x].tolist() == [int(c == k) for k in range(3)]
+            assert avg[y, x, c] == f[y - 2, x + 2]
+    assert mask[2:, :13].all() and not mask[:2].any()
