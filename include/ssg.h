@@ -263,12 +263,29 @@ int ssg_accumulation_clear(ssg_accumulation *a);
  * SSG_32FC1 device images of one size; weights: n floats (host). */
 int ssg_frame_sharpness(int n, const ssg_image *frames, float *weights);
 
+/* ---- FFT phase correlation (no counterpart in the reference, which has none on this path;
+ * its nearest code is core/proc/fft.cc:17-160).  Optional 2x2 binning (the luminance of raw
+ * Bayer frames), mean removal, separable Hann apodisation, zero padding to powers of two, normalised
+ * cross-power spectrum weighted so the correlation peak is a gaussian of sigma 1 px, integer peak
+ * and 3-point gaussian refinement per axis (FP64).
+ * Estimates T with ref(x) = frame(x + T): the parameters of a translation transform. ------ */
+typedef struct ssg_phasecorr ssg_phasecorr;
+/* width/height: frame size; binning: 1, or 2 (2x2 cell means).  The padded binned size must be
+ * at most 4096 per axis. */
+int ssg_phasecorr_create(int32_t width, int32_t height, int32_t binning, ssg_phasecorr **out);
+void ssg_phasecorr_destroy(ssg_phasecorr *pc);
+/* Reference frame: SSG_32FC1 or SSG_16UC1, host or device. */
+int ssg_phasecorr_set_reference(ssg_phasecorr *pc, const ssg_image *ref);
+/* n device frames (SSG_32FC1 or SSG_16UC1, one size and step): out[3i] = Tx, out[3i+1] = Ty,
+ * out[3i+2] = correlation peak (1 for an exact circular shift). */
+int ssg_phasecorr_estimate(ssg_phasecorr *pc, int n, const ssg_image *frames, double *out);
+
 /* ---- kernel timing (no reference counterpart; the reference's INSTRUMENT_REGION timers,
  * core/debug.h:183-188, are its closest analogue).  When enabled, every launch of a hot
  * kernel is bracketed by HIP events on the handle's stream; algorithmic bytes follow
  * the per-pixel minimum traffic of each kernel. ------------------------------------------ */
 enum { SSG_PROF_ECC_TRANSLATION = 0, SSG_PROF_ECC_MAIN = 1, SSG_PROF_CORRELATION = 2, SSG_PROF_PYRAMID = 3,
-       SSG_PROF_ACCUMULATE = 4, SSG_PROF_ECC_SOLVE = 5, SSG_PROF_COUNT = 8 };
+       SSG_PROF_ACCUMULATE = 4, SSG_PROF_ECC_SOLVE = 5, SSG_PROF_FFT = 6, SSG_PROF_COUNT = 8 };
 typedef struct ssg_profile {
   int64_t launches[SSG_PROF_COUNT];
   double ms[SSG_PROF_COUNT];     /* summed kernel durations */
@@ -278,6 +295,8 @@ int ssg_registration_set_profiling(ssg_registration *r, int enable);
 int ssg_registration_get_profile(ssg_registration *r, ssg_profile *out, int reset);
 int ssg_accumulation_set_profiling(ssg_accumulation *a, int enable);
 int ssg_accumulation_get_profile(ssg_accumulation *a, ssg_profile *out, int reset);
+int ssg_phasecorr_set_profiling(ssg_phasecorr *pc, int enable);
+int ssg_phasecorr_get_profile(ssg_phasecorr *pc, ssg_profile *out, int reset);
 
 /* ---- synthetic planetary-disk frames (benchmark and test inputs; no reference counterpart) -- */
 typedef struct ssg_synth_params {

@@ -292,6 +292,46 @@ class c_bayer_average:
         _lib.check(self._lib.ssg_accumulation_clear(self._h), "clear")
 
 
+class phase_correlation:
+    """FFT phase-correlation translation estimate (the pre-alignment of the turbulence
+    and Bayer configurations; the reference has none).  estimate() returns, per frame,
+    (Tx, Ty, peak) with ref(x) = frame(x + T)."""
+
+    def __init__(self, width: int, height: int, binning: int = 1):
+        self._lib = _lib.load()
+        h = C.c_void_p()
+        _lib.check(self._lib.ssg_phasecorr_create(width, height, binning, C.byref(h)), "ssg_phasecorr_create")
+        self._h = h
+        self.w, self.h, self.binning = width, height, binning
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            self._lib.ssg_phasecorr_destroy(h)
+            self._h = None
+
+    def set_reference(self, ref):
+        v = image_view(ref)
+        _lib.check(self._lib.ssg_phasecorr_set_reference(self._h, C.byref(v)), "set_reference")
+
+    def estimate(self, frames) -> np.ndarray:
+        views = image_views(frames)
+        out = np.zeros((len(views), 3), np.float64)
+        _lib.check(self._lib.ssg_phasecorr_estimate(self._h, len(views), views,
+                                                    out.ctypes.data_as(C.POINTER(C.c_double))), "estimate")
+        return out
+
+    def statuses(self, shifts) -> C.Array:
+        """Translation registration statuses (ok, params = T) for the accumulators."""
+        st = (_lib.RegistrationStatus * len(shifts))()
+        for i, (tx, ty, _) in enumerate(shifts):
+            st[i].ok = 1
+            st[i].nparams = 2
+            st[i].params[0], st[i].params[1] = float(tx), float(ty)
+            st[i].matrix[:] = [1.0, 0.0, float(np.float32(tx)), 0.0, 1.0, float(np.float32(ty))]
+        return st
+
+
 def frame_sharpness(frames) -> np.ndarray:
     """Per-frame weight (mean squared 3x3 laplacian) of device frames."""
     lib = _lib.load()

@@ -109,6 +109,7 @@ class RegistrationStatus(C.Structure):
 
 
 PROF_ECC_TRANSLATION, PROF_ECC_MAIN, PROF_CORRELATION, PROF_PYRAMID, PROF_ACCUMULATE = 0, 1, 2, 3, 4
+PROF_ECC_SOLVE, PROF_FFT = 5, 6
 
 
 class Profile(C.Structure):
@@ -161,6 +162,12 @@ _SIGS = [
     ("ssg_accumulation_get_counts", C.c_int, [C.c_void_p, _P(Image)]),
     ("ssg_accumulation_accumulated_frames", C.c_int, [C.c_void_p, _P(C.c_int32)]),
     ("ssg_accumulation_clear", C.c_int, [C.c_void_p]),
+    ("ssg_phasecorr_create", C.c_int, [C.c_int32, C.c_int32, C.c_int32, _P(C.c_void_p)]),
+    ("ssg_phasecorr_destroy", None, [C.c_void_p]),
+    ("ssg_phasecorr_set_reference", C.c_int, [C.c_void_p, _P(Image)]),
+    ("ssg_phasecorr_estimate", C.c_int, [C.c_void_p, C.c_int, _P(Image), _P(C.c_double)]),
+    ("ssg_phasecorr_set_profiling", C.c_int, [C.c_void_p, C.c_int]),
+    ("ssg_phasecorr_get_profile", C.c_int, [C.c_void_p, _P(Profile), C.c_int]),
     ("ssg_accumulation_set_bayer_pattern", C.c_int, [C.c_void_p, C.c_int32]),
     ("ssg_accumulation_set_remap", C.c_int, [C.c_void_p, _P(Image)]),
     ("ssg_accumulation_set_input_bpp", C.c_int, [C.c_void_p, C.c_int32]),

@@ -13,6 +13,9 @@
 #include <string>
 #include <vector>
 
+#include <cmath>
+#include <memory>
+
 #include "../../include/ssg.h"
 #include "ssg_engine.hpp"
 
@@ -1476,6 +1479,171 @@ extern "C" int ssg_accumulation_get_profile(ssg_accumulation* a, ssg_profile* ou
   if (!a || !out) return fail(SSG_E_INVALID_ARG, "null argument");
   *out = a->prof.acc;
   if (reset) a->prof.acc = ssg_profile{};
+  return SSG_OK;
+}
+
+// ---- FFT phase correlation ------------------------------------------------------------------------
+struct ssg_phasecorr {
+  int w = 0, h = 0, binning = 1, bw = 0, bh = 0, M = 0, N = 0;
+  hipStream_t stream = nullptr;
+  DevBuf wx, wy, tw_m, tw_n, ref_spec, buf, tmp, frame_ptrs, out, mean;
+  int chunk = 1;
+  bool have_ref = false;
+  Profiler prof;
+  ~ssg_phasecorr() {
+    if (stream) {
+      (void)hipStreamSynchronize(stream);
+      (void)hipStreamDestroy(stream);
+    }
+  }
+};
+
+namespace {
+int pow2_at_least(int n) {
+  int p = 1;
+  while (p < n) p <<= 1;
+  return p;
+}
+
+// frames -> spectra in pc->tmp ([M][N] per frame)
+int pc_spectra(ssg_phasecorr* pc, int n, const void* const* dev_ptrs, bool u16, int pitch) {
+  hipStream_t s = pc->stream;
+  CHK(pc->mean.ensure(sizeof(float) * n));
+  HIPCHK(launch_pc_load(dev_ptrs, u16, pitch, pc->bw, pc->bh, pc->binning, pc->mean.as<float>(), pc->wx.as<float>(),
+                        pc->wy.as<float>(), pc->buf.as<float2>(), pc->M, pc->N, n, s));
+  HIPCHK(launch_fft2(pc->buf.as<float2>(), pc->tmp.as<float2>(), pc->M, pc->N, n, pc->tw_m.as<float2>(),
+                     pc->tw_n.as<float2>(), false, s));
+  return SSG_OK;
+}
+}  // namespace
+
+extern "C" int ssg_phasecorr_create(int32_t width, int32_t height, int32_t binning, ssg_phasecorr** out) {
+  if (!out) return fail(SSG_E_INVALID_ARG, "out is null");
+  *out = nullptr;
+  if (binning != 1 && binning != 2) return fail(SSG_E_INVALID_ARG, "binning must be 1 or 2");
+  if (width < 4 * binning || height < 4 * binning) return fail(SSG_E_INVALID_ARG, "frame %dx%d too small", width, height);
+  auto pc = std::make_unique<ssg_phasecorr>();
+  pc->w = width;
+  pc->h = height;
+  pc->binning = binning;
+  pc->bw = width / binning;
+  pc->bh = height / binning;
+  pc->M = pow2_at_least(pc->bw);
+  pc->N = pow2_at_least(pc->bh);
+  if (pc->M > 4096 || pc->N > 4096)
+    return fail(SSG_E_UNSUPPORTED, "padded FFT size %dx%d above 4096", pc->M, pc->N);
+  HIPCHK(hipStreamCreateWithFlags(&pc->stream, hipStreamNonBlocking));
+  std::vector<float> wx(pc->bw), wy(pc->bh);
+  auto hann = [](int n, int i) { return n == 1 ? 1.0 : 0.5 - 0.5 * std::cos(2.0 * M_PI * i / (n - 1)); };
+  for (int i = 0; i < pc->bw; ++i) wx[i] = (float)hann(pc->bw, i);
+  for (int i = 0; i < pc->bh; ++i) wy[i] = (float)hann(pc->bh, i);
+  auto twiddles = [](int n) {
+    std::vector<float2> t(std::max(1, n / 2));
+    for (int k = 0; k < n / 2; ++k) t[k] = make_float2((float)std::cos(-2.0 * M_PI * k / n), (float)std::sin(-2.0 * M_PI * k / n));
+    return t;
+  };
+  const auto tm = twiddles(pc->M), tn = twiddles(pc->N);
+  CHK(pc->wx.ensure(sizeof(float) * wx.size()));
+  CHK(pc->wy.ensure(sizeof(float) * wy.size()));
+  CHK(pc->tw_m.ensure(sizeof(float2) * tm.size()));
+  CHK(pc->tw_n.ensure(sizeof(float2) * tn.size()));
+  HIPCHK(hipMemcpyAsync(pc->wx.p, wx.data(), sizeof(float) * wx.size(), hipMemcpyHostToDevice, pc->stream));
+  HIPCHK(hipMemcpyAsync(pc->wy.p, wy.data(), sizeof(float) * wy.size(), hipMemcpyHostToDevice, pc->stream));
+  HIPCHK(hipMemcpyAsync(pc->tw_m.p, tm.data(), sizeof(float2) * tm.size(), hipMemcpyHostToDevice, pc->stream));
+  HIPCHK(hipMemcpyAsync(pc->tw_n.p, tn.data(), sizeof(float2) * tn.size(), hipMemcpyHostToDevice, pc->stream));
+  // frames per launch: two complex planes per frame, at most about 2 GiB in flight
+  const size_t per = (size_t)pc->M * pc->N * sizeof(float2);
+  pc->chunk = (int)std::max<size_t>(1, std::min<size_t>(64, ((size_t)2 << 30) / (2 * per)));
+  CHK(pc->ref_spec.ensure(per));
+  HIPCHK(hipStreamSynchronize(pc->stream));
+  *out = pc.release();
+  return SSG_OK;
+}
+
+extern "C" void ssg_phasecorr_destroy(ssg_phasecorr* pc) { delete pc; }
+
+extern "C" int ssg_phasecorr_set_reference(ssg_phasecorr* pc, const ssg_image* ref) {
+  if (!pc) return fail(SSG_E_INVALID_ARG, "null handle");
+  CHK(check_image(ref, "reference"));
+  if (ref->width != pc->w || ref->height != pc->h) return fail(SSG_E_INVALID_ARG, "reference size");
+  if (ref->type != SSG_32FC1 && ref->type != SSG_16UC1)
+    return fail(SSG_E_INVALID_ARG, "phase correlation takes SSG_32FC1 or SSG_16UC1 frames");
+  hipStream_t s = pc->stream;
+  DevBuf t;
+  const void* p;
+  size_t pb;
+  CHK(device_view(ref, t, s, &p, &pb));
+  const size_t per = (size_t)pc->M * pc->N * sizeof(float2);
+  CHK(pc->buf.ensure(per));
+  CHK(pc->tmp.ensure(per));
+  CHK(pc->frame_ptrs.ensure(sizeof(void*)));
+  HIPCHK(hipMemcpyAsync(pc->frame_ptrs.p, &p, sizeof(void*), hipMemcpyHostToDevice, s));
+  const bool u16 = ref->type == SSG_16UC1;
+  CHK(pc_spectra(pc, 1, pc->frame_ptrs.as<const void*>(), u16, (int)(pb / (u16 ? 2 : 4))));
+  HIPCHK(hipMemcpyAsync(pc->ref_spec.p, pc->tmp.p, per, hipMemcpyDeviceToDevice, s));
+  HIPCHK(hipStreamSynchronize(s));
+  pc->have_ref = true;
+  return SSG_OK;
+}
+
+extern "C" int ssg_phasecorr_estimate(ssg_phasecorr* pc, int n, const ssg_image* frames, double* out) {
+  if (!pc) return fail(SSG_E_INVALID_ARG, "null handle");
+  if (n <= 0) return SSG_OK;
+  if (!frames || !out) return fail(SSG_E_INVALID_ARG, "null frames or out");
+  if (!pc->have_ref) return fail(SSG_E_NOT_READY, "no reference frame set");
+  const int type = frames[0].type;
+  if (type != SSG_32FC1 && type != SSG_16UC1)
+    return fail(SSG_E_INVALID_ARG, "phase correlation takes SSG_32FC1 or SSG_16UC1 frames");
+  const int es = type == SSG_16UC1 ? 2 : 4;
+  for (int f = 0; f < n; ++f) {
+    CHK(check_image(&frames[f], "frame"));
+    if (!frames[f].on_device) return fail(SSG_E_INVALID_ARG, "frame %d: batched frames must be device images", f);
+    if (frames[f].width != pc->w || frames[f].height != pc->h || frames[f].type != type ||
+        frames[f].step != frames[0].step)
+      return fail(SSG_E_INVALID_ARG, "frame %d: frames share the configured size, type and step", f);
+  }
+  if (frames[0].step % es) return fail(SSG_E_INVALID_ARG, "frame step must be a multiple of the element size");
+  hipStream_t s = pc->stream;
+  const int chunk = std::min(pc->chunk, n);
+  const long long per = (long long)pc->M * pc->N;
+  CHK(pc->buf.ensure(sizeof(float2) * per * chunk));
+  CHK(pc->tmp.ensure(sizeof(float2) * per * chunk));
+  CHK(pc->frame_ptrs.ensure(sizeof(void*) * n));
+  CHK(pc->out.ensure(sizeof(double) * 3 * n));
+  std::vector<const void*> ptrs(n);
+  for (int f = 0; f < n; ++f) ptrs[f] = frames[f].data;
+  HIPCHK(hipMemcpyAsync(pc->frame_ptrs.p, ptrs.data(), sizeof(void*) * n, hipMemcpyHostToDevice, s));
+  const int pitch = (int)(frames[0].step / es);
+  pc->prof.start(s);
+  for (int f0 = 0; f0 < n; f0 += chunk) {
+    const int m = std::min(chunk, n - f0);
+    CHK(pc_spectra(pc, m, pc->frame_ptrs.as<const void*>() + f0, type == SSG_16UC1, pitch));
+    HIPCHK(launch_pc_cross(pc->tmp.as<float2>(), pc->ref_spec.as<float2>(), pc->M, pc->N, m, s));
+    HIPCHK(launch_fft2(pc->tmp.as<float2>(), pc->buf.as<float2>(), pc->M, pc->N, m, pc->tw_m.as<float2>(),
+                       pc->tw_n.as<float2>(), true, s));
+    HIPCHK(launch_pc_peak(pc->buf.as<float2>(), pc->M, pc->N, pc->binning, m, pc->out.as<double>() + 3 * f0, s));
+  }
+  pc->prof.stop(s, SSG_PROF_FFT);
+  HIPCHK(hipMemcpyAsync(out, pc->out.p, sizeof(double) * 3 * n, hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  // algorithmic bytes: each frame read once (es B per pixel), plus the passes over the
+  // complex planes: load write, 2 x (row FFT r+w, transpose r+w, row FFT r+w), cross
+  // r+w + reference read, peak read -- 17 plane sweeps of 8 B per padded element
+  pc->prof.add_bytes(SSG_PROF_FFT, (double)n * ((double)es * pc->w * pc->h + 17.0 * 8.0 * per));
+  pc->prof.collect();
+  return SSG_OK;
+}
+
+extern "C" int ssg_phasecorr_set_profiling(ssg_phasecorr* pc, int enable) {
+  if (!pc) return fail(SSG_E_INVALID_ARG, "null handle");
+  pc->prof.on = enable != 0;
+  return SSG_OK;
+}
+
+extern "C" int ssg_phasecorr_get_profile(ssg_phasecorr* pc, ssg_profile* out, int reset) {
+  if (!pc || !out) return fail(SSG_E_INVALID_ARG, "null argument");
+  *out = pc->prof.acc;
+  if (reset) pc->prof.acc = ssg_profile{};
   return SSG_OK;
 }
 

@@ -220,6 +220,14 @@ hipError_t launch_bayer_compute(const BayerState& st, float* avg, int avg_pitch,
                                 hipStream_t s);
 hipError_t launch_bayer_counters(const BayerState& st, float* out, int out_pitch, hipStream_t s);
 
+// FFT phase correlation (ssg_fft.hip)
+hipError_t launch_fft2(float2* buf, float2* tmp, int M, int N, int nframes, const float2* tw_m, const float2* tw_n,
+                       bool inverse, hipStream_t s);
+hipError_t launch_pc_load(const void* const* frames, bool u16, int pitch, int bw, int bh, int binning, float* mean,
+                          const float* wx, const float* wy, float2* out, int M, int N, int nframes, hipStream_t s);
+hipError_t launch_pc_cross(float2* spec, const float2* ref, int M, int N, int nframes, hipStream_t s);
+hipError_t launch_pc_peak(const float2* r, int M, int N, int binning, int nframes, double* out, hipStream_t s);
+
 // synthetic frames (ssg_synth.hip)
 struct SynthFrame {
   double A[6];  // scene coordinate = A * (x, y, 1): maps frame pixels to the scene

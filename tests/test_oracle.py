@@ -577,3 +577,36 @@ def test_bayer_even_shift_keeps_the_phase(oracle32):
             assert st[2][y, x].tolist() == [int(c == k) for k in range(3)]
             assert avg[y, x, c] == f[y - 2, x + 2]
     assert mask[2:, :13].all() and not mask[:2].any()
+
+
+# ---- FFT phase correlation (oracle/phasecorr.py; the reference has none) --------------------------
+def test_phasecorr_circular_shift_is_exact():
+    from oracle import phasecorr as pc
+    r = rng(90)
+    ref = r.normal(size=(64, 128))
+    for dx, dy in ((5, -3), (-17, 9), (0, 0), (63, -31)):
+        frame = np.roll(np.roll(ref, dy, axis=0), dx, axis=1)  # frame(x) = ref(x - d)
+        tx, ty, peak = pc.estimate(pc.spectrum(ref, window=False), frame, window=False)
+        # an exact circular shift gives the weighted delta: a symmetric gaussian peak at the
+        # shift (no sub-pixel offset) of height mean(weight)
+        assert abs(tx - dx) < 1e-9 and abs(ty - dy) < 1e-9
+        assert abs(peak - pc.peak_weight(64, 128).mean()) < 1e-12
+
+
+@pytest.mark.parametrize("binning,tol", [(1, 0.1), (2, 0.3)])
+def test_phasecorr_recovers_known_subpixel_shift(binning, tol):
+    """ref(x) = frame(x + T) for a textured scene (600 blobs, sigma 1.2-3 px, 1% noise)
+    rendered under a known shift; the tolerance is the parabolic-fit bias (binned
+    pixels are 2 px)."""
+    from oracle import phasecorr as pc
+    w, h = 256, 192
+    render = synth_np.blob_scene(w, h, seed=91, n=600, smin=1.2, smax=3.0)
+    r = rng(92)
+    ref = render([1, 0, 0, 0, 1, 0]) + 0.01 * r.normal(size=(h, w))
+    spec = pc.spectrum(ref, binning)
+    for T in ((3.3, -1.7), (-7.6, 4.25), (0.4, 0.0)):
+        frame = render([1, 0, -T[0], 0, 1, -T[1]]) + 0.01 * r.normal(size=(h, w))  # frame(x) = scene(x - T)
+        tx, ty, peak = pc.estimate(spec, frame, binning)
+        # frame(x + T) = scene(x) = ref(x): the estimate is T
+        assert abs(tx - T[0]) < tol and abs(ty - T[1]) < tol, (T, tx, ty)
+        assert 0.1 < peak <= 1.0
