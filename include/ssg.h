@@ -201,9 +201,12 @@ int ssg_remap(const ssg_image *src, const ssg_image *src_mask, const ssg_image *
 typedef struct ssg_accumulation ssg_accumulation;
 
 /* create_frame_accumulation (c_image_stacking_pipeline.cc:450-466): method is
- * SSG_ACC_WEIGHTED_AVERAGE or SSG_ACC_AVERAGE (both c_weigthed_average), or
+ * SSG_ACC_WEIGHTED_AVERAGE or SSG_ACC_AVERAGE (both c_weigthed_average),
  * SSG_ACC_BAYER_AVERAGE (c_bayer_average, c_frame_accumulation.h:222-262; channels = 1,
- * the raw input; the accumulator has the reference's 3 channels, BGR order). */
+ * the raw input; the accumulator has the reference's 3 channels, BGR order), or
+ * SSG_ACC_FOCUS_STACK (c_laplacian_pyramid_focus_stacking, c_frame_accumulation.h:141-184;
+ * frames SSG_32FC1/SSG_32FC3 added without masks; compute() writes no counters and a mask
+ * of 255, get_acc_counters() writes nothing, as the reference releases accw). */
 int ssg_accumulation_create(int32_t method, int32_t width, int32_t height, int32_t channels,
                             ssg_accumulation **out);
 void ssg_accumulation_destroy(ssg_accumulation *a);
@@ -217,6 +220,33 @@ int ssg_accumulation_set_remap(ssg_accumulation *a, const ssg_image *map);
 /* SSG_16UC1 raw frames are read as value / 2^bpp, the pipeline's conversion
  * (c_image_stacking_pipeline_base.cc:269-273); default 16. */
 int ssg_accumulation_set_input_bpp(ssg_accumulation *a, int32_t bpp);
+
+/* c_laplacian_pyramid_focus_stacking::options (c_frame_accumulation.h:155-161) and the weight
+ * source of the focus configuration.  SSG_FOCUS_SELECT_MAX_ENERGY and SSG_FOCUS_WEIGHTED_AVERAGE
+ * are the reference's fusing policies (c_frame_accumulation.cc:491-681).  SSG_FOCUS_LPG_WEIGHTS
+ * (no reference counterpart) weights every laplacian level by W = gaussian(lpg(k, p), lpg_sigma)
+ * reduced to that level with pyrDown, on every level including the top one, and divides by the
+ * summed weights. */
+enum { SSG_FOCUS_SELECT_MAX_ENERGY = 0, SSG_FOCUS_WEIGHTED_AVERAGE = 1, SSG_FOCUS_LPG_WEIGHTS = 2 };
+typedef struct ssg_focus_stacking_options {
+  int32_t fusing_policy;      /* SSG_FOCUS_*; reference default select_max_energy */
+  int32_t inpaint_mask_holes; /* reference default 1; masks are not taken on this path */
+  int32_t avgchannel;         /* reference default 1 */
+  int32_t kradius;            /* reference default 0 */
+  double ksigma;              /* reference default 0 */
+  double lpg_k, lpg_p, lpg_sigma; /* SSG_FOCUS_LPG_WEIGHTS: 6, 2, 5 */
+  int32_t max_levels;         /* 0: levels while the next level's smaller side is > 8
+                                 (laplacian_pyramid.cc:62-69); else at most this many levels */
+} ssg_focus_stacking_options;
+void ssg_default_focus_stacking_options(ssg_focus_stacking_options *o);
+/* Options of a SSG_ACC_FOCUS_STACK accumulator, before its first add(). */
+int ssg_accumulation_set_focus_options(ssg_accumulation *a, const ssg_focus_stacking_options *o);
+
+/* lpg() (core/proc/lpg.cc:277-344) with dscale = uscale = 0: the sharpness map
+ * alpha*lap5x5^2 + beta*grad5x5^2 + 1e-9 of the channel average of src, raised to p
+ * (alpha = 625 k/(k+1), beta = (100/36)^2/(k+1)).  src: SSG_32FC1/SSG_32FC3; dst: SSG_32FC1 of the
+ * same size (host or device); sharpness (optional): the map's mean. */
+int ssg_lpg(const ssg_image *src, ssg_image *dst, double k, double p, double *sharpness);
 
 /* c_frame_accumulation::add(src, weights) (c_frame_accumulation.cc:182-221, bayer :1169-1196):
  * weights may be NULL, a SSG_8UC1 mask (nonzero = 1) or a SSG_32FC1 weight map.  Bayer

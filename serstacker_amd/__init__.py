@@ -332,6 +332,60 @@ class phase_correlation:
         return st
 
 
+class c_laplacian_pyramid_focus_stacking:
+    """c_laplacian_pyramid_focus_stacking (core/average/c_frame_accumulation.h:141-184);
+    keyword options are the fields of ssg_focus_stacking_options."""
+
+    def __init__(self, width: int, height: int, channels: int = 1, **opts):
+        self._lib = _lib.load()
+        h = C.c_void_p()
+        _lib.check(self._lib.ssg_accumulation_create(_lib.ACC_FOCUS_STACK, width, height, channels, C.byref(h)),
+                   "ssg_accumulation_create")
+        self._h = h
+        self.w, self.h, self.cn = width, height, channels
+        o = _lib.FocusStackingOptions()
+        self._lib.ssg_default_focus_stacking_options(C.byref(o))
+        for k, v in opts.items():
+            setattr(o, k, v)
+        _lib.check(self._lib.ssg_accumulation_set_focus_options(self._h, C.byref(o)), "set_focus_options")
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            self._lib.ssg_accumulation_destroy(h)
+            self._h = None
+
+    def add(self, src) -> bool:
+        sv = image_view(src)
+        rc = self._lib.ssg_accumulation_add(self._h, C.byref(sv), None)
+        if rc:
+            log.error("add: %s", self._lib.ssg_last_error().decode())
+            return False
+        return True
+
+    def compute(self):
+        avg = np.zeros((self.h, self.w) if self.cn == 1 else (self.h, self.w, self.cn), np.float32)
+        av = _lib.host_image(avg)
+        _lib.check(self._lib.ssg_accumulation_compute(self._h, C.byref(av), None, 1.0), "compute")
+        return avg
+
+    def accumulated_frames(self) -> int:
+        n = C.c_int32()
+        _lib.check(self._lib.ssg_accumulation_accumulated_frames(self._h, C.byref(n)), "accumulated_frames")
+        return n.value
+
+
+def lpg(src, k: float = 6.0, p: float = 2.0):
+    """lpg() (core/proc/lpg.cc:277-344), dscale = uscale = 0: (map, mean sharpness)."""
+    lib = _lib.load()
+    h, w = src.shape[:2]
+    out = np.zeros((h, w), np.float32)
+    sv, ov = image_view(src), _lib.host_image(out)
+    sh = C.c_double()
+    _lib.check(lib.ssg_lpg(C.byref(sv), C.byref(ov), k, p, C.byref(sh)), "ssg_lpg")
+    return out, sh.value
+
+
 def frame_sharpness(frames) -> np.ndarray:
     """Per-frame weight (mean squared 3x3 laplacian) of device frames."""
     lib = _lib.load()

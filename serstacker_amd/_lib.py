@@ -17,6 +17,7 @@ LIB_PATH = os.path.join(HERE, "libssg.so")
 SSG_OK = 0
 SSG_8UC1, SSG_16UC1, SSG_32SC1, SSG_32FC1, SSG_32FC2, SSG_32SC3, SSG_32FC3 = 0, 2, 4, 5, 13, 20, 21
 COLORID_BAYER_RGGB, COLORID_BAYER_GRBG, COLORID_BAYER_GBRG, COLORID_BAYER_BGGR = 8, 9, 10, 11
+FOCUS_SELECT_MAX_ENERGY, FOCUS_WEIGHTED_AVERAGE, FOCUS_LPG_WEIGHTS = 0, 1, 2
 MOTION_TRANSLATION, MOTION_EUCLIDEAN, MOTION_SCALED_EUCLIDEAN, MOTION_AFFINE = 0, 1, 2, 3
 ECC_ALIGN_FORWARD_ADDITIVE, ECC_ALIGN_INVERSE_COMPOSITIONAL, ECC_ALIGN_LM, ECC_ALIGN_INVERSE_COMPOSITIONAL_LM = 0, 1, 2, 3
 INTER_NEAREST, INTER_LINEAR, INTER_CUBIC, INTER_AREA = 0, 1, 2, 3
@@ -112,6 +113,12 @@ PROF_ECC_TRANSLATION, PROF_ECC_MAIN, PROF_CORRELATION, PROF_PYRAMID, PROF_ACCUMU
 PROF_ECC_SOLVE, PROF_FFT = 5, 6
 
 
+class FocusStackingOptions(C.Structure):
+    _fields_ = [("fusing_policy", C.c_int32), ("inpaint_mask_holes", C.c_int32), ("avgchannel", C.c_int32),
+                ("kradius", C.c_int32), ("ksigma", C.c_double), ("lpg_k", C.c_double), ("lpg_p", C.c_double),
+                ("lpg_sigma", C.c_double), ("max_levels", C.c_int32)]
+
+
 class Profile(C.Structure):
     _fields_ = [("launches", C.c_int64 * 8), ("ms", C.c_double * 8), ("bytes", C.c_double * 8)]
 
@@ -168,6 +175,9 @@ _SIGS = [
     ("ssg_phasecorr_estimate", C.c_int, [C.c_void_p, C.c_int, _P(Image), _P(C.c_double)]),
     ("ssg_phasecorr_set_profiling", C.c_int, [C.c_void_p, C.c_int]),
     ("ssg_phasecorr_get_profile", C.c_int, [C.c_void_p, _P(Profile), C.c_int]),
+    ("ssg_default_focus_stacking_options", None, [_P(FocusStackingOptions)]),
+    ("ssg_accumulation_set_focus_options", C.c_int, [C.c_void_p, _P(FocusStackingOptions)]),
+    ("ssg_lpg", C.c_int, [_P(Image), _P(Image), C.c_double, C.c_double, _P(C.c_double)]),
     ("ssg_accumulation_set_bayer_pattern", C.c_int, [C.c_void_p, C.c_int32]),
     ("ssg_accumulation_set_remap", C.c_int, [C.c_void_p, _P(Image)]),
     ("ssg_accumulation_set_input_bpp", C.c_int, [C.c_void_p, C.c_int32]),

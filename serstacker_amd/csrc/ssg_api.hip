@@ -1034,10 +1034,24 @@ struct ssg_accumulation {
   int bpp = 16;
   DevBuf rmap;
   int rmap_pitch = 0;  // floats per row; 0 = no remap set
+  // c_laplacian_pyramid_focus_stacking: level-major pyramids, the channel planes of a
+  // level contiguous: element offset of (level i, channel c) = cn * loff[i] + c * n_i
+  ssg_focus_stacking_options fo{};
+  std::vector<int> lw, lh;
+  std::vector<long long> loff;
+  long long P = 0;  // elements of one single-plane pyramid
+  int L = 0, wc = 1;
+  DevBuf facc, fw, fp, fup, ft0, ft1, ft2, ft3, fgray, fm, fW, fptrs, fstd;
+  std::vector<const float*> hptrs;
+  int slot = 0;
+  SepKernel G{}, Wk{};
+  bool hasG = false;
   bool bayer() const { return method == SSG_ACC_BAYER_AVERAGE; }
-  // accumulator planes: weighted average (2*cn + 2) f32 planes; bayer acc[3] + cntr[3]
-  size_t acc_bytes() const { return sizeof(float) * plane * (bayer() ? 6 : 2 * cn + 2); }
-  size_t cnt_bytes() const { return sizeof(unsigned) * plane * (bayer() ? 3 : 1); }
+  bool focus() const { return method == SSG_ACC_FOCUS_STACK; }
+  // accumulator planes: weighted average (2*cn + 2) f32 planes; bayer acc[3] + cntr[3];
+  // focus stacking keeps its own pyramids
+  size_t acc_bytes() const { return sizeof(float) * (focus() ? 1 : plane * (bayer() ? 6 : 2 * cn + 2)); }
+  size_t cnt_bytes() const { return sizeof(unsigned) * (focus() ? 1 : plane * (bayer() ? 3 : 1)); }
   BayerState bayer_state() {
     BayerState st;
     st.acc = acc.as<float>();
@@ -1055,8 +1069,11 @@ extern "C" int ssg_accumulation_create(int32_t method, int32_t width, int32_t he
                                        ssg_accumulation** out) {
   if (!out) return fail(SSG_E_INVALID_ARG, "out is null");
   *out = nullptr;
-  if (method != SSG_ACC_WEIGHTED_AVERAGE && method != SSG_ACC_AVERAGE && method != SSG_ACC_BAYER_AVERAGE)
+  if (method != SSG_ACC_WEIGHTED_AVERAGE && method != SSG_ACC_AVERAGE && method != SSG_ACC_BAYER_AVERAGE &&
+      method != SSG_ACC_FOCUS_STACK)
     return fail(SSG_E_UNSUPPORTED, "accumulation method %d", method);
+  if (method == SSG_ACC_FOCUS_STACK && (width < 5 || height < 5))
+    return fail(SSG_E_INVALID_ARG, "focus stacking needs frames of at least 5x5");
   if (width <= 0 || height <= 0 || (channels != 1 && channels != 3))
     return fail(SSG_E_INVALID_ARG, "accumulator %dx%dx%d", width, height, channels);
   if (method == SSG_ACC_BAYER_AVERAGE && channels != 1)
@@ -1067,6 +1084,7 @@ extern "C" int ssg_accumulation_create(int32_t method, int32_t width, int32_t he
   a->h = height;
   a->cn = channels;
   a->plane = (long long)width * height;
+  ssg_default_focus_stacking_options(&a->fo);
   hipError_t e = hipStreamCreateWithFlags(&a->stream, hipStreamNonBlocking);
   if (e != hipSuccess) {
     delete a;
@@ -1171,6 +1189,324 @@ int bayer_add(ssg_accumulation* a, const ssg_image* src, const ssg_image* weight
 }
 }  // namespace
 
+extern "C" void ssg_default_focus_stacking_options(ssg_focus_stacking_options* o) {
+  if (!o) return;
+  *o = ssg_focus_stacking_options{};
+  o->fusing_policy = SSG_FOCUS_SELECT_MAX_ENERGY;  // c_frame_accumulation.h:156-161
+  o->inpaint_mask_holes = 1;
+  o->avgchannel = 1;
+  o->kradius = 0;
+  o->ksigma = 0;
+  o->lpg_k = 6;
+  o->lpg_p = 2;
+  o->lpg_sigma = 5;
+  o->max_levels = 0;
+}
+
+extern "C" int ssg_accumulation_set_focus_options(ssg_accumulation* a, const ssg_focus_stacking_options* o) {
+  if (!a || !o) return fail(SSG_E_INVALID_ARG, "null argument");
+  if (!a->focus()) return fail(SSG_E_INVALID_ARG, "not a focus stacking accumulator");
+  if (a->accumulated) return fail(SSG_E_INVALID_ARG, "options are set before the first add()");
+  if (o->fusing_policy < SSG_FOCUS_SELECT_MAX_ENERGY || o->fusing_policy > SSG_FOCUS_LPG_WEIGHTS)
+    return fail(SSG_E_INVALID_ARG, "fusing_policy %d", o->fusing_policy);
+  if (o->max_levels < 0 || o->kradius < 0) return fail(SSG_E_INVALID_ARG, "max_levels / kradius");
+  if (o->fusing_policy == SSG_FOCUS_LPG_WEIGHTS && !(o->lpg_sigma > 0))
+    return fail(SSG_E_INVALID_ARG, "lpg_sigma must be > 0");
+  a->fo = *o;
+  return SSG_OK;
+}
+
+namespace {
+// cv::getGaussianKernel (fixed tables for sigma <= 0 and n <= 7), float taps
+void gaussian_taps(int n, double sigma, float* out) {
+  static const float small_tab[4][7] = {{1.f},
+                                        {0.25f, 0.5f, 0.25f},
+                                        {0.0625f, 0.25f, 0.375f, 0.25f, 0.0625f},
+                                        {0.03125f, 0.109375f, 0.21875f, 0.28125f, 0.21875f, 0.109375f, 0.03125f}};
+  const float* fixed = (n % 2 == 1 && n <= 7 && sigma <= 0) ? small_tab[n >> 1] : nullptr;
+  const double sx = sigma > 0 ? sigma : ((n - 1) * 0.5 - 1) * 0.3 + 0.8;
+  const double scale2X = -0.5 / (sx * sx);
+  std::vector<double> cd(n);
+  double sum = 0;
+  for (int i = 0; i < n; ++i) {
+    const double x = i - (n - 1) * 0.5;
+    cd[i] = fixed ? (double)fixed[i] : exp(scale2X * x * x);
+    sum += cd[i];
+  }
+  sum = 1. / sum;
+  for (int i = 0; i < n; ++i) out[i] = (float)(cd[i] * sum);
+}
+
+SepKernel square_sep(const float* k, int n, float delta) {
+  SepKernel K{};
+  for (int i = 0; i < n; ++i) K.kx[i] = K.ky[i] = k[i];
+  K.nx = K.ny = n;
+  bool sym = true, asym = true;
+  for (int i = 0; i < n; ++i) {
+    if ((double)k[i] != (double)k[n - 1 - i]) sym = false;
+    if ((double)k[i] != -(double)k[n - 1 - i]) asym = false;
+  }
+  K.tx = K.ty = sym ? 1 : (asym ? 2 : 0);
+  K.delta = delta;
+  return K;
+}
+
+// separable filter of nplanes dense planes (w x h) from src into dst (REFLECT101)
+int blur_planes(ssg_accumulation* a, const float* src, float* dst, int w, int h, int nplanes, const SepKernel& K) {
+  const size_t n = (size_t)w * h;
+  if (a->slot + nplanes > (int)a->hptrs.size()) return fail(SSG_E_INVALID_ARG, "too many filter passes");
+  for (int c = 0; c < nplanes; ++c) a->hptrs[a->slot + c] = src + c * n;
+  const float** dp = a->fptrs.as<const float*>() + a->slot;
+  HIPCHK(hipMemcpyAsync((void*)dp, &a->hptrs[a->slot], sizeof(float*) * nplanes, hipMemcpyHostToDevice, a->stream));
+  a->slot += nplanes;
+  HIPCHK(launch_sepfilter(dp, w, dst, (long long)n, w, w, h, nplanes, K, SSG_BORDER_REFLECT101, a->stream));
+  return SSG_OK;
+}
+
+int focus_prepare(ssg_accumulation* a) {
+  const ssg_focus_stacking_options& o = a->fo;
+  a->lw.assign(1, a->w);
+  a->lh.assign(1, a->h);
+  for (;;) {
+    const int nw = (a->lw.back() + 1) / 2, nh = (a->lh.back() + 1) / 2;
+    if (std::min(nw, nh) <= 8 || (o.max_levels && (int)a->lw.size() >= o.max_levels)) break;
+    a->lw.push_back(nw);
+    a->lh.push_back(nh);
+  }
+  a->L = (int)a->lw.size();
+  a->loff.assign(a->L + 1, 0);
+  for (int i = 0; i < a->L; ++i) a->loff[i + 1] = a->loff[i] + (long long)a->lw[i] * a->lh[i];
+  a->P = a->loff[a->L];
+  const int cn = a->cn;
+  a->wc = (o.fusing_policy == SSG_FOCUS_WEIGHTED_AVERAGE && o.avgchannel && cn > 1) ? 1 : cn;
+  const size_t n0 = (size_t)a->w * a->h;
+  CHK(a->facc.ensure(sizeof(float) * cn * a->P));
+  CHK(a->fp.ensure(sizeof(float) * cn * a->P));
+  CHK(a->fup.ensure(sizeof(float) * cn * n0));
+  CHK(a->ft0.ensure(sizeof(float) * cn * n0));
+  CHK(a->ft1.ensure(sizeof(float) * cn * n0));
+  CHK(a->ft2.ensure(sizeof(float) * cn * n0));
+  CHK(a->ft3.ensure(sizeof(float) * cn * n0));
+  CHK(a->fstd.ensure(sizeof(double) * 2 * cn));
+  CHK(a->fptrs.ensure(sizeof(float*) * 256));
+  a->hptrs.assign(256, nullptr);
+  if (o.fusing_policy == SSG_FOCUS_LPG_WEIGHTS) {
+    CHK(a->fw.ensure(sizeof(float) * a->P));
+    CHK(a->fW.ensure(sizeof(float) * a->P));
+    CHK(a->fgray.ensure(sizeof(float) * n0));
+    CHK(a->fm.ensure(sizeof(float) * n0));
+    HIPCHK(hipMemsetAsync(a->facc.p, 0, sizeof(float) * cn * a->P, a->stream));
+    HIPCHK(hipMemsetAsync(a->fw.p, 0, sizeof(float) * a->P, a->stream));
+    const int nk = (int)std::floor(o.lpg_sigma * 8 + 1 + 0.5) | 1;  // cv::GaussianBlur on float images
+    if (nk > SEP_MAX_TAPS) return fail(SSG_E_UNSUPPORTED, "lpg_sigma %g needs %d taps (max %d)", o.lpg_sigma, nk,
+                                       SEP_MAX_TAPS);
+    std::vector<float> t(nk);
+    gaussian_taps(nk, o.lpg_sigma, t.data());
+    a->Wk = square_sep(t.data(), nk, 0.f);
+  } else if (o.fusing_policy == SSG_FOCUS_WEIGHTED_AVERAGE) {
+    CHK(a->fw.ensure(sizeof(float) * a->wc * a->P));
+  }
+  a->hasG = o.ksigma > 0 || o.kradius > 0;
+  if (a->hasG) {
+    const int nk = std::max(0, 2 * o.kradius + 1);
+    if (nk > SEP_MAX_TAPS) return fail(SSG_E_UNSUPPORTED, "kradius %d", o.kradius);
+    std::vector<float> t(nk);
+    gaussian_taps(nk, std::max(0.0, o.ksigma), t.data());
+    a->G = square_sep(t.data(), nk, 0.f);
+  }
+  return SSG_OK;
+}
+
+// c_laplacian_pyramid_focus_stacking::add (c_frame_accumulation.cc:491-627)
+int focus_add(ssg_accumulation* a, const ssg_image* src) {
+  const int cn = a->cn;
+  if (src->type != (cn == 3 ? SSG_32FC3 : SSG_32FC1))
+    return fail(SSG_E_INVALID_ARG, "focus frames are f32 with the accumulator's %d channels", cn);
+  const bool first = a->accumulated == 0;
+  if (first) CHK(focus_prepare(a));
+  hipStream_t s = a->stream;
+  a->slot = 0;
+  const ssg_focus_stacking_options& o = a->fo;
+  DevBuf ts;
+  const void* sp;
+  size_t spb;
+  CHK(device_view(src, ts, s, &sp, &spb));
+  float* fp = a->fp.as<float>();
+  float* acc = a->facc.as<float>();
+  const size_t n0 = (size_t)a->w * a->h;
+  auto lvl = [&](float* base, int i, int planes) { return base + (size_t)planes * a->loff[i]; };
+  auto nl = [&](int i) { return (size_t)a->lw[i] * a->lh[i]; };
+  HIPCHK(launch_split((const float*)sp, (int)(spb / 4), a->w, a->h, cn, fp, s));
+  if (o.fusing_policy == SSG_FOCUS_LPG_WEIGHTS) {
+    // W = gaussian(lpg(gray)), reduced to every level
+    const double k = o.lpg_k;
+    const float l2 = (100.f / 4.f) * (100.f / 4.f), g2 = (100.f / 36.f) * (100.f / 36.f);
+    const float alpha = (float)(k / (k + 1)) * l2, beta = (float)(1. / (k + 1)) * g2;
+    HIPCHK(launch_gray(fp, n0, cn, a->fgray.as<float>(), s));
+    HIPCHK(launch_lpg(a->fgray.as<float>(), a->w, a->h, alpha, beta, 1e-9f, (float)o.lpg_p, a->fm.as<float>(), s));
+    CHK(blur_planes(a, a->fm.as<float>(), a->fW.as<float>(), a->w, a->h, 1, a->Wk));
+    for (int i = 0; i + 1 < a->L; ++i)
+      HIPCHK(launch_pyrdown(lvl(a->fW.as<float>(), i, 1), (long long)nl(i), a->lw[i], a->lw[i], a->lh[i],
+                            lvl(a->fW.as<float>(), i + 1, 1), (long long)nl(i + 1), a->lw[i + 1], a->lw[i + 1],
+                            a->lh[i + 1], 1, SSG_BORDER_REFLECT101, s));
+  }
+  // build_laplacian_pyramid (laplacian_pyramid.cc:44-88)
+  for (int i = 0; i + 1 < a->L; ++i) {
+    HIPCHK(launch_pyrdown(lvl(fp, i, cn), (long long)nl(i), a->lw[i], a->lw[i], a->lh[i], lvl(fp, i + 1, cn),
+                          (long long)nl(i + 1), a->lw[i + 1], a->lw[i + 1], a->lh[i + 1], cn, SSG_BORDER_REFLECT101, s));
+    HIPCHK(launch_pyrup(lvl(fp, i + 1, cn), (long long)nl(i + 1), a->lw[i + 1], a->lh[i + 1], a->fup.as<float>(),
+                        (long long)nl(i), a->lw[i], a->lh[i], cn, s));
+    HIPCHK(launch_ew(0, lvl(fp, i, cn), a->fup.as<float>(), lvl(fp, i, cn), cn * nl(i), s));
+  }
+  const int top = a->L - 1;
+  if (o.fusing_policy == SSG_FOCUS_LPG_WEIGHTS) {
+    for (int i = 0; i < a->L; ++i)
+      HIPCHK(launch_wacc(lvl(acc, i, cn), lvl(a->fw.as<float>(), i, 1), lvl(a->fW.as<float>(), i, 1), lvl(fp, i, cn),
+                         nl(i), cn, s));
+  } else {
+    if (!first) {
+      // top level: blend weighted by the channel-summed standard deviations (:579-588)
+      HIPCHK(launch_plane_stdev(lvl(acc, top, cn), nl(top), cn, a->fstd.as<double>(), s));
+      HIPCHK(launch_plane_stdev(lvl(fp, top, cn), nl(top), cn, a->fstd.as<double>() + cn, s));
+      double sd[6];
+      HIPCHK(hipMemcpyAsync(sd, a->fstd.p, sizeof(double) * 2 * cn, hipMemcpyDeviceToHost, s));
+      HIPCHK(hipStreamSynchronize(s));
+      double s0 = 0, s1 = 0;
+      for (int c = 0; c < cn; ++c) {
+        s0 += sd[c];
+        s1 += sd[cn + c];
+      }
+      HIPCHK(launch_blend(lvl(acc, top, cn), lvl(fp, top, cn), (float)(s0 / (s0 + s1)), (float)(s1 / (s0 + s1)),
+                          cn * nl(top), s));
+    } else {
+      HIPCHK(hipMemcpyAsync(lvl(acc, top, cn), lvl(fp, top, cn), sizeof(float) * cn * nl(top),
+                            hipMemcpyDeviceToDevice, s));
+    }
+    for (int i = 0; i < top; ++i) {
+      float* L = lvl(fp, i, cn);
+      float* A = lvl(acc, i, cn);
+      const size_t n = nl(i);
+      if (o.fusing_policy == SSG_FOCUS_SELECT_MAX_ENERGY) {
+        if (first) {
+          HIPCHK(hipMemcpyAsync(A, L, sizeof(float) * cn * n, hipMemcpyDeviceToDevice, s));
+          continue;
+        }
+        float* w0 = a->ft0.as<float>();
+        float* w1 = a->ft1.as<float>();
+        HIPCHK(launch_ew(2, A, nullptr, w0, cn * n, s));
+        HIPCHK(launch_ew(2, L, nullptr, w1, cn * n, s));
+        if (a->hasG) {
+          CHK(blur_planes(a, w0, a->ft2.as<float>(), a->lw[i], a->lh[i], cn, a->G));
+          CHK(blur_planes(a, w1, a->ft3.as<float>(), a->lw[i], a->lh[i], cn, a->G));
+          w0 = a->ft2.as<float>();
+          w1 = a->ft3.as<float>();
+        }
+        HIPCHK(launch_select(A, L, w0, w1, cn * n, s));
+      } else {
+        // compute_energy (:511-531), then acc += lap*w, wwp += w
+        float* ww = a->ft0.as<float>();
+        HIPCHK(launch_energy(L, ww, n, cn, o.avgchannel, s));
+        if (a->hasG) {
+          SepKernel K = a->G;
+          K.delta = 1e-12f;
+          CHK(blur_planes(a, ww, a->ft1.as<float>(), a->lw[i], a->lh[i], a->wc, K));
+          ww = a->ft1.as<float>();
+        }
+        HIPCHK(launch_mulw(L, ww, a->wc, n, cn, s));
+        float* Wp = lvl(a->fw.as<float>(), i, a->wc);
+        if (first) {
+          HIPCHK(hipMemcpyAsync(A, L, sizeof(float) * cn * n, hipMemcpyDeviceToDevice, s));
+          HIPCHK(hipMemcpyAsync(Wp, ww, sizeof(float) * a->wc * n, hipMemcpyDeviceToDevice, s));
+        } else {
+          HIPCHK(launch_ew(1, L, A, A, cn * n, s));
+          HIPCHK(launch_ew(1, ww, Wp, Wp, a->wc * n, s));
+        }
+      }
+    }
+  }
+  HIPCHK(hipStreamSynchronize(s));
+  ++a->accumulated;
+  return SSG_OK;
+}
+
+// c_laplacian_pyramid_focus_stacking::compute (:631-662) + reconstruct_laplacian_pyramid (:90-127):
+// the reconstructed image is left in level 0 of fp
+int focus_compute(ssg_accumulation* a) {
+  const int cn = a->cn;
+  hipStream_t s = a->stream;
+  float* fp = a->fp.as<float>();
+  float* acc = a->facc.as<float>();
+  auto lvl = [&](float* base, int i, int planes) { return base + (size_t)planes * a->loff[i]; };
+  auto nl = [&](int i) { return (size_t)a->lw[i] * a->lh[i]; };
+  const int top = a->L - 1;
+  const int pol = a->fo.fusing_policy;
+  // levels to reconstruct: lv_i in fp (select: the accumulator itself)
+  std::vector<const float*> lv(a->L);
+  for (int i = 0; i < a->L; ++i) {
+    if (pol == SSG_FOCUS_SELECT_MAX_ENERGY || (pol == SSG_FOCUS_WEIGHTED_AVERAGE && i == top)) {
+      lv[i] = lvl(acc, i, cn);
+    } else {
+      const int wc = pol == SSG_FOCUS_LPG_WEIGHTS ? 1 : a->wc;
+      HIPCHK(launch_div(lvl(acc, i, cn), lvl(a->fw.as<float>(), i, wc), wc, lvl(fp, i, cn), nl(i), cn, s));
+      lv[i] = lvl(fp, i, cn);
+    }
+  }
+  // cur = top; cur = pyrUp(cur) + lv_i, into fp level i
+  if (lv[top] != lvl(fp, top, cn))
+    HIPCHK(hipMemcpyAsync(lvl(fp, top, cn), lv[top], sizeof(float) * cn * nl(top), hipMemcpyDeviceToDevice, s));
+  for (int i = top - 1; i >= 0; --i) {
+    HIPCHK(launch_pyrup(lvl(fp, i + 1, cn), (long long)nl(i + 1), a->lw[i + 1], a->lh[i + 1], a->fup.as<float>(),
+                        (long long)nl(i), a->lw[i], a->lh[i], cn, s));
+    HIPCHK(launch_ew(1, a->fup.as<float>(), lv[i], lvl(fp, i, cn), cn * nl(i), s));
+  }
+  return SSG_OK;
+}
+}  // namespace
+
+extern "C" int ssg_lpg(const ssg_image* src, ssg_image* dst, double k, double p, double* sharpness) {
+  CHK(check_image(src, "src"));
+  CHK(check_image(dst, "dst"));
+  if (src->type != SSG_32FC1 && src->type != SSG_32FC3) return fail(SSG_E_INVALID_ARG, "src must be SSG_32FC1/3");
+  if (dst->type != SSG_32FC1 || dst->width != src->width || dst->height != src->height)
+    return fail(SSG_E_INVALID_ARG, "dst must be SSG_32FC1 of the source size");
+  if (src->width < 5 || src->height < 5) return fail(SSG_E_INVALID_ARG, "lpg needs at least 5x5 pixels");
+  const int w = src->width, h = src->height, cn = channels_of(src->type);
+  hipStream_t s = nullptr;
+  HIPCHK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+  struct StreamGuard {
+    hipStream_t s;
+    ~StreamGuard() {
+      (void)hipStreamSynchronize(s);
+      (void)hipStreamDestroy(s);
+    }
+  } guard{s};
+  DevBuf ts, planes, g, m, sums;
+  const void* sp;
+  size_t spb;
+  CHK(device_view(src, ts, s, &sp, &spb));
+  const size_t n = (size_t)w * h;
+  CHK(planes.ensure(sizeof(float) * cn * n));
+  CHK(g.ensure(sizeof(float) * n));
+  CHK(m.ensure(sizeof(float) * n));
+  HIPCHK(launch_split((const float*)sp, (int)(spb / 4), w, h, cn, planes.as<float>(), s));
+  HIPCHK(launch_gray(planes.as<float>(), n, cn, g.as<float>(), s));
+  const float l2 = (100.f / 4.f) * (100.f / 4.f), g2 = (100.f / 36.f) * (100.f / 36.f);
+  HIPCHK(launch_lpg(g.as<float>(), w, h, (float)(k / (k + 1)) * l2, (float)(1. / (k + 1)) * g2, 1e-9f, (float)p,
+                    m.as<float>(), s));
+  if (sharpness) {
+    CHK(sums.ensure(sizeof(double) * (h + 1)));
+    HIPCHK(launch_sum_f32(m.as<float>(), w, w, h, sums.as<double>(), s));
+    double total = 0;
+    HIPCHK(hipMemcpyAsync(&total, sums.as<double>() + h, sizeof(double), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    *sharpness = total / (double)n;
+  }
+  HIPCHK(hipMemcpy2DAsync(dst->data, (size_t)dst->step, m.p, (size_t)w * 4, (size_t)w * 4, h,
+                          dst->on_device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  return SSG_OK;
+}
+
 extern "C" int ssg_accumulation_add(ssg_accumulation* a, const ssg_image* src, const ssg_image* weights) {
   if (!a) return fail(SSG_E_INVALID_ARG, "null handle");
   CHK(check_image(src, "src"));
@@ -1178,6 +1514,10 @@ extern "C" int ssg_accumulation_add(ssg_accumulation* a, const ssg_image* src, c
     return fail(SSG_E_INVALID_ARG, "current frame (%dx%d) and accumulator (%dx%d) sizes not match", src->width, src->height,
                 a->w, a->h);
   if (a->bayer()) return bayer_add(a, src, weights);
+  if (a->focus()) {
+    if (weights) return fail(SSG_E_UNSUPPORTED, "focus stacking takes frames without masks on this path");
+    return focus_add(a, src);
+  }
   if (channels_of(src->type) != a->cn || (src->type != SSG_32FC1 && src->type != SSG_32FC3))
     return fail(SSG_E_INVALID_ARG, "frame type %d does not match the accumulator (%d channels, f32)", src->type, a->cn);
   hipStream_t s = a->stream;
@@ -1271,6 +1611,7 @@ extern "C" int ssg_accumulation_add_registered_batch(ssg_accumulation* a, int n,
   if (motion_type < SSG_MOTION_TRANSLATION || motion_type > SSG_MOTION_AFFINE)
     return fail(SSG_E_INVALID_ARG, "motion_type %d", motion_type);
   if (a->bayer()) return bayer_batch(a, n, frames, statuses, motion_type, frame_weights, interpolation);
+  if (a->focus()) return fail(SSG_E_UNSUPPORTED, "focus stacking adds frames one by one (ssg_accumulation_add)");
   const int sw = frames[0].width, sh = frames[0].height;
   const int64_t step = frames[0].step;
   for (int f = 0; f < n; ++f) {
@@ -1384,6 +1725,10 @@ extern "C" int ssg_accumulation_compute(ssg_accumulation* a, ssg_image* avg, ssg
   }
   if (a->bayer()) {
     HIPCHK(launch_bayer_compute(a->bayer_state(), ap, (int)(apb / 4), mp, (int)mpb, s));
+  } else if (a->focus()) {
+    CHK(focus_compute(a));
+    if (ap) HIPCHK(launch_merge(a->fp.as<float>(), a->w, a->h, a->cn, ap, (int)(apb / 4), s));
+    if (mp) HIPCHK(hipMemset2DAsync(mp, mpb, 255, a->w, a->h, s));
   } else {
     HIPCHK(launch_acc_compute(a->acc.as<float>(), nullptr, a->w, a->h, a->cn, a->plane, ap, (int)(apb / 4), mp,
                               (int)mpb, dscale, s));
@@ -1398,6 +1743,7 @@ extern "C" int ssg_accumulation_compute(ssg_accumulation* a, ssg_image* avg, ssg
 
 extern "C" int ssg_accumulation_get_acc_counters(ssg_accumulation* a, ssg_image* accw) {
   if (!a) return fail(SSG_E_INVALID_ARG, "null handle");
+  if (a->focus()) return SSG_OK;  // the reference releases accw (c_frame_accumulation.cc:683-687)
   CHK(check_image(accw, "accw"));
   if (a->bayer()) {
     if (accw->type != SSG_32FC3 || accw->width != a->w || accw->height != a->h)

@@ -158,10 +158,12 @@ struct BayerBatchArgs {
 
 // ---- launchers --------------------------------------------------------------------------------
 // pyramids (ssg_pyr.hip)
+constexpr int SEP_MAX_TAPS = 49;
 struct SepKernel {
-  float kx[32];
-  float ky[32];
+  float kx[SEP_MAX_TAPS];
+  float ky[SEP_MAX_TAPS];
   int nx, ny, tx, ty;  // taps and symmetry type (0 other, 1 symmetric, 2 antisymmetric)
+  float delta;         // added to every output (cv::sepFilter2D's delta)
 };
 hipError_t launch_sepfilter(const float* const* src_ptrs, int src_pitch, float* dst, long long dst_fstride,
                             int dst_pitch, int w, int h, int nframes, const SepKernel& k, int border,
@@ -219,6 +221,23 @@ hipError_t launch_bayer_batch(int motion, bool u16, const BayerBatchArgs& a, hip
 hipError_t launch_bayer_compute(const BayerState& st, float* avg, int avg_pitch, uint8_t* mask, int mask_pitch,
                                 hipStream_t s);
 hipError_t launch_bayer_counters(const BayerState& st, float* out, int out_pitch, hipStream_t s);
+
+// focus stacking (ssg_focus.hip)
+hipError_t launch_split(const float* src, int pitch, int w, int h, int cn, float* planes, hipStream_t s);
+hipError_t launch_merge(const float* planes, int w, int h, int cn, float* dst, int pitch, hipStream_t s);
+hipError_t launch_gray(const float* planes, size_t n, int cn, float* g, hipStream_t s);
+hipError_t launch_lpg(const float* g, int w, int h, float alpha, float beta, float eps, float p, float* m,
+                      hipStream_t s);
+hipError_t launch_pyrup(const float* src, long long src_fstride, int sw, int sh, float* dst, long long dst_fstride,
+                        int dw, int dh, int nplanes, hipStream_t s);
+hipError_t launch_ew(int op, const float* a, const float* b, float* d, size_t n, hipStream_t s);  // 0 a-b, 1 a+b, 2 |a|
+hipError_t launch_div(const float* a, const float* w, int wc, float* d, size_t n, int cn, hipStream_t s);
+hipError_t launch_wacc(float* acc, float* wsum, const float* W, const float* L, size_t n, int cn, hipStream_t s);
+hipError_t launch_select(float* acc, const float* pyr, const float* w0, const float* w1, size_t n, hipStream_t s);
+hipError_t launch_energy(const float* lap, float* w, size_t n, int cn, int avgc, hipStream_t s);
+hipError_t launch_mulw(float* lap, const float* w, int wc, size_t n, int cn, hipStream_t s);
+hipError_t launch_blend(float* a, const float* b, float alpha, float beta, size_t n, hipStream_t s);
+hipError_t launch_plane_stdev(const float* planes, size_t n, int nplanes, double* out, hipStream_t s);
 
 // FFT phase correlation (ssg_fft.hip)
 hipError_t launch_fft2(float2* buf, float2* tmp, int M, int N, int nframes, const float2* tw_m, const float2* tw_n,

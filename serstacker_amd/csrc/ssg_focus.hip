@@ -1,0 +1,256 @@
+// ssg_focus.hip -- focus stacking on gfx950: the lpg sharpness map
+// (core/proc/lpg.cc:63-127, 277-344), laplacian pyramids
+// (core/proc/laplacian_pyramid.cc:44-127) and the per-level fusion of
+// c_laplacian_pyramid_focus_stacking (core/average/c_frame_accumulation.cc:491-681).
+//
+// Images are planar f32 (one plane per channel, dense rows).  Every elementwise
+// formula keeps the reference's order of operations with no fused multiply-add
+// (the library builds with -ffp-contract=off), so the results reproduce the
+// FP32 oracle bit for bit.
+#include "ssg_engine.hpp"
+
+namespace ssg {
+
+namespace {
+
+__device__ __forceinline__ size_t gidx() { return (size_t)blockIdx.x * 256 + threadIdx.x; }
+
+// interleaved cn-channel rows (pitch in floats) -> cn dense planes
+__global__ void __launch_bounds__(256) k_split(const float* src, int pitch, int w, int h, int cn, float* planes) {
+  const size_t i = gidx();
+  const size_t n = (size_t)w * h;
+  if (i >= n) return;
+  const int y = (int)(i / w), x = (int)(i - (size_t)y * w);
+  for (int c = 0; c < cn; ++c) planes[(size_t)c * n + i] = src[(size_t)y * pitch + (size_t)x * cn + c];
+}
+
+__global__ void __launch_bounds__(256) k_merge(const float* planes, int w, int h, int cn, float* dst, int pitch) {
+  const size_t i = gidx();
+  const size_t n = (size_t)w * h;
+  if (i >= n) return;
+  const int y = (int)(i / w), x = (int)(i - (size_t)y * w);
+  for (int c = 0; c < cn; ++c) dst[(size_t)y * pitch + (size_t)x * cn + c] = planes[(size_t)c * n + i];
+}
+
+// reduce_color_channels(REDUCE_AVG): (c0 + c1) + c2 in float, times 1/3 in double
+__global__ void __launch_bounds__(256) k_gray(const float* planes, size_t n, int cn, float* g) {
+  const size_t i = gidx();
+  if (i >= n) return;
+  if (cn == 1) {
+    g[i] = planes[i];
+    return;
+  }
+  const float s = (planes[i] + planes[n + i]) + planes[2 * n + i];
+  g[i] = (float)((double)s * (1.0 / 3.0));
+}
+
+// compute_lpg_5x5_parallel (lpg.cc:63-127) at every pixel of the clamped grid
+// (its edge rows/columns copy the nearest computed pixel), then pow(m, p)
+__global__ void __launch_bounds__(256) k_lpg(const float* g, int w, int h, float alpha, float beta, float eps,
+                                             float p, float* m) {
+  const size_t i = gidx();
+  if (i >= (size_t)w * h) return;
+  const int y = (int)(i / w), x = (int)(i - (size_t)y * w);
+  const int cy = min(max(y, 2), h - 3), cx = min(max(x, 2), w - 3);
+  auto R = [&](int dy, int dx) { return g[(size_t)(cy + dy) * w + cx + dx]; };
+  const float two = 2.f, four = 4.f;
+  auto col = [&](int dx) { return (((R(-2, dx) + two * R(-1, dx)) + four * R(0, dx)) + two * R(1, dx)) + R(2, dx); };
+  auto row = [&](int dy) { return (((R(dy, -2) + two * R(dy, -1)) + four * R(dy, 0)) + two * R(dy, 1)) + R(dy, 2); };
+  auto c3 = [&](int dx) { return (R(-1, dx) + two * R(0, dx)) + R(1, dx); };
+  auto r3 = [&](int dy) { return (R(dy, -1) + two * R(dy, 0)) + R(dy, 1); };
+  const float gx = (col(2) - col(-2)) + two * (c3(1) - c3(-1));
+  const float gy = (row(2) - row(-2)) + two * (r3(1) - r3(-1));
+  const float grad = gx * gx + gy * gy;
+  const float t = ((16.f * R(0, 0) - two * (((R(-1, 0) + R(1, 0)) + R(0, -1)) + R(0, 1))) -
+                   1.f * (((R(-1, -1) + R(-1, 1)) + R(1, -1)) + R(1, 1))) -
+                  1.f * (((R(-2, 0) + R(2, 0)) + R(0, -2)) + R(0, 2));
+  float v = (alpha * (t * t) + beta * grad) + eps;
+  if (p == 2.f) v = v * v;
+  else if (p != 0.f && p != 1.f) v = powf(v, p);
+  m[i] = v;
+}
+
+// cv::pyrUp (float) of nplanes planes: the oracle's canonical form -- horizontally
+// upsampled rows (left REFLECT101, right replicated), then the vertical pass; an odd
+// extra column repeats the last one, an odd extra row repeats row 2*sh-2
+__device__ __forceinline__ float hup(const float* S, int sw, int xo) {
+  // horizontally upsampled value at output column xo of source row S
+  if (sw == 1) return S[0] * 8.f;
+  if (xo >= 2 * sw) xo = 2 * sw - 1;
+  const int x = xo >> 1;
+  if (xo & 1) {
+    if (x == sw - 1) return S[sw - 1] * 8.f;
+    return (S[x] + S[x + 1]) * 4.f;
+  }
+  if (x == 0) return fmaf(S[0], 6.f, S[1] + S[1]);
+  if (x == sw - 1) return fmaf(S[sw - 1], 6.f, S[sw - 2] + S[sw - 1]);
+  return fmaf(S[x], 6.f, S[x - 1] + S[x + 1]);
+}
+
+__global__ void __launch_bounds__(256) k_pyrup(const float* src, long long src_fstride, int sw, int sh, float* dst,
+                                               long long dst_fstride, int dw, int dh) {
+  const size_t i = gidx();
+  if (i >= (size_t)dw * dh) return;
+  const int yo = (int)(i / dw), xo = (int)(i - (size_t)yo * dw);
+  const float* S = src + blockIdx.y * src_fstride;
+  float* D = dst + blockIdx.y * dst_fstride;
+  const int ye = (dh > 2 * sh && yo == 2 * sh) ? 2 * sh - 2 : yo;  // the extra odd row copies row 2 sh - 2
+  const int y = ye >> 1;
+  auto srow = [&](int r) {  // source rows -1..sh: REFLECT101 above, replicated below
+    if (r < 0) r = sh > 1 ? 1 : 0;
+    if (r >= sh) r = sh - 1;
+    return S + (size_t)r * sw;
+  };
+  const float scale = 1.f / 64.f;
+  float v;
+  if (ye & 1) {
+    // a row 2y+1 that exists only as min(2y+1, dh-1) when dh is odd and smaller is
+    // written by the even branch below (the oracle writes t1 first, then t0)
+    v = (hup(srow(y), sw, xo) + hup(srow(y + 1), sw, xo)) * 4.f * scale;
+  } else {
+    const float r0 = hup(srow(y - 1), sw, xo), r1 = hup(srow(y), sw, xo), r2 = hup(srow(y + 1), sw, xo);
+    v = fmaf(r1, 6.f, r0 + r2) * scale;
+  }
+  D[(size_t)yo * dw + xo] = v;
+}
+
+// elementwise helpers over n elements
+__global__ void __launch_bounds__(256) k_sub(const float* a, const float* b, float* d, size_t n) {
+  const size_t i = gidx();
+  if (i < n) d[i] = a[i] - b[i];
+}
+__global__ void __launch_bounds__(256) k_add(const float* a, const float* b, float* d, size_t n) {
+  const size_t i = gidx();
+  if (i < n) d[i] = a[i] + b[i];
+}
+// d[c*n + i] = a[c*n + i] / w[(wc > 1 ? c : 0)*n + i]
+__global__ void __launch_bounds__(256) k_div(const float* a, const float* w, int wc, float* d, size_t n, int cn) {
+  const size_t i = gidx();
+  if (i >= n) return;
+  for (int c = 0; c < cn; ++c) d[(size_t)c * n + i] = a[(size_t)c * n + i] / w[(size_t)(wc > 1 ? c : 0) * n + i];
+}
+// lpg weights: acc[c] = acc[c] + W * L[c]; wsum = wsum + W
+__global__ void __launch_bounds__(256) k_wacc(float* acc, float* wsum, const float* W, const float* L, size_t n,
+                                              int cn) {
+  const size_t i = gidx();
+  if (i >= n) return;
+  const float w = W[i];
+  for (int c = 0; c < cn; ++c) acc[(size_t)c * n + i] = acc[(size_t)c * n + i] + w * L[(size_t)c * n + i];
+  wsum[i] = wsum[i] + w;
+}
+// |x| of cn planes
+__global__ void __launch_bounds__(256) k_abs(const float* a, float* d, size_t n) {
+  const size_t i = gidx();
+  if (i < n) d[i] = fabsf(a[i]);
+}
+// select_max_energy: acc = w1 > w0 ? pyr : acc
+__global__ void __launch_bounds__(256) k_select(float* acc, const float* pyr, const float* w0, const float* w1,
+                                                size_t n) {
+  const size_t i = gidx();
+  if (i < n && w1[i] > w0[i]) acc[i] = pyr[i];
+}
+// compute_energy, first half: w = lap^2 per channel, or (sum of channels)^2
+__global__ void __launch_bounds__(256) k_energy(const float* lap, float* w, size_t n, int cn, int avgc) {
+  const size_t i = gidx();
+  if (i >= n) return;
+  if (!avgc || cn == 1) {
+    for (int c = 0; c < cn; ++c) w[(size_t)c * n + i] = lap[(size_t)c * n + i] * lap[(size_t)c * n + i];
+  } else {
+    const float s = (lap[i] + lap[n + i]) + lap[2 * n + i];
+    w[i] = s * s;
+  }
+}
+// lap[c] = lap[c] * w[c or 0]
+__global__ void __launch_bounds__(256) k_mulw(float* lap, const float* w, int wc, size_t n, int cn) {
+  const size_t i = gidx();
+  if (i >= n) return;
+  for (int c = 0; c < cn; ++c) lap[(size_t)c * n + i] = lap[(size_t)c * n + i] * w[(size_t)(wc > 1 ? c : 0) * n + i];
+}
+// addWeighted(a, alpha, b, beta, 0): (a*alpha + b*beta) in float
+__global__ void __launch_bounds__(256) k_blend(float* a, const float* b, float alpha, float beta, size_t n) {
+  const size_t i = gidx();
+  if (i < n) a[i] = a[i] * alpha + b[i] * beta;
+}
+// meanStdDev of each plane, sequential double sums (small top levels)
+__global__ void k_plane_stdev(const float* planes, size_t n, int nplanes, double* out) {
+  const int c = threadIdx.x;
+  if (c >= nplanes) return;
+  double s = 0, sq = 0;
+  for (size_t i = 0; i < n; ++i) {
+    const double v = planes[(size_t)c * n + i];
+    s += v;
+    sq += v * v;
+  }
+  const double scale = 1.0 / (double)n;
+  s *= scale;
+  const double var = sq * scale - s * s;
+  out[c] = sqrt(var > 0 ? var : 0.0);
+}
+
+inline dim3 g1(size_t n, int y = 1) { return dim3((unsigned)((n + 255) / 256), y); }
+
+}  // namespace
+
+hipError_t launch_split(const float* src, int pitch, int w, int h, int cn, float* planes, hipStream_t s) {
+  hipLaunchKernelGGL(k_split, g1((size_t)w * h), dim3(256), 0, s, src, pitch, w, h, cn, planes);
+  return hipGetLastError();
+}
+hipError_t launch_merge(const float* planes, int w, int h, int cn, float* dst, int pitch, hipStream_t s) {
+  hipLaunchKernelGGL(k_merge, g1((size_t)w * h), dim3(256), 0, s, planes, w, h, cn, dst, pitch);
+  return hipGetLastError();
+}
+hipError_t launch_gray(const float* planes, size_t n, int cn, float* g, hipStream_t s) {
+  hipLaunchKernelGGL(k_gray, g1(n), dim3(256), 0, s, planes, n, cn, g);
+  return hipGetLastError();
+}
+hipError_t launch_lpg(const float* g, int w, int h, float alpha, float beta, float eps, float p, float* m,
+                      hipStream_t s) {
+  if (w < 5 || h < 5) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_lpg, g1((size_t)w * h), dim3(256), 0, s, g, w, h, alpha, beta, eps, p, m);
+  return hipGetLastError();
+}
+hipError_t launch_pyrup(const float* src, long long src_fstride, int sw, int sh, float* dst, long long dst_fstride,
+                        int dw, int dh, int nplanes, hipStream_t s) {
+  hipLaunchKernelGGL(k_pyrup, g1((size_t)dw * dh, nplanes), dim3(256), 0, s, src, src_fstride, sw, sh, dst,
+                     dst_fstride, dw, dh);
+  return hipGetLastError();
+}
+hipError_t launch_ew(int op, const float* a, const float* b, float* d, size_t n, hipStream_t s) {
+  switch (op) {
+    case 0: hipLaunchKernelGGL(k_sub, g1(n), dim3(256), 0, s, a, b, d, n); break;
+    case 1: hipLaunchKernelGGL(k_add, g1(n), dim3(256), 0, s, a, b, d, n); break;
+    case 2: hipLaunchKernelGGL(k_abs, g1(n), dim3(256), 0, s, a, d, n); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+hipError_t launch_div(const float* a, const float* w, int wc, float* d, size_t n, int cn, hipStream_t s) {
+  hipLaunchKernelGGL(k_div, g1(n), dim3(256), 0, s, a, w, wc, d, n, cn);
+  return hipGetLastError();
+}
+hipError_t launch_wacc(float* acc, float* wsum, const float* W, const float* L, size_t n, int cn, hipStream_t s) {
+  hipLaunchKernelGGL(k_wacc, g1(n), dim3(256), 0, s, acc, wsum, W, L, n, cn);
+  return hipGetLastError();
+}
+hipError_t launch_select(float* acc, const float* pyr, const float* w0, const float* w1, size_t n, hipStream_t s) {
+  hipLaunchKernelGGL(k_select, g1(n), dim3(256), 0, s, acc, pyr, w0, w1, n);
+  return hipGetLastError();
+}
+hipError_t launch_energy(const float* lap, float* w, size_t n, int cn, int avgc, hipStream_t s) {
+  hipLaunchKernelGGL(k_energy, g1(n), dim3(256), 0, s, lap, w, n, cn, avgc);
+  return hipGetLastError();
+}
+hipError_t launch_mulw(float* lap, const float* w, int wc, size_t n, int cn, hipStream_t s) {
+  hipLaunchKernelGGL(k_mulw, g1(n), dim3(256), 0, s, lap, w, wc, n, cn);
+  return hipGetLastError();
+}
+hipError_t launch_blend(float* a, const float* b, float alpha, float beta, size_t n, hipStream_t s) {
+  hipLaunchKernelGGL(k_blend, g1(n), dim3(256), 0, s, a, b, alpha, beta, n);
+  return hipGetLastError();
+}
+hipError_t launch_plane_stdev(const float* planes, size_t n, int nplanes, double* out, hipStream_t s) {
+  hipLaunchKernelGGL(k_plane_stdev, dim3(1), dim3(64), 0, s, planes, n, nplanes, out);
+  return hipGetLastError();
+}
+
+}  // namespace ssg

@@ -13,7 +13,7 @@ namespace ssg {
 // (the OpenCV split SymmRowSmallFilter / RowFilter / SymmColumnFilter,
 // filter.simd.hpp, with fused multiply-adds).
 // ---------------------------------------------------------------------------
-constexpr int SF_TW = 64, SF_TH = 16, SF_MAXR = 15;
+constexpr int SF_TW = 64, SF_TH = 16, SF_MAXR = (SEP_MAX_TAPS - 1) / 2;
 
 __device__ __forceinline__ float row_filter(const float* c, const SepKernel& K) {
   // c points at the centre tap
@@ -39,16 +39,16 @@ __device__ __forceinline__ float col_filter(const float* c, int stride, const Se
   const int n = K.ny, a = n / 2;
   const float* k = K.ky;
   if (K.ty == 1) {
-    float s = fmaf(c[0], k[a], 0.f);
+    float s = fmaf(c[0], k[a], K.delta);
     for (int i = 1; i <= a; ++i) s = fmaf(c[-i * stride] + c[i * stride], k[a + i], s);
     return s;
   }
   if (K.ty == 2) {
-    float s = 0.f;
+    float s = K.delta;
     for (int i = 1; i <= a; ++i) s = fmaf(c[i * stride] - c[-i * stride], k[a + i], s);
     return s;
   }
-  float s = 0.f;
+  float s = K.delta;
   for (int i = 0; i < n; ++i) s = fmaf(c[(i - a) * stride], k[i], s);
   return s;
 }
@@ -114,18 +114,18 @@ __device__ __forceinline__ float col_filter_r(const float* win, const SepKernel&
   constexpr int n = 2 * R + 1;
   const float* k = K.ky;
   if (K.ty == 1) {
-    float s = fmaf(win[R], k[R], 0.f);
+    float s = fmaf(win[R], k[R], K.delta);
 #pragma unroll
     for (int i = 1; i <= R; ++i) s = fmaf(win[R - i] + win[R + i], k[R + i], s);
     return s;
   }
   if (K.ty == 2) {
-    float s = 0.f;
+    float s = K.delta;
 #pragma unroll
     for (int i = 1; i <= R; ++i) s = fmaf(win[R + i] - win[R - i], k[R + i], s);
     return s;
   }
-  float s = 0.f;
+  float s = K.delta;
 #pragma unroll
   for (int i = 0; i < n; ++i) s = fmaf(win[i], k[i], s);
   return s;

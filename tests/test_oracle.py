@@ -610,3 +610,79 @@ def test_phasecorr_recovers_known_subpixel_shift(binning, tol):
         # frame(x + T) = scene(x) = ref(x): the estimate is T
         assert abs(tx - T[0]) < tol and abs(ty - T[1]) < tol, (T, tx, ty)
         assert 0.1 < peak <= 1.0
+
+
+# ---- focus stacking: lpg, laplacian pyramids, fusion (oracle/focus.py) -----------------------------
+def _lpg_brute(g, k, p):
+    """float64 loops over lpg.cc:63-127 (interior), then the edge replication and pow."""
+    g = g.astype(np.float64)
+    h, w = g.shape
+    alpha = k / (k + 1) * 25.0 ** 2
+    beta = 1 / (k + 1) * (100 / 36) ** 2
+    m = np.zeros((h, w))
+    for y in range(2, h - 2):
+        r = [g[y + d] for d in (-2, -1, 0, 1, 2)]
+        for x in range(2, w - 2):
+            gx = (sum(c * r[j][x + 2] for j, c in enumerate((1, 2, 4, 2, 1)))
+                  - sum(c * r[j][x - 2] for j, c in enumerate((1, 2, 4, 2, 1)))
+                  + 2 * ((r[1][x + 1] + 2 * r[2][x + 1] + r[3][x + 1]) - (r[1][x - 1] + 2 * r[2][x - 1] + r[3][x - 1])))
+            gy = (sum(c * r[4][x + d] for d, c in zip((-2, -1, 0, 1, 2), (1, 2, 4, 2, 1)))
+                  - sum(c * r[0][x + d] for d, c in zip((-2, -1, 0, 1, 2), (1, 2, 4, 2, 1)))
+                  + 2 * ((r[3][x - 1] + 2 * r[3][x] + r[3][x + 1]) - (r[1][x - 1] + 2 * r[1][x] + r[1][x + 1])))
+            lap = (16 * r[2][x] - 2 * (r[1][x] + r[3][x] + r[2][x - 1] + r[2][x + 1])
+                   - (r[1][x - 1] + r[1][x + 1] + r[3][x - 1] + r[3][x + 1]) - (r[0][x] + r[4][x] + r[2][x - 2] + r[2][x + 2]))
+            m[y, x] = alpha * lap * lap + beta * (gx * gx + gy * gy) + 1e-9
+    m[:, :2] = m[:, 2:3]; m[:, -2:] = m[:, -3:-2]
+    m[:2] = m[2]; m[-2:] = m[-3]
+    return m ** p
+
+
+def test_lpg_brute_force_and_closed_forms():
+    from oracle import focus
+    r = rng(100)
+    g = r.uniform(0, 1, (13, 17)).astype(np.float32)
+    np.testing.assert_allclose(focus.lpg(g, 6.0, 2.0), _lpg_brute(g, 6.0, 2.0), rtol=2e-5)
+    np.testing.assert_allclose(focus.lpg(g, 3.0, 1.0), _lpg_brute(g, 3.0, 1.0), rtol=2e-5)
+    # constant image: only eps remains; a ramp a*x: no laplacian, gx = 56 a
+    assert np.all(focus.lpg(np.full((9, 9), 0.3, np.float32), 6, 1) == np.float32(1e-9))
+    yy, xx = np.mgrid[0:12, 0:15].astype(np.float32)
+    m = focus.lpg(np.float32(0.02) * xx, 6.0, 1.0)
+    np.testing.assert_allclose(m, (100 / 36) ** 2 / 7 * (56 * 0.02) ** 2, rtol=1e-5)
+
+
+def test_laplacian_pyramid_reconstructs(oracle32):
+    from oracle import focus
+    img = rng(101).uniform(0, 1, (70, 90)).astype(np.float32)
+    sizes = focus.level_sizes(90, 70)
+    assert sizes == [(90, 70), (45, 35), (23, 18), (12, 9)]  # stop when the next smaller side <= 8
+    pyr = focus.build_laplacian(oracle32, img, sizes)
+    np.testing.assert_allclose(focus.reconstruct(oracle32, pyr), img, atol=2e-6)
+    assert focus.level_sizes(90, 70, max_levels=2) == [(90, 70), (45, 35)]
+
+
+@pytest.mark.parametrize("policy", [0, 1, 2])
+def test_focus_of_identical_frames_is_the_frame(oracle32, policy):
+    from oracle import focus
+    img = rng(102).uniform(0.2, 1, (48, 64)).astype(np.float32)
+    fs = focus.FocusStack(oracle32, policy)
+    for _ in range(3):
+        fs.add([img])
+    np.testing.assert_allclose(fs.compute()[0], img, atol=1e-4)
+
+
+@pytest.mark.parametrize("policy", [0, 1, 2])
+def test_focus_picks_the_sharp_halves(oracle32, policy):
+    """A is sharp on the left and blurred on the right, B the opposite: the fused
+    image is closer to the sharp scene than either input (a min-selection or a
+    swapped weight fails this)."""
+    from oracle import focus
+    r = rng(103)
+    sharp = ndimage.gaussian_filter(r.uniform(0, 1, (64, 96)), 0.7).astype(np.float32)
+    blur = ndimage.gaussian_filter(sharp, 3.0).astype(np.float32)
+    A = sharp.copy(); A[:, 48:] = blur[:, 48:]
+    B = sharp.copy(); B[:, :48] = blur[:, :48]
+    fs = focus.FocusStack(oracle32, policy, lpg_sigma=2.0)
+    fs.add([A]); fs.add([B])
+    out = fs.compute()[0]
+    err = lambda x: float(np.sqrt(np.mean((x[8:-8, 8:-8] - sharp[8:-8, 8:-8]) ** 2)))
+    assert err(out) < 0.6 * min(err(A), err(B)), (err(out), err(A), err(B))
