@@ -248,6 +248,16 @@ int ssg_accumulation_set_focus_options(ssg_accumulation *a, const ssg_focus_stac
  * same size (host or device); sharpness (optional): the map's mean. */
 int ssg_lpg(const ssg_image *src, ssg_image *dst, double k, double p, double *sharpness);
 
+/* compute_local_variance_map (core/proc/sharpness_measure/c_local_variance_sharpness_measure.cc:193-249),
+ * the per-pixel weight of weighted-average stacking (c_image_stacking_pipeline.cc:2013-2020), for the
+ * gray channel with uscale 0 (the pipeline's settings, c_image_stacking_pipeline.h:94-99: dscale 1,
+ * kradius 1): G = morphological gradient of the pyrDown'ed frame, W = |G|_1,
+ * Q = 20^3 sum(G^4) / W (summed in double; the reference's float atomic sum varies with its thread
+ * schedule), map = G^3 20^3 + 0.05 Q resized INTER_LINEAR to the frame size.  image: SSG_32FC1 or
+ * SSG_32FC3 (BGR); map: SSG_32FC1 of the image size or NULL; q: the quality Q or NULL.  A frame
+ * without gradient gives Q = 0 and a zero map. */
+int ssg_local_variance_map(const ssg_image *image, int32_t kradius, int32_t dscale, ssg_image *map, double *q);
+
 /* c_frame_accumulation::add(src, weights) (c_frame_accumulation.cc:182-221, bayer :1169-1196):
  * weights may be NULL, a SSG_8UC1 mask (nonzero = 1) or a SSG_32FC1 weight map.  Bayer
  * sources are SSG_32FC1 or SSG_16UC1 raw frames of the accumulator size. */
@@ -257,15 +267,19 @@ int ssg_accumulation_add(ssg_accumulation *a, const ssg_image *src, const ssg_im
  * per-frame sequence custom_remap -> weights*mask/255 -> add of
  * c_image_stacking_pipeline.cc:1644-1776 for every frame at once).
  * statuses[i].ok == 0 skips frame i.  frame_weights: per-frame scalar weight
- * or NULL (1).  Frames: SSG_32FC1/SSG_32FC3 device images.
+ * or NULL (1), remapped as a constant image; weight_maps: n per-frame weight images on
+ * the frames' own grid (SSG_32FC1 device images, e.g. ssg_local_variance_map of the
+ * unaligned frames, c_image_stacking_pipeline.cc:1466-1472) remapped with the frame's
+ * interpolation and BORDER_CONSTANT 0 (:1653-1660), or NULL; the weight is then gated
+ * by the mask (:1704-1712).  Frames: SSG_32FC1/SSG_32FC3 device images.
  * Bayer accumulators: the pipeline's set_remap(current_remap) + add(raw,
  * current_mask) (:1747-1748) for every frame, frames in order: raw SSG_32FC1 or
  * SSG_16UC1 device frames, the remap of each status's transform, the base_remap
- * mask (LINEAR only), frame_weights NULL. */
+ * mask (LINEAR only), frame_weights and weight_maps NULL. */
 int ssg_accumulation_add_registered_batch(ssg_accumulation *a, int n, const ssg_image *frames,
                                           const ssg_registration_status *statuses, int32_t motion_type,
-                                          const float *frame_weights, int32_t interpolation,
-                                          int32_t border_mode, double border_value);
+                                          const float *frame_weights, const ssg_image *weight_maps,
+                                          int32_t interpolation, int32_t border_mode, double border_value);
 
 /* c_frame_accumulation::compute(avg, mask, dscale, ddepth) (c_frame_accumulation.cc:223-248):
  * avg = sum / weight (FP64 division), mask = weight > 0 (SSG_8UC1, optional).

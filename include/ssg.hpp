@@ -173,12 +173,14 @@ class c_weigthed_average : public c_frame_accumulation {
   bool add_registered_batch(const std::vector<image> &frames, const std::vector<registration_status> &st,
                             int motion_type, const std::vector<float> *frame_weights = nullptr,
                             int interpolation = SSG_INTER_LINEAR, int border_mode = SSG_BORDER_REFLECT101,
-                            double border_value = 0) {
+                            double border_value = 0, const std::vector<image> *weight_maps = nullptr) {
     if (!a_ || st.size() != frames.size()) return false;
-    std::vector<ssg_image> f(frames.begin(), frames.end());
+    std::vector<ssg_image> f(frames.begin(), frames.end()), wm;
+    if (weight_maps) wm.assign(weight_maps->begin(), weight_maps->end());
     return ssg_accumulation_add_registered_batch(a_, (int)f.size(), f.data(), st.data(), motion_type,
-                                                 frame_weights ? frame_weights->data() : nullptr, interpolation,
-                                                 border_mode, border_value) == SSG_OK;
+                                                 frame_weights ? frame_weights->data() : nullptr,
+                                                 weight_maps ? wm.data() : nullptr, interpolation, border_mode,
+                                                 border_value) == SSG_OK;
   }
 
   bool compute(image &avg, image *mask = nullptr, double dscale = 1.0) const override {

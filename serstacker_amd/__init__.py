@@ -187,11 +187,13 @@ class c_weigthed_average:
         return True
 
     def add_registered_batch(self, frames, statuses, motion_type, frame_weights=None,
-                             interpolation=_lib.INTER_LINEAR, border_mode=_lib.BORDER_REFLECT101, border_value=0.0):
+                             interpolation=_lib.INTER_LINEAR, border_mode=_lib.BORDER_REFLECT101, border_value=0.0,
+                             weight_maps=None):
         views = image_views(frames)
         wts = (C.c_float * len(frames))(*frame_weights) if frame_weights is not None else None
+        wmv = image_views(weight_maps) if weight_maps is not None else None
         rc = self._lib.ssg_accumulation_add_registered_batch(self._h, len(frames), views, statuses, motion_type, wts,
-                                                             interpolation, border_mode, border_value)
+                                                             wmv, interpolation, border_mode, border_value)
         _lib.check(rc, "add_registered_batch")
 
     def compute(self, dscale: float = 1.0):
@@ -261,7 +263,7 @@ class c_bayer_average:
     def add_registered_batch(self, frames, statuses, motion_type):
         views = image_views(frames)
         rc = self._lib.ssg_accumulation_add_registered_batch(self._h, len(views), views, statuses, motion_type, None,
-                                                             _lib.INTER_LINEAR, _lib.BORDER_REFLECT101, 0.0)
+                                                             None, _lib.INTER_LINEAR, _lib.BORDER_REFLECT101, 0.0)
         _lib.check(rc, "add_registered_batch")
 
     def compute(self):
@@ -384,6 +386,19 @@ def lpg(src, k: float = 6.0, p: float = 2.0):
     sh = C.c_double()
     _lib.check(lib.ssg_lpg(C.byref(sv), C.byref(ov), k, p, C.byref(sh)), "ssg_lpg")
     return out, sh.value
+
+
+def local_variance_map(image, kradius: int = 1, dscale: int = 1):
+    """compute_local_variance_map (c_local_variance_sharpness_measure.cc:193-249):
+    (per-pixel weight map at the image size, quality Q)."""
+    lib = _lib.load()
+    h, w = image.shape[:2]
+    out = np.zeros((h, w), np.float32)
+    iv, ov = image_view(image), _lib.host_image(out)
+    q = C.c_double()
+    _lib.check(lib.ssg_local_variance_map(C.byref(iv), kradius, dscale, C.byref(ov), C.byref(q)),
+               "ssg_local_variance_map")
+    return out, q.value
 
 
 def frame_sharpness(frames) -> np.ndarray:

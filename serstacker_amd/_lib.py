@@ -163,7 +163,8 @@ _SIGS = [
     ("ssg_accumulation_destroy", None, [C.c_void_p]),
     ("ssg_accumulation_add", C.c_int, [C.c_void_p, _P(Image), _P(Image)]),
     ("ssg_accumulation_add_registered_batch", C.c_int, [C.c_void_p, C.c_int, _P(Image), _P(RegistrationStatus),
-                                                        C.c_int32, _P(C.c_float), C.c_int32, C.c_int32, C.c_double]),
+                                                        C.c_int32, _P(C.c_float), _P(Image), C.c_int32, C.c_int32,
+                                                        C.c_double]),
     ("ssg_accumulation_compute", C.c_int, [C.c_void_p, _P(Image), _P(Image), C.c_double]),
     ("ssg_accumulation_get_acc_counters", C.c_int, [C.c_void_p, _P(Image)]),
     ("ssg_accumulation_get_counts", C.c_int, [C.c_void_p, _P(Image)]),
@@ -178,6 +179,7 @@ _SIGS = [
     ("ssg_default_focus_stacking_options", None, [_P(FocusStackingOptions)]),
     ("ssg_accumulation_set_focus_options", C.c_int, [C.c_void_p, _P(FocusStackingOptions)]),
     ("ssg_lpg", C.c_int, [_P(Image), _P(Image), C.c_double, C.c_double, _P(C.c_double)]),
+    ("ssg_local_variance_map", C.c_int, [_P(Image), C.c_int32, C.c_int32, _P(Image), _P(C.c_double)]),
     ("ssg_accumulation_set_bayer_pattern", C.c_int, [C.c_void_p, C.c_int32]),
     ("ssg_accumulation_set_remap", C.c_int, [C.c_void_p, _P(Image)]),
     ("ssg_accumulation_set_input_bpp", C.c_int, [C.c_void_p, C.c_int32]),

@@ -134,6 +134,45 @@ __device__ __forceinline__ bool cub_general(const float* __restrict__ img, int p
   return true;
 }
 
+// cv::remap of a constant image of value c with BORDER_CONSTANT 0 at (X, Y): the
+// pipeline remaps its per-frame weight image that way (c_image_stacking_pipeline.cc:1653-1660)
+template <int INTERP>
+__device__ __forceinline__ float const_sample(float c, int w, int h, int X, int Y) {
+  if (INTERP == INTER_CUBIC) {
+    const int bx = sat_short(X >> 5) - 1, by = sat_short(Y >> 5) - 1;
+    const float* cx = c_cubic_tab[X & 31];
+    const float* cy = c_cubic_tab[Y & 31];
+    if ((unsigned)bx < (unsigned)max(w - 3, 0) && (unsigned)by < (unsigned)max(h - 3, 0)) {
+      float sum = 0.f;
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        float t = c * (cy[j] * cx[0]);
+        t = fmaf(c, cy[j] * cx[1], t);
+        t = fmaf(c, cy[j] * cx[2], t);
+        t = fmaf(c, cy[j] * cx[3], t);
+        sum = j == 0 ? t : sum + t;
+      }
+      return sum;
+    }
+    if (bx >= w || bx + 4 <= 0 || by >= h || by + 4 <= 0) return 0.f;
+    float sum = 0.f;
+    for (int j = 0; j < 4; ++j) {
+      if ((unsigned)(by + j) >= (unsigned)h) continue;
+      for (int i = 0; i < 4; ++i)
+        if ((unsigned)(bx + i) < (unsigned)w) sum = fmaf(c - 0.f, cy[j] * cx[i], sum);
+    }
+    return sum;
+  }
+  const int ix = sat_short(X >> 5), iy = sat_short(Y >> 5);
+  float w0, w1, w2, w3;
+  bilinear_w(X & 31, Y & 31, w0, w1, w2, w3);
+  if ((unsigned)ix < (unsigned)(w - 1) && (unsigned)iy < (unsigned)(h - 1)) return bilinear_sum(c, c, c, c, w0, w1, w2, w3);
+  if (ix >= w || ix + 1 < 0 || iy >= h || iy + 1 < 0) return 0.f;
+  const bool x0 = (unsigned)ix < (unsigned)w, x1 = (unsigned)(ix + 1) < (unsigned)w;
+  const bool y0 = (unsigned)iy < (unsigned)h, y1 = (unsigned)(iy + 1) < (unsigned)h;
+  return bilinear_sum(x0 && y0 ? c : 0.f, x1 && y0 ? c : 0.f, x0 && y1 ? c : 0.f, x1 && y1 ? c : 0.f, w0, w1, w2, w3);
+}
+
 // remap(mask or 255, INTER_CUBIC, BORDER_CONSTANT 0) at (X, Y), 15-bit fixed point
 __device__ __forceinline__ int remap_mask_u8_cubic(const uint8_t* __restrict__ m, int pitch, int w, int h, int X,
                                                    int Y) {
@@ -171,7 +210,11 @@ __device__ __forceinline__ int mask_value(const uint8_t* m, int pitch, int w, in
 constexpr int AC_TW = 64, AC_RPT = 4, AC_WAVES = 4, AC_TH = AC_RPT * AC_WAVES;
 constexpr int AC_GROUP = 16;  // frames summed plainly before folding into the compensated pair
 
-template <int MOTION, int INTERP, int CN>
+// WMAP: per-frame weight images (frames' own grid, f32) remapped with the
+// frame's interpolation and BORDER_CONSTANT 0; else a per-frame scalar weight,
+// remapped as a constant image.  Either is then gated by the mask (weights *
+// mask / 255, c_image_stacking_pipeline.cc:1704-1712, the mask being 0 or 255).
+template <int MOTION, int INTERP, int CN, bool WMAP>
 __global__ void __launch_bounds__(256) k_accumulate(AccumArgs a) {
   __shared__ uint8_t mk[AC_TH + 4][AC_TW + 4];
   __shared__ uint8_t rmk[AC_TH + 4][AC_TW];
@@ -256,9 +299,25 @@ __global__ void __launch_bounds__(256) k_accumulate(AccumArgs a) {
                                    : lin_general<CN>(img, a.src_pitch, a.sw, a.sh, X, Y, a.border, a.cval, v);
       }
       if (!ok) continue;
+      float wp;
+      if (WMAP) {
+        const float* __restrict__ wimg = a.wmaps[k];
+        float wv[1];
+        if (interior) {
+          if (INTERP == INTER_CUBIC) cub_fast<1>(wimg, a.wmap_pitch, X, Y, wv);
+          else lin_fast<1>(wimg, a.wmap_pitch, X, Y, wv);
+        } else if (INTERP == INTER_CUBIC) {
+          cub_general<1>(wimg, a.wmap_pitch, a.sw, a.sh, X, Y, BORDER_CONSTANT, 0.f, wv);
+        } else {
+          lin_general<1>(wimg, a.wmap_pitch, a.sw, a.sh, X, Y, BORDER_CONSTANT, 0.f, wv);
+        }
+        wp = wv[0];
+      } else {
+        wp = const_sample<INTERP>(wk, a.sw, a.sh, X, Y);
+      }
 #pragma unroll
-      for (int c = 0; c < CN; ++c) sg[r][c] = fmaf(wk, v[c], sg[r][c]);
-      wg[r] += wk;
+      for (int c = 0; c < CN; ++c) sg[r][c] = fmaf(wp, v[c], sg[r][c]);
+      wg[r] += wp;
       cnt[r] += 1;
     }
     if (!interior) __syncthreads();
@@ -307,7 +366,11 @@ hipError_t launch_accumulate(int motion, const AccumArgs& a, hipStream_t s) {
   const int nch = a.nchunks;
   dim3 grid((a.dw + AC_TW - 1) / AC_TW, (a.dh + AC_TH - 1) / AC_TH, nch);
   const bool cubic = a.interp == INTER_CUBIC;
-#define SSG_ACC(M, I, C) hipLaunchKernelGGL((k_accumulate<M, I, C>), grid, dim3(256), 0, s, a)
+#define SSG_ACC(M, I, C)                                                                        \
+  do {                                                                                          \
+    if (a.wmaps) hipLaunchKernelGGL((k_accumulate<M, I, C, true>), grid, dim3(256), 0, s, a);   \
+    else hipLaunchKernelGGL((k_accumulate<M, I, C, false>), grid, dim3(256), 0, s, a);          \
+  } while (0)
 #define SSG_ACC_IC(M)                                   \
   do {                                                  \
     if (cubic) {                                        \

@@ -1463,6 +1463,96 @@ int focus_compute(ssg_accumulation* a) {
 }
 }  // namespace
 
+extern "C" int ssg_local_variance_map(const ssg_image* image, int32_t kradius, int32_t dscale, ssg_image* map,
+                                      double* q) {
+  CHK(check_image(image, "image"));
+  if (image->type != SSG_32FC1 && image->type != SSG_32FC3) return fail(SSG_E_INVALID_ARG, "image must be SSG_32FC1/3");
+  if (map) {
+    CHK(check_image(map, "map"));
+    if (map->type != SSG_32FC1 || map->width != image->width || map->height != image->height)
+      return fail(SSG_E_INVALID_ARG, "map must be SSG_32FC1 of the image size");
+  }
+  if (dscale < 0 || dscale > 16) return fail(SSG_E_INVALID_ARG, "dscale %d", dscale);
+  const int w = image->width, h = image->height;
+  hipStream_t s = nullptr;
+  HIPCHK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+  struct StreamGuard {
+    hipStream_t s;
+    ~StreamGuard() {
+      (void)hipStreamSynchronize(s);
+      (void)hipStreamDestroy(s);
+    }
+  } guard{s};
+  DevBuf ts, a, b, part, out;
+  const void* sp;
+  size_t spb;
+  CHK(device_view(image, ts, s, &sp, &spb));
+  const size_t n = (size_t)w * h;
+  CHK(a.ensure(sizeof(float) * n));
+  CHK(b.ensure(sizeof(float) * n));
+  // gray, dense
+  if (image->type == SSG_32FC3) {
+    HIPCHK(launch_bgr2gray((const float*)sp, (int)(spb / 4), w, h, a.as<float>(), s));
+  } else {
+    HIPCHK(hipMemcpy2DAsync(a.p, (size_t)w * 4, sp, spb, (size_t)w * 4, h, hipMemcpyDeviceToDevice, s));
+  }
+  // pdownscale (:30-52): pyrDown while the image keeps a side >= 4
+  int cw = w, ch = h;
+  float* cur = a.as<float>();
+  float* oth = b.as<float>();
+  if (dscale > 0 && std::min(cw, ch) >= 4) {
+    for (int l = 0; l < dscale; ++l) {
+      const int nw = (cw + 1) / 2, nh = (ch + 1) / 2;
+      HIPCHK(launch_pyrdown(cur, 0, cw, cw, ch, oth, 0, nw, nw, nh, 1, SSG_BORDER_REFLECT101, s));
+      std::swap(cur, oth);
+      cw = nw;
+      ch = nh;
+      if (std::min(cw, ch) < 4) break;
+    }
+  }
+  const size_t nl = (size_t)cw * ch;
+  HIPCHK(launch_morph_grad(cur, cw, ch, std::max(1, (int)kradius), oth, s));
+  float* G = oth;
+  const int nb = lvm_sum_blocks(nl);
+  CHK(part.ensure(sizeof(double) * 2 * nb));
+  HIPCHK(launch_lvm_sums(G, nl, part.as<double>(), s));
+  std::vector<double> hp(2 * nb);
+  HIPCHK(hipMemcpyAsync(hp.data(), part.p, sizeof(double) * 2 * nb, hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  double W = 0, S4 = 0;
+  for (int i = 0; i < nb; ++i) {
+    W += hp[2 * i];
+    S4 += hp[2 * i + 1];
+  }
+  const double ds = 20.0;  // 20 * maxval(CV_32F) / maxval(CV_32F)
+  const double Q = W > 0 ? ds * ds * ds / W * S4 : 0.0;
+  if (q) *q = Q;
+  if (!map) return SSG_OK;
+  if (!(W > 0)) {
+    HIPCHK(hipMemset2DAsync(map->data, (size_t)map->step, 0, (size_t)w * 4, h, s));
+    HIPCHK(hipStreamSynchronize(s));
+    return SSG_OK;
+  }
+  HIPCHK(launch_lvm_map(G, nl, (float)(ds * ds * ds), (float)(0.05 * Q), cur, s));
+  float* dptr = (float*)map->data;
+  size_t dpb = (size_t)map->step;
+  if (!map->on_device || (cw == w && ch == h)) {
+    CHK(out.ensure(sizeof(float) * n));
+    dptr = out.as<float>();
+    dpb = (size_t)w * 4;
+  }
+  if (cw != w || ch != h) {
+    HIPCHK(launch_resize_linear(cur, cw, ch, dptr, (int)(dpb / 4), w, h, s));
+  } else {
+    HIPCHK(hipMemcpyAsync(dptr, cur, sizeof(float) * n, hipMemcpyDeviceToDevice, s));
+  }
+  if (dptr != map->data)
+    HIPCHK(hipMemcpy2DAsync(map->data, (size_t)map->step, dptr, dpb, (size_t)w * 4, h,
+                            map->on_device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  return SSG_OK;
+}
+
 extern "C" int ssg_lpg(const ssg_image* src, ssg_image* dst, double k, double p, double* sharpness) {
   CHK(check_image(src, "src"));
   CHK(check_image(dst, "dst"));
@@ -1601,8 +1691,9 @@ int bayer_batch(ssg_accumulation* a, int n, const ssg_image* frames, const ssg_r
 
 extern "C" int ssg_accumulation_add_registered_batch(ssg_accumulation* a, int n, const ssg_image* frames,
                                                      const ssg_registration_status* statuses, int32_t motion_type,
-                                                     const float* frame_weights, int32_t interpolation,
-                                                     int32_t border_mode, double border_value) {
+                                                     const float* frame_weights, const ssg_image* weight_maps,
+                                                     int32_t interpolation, int32_t border_mode,
+                                                     double border_value) {
   if (!a) return fail(SSG_E_INVALID_ARG, "null handle");
   if (n <= 0) return SSG_OK;
   if (!frames || !statuses) return fail(SSG_E_INVALID_ARG, "null frames or statuses");
@@ -1610,7 +1701,11 @@ extern "C" int ssg_accumulation_add_registered_batch(ssg_accumulation* a, int n,
     return fail(SSG_E_UNSUPPORTED, "interpolation %d", interpolation);
   if (motion_type < SSG_MOTION_TRANSLATION || motion_type > SSG_MOTION_AFFINE)
     return fail(SSG_E_INVALID_ARG, "motion_type %d", motion_type);
-  if (a->bayer()) return bayer_batch(a, n, frames, statuses, motion_type, frame_weights, interpolation);
+  if (a->bayer()) {
+    if (weight_maps) return fail(SSG_E_INVALID_ARG, "bayer batches take the base_remap mask only (weight_maps NULL)");
+    return bayer_batch(a, n, frames, statuses, motion_type, frame_weights, interpolation);
+  }
+  if (frame_weights && weight_maps) return fail(SSG_E_INVALID_ARG, "pass frame_weights or weight_maps, not both");
   if (a->focus()) return fail(SSG_E_UNSUPPORTED, "focus stacking adds frames one by one (ssg_accumulation_add)");
   const int sw = frames[0].width, sh = frames[0].height;
   const int64_t step = frames[0].step;
@@ -1621,6 +1716,14 @@ extern "C" int ssg_accumulation_add_registered_batch(ssg_accumulation* a, int n,
       return fail(SSG_E_INVALID_ARG, "frame %d: all frames of a batch share size, step and type", f);
     if (channels_of(frames[f].type) != a->cn) return fail(SSG_E_INVALID_ARG, "frame %d: channel count", f);
     if (step % 4) return fail(SSG_E_INVALID_ARG, "frame step must be a multiple of 4");
+  }
+  if (weight_maps) {
+    for (int f = 0; f < n; ++f) {
+      CHK(check_image(&weight_maps[f], "weight map"));
+      if (!weight_maps[f].on_device || weight_maps[f].type != SSG_32FC1 || weight_maps[f].width != sw ||
+          weight_maps[f].height != sh || weight_maps[f].step != weight_maps[0].step || weight_maps[0].step % 4)
+        return fail(SSG_E_INVALID_ARG, "weight map %d: SSG_32FC1 device images of the frame size and one step", f);
+    }
   }
   hipStream_t s = a->stream;
   std::vector<MapCoef> maps(n);
@@ -1639,7 +1742,13 @@ extern "C" int ssg_accumulation_add_registered_batch(ssg_accumulation* a, int n,
   CHK(a->maps.ensure(sizeof(MapCoef) * n));
   CHK(a->weights.ensure(sizeof(float) * n));
   CHK(a->valid.ensure(sizeof(int) * n));
-  CHK(a->frame_ptrs.ensure(sizeof(void*) * n));
+  CHK(a->frame_ptrs.ensure(sizeof(void*) * n * (weight_maps ? 2 : 1)));
+  if (weight_maps) {
+    std::vector<const float*> wp(n);
+    for (int f = 0; f < n; ++f) wp[f] = (const float*)weight_maps[f].data;
+    HIPCHK(hipMemcpyAsync(a->frame_ptrs.as<const float*>() + n, wp.data(), sizeof(void*) * n, hipMemcpyHostToDevice,
+                          s));
+  }
   HIPCHK(hipMemcpyAsync(a->maps.p, maps.data(), sizeof(MapCoef) * n, hipMemcpyHostToDevice, s));
   HIPCHK(hipMemcpyAsync(a->weights.p, w.data(), sizeof(float) * n, hipMemcpyHostToDevice, s));
   HIPCHK(hipMemcpyAsync(a->valid.p, valid.data(), sizeof(int) * n, hipMemcpyHostToDevice, s));
@@ -1660,6 +1769,8 @@ extern "C" int ssg_accumulation_add_registered_batch(ssg_accumulation* a, int n,
   g.cn = a->cn;
   g.maps = a->maps.as<MapCoef>();
   g.weights = a->weights.as<float>();
+  g.wmaps = weight_maps ? a->frame_ptrs.as<const float*>() + n : nullptr;
+  g.wmap_pitch = weight_maps ? (int)(weight_maps[0].step / 4) : 0;
   g.valid = a->valid.as<int>();
   g.nframes = n;
   g.per_chunk = per;

@@ -5,6 +5,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <algorithm>
+
 #include "ssg_device.hpp"
 
 namespace ssg {
@@ -123,7 +125,9 @@ struct AccumArgs {
   const float* const* frames;  // n device pointers
   int src_pitch, sw, sh, cn;   // all frames share the geometry
   const MapCoef* maps;         // per frame
-  const float* weights;        // per frame scalar weight
+  const float* weights;        // per frame scalar weight (when wmaps is null)
+  const float* const* wmaps;   // per frame weight image on the frame grid (f32) or null
+  int wmap_pitch;              // elements per row of the weight images
   const int* valid;            // per frame 0/1
   int nframes, per_chunk;      // frames [c*per_chunk, min((c+1)*per_chunk, nframes)) go to chunk c
   int dw, dh;                  // accumulator size
@@ -238,6 +242,15 @@ hipError_t launch_energy(const float* lap, float* w, size_t n, int cn, int avgc,
 hipError_t launch_mulw(float* lap, const float* w, int wc, size_t n, int cn, hipStream_t s);
 hipError_t launch_blend(float* a, const float* b, float alpha, float beta, size_t n, hipStream_t s);
 hipError_t launch_plane_stdev(const float* planes, size_t n, int nplanes, double* out, hipStream_t s);
+
+// local-variance sharpness map (ssg_lvm.hip)
+hipError_t launch_bgr2gray(const float* src, int pitch, int w, int h, float* g, hipStream_t s);
+hipError_t launch_morph_grad(const float* m, int w, int h, int r, float* g, hipStream_t s);
+int lvm_sum_blocks(size_t n);
+hipError_t launch_lvm_sums(const float* g, size_t n, double* partial, hipStream_t s);
+hipError_t launch_lvm_map(const float* g, size_t n, float map_scale, float add, float* m, hipStream_t s);
+hipError_t launch_resize_linear(const float* src, int sw, int sh, float* dst, int dst_pitch, int dw, int dh,
+                                hipStream_t s);
 
 // FFT phase correlation (ssg_fft.hip)
 hipError_t launch_fft2(float2* buf, float2* tmp, int M, int N, int nframes, const float2* tw_m, const float2* tw_n,

@@ -686,3 +686,38 @@ def test_focus_picks_the_sharp_halves(oracle32, policy):
     out = fs.compute()[0]
     err = lambda x: float(np.sqrt(np.mean((x[8:-8, 8:-8] - sharp[8:-8, 8:-8]) ** 2)))
     assert err(out) < 0.6 * min(err(A), err(B)), (err(out), err(A), err(B))
+
+
+# ---- local-variance sharpness map (oracle/lvm.py) ---------------------------------------------------
+def test_lvm_morph_gradient_matches_scipy():
+    from oracle import lvm
+    m = rng(110).normal(size=(21, 30)).astype(np.float32)
+    for r in (1, 2):
+        k = 2 * r + 1
+        want = ndimage.grey_dilation(m, size=(k, k), mode="nearest") - ndimage.grey_erosion(m, size=(k, k), mode="nearest")
+        np.testing.assert_array_equal(lvm.morph_gradient(m, r), want)
+
+
+def test_lvm_resize_linear_closed_forms():
+    from oracle import lvm
+    # constants stay constant; a ramp is reproduced at the half-pixel-centre positions
+    np.testing.assert_array_equal(lvm.resize_linear(np.full((7, 9), 2.5, np.float32), 17, 13), 2.5)
+    src = np.tile(np.arange(9, dtype=np.float32) * 2, (7, 1))
+    out = lvm.resize_linear(src, 18, 7)
+    xs = (np.arange(18) + 0.5) * 0.5 - 0.5
+    want = 2 * np.clip(xs, 0, 8)
+    np.testing.assert_allclose(out[3], want, atol=1e-6)
+
+
+def test_lvm_closed_forms(oracle32):
+    from oracle import lvm
+    # a flat frame has no gradient: Q = 0 and a zero map
+    m, q = lvm.local_variance_map(oracle32, np.full((32, 40), 0.4, np.float32))
+    assert q == 0 and not m.any()
+    # a vertical step: G is the step height on the 2 columns around the edge (3x3 gradient)
+    img = np.zeros((24, 32), np.float32); img[:, 16:] = 0.5
+    m, q = lvm.local_variance_map(oracle32, img, kradius=1, dscale=0)
+    G = np.zeros_like(img); G[:, 15:17] = 0.5
+    W = float(np.abs(G).sum())
+    assert abs(q - 8000.0 / W * float((G.astype(np.float64) ** 4).sum())) < 1e-9 * q
+    np.testing.assert_allclose(m, G ** 3 * 8000 + np.float32(0.05 * q), rtol=1e-6)
