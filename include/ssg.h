@@ -324,12 +324,60 @@ int ssg_phasecorr_set_reference(ssg_phasecorr *pc, const ssg_image *ref);
  * out[3i+2] = correlation peak (1 for an exact circular shift). */
 int ssg_phasecorr_estimate(ssg_phasecorr *pc, int n, const ssg_image *frames, double *out);
 
+/* ---- dense ECC flow: c_eccflow (core/proc/image_registration/ecc2.h:548-662, ecc2.cc:2103-2865).
+ * Coarse-to-fine smooth optical flow: per pyramid level and iteration the current level is
+ * warped by the flow, the residual products It*Ix, It*Iy are averaged over cells of
+ * 2^support_scale px (INTER_AREA + 3x3 gaussian), a 2x2 system is solved per cell and the
+ * update is upsampled (INTER_CUBIC) onto the flow. ----------------------------------------- */
+/* ECCFlowDownscaleMethod, ecc2.h:507-511 */
+enum { SSG_ECCFLOW_DOWNSCALE_RECURSIVE_RESIZE = 0, SSG_ECCFLOW_DOWNSCALE_FULL_RESIZE = 1,
+       SSG_ECCFLOW_DOWNSCALE_PYRAMID = 2 };
+/* c_eccflow_options, ecc2.h:515-527 (same fields, order and defaults) */
+typedef struct ssg_eccflow_options {
+  double input_smooth_sigma;     /* carried, unused (as in c_eccflow) */
+  double reference_smooth_sigma; /* carried, unused (as in c_eccflow) */
+  double update_multiplier;
+  double scale_factor;           /* RecursiveResize / FullResize level step */
+  double noise_level;            /* < 0: 1e-3 */
+  int32_t max_iterations;
+  int32_t support_scale;
+  int32_t min_image_size;
+  int32_t max_pyramid_level;     /* < 0: no limit */
+  int32_t downscale;             /* SSG_ECCFLOW_DOWNSCALE_* */
+} ssg_eccflow_options;
+typedef struct ssg_eccflow ssg_eccflow;
+void ssg_default_eccflow_options(ssg_eccflow_options *o);
+int ssg_eccflow_create(const ssg_eccflow_options *o, ssg_eccflow **out);
+void ssg_eccflow_destroy(ssg_eccflow *f);
+/* c_eccflow::set_reference_image (ecc2.cc:2489-2672).  image: SSG_32FC1, mask: SSG_8UC1 or
+ * NULL; host or device. */
+int ssg_eccflow_set_reference_image(ssg_eccflow *f, const ssg_image *image, const ssg_image *mask);
+/* Pyramid geometry of the reference (c_eccflow::current_pyramid, ecc2.cc:2215): the number of
+ * levels, and a level's size. */
+int ssg_eccflow_num_levels(const ssg_eccflow *f, int32_t *n);
+int ssg_eccflow_level_size(const ssg_eccflow *f, int32_t level, int32_t *width, int32_t *height);
+/* Reference pyramid entry planes (pyramid_entry, ecc2.h:555-560): which = 0 image, 1 Ix, 2 Iy
+ * (level size), 3 a00, 4 a01, 5 a11, 6 idet (the D cells, cell size).  out: SSG_32FC1 of that
+ * size, host or device. */
+int ssg_eccflow_reference_plane(const ssg_eccflow *f, int32_t level, int32_t which, ssg_image *out);
+/* c_eccflow::compute(input, rmap, mask) (ecc2.cc:2856-2865): input SSG_32FC1 at the reference
+ * size; input_mask SSG_8UC1 or NULL; rmap_in the initial remap (SSG_32FC2 at the reference
+ * size) or NULL for the zero flow; rmap_out receives identity + flow (SSG_32FC2, may be the
+ * same memory as rmap_in).  Host or device images. */
+int ssg_eccflow_compute(ssg_eccflow *f, const ssg_image *input, const ssg_image *input_mask,
+                        const ssg_image *rmap_in, ssg_image *rmap_out);
+/* Batched compute over n device frames of the reference size (no reference counterpart):
+ * input_masks and rmaps_in are NULL or n entries; rmaps_out has n device SSG_32FC2 maps. */
+int ssg_eccflow_compute_batch(ssg_eccflow *f, int n, const ssg_image *inputs, const ssg_image *input_masks,
+                              const ssg_image *rmaps_in, ssg_image *rmaps_out);
+
 /* ---- kernel timing (no reference counterpart; the reference's INSTRUMENT_REGION timers,
  * core/debug.h:183-188, are its closest analogue).  When enabled, every launch of a hot
  * kernel is bracketed by HIP events on the handle's stream; algorithmic bytes follow
  * the per-pixel minimum traffic of each kernel. ------------------------------------------ */
 enum { SSG_PROF_ECC_TRANSLATION = 0, SSG_PROF_ECC_MAIN = 1, SSG_PROF_CORRELATION = 2, SSG_PROF_PYRAMID = 3,
-       SSG_PROF_ACCUMULATE = 4, SSG_PROF_ECC_SOLVE = 5, SSG_PROF_FFT = 6, SSG_PROF_COUNT = 8 };
+       SSG_PROF_ACCUMULATE = 4, SSG_PROF_ECC_SOLVE = 5, SSG_PROF_FFT = 6, SSG_PROF_ECCFLOW = 7,
+       SSG_PROF_COUNT = 8 };
 typedef struct ssg_profile {
   int64_t launches[SSG_PROF_COUNT];
   double ms[SSG_PROF_COUNT];     /* summed kernel durations */
@@ -341,6 +389,8 @@ int ssg_accumulation_set_profiling(ssg_accumulation *a, int enable);
 int ssg_accumulation_get_profile(ssg_accumulation *a, ssg_profile *out, int reset);
 int ssg_phasecorr_set_profiling(ssg_phasecorr *pc, int enable);
 int ssg_phasecorr_get_profile(ssg_phasecorr *pc, ssg_profile *out, int reset);
+int ssg_eccflow_set_profiling(ssg_eccflow *f, int enable);
+int ssg_eccflow_get_profile(ssg_eccflow *f, ssg_profile *out, int reset);
 
 /* ---- synthetic planetary-disk frames (benchmark and test inputs; no reference counterpart) -- */
 typedef struct ssg_synth_params {

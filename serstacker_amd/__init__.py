@@ -27,7 +27,7 @@ from ._lib import (  # noqa: F401
 log = logging.getLogger("serstacker_amd")
 
 __all__ = ["c_image_registration_options", "c_frame_registration", "c_weigthed_average", "image_view",
-           "synth_params", "synth_render", "SsgError"]
+           "c_eccflow", "synth_params", "synth_render", "SsgError"]
 
 
 def _is_torch(x) -> bool:
@@ -332,6 +332,93 @@ class phase_correlation:
             st[i].params[0], st[i].params[1] = float(tx), float(ty)
             st[i].matrix[:] = [1.0, 0.0, float(np.float32(tx)), 0.0, 1.0, float(np.float32(ty))]
         return st
+
+
+class c_eccflow:
+    """c_eccflow (core/proc/image_registration/ecc2.h:548-662): dense coarse-to-fine ECC flow.
+    Keyword options are the fields of c_eccflow_options (ecc2.h:515-527); compute() returns
+    the refined remap identity + flow (H, W, 2)."""
+
+    def __init__(self, **opts):
+        self._lib = _lib.load()
+        o = _lib.EccflowOptions()
+        self._lib.ssg_default_eccflow_options(C.byref(o))
+        for k, v in opts.items():
+            if not hasattr(o, k):
+                raise AttributeError(f"c_eccflow_options has no field {k!r}")
+            setattr(o, k, v)
+        self.options = o
+        h = C.c_void_p()
+        _lib.check(self._lib.ssg_eccflow_create(C.byref(o), C.byref(h)), "ssg_eccflow_create")
+        self._h = h
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            self._lib.ssg_eccflow_destroy(h)
+            self._h = None
+
+    def set_reference_image(self, image, mask=None):
+        v = image_view(image)
+        m = image_view(mask)
+        _lib.check(self._lib.ssg_eccflow_set_reference_image(self._h, C.byref(v), C.byref(m) if m else None),
+                   "set_reference_image")
+
+    def num_levels(self) -> int:
+        n = C.c_int32()
+        _lib.check(self._lib.ssg_eccflow_num_levels(self._h, C.byref(n)), "num_levels")
+        return n.value
+
+    def level_size(self, level: int):
+        w, h = C.c_int32(), C.c_int32()
+        _lib.check(self._lib.ssg_eccflow_level_size(self._h, level, C.byref(w), C.byref(h)), "level_size")
+        return w.value, h.value
+
+    PLANES = ("image", "Ix", "Iy", "a00", "a01", "a11", "idet")
+
+    def reference_plane(self, level: int, which: str) -> np.ndarray:
+        """A plane of the reference pyramid entry: image, Ix, Iy (level size) or a00, a01, a11,
+        idet (the D cells)."""
+        k = self.PLANES.index(which)
+        w, h = self.level_size(level)
+        if k >= 3:
+            for _ in range(self.options.support_scale):
+                w, h = (w + 1) // 2, (h + 1) // 2
+        out = np.zeros((h, w), np.float32)
+        v = _lib.host_image(out)
+        _lib.check(self._lib.ssg_eccflow_reference_plane(self._h, level, k, C.byref(v)), "reference_plane")
+        return out
+
+    def compute(self, image, rmap=None, mask=None, out=None):
+        """c_eccflow::compute(input, rmap, mask): the refined remap.  out: a device tensor
+        (H, W, 2) to write into, else a host array is returned."""
+        v = image_view(image)
+        m = image_view(mask)
+        r = image_view(rmap)
+        if out is None:
+            res = np.zeros((v.height, v.width, 2), np.float32)
+            o = _lib.host_image(res)
+        else:
+            res, o = out, image_view(out)
+        _lib.check(self._lib.ssg_eccflow_compute(self._h, C.byref(v), C.byref(m) if m else None,
+                                                 C.byref(r) if r else None, C.byref(o)), "compute")
+        return res
+
+    def compute_batch(self, frames, outs, rmaps=None, masks=None):
+        """Batched compute over device frames into device maps outs[i] (H, W, 2)."""
+        fv = image_views(frames)
+        ov = image_views(outs)
+        rv = image_views(rmaps) if rmaps is not None else None
+        mv = image_views(masks) if masks is not None else None
+        _lib.check(self._lib.ssg_eccflow_compute_batch(self._h, len(fv), fv, mv, rv, ov), "compute_batch")
+
+    def set_profiling(self, enable: bool = True):
+        _lib.check(self._lib.ssg_eccflow_set_profiling(self._h, int(enable)), "set_profiling")
+
+    def profile(self, reset: bool = False) -> _lib.Profile:
+        p = _lib.Profile()
+        _lib.check(self._lib.ssg_eccflow_get_profile(self._h, C.byref(p), int(reset)), "get_profile")
+        return p
 
 
 class c_laplacian_pyramid_focus_stacking:

@@ -110,7 +110,23 @@ class RegistrationStatus(C.Structure):
 
 
 PROF_ECC_TRANSLATION, PROF_ECC_MAIN, PROF_CORRELATION, PROF_PYRAMID, PROF_ACCUMULATE = 0, 1, 2, 3, 4
-PROF_ECC_SOLVE, PROF_FFT = 5, 6
+PROF_ECC_SOLVE, PROF_FFT, PROF_ECCFLOW = 5, 6, 7
+ECCFLOW_DOWNSCALE_RECURSIVE_RESIZE, ECCFLOW_DOWNSCALE_FULL_RESIZE, ECCFLOW_DOWNSCALE_PYRAMID = 0, 1, 2
+
+
+class EccflowOptions(C.Structure):
+    _fields_ = [
+        ("input_smooth_sigma", C.c_double),
+        ("reference_smooth_sigma", C.c_double),
+        ("update_multiplier", C.c_double),
+        ("scale_factor", C.c_double),
+        ("noise_level", C.c_double),
+        ("max_iterations", C.c_int32),
+        ("support_scale", C.c_int32),
+        ("min_image_size", C.c_int32),
+        ("max_pyramid_level", C.c_int32),
+        ("downscale", C.c_int32),
+    ]
 
 
 class FocusStackingOptions(C.Structure):
@@ -188,6 +204,17 @@ _SIGS = [
     ("ssg_registration_get_profile", C.c_int, [C.c_void_p, _P(Profile), C.c_int]),
     ("ssg_accumulation_set_profiling", C.c_int, [C.c_void_p, C.c_int]),
     ("ssg_accumulation_get_profile", C.c_int, [C.c_void_p, _P(Profile), C.c_int]),
+    ("ssg_default_eccflow_options", None, [_P(EccflowOptions)]),
+    ("ssg_eccflow_create", C.c_int, [_P(EccflowOptions), _P(C.c_void_p)]),
+    ("ssg_eccflow_destroy", None, [C.c_void_p]),
+    ("ssg_eccflow_set_reference_image", C.c_int, [C.c_void_p, _P(Image), _P(Image)]),
+    ("ssg_eccflow_num_levels", C.c_int, [C.c_void_p, _P(C.c_int32)]),
+    ("ssg_eccflow_level_size", C.c_int, [C.c_void_p, C.c_int32, _P(C.c_int32), _P(C.c_int32)]),
+    ("ssg_eccflow_reference_plane", C.c_int, [C.c_void_p, C.c_int32, C.c_int32, _P(Image)]),
+    ("ssg_eccflow_compute", C.c_int, [C.c_void_p, _P(Image), _P(Image), _P(Image), _P(Image)]),
+    ("ssg_eccflow_compute_batch", C.c_int, [C.c_void_p, C.c_int, _P(Image), _P(Image), _P(Image), _P(Image)]),
+    ("ssg_eccflow_set_profiling", C.c_int, [C.c_void_p, C.c_int]),
+    ("ssg_eccflow_get_profile", C.c_int, [C.c_void_p, _P(Profile), C.c_int]),
     ("ssg_synth_render", C.c_int, [_P(SynthParams), C.c_int, C.c_int, _P(Image), _P(C.c_double)]),
 ]
 

@@ -2138,3 +2138,620 @@ extern "C" int ssg_frame_sharpness(int n, const ssg_image* frames, float* weight
   }
   return SSG_OK;
 }
+
+// =====================================================================================================
+// dense ECC flow: c_eccflow (core/proc/image_registration/ecc2.cc:2103-2865)
+// =====================================================================================================
+namespace {
+// Geometry tables, packed into one device buffer per handle.
+struct TabBuild {
+  std::vector<char> bytes;
+  template <class T>
+  size_t put(const std::vector<T>& v) {
+    const size_t off = (bytes.size() + 15) / 16 * 16;
+    bytes.resize(off + sizeof(T) * std::max<size_t>(1, v.size()));
+    if (!v.empty()) memcpy(bytes.data() + off, v.data(), sizeof(T) * v.size());
+    return off;
+  }
+};
+
+// cv::resize INTER_AREA geometry (resize.cpp: is_area_fast, computeResizeAreaTab) in double
+struct AreaHost {
+  AreaGeom g{};
+  size_t o[6] = {0, 0, 0, 0, 0, 0};  // xoff, xsi, xal, yoff, ysi, yal
+};
+
+bool area_fast_axis(int ssize, int dsize, int* iscale) {
+  const double scale = 1.0 / ((double)dsize / ssize);
+  const int is = (int)std::lrint(scale);
+  *iscale = is;
+  return std::abs(scale - is) < 2.220446049250313e-16;
+}
+
+void area_axis(int ssize, int dsize, std::vector<int>& off, std::vector<int>& si, std::vector<float>& al) {
+  const double scale = 1.0 / ((double)dsize / ssize);
+  off.assign(dsize + 1, 0);
+  for (int dx = 0; dx < dsize; ++dx) {
+    off[dx] = (int)si.size();
+    const double fsx1 = dx * scale, fsx2 = fsx1 + scale;
+    const double cell = std::min(scale, ssize - fsx1);
+    int sx1 = (int)std::ceil(fsx1), sx2 = (int)std::floor(fsx2);
+    sx2 = std::min(sx2, ssize - 1);
+    sx1 = std::min(sx1, sx2);
+    if (sx1 - fsx1 > 1e-3) {
+      si.push_back(sx1 - 1);
+      al.push_back((float)((sx1 - fsx1) / cell));
+    }
+    for (int sx = sx1; sx < sx2; ++sx) {
+      si.push_back(sx);
+      al.push_back((float)(1.0 / cell));
+    }
+    if (fsx2 - sx2 > 1e-3) {
+      si.push_back(sx2);
+      al.push_back((float)(std::min(std::min(fsx2 - sx2, 1.), cell) / cell));
+    }
+  }
+  off[dsize] = (int)si.size();
+}
+
+AreaHost area_host(int sw, int sh, int dw, int dh, TabBuild& tb) {
+  AreaHost a;
+  a.g.sw = sw;
+  a.g.sh = sh;
+  a.g.dw = dw;
+  a.g.dh = dh;
+  const bool fx = area_fast_axis(sw, dw, &a.g.isx), fy = area_fast_axis(sh, dh, &a.g.isy);
+  a.g.fast = fx && fy;
+  if (!a.g.fast) {
+    std::vector<int> xo, xs, yo, ys;
+    std::vector<float> xa, ya;
+    area_axis(sw, dw, xo, xs, xa);
+    area_axis(sh, dh, yo, ys, ya);
+    a.o[0] = tb.put(xo);
+    a.o[1] = tb.put(xs);
+    a.o[2] = tb.put(xa);
+    a.o[3] = tb.put(yo);
+    a.o[4] = tb.put(ys);
+    a.o[5] = tb.put(ya);
+  }
+  return a;
+}
+
+void area_bind(AreaHost& a, const char* base) {
+  if (a.g.fast) return;
+  a.g.xoff = (const int*)(base + a.o[0]);
+  a.g.xsi = (const int*)(base + a.o[1]);
+  a.g.xal = (const float*)(base + a.o[2]);
+  a.g.yoff = (const int*)(base + a.o[3]);
+  a.g.ysi = (const int*)(base + a.o[4]);
+  a.g.yal = (const float*)(base + a.o[5]);
+}
+
+// cv::resize INTER_CUBIC axis: floor of fx = (float)((d+0.5) scale - 0.5) and
+// interpolateCubic(fx - floor) (resize.cpp)
+struct CubicHost {
+  int sw, sh, dw, dh;
+  size_t o[4];  // xs, xa, ys, ya
+};
+
+void cubic_axis(int ssize, int dsize, std::vector<int>& s, std::vector<float4>& c) {
+  const double scale = 1.0 / ((double)dsize / ssize);
+  for (int d = 0; d < dsize; ++d) {
+    float fx = (float)((d + 0.5) * scale - 0.5);
+    const int sx = (int)std::floor(fx);
+    fx -= (float)sx;
+    float k[4];
+    cubic_coeffs(fx, k);
+    s.push_back(sx);
+    c.push_back(make_float4(k[0], k[1], k[2], k[3]));
+  }
+}
+
+CubicHost cubic_host(int sw, int sh, int dw, int dh, TabBuild& tb) {
+  CubicHost h{sw, sh, dw, dh, {0, 0, 0, 0}};
+  std::vector<int> xs, ys;
+  std::vector<float4> xa, ya;
+  cubic_axis(sw, dw, xs, xa);
+  cubic_axis(sh, dh, ys, ya);
+  h.o[0] = tb.put(xs);
+  h.o[1] = tb.put(xa);
+  h.o[2] = tb.put(ys);
+  h.o[3] = tb.put(ya);
+  return h;
+}
+
+FlowCubicArgs cubic_args(const CubicHost& h, const char* base) {
+  FlowCubicArgs a{};
+  a.sw = h.sw;
+  a.sh = h.sh;
+  a.dw = h.dw;
+  a.dh = h.dh;
+  a.xs = (const int*)(base + h.o[0]);
+  a.xa = (const float4*)(base + h.o[1]);
+  a.ys = (const int*)(base + h.o[2]);
+  a.ya = (const float4*)(base + h.o[3]);
+  a.rx = a.ry = 1.0;
+  return a;
+}
+
+// ecc_differentiate (ecc2.cc:142-169): d5 = [1 -8 0 8 -1]/12 along one axis, [1 2 1]/4 along the other
+SepKernel diff_sep(bool along_x) {
+  const float d5[5] = {1.f / 12.f, -2.f / 3.f, 0.f, 2.f / 3.f, -1.f / 12.f};
+  const float s3[3] = {0.25f, 0.5f, 0.25f};
+  SepKernel K{};
+  float* kd = along_x ? K.kx : K.ky;
+  float* ks = along_x ? K.ky : K.kx;
+  for (int i = 0; i < 5; ++i) kd[i] = d5[i];
+  for (int i = 0; i < 3; ++i) ks[i] = s3[i];
+  if (along_x) {
+    K.nx = 5, K.tx = 2, K.ny = 3, K.ty = 1;
+  } else {
+    K.nx = 3, K.tx = 1, K.ny = 5, K.ty = 2;
+  }
+  K.delta = 0.f;
+  return K;
+}
+}  // namespace
+
+struct ssg_eccflow {
+  ssg_eccflow_options o{};
+  hipStream_t stream = nullptr;
+  int L = 0, w0 = 0, h0 = 0;
+  std::vector<int> lw, lh, cw, ch, src;  // level and cell sizes, source level of each level
+  std::vector<size_t> loff, coff;        // element offsets of the levels / cell grids when packed
+  size_t lsum = 0, csum = 0;
+  bool have_ref = false, ref_mask = false;
+  DevBuf ref, rmask, Ix, Iy, D, tabs, ref_tmp, ref_ptr;
+  std::vector<AreaHost> cell_geom, down_geom;
+  std::vector<CubicHost> up_tab, lvl_tab;  // cells -> level; level l+1 -> level l
+  CubicHost init_tab{};                   // level 0 -> coarsest
+  int chunk = 1;
+  DevBuf cur, cmask, uvA, uvB, itxy, cells, cuv, ptrs;
+  Profiler prof;
+  ~ssg_eccflow() {
+    if (stream) {
+      (void)hipStreamSynchronize(stream);
+      (void)hipStreamDestroy(stream);
+    }
+  }
+};
+
+extern "C" void ssg_default_eccflow_options(ssg_eccflow_options* o) {
+  if (!o) return;
+  memset(o, 0, sizeof(*o));
+  o->input_smooth_sigma = 0;
+  o->reference_smooth_sigma = 0;
+  o->update_multiplier = 1.5;
+  o->scale_factor = 0.5;
+  o->noise_level = -1;
+  o->max_iterations = 1;
+  o->support_scale = 5;
+  o->min_image_size = 4;
+  o->max_pyramid_level = -1;
+  o->downscale = SSG_ECCFLOW_DOWNSCALE_RECURSIVE_RESIZE;
+}
+
+extern "C" int ssg_eccflow_create(const ssg_eccflow_options* o, ssg_eccflow** out) {
+  if (!out) return fail(SSG_E_INVALID_ARG, "out is null");
+  *out = nullptr;
+  auto f = std::make_unique<ssg_eccflow>();
+  if (o) f->o = *o;
+  else ssg_default_eccflow_options(&f->o);
+  const auto& op = f->o;
+  if (op.downscale < SSG_ECCFLOW_DOWNSCALE_RECURSIVE_RESIZE || op.downscale > SSG_ECCFLOW_DOWNSCALE_PYRAMID)
+    return fail(SSG_E_INVALID_ARG, "unknown downscale method %d", op.downscale);
+  if (op.support_scale < 0 || op.support_scale > 12) return fail(SSG_E_INVALID_ARG, "support_scale %d", op.support_scale);
+  if (op.max_iterations < 0) return fail(SSG_E_INVALID_ARG, "max_iterations %d", op.max_iterations);
+  if (op.downscale != SSG_ECCFLOW_DOWNSCALE_PYRAMID && !(op.scale_factor > 0 && op.scale_factor < 1))
+    return fail(SSG_E_INVALID_ARG, "scale_factor must be in (0, 1)");
+  HIPCHK(hipStreamCreateWithFlags(&f->stream, hipStreamNonBlocking));
+  *out = f.release();
+  return SSG_OK;
+}
+
+extern "C" void ssg_eccflow_destroy(ssg_eccflow* f) { delete f; }
+
+namespace {
+// Level sizes of c_eccflow::set_reference_image (ecc2.cc:2530-2595)
+void eccflow_levels(ssg_eccflow* f, int w0, int h0) {
+  const auto& o = f->o;
+  const int min_size = std::max(4, o.min_image_size);
+  const bool big_aspect = std::max(w0, h0) / std::min(w0, h0) >= 2;
+  f->lw.assign(1, w0);
+  f->lh.assign(1, h0);
+  f->src.assign(1, -1);
+  for (int level = 0;; ++level) {
+    if (o.max_pyramid_level >= 0 && level >= o.max_pyramid_level) break;
+    const int pw = f->lw.back(), ph = f->lh.back();
+    int nw, nh, s;
+    if (o.downscale == SSG_ECCFLOW_DOWNSCALE_PYRAMID) {
+      nw = std::max(o.min_image_size, (pw + 1) / 2);
+      nh = std::max(o.min_image_size, (ph + 1) / 2);
+      if ((nw == pw && nh == ph) || std::min(nw, nh) <= min_size) break;
+      s = level;
+    } else {
+      nw = std::max(o.min_image_size, (int)((pw + 1) * o.scale_factor));
+      nh = std::max(o.min_image_size, (int)((ph + 1) * o.scale_factor));
+      if ((nw == pw && nh == ph) || std::max(nw, nh) <= min_size) break;
+      s = (o.downscale == SSG_ECCFLOW_DOWNSCALE_FULL_RESIZE || (big_aspect && std::min(nw, nh) <= min_size + 1)) ? 0
+                                                                                                                 : level;
+    }
+    f->lw.push_back(nw);
+    f->lh.push_back(nh);
+    f->src.push_back(s);
+  }
+  f->L = (int)f->lw.size();
+  f->cw.resize(f->L);
+  f->ch.resize(f->L);
+  f->loff.resize(f->L);
+  f->coff.resize(f->L);
+  f->lsum = f->csum = 0;
+  for (int l = 0; l < f->L; ++l) {
+    int cw = f->lw[l], ch = f->lh[l];
+    for (int i = 0; i < o.support_scale; ++i) cw = (cw + 1) / 2, ch = (ch + 1) / 2;
+    f->cw[l] = cw;
+    f->ch[l] = ch;
+    f->loff[l] = f->lsum;
+    f->coff[l] = f->csum;
+    f->lsum += (size_t)f->lw[l] * f->lh[l];
+    f->csum += (size_t)cw * ch;
+  }
+}
+
+// level l of frame-strided pyramids from its source level (pyrDown or INTER_AREA; masks nearest)
+int eccflow_downscale(ssg_eccflow* f, float* pyr, long long fs, uint8_t* mpyr, int l, int n) {
+  hipStream_t s = f->stream;
+  const int sl = f->src[l];
+  const int sw = f->lw[sl], sh = f->lh[sl], dw = f->lw[l], dh = f->lh[l];
+  if (f->o.downscale == SSG_ECCFLOW_DOWNSCALE_PYRAMID)
+    HIPCHK(launch_pyrdown(pyr + f->loff[sl], fs, sw, sw, sh, pyr + f->loff[l], fs, dw, dw, dh, n, BORDER_REFLECT101, s));
+  else
+    HIPCHK(launch_area(1, pyr + f->loff[sl], fs, sw, pyr + f->loff[l], fs, dw, f->down_geom[l].g, n, s));
+  if (mpyr) HIPCHK(launch_nearest_u8(mpyr + f->loff[sl], fs, sw, sh, mpyr + f->loff[l], fs, dw, dh, n, s));
+  return SSG_OK;
+}
+}  // namespace
+
+extern "C" int ssg_eccflow_set_reference_image(ssg_eccflow* f, const ssg_image* image, const ssg_image* mask) {
+  if (!f) return fail(SSG_E_INVALID_ARG, "null handle");
+  CHK(check_image(image, "reference image"));
+  if (image->type != SSG_32FC1) return fail(SSG_E_INVALID_ARG, "reference image: SSG_32FC1 expected");
+  if (mask) {
+    CHK(check_image(mask, "reference mask"));
+    if (mask->type != SSG_8UC1 || mask->width != image->width || mask->height != image->height)
+      return fail(SSG_E_INVALID_ARG, "reference mask: SSG_8UC1 of the image size expected");
+  }
+  f->have_ref = false;
+  hipStream_t s = f->stream;
+  const int w0 = image->width, h0 = image->height;
+  eccflow_levels(f, w0, h0);
+  f->w0 = w0;
+  f->h0 = h0;
+  // tables
+  TabBuild tb;
+  f->cell_geom.clear();
+  f->down_geom.clear();
+  f->up_tab.clear();
+  f->lvl_tab.clear();
+  for (int l = 0; l < f->L; ++l) {
+    f->cell_geom.push_back(area_host(f->lw[l], f->lh[l], f->cw[l], f->ch[l], tb));
+    f->down_geom.push_back(l > 0 && f->o.downscale != SSG_ECCFLOW_DOWNSCALE_PYRAMID
+                               ? area_host(f->lw[f->src[l]], f->lh[f->src[l]], f->lw[l], f->lh[l], tb)
+                               : AreaHost{});
+    f->up_tab.push_back(cubic_host(f->cw[l], f->ch[l], f->lw[l], f->lh[l], tb));
+    f->lvl_tab.push_back(l + 1 < f->L ? cubic_host(f->lw[l + 1], f->lh[l + 1], f->lw[l], f->lh[l], tb) : CubicHost{});
+  }
+  f->init_tab = cubic_host(w0, h0, f->lw[f->L - 1], f->lh[f->L - 1], tb);
+  CHK(f->tabs.ensure(tb.bytes.size()));
+  HIPCHK(hipMemcpyAsync(f->tabs.p, tb.bytes.data(), tb.bytes.size(), hipMemcpyHostToDevice, s));
+  const char* base = (const char*)f->tabs.p;
+  for (auto& g : f->cell_geom) area_bind(g, base);
+  for (auto& g : f->down_geom) area_bind(g, base);
+  // reference pyramid
+  f->ref_mask = mask != nullptr;
+  CHK(f->ref.ensure(sizeof(float) * f->lsum));
+  CHK(f->Ix.ensure(sizeof(float) * f->lsum));
+  CHK(f->Iy.ensure(sizeof(float) * f->lsum));
+  CHK(f->D.ensure(sizeof(float4) * f->csum));
+  if (mask) CHK(f->rmask.ensure(f->lsum));
+  {
+    DevBuf t;
+    const void* p;
+    size_t pb;
+    CHK(device_view(image, t, s, &p, &pb));
+    HIPCHK(hipMemcpy2DAsync(f->ref.p, sizeof(float) * w0, p, pb, sizeof(float) * w0, h0, hipMemcpyDeviceToDevice, s));
+    if (mask) {
+      DevBuf tm;
+      CHK(device_view(mask, tm, s, &p, &pb));
+      HIPCHK(hipMemcpy2DAsync(f->rmask.p, w0, p, pb, w0, h0, hipMemcpyDeviceToDevice, s));
+    }
+    HIPCHK(hipStreamSynchronize(s));
+  }
+  for (int l = 1; l < f->L; ++l)
+    CHK(eccflow_downscale(f, f->ref.as<float>(), 0, mask ? f->rmask.as<uint8_t>() : nullptr, l, 1));
+  // per level: Ix, Iy, D
+  const size_t n0 = (size_t)w0 * h0, c0 = (size_t)f->cw[0] * f->ch[0];
+  DevBuf P2, P1, A2, A1;
+  CHK(P2.ensure(sizeof(float2) * n0));
+  CHK(P1.ensure(sizeof(float) * n0));
+  CHK(A2.ensure(sizeof(float2) * c0));
+  CHK(A1.ensure(sizeof(float) * c0));
+  CHK(f->ref_ptr.ensure(sizeof(void*) * f->L));
+  std::vector<const float*> lp(f->L);
+  for (int l = 0; l < f->L; ++l) lp[l] = f->ref.as<float>() + f->loff[l];
+  HIPCHK(hipMemcpyAsync(f->ref_ptr.p, lp.data(), sizeof(void*) * f->L, hipMemcpyHostToDevice, s));
+  const double noise = f->o.noise_level >= 0 ? f->o.noise_level : 1e-3;
+  const SepKernel Kx = diff_sep(true), Ky = diff_sep(false);
+  for (int l = 0; l < f->L; ++l) {
+    const int w = f->lw[l], h = f->lh[l];
+    const float* const* sp = f->ref_ptr.as<const float*>() + l;
+    HIPCHK(launch_sepfilter(sp, w, f->Ix.as<float>() + f->loff[l], 0, w, w, h, 1, Kx, BORDER_REPLICATE, s));
+    HIPCHK(launch_sepfilter(sp, w, f->Iy.as<float>() + f->loff[l], 0, w, w, h, 1, Ky, BORDER_REPLICATE, s));
+    HIPCHK(launch_flow_products(f->Ix.as<float>() + f->loff[l], f->Iy.as<float>() + f->loff[l], w * h,
+                                P2.as<float2>(), P1.as<float>(), s));
+    const AreaGeom& g = f->cell_geom[l].g;
+    HIPCHK(launch_area(2, P2.as<float>(), 0, 2 * w, A2.as<float>(), 0, 2 * f->cw[l], g, 1, s));
+    HIPCHK(launch_area(1, P1.as<float>(), 0, w, A1.as<float>(), 0, f->cw[l], g, 1, s));
+    const float reg = noise > 0 ? (float)std::pow(1e-5 * noise / (1 << l), 4) : 0.f;
+    HIPCHK(launch_flow_D(A2.as<float>(), A1.as<float>(), f->cw[l], f->ch[l], (float)f->o.update_multiplier, reg,
+                         f->D.as<float4>() + f->coff[l], s));
+  }
+  HIPCHK(hipStreamSynchronize(s));
+  // frames per launch: workspace of about 2 GiB at most
+  const size_t per = sizeof(float) * f->lsum + f->lsum + 3 * sizeof(float2) * n0 + 2 * sizeof(float2) * c0;
+  f->chunk = (int)std::max<size_t>(1, std::min<size_t>(64, ((size_t)2 << 30) / per));
+  f->have_ref = true;
+  return SSG_OK;
+}
+
+extern "C" int ssg_eccflow_num_levels(const ssg_eccflow* f, int32_t* n) {
+  if (!f || !n) return fail(SSG_E_INVALID_ARG, "null argument");
+  if (!f->have_ref) return fail(SSG_E_NOT_READY, "no reference image set");
+  *n = f->L;
+  return SSG_OK;
+}
+
+extern "C" int ssg_eccflow_level_size(const ssg_eccflow* f, int32_t level, int32_t* width, int32_t* height) {
+  if (!f || !width || !height) return fail(SSG_E_INVALID_ARG, "null argument");
+  if (!f->have_ref) return fail(SSG_E_NOT_READY, "no reference image set");
+  if (level < 0 || level >= f->L) return fail(SSG_E_INVALID_ARG, "level %d of %d", level, f->L);
+  *width = f->lw[level];
+  *height = f->lh[level];
+  return SSG_OK;
+}
+
+extern "C" int ssg_eccflow_reference_plane(const ssg_eccflow* f, int32_t level, int32_t which, ssg_image* out) {
+  if (!f || !out) return fail(SSG_E_INVALID_ARG, "null argument");
+  if (!f->have_ref) return fail(SSG_E_NOT_READY, "no reference image set");
+  if (level < 0 || level >= f->L) return fail(SSG_E_INVALID_ARG, "level %d of %d", level, f->L);
+  if (which < 0 || which > 6) return fail(SSG_E_INVALID_ARG, "plane %d", which);
+  CHK(check_image(out, "out"));
+  const bool cellp = which >= 3;
+  const int w = cellp ? f->cw[level] : f->lw[level], h = cellp ? f->ch[level] : f->lh[level];
+  if (out->type != SSG_32FC1 || out->width != w || out->height != h)
+    return fail(SSG_E_INVALID_ARG, "out: SSG_32FC1 of %dx%d expected", w, h);
+  const hipMemcpyKind kind = out->on_device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;
+  if (!cellp) {
+    const DevBuf& b = which == 0 ? f->ref : (which == 1 ? f->Ix : f->Iy);
+    HIPCHK(hipMemcpy2DAsync(out->data, (size_t)out->step, b.as<float>() + f->loff[level], sizeof(float) * w,
+                            sizeof(float) * w, h, kind, f->stream));
+  } else {
+    // component (which - 3) of the float4 cells, one row of w cells at a time
+    const float* p = (const float*)(f->D.as<float4>() + f->coff[level]) + (which - 3);
+    for (int y = 0; y < h; ++y)
+      HIPCHK(hipMemcpy2DAsync((char*)out->data + (size_t)y * out->step, sizeof(float), p + (size_t)y * w * 4,
+                              sizeof(float4), sizeof(float), w, kind, f->stream));
+  }
+  HIPCHK(hipStreamSynchronize(f->stream));
+  return SSG_OK;
+}
+
+namespace {
+// The level loop of c_eccflow::compute_uv (ecc2.cc:2770-2835) over n device frames given by
+// host arrays of device pointers (pitches in elements).
+int eccflow_run(ssg_eccflow* f, int n, const float* const* in, int in_pitch, const uint8_t* const* masks,
+                int mask_pitch, const float2* const* rin, int rin_pitch, float2* const* rout, int rout_pitch) {
+  hipStream_t s = f->stream;
+  const int L = f->L, w0 = f->w0, h0 = f->h0;
+  const size_t n0 = (size_t)w0 * h0, c0 = (size_t)f->cw[0] * f->ch[0];
+  const int chunk = std::min(f->chunk, n);
+  const bool hm = masks != nullptr;
+  CHK(f->cur.ensure(sizeof(float) * f->lsum * chunk));
+  if (hm) CHK(f->cmask.ensure(f->lsum * chunk));
+  CHK(f->uvA.ensure(sizeof(float2) * n0 * chunk));
+  CHK(f->uvB.ensure(sizeof(float2) * n0 * chunk));
+  CHK(f->itxy.ensure(sizeof(float2) * n0 * chunk));
+  CHK(f->cells.ensure(sizeof(float2) * c0 * chunk));
+  CHK(f->cuv.ensure(sizeof(float2) * c0 * chunk));
+  CHK(f->ptrs.ensure(sizeof(void*) * 4 * (size_t)n));
+  std::vector<const void*> hp(4 * (size_t)n, nullptr);
+  for (int i = 0; i < n; ++i) {
+    hp[i] = in[i];
+    hp[n + i] = hm ? (const void*)masks[i] : nullptr;
+    hp[2 * n + i] = rin ? (const void*)rin[i] : nullptr;
+    hp[3 * n + i] = rout[i];
+  }
+  HIPCHK(hipMemcpyAsync(f->ptrs.p, hp.data(), sizeof(void*) * hp.size(), hipMemcpyHostToDevice, s));
+  const void* const* dp = f->ptrs.as<const void*>();
+  const char* base = (const char*)f->tabs.p;
+  const long long fs = (long long)f->lsum;
+  const long long ufs = (long long)n0, cfs = (long long)c0;
+  f->prof.start(s);
+  for (int f0 = 0; f0 < n; f0 += chunk) {
+    const int m = std::min(chunk, n - f0);
+    float* cur = f->cur.as<float>();
+    uint8_t* cm = hm ? f->cmask.as<uint8_t>() : nullptr;
+    // current pyramid (setup_input_image, ecc2.cc:2674-2767)
+    HIPCHK(launch_copy_frames((const float* const*)(dp + f0), in_pitch, cur, fs, w0, w0, h0, m, s));
+    if (hm)
+      HIPCHK(launch_mask_prepare((const uint8_t* const*)(dp + n + f0), mask_pitch, cm, fs, w0, w0, h0, w0, h0, m, s));
+    for (int l = 1; l < L; ++l) CHK(eccflow_downscale(f, cur, fs, cm, l, m));
+    // initial flow at the coarsest level
+    float2* ua = f->uvA.as<float2>();
+    float2* ub = f->uvB.as<float2>();
+    if (rin) {
+      FlowCubicArgs a = cubic_args(f->init_tab, base);
+      a.maps = (const float2* const*)(dp + 2 * n + f0);
+      a.map_pitch = rin_pitch;
+      a.dst = ua;
+      a.dfs = ufs;
+      a.rx = (double)f->lw[L - 1] / (double)w0;
+      a.ry = (double)f->lh[L - 1] / (double)h0;
+      HIPCHK(launch_flow_cubic(CUBIC_INIT, a, m, s));
+    } else {
+      HIPCHK(hipMemsetAsync(ua, 0, sizeof(float2) * n0 * m, s));
+    }
+    for (int l = L - 1; l >= 0; --l) {
+      const int w = f->lw[l], h = f->lh[l];
+      if (l < L - 1) {
+        FlowCubicArgs a = cubic_args(f->lvl_tab[l], base);
+        a.src = ua;
+        a.sfs = ufs;
+        a.dst = ub;
+        a.dfs = ufs;
+        a.rx = (double)w / (double)f->lw[l + 1];
+        a.ry = (double)h / (double)f->lh[l + 1];
+        HIPCHK(launch_flow_cubic(CUBIC_SCALED, a, m, s));
+        std::swap(ua, ub);
+      }
+      FlowLevelArgs la{};
+      la.cur = cur + f->loff[l];
+      la.cfs = fs;
+      la.cmask = hm ? cm + f->loff[l] : nullptr;
+      la.mfs = fs;
+      la.ref = f->ref.as<float>() + f->loff[l];
+      la.Ix = f->Ix.as<float>() + f->loff[l];
+      la.Iy = f->Iy.as<float>() + f->loff[l];
+      la.rmask = f->ref_mask ? f->rmask.as<uint8_t>() + f->loff[l] : nullptr;
+      la.w = w;
+      la.h = h;
+      FlowCubicArgs up = cubic_args(f->up_tab[l], base);
+      up.src = f->cuv.as<float2>();
+      up.sfs = cfs;
+      up.dst = ua;
+      up.dfs = ufs;
+      for (int it = 0; it < f->o.max_iterations; ++it) {
+        HIPCHK(launch_flow_residual(la, ua, ufs, f->itxy.as<float2>(), ufs, m, s));
+        HIPCHK(launch_area(2, f->itxy.as<float>(), 2 * ufs, 2 * w, f->cells.as<float>(), 2 * cfs, 2 * f->cw[l],
+                           f->cell_geom[l].g, m, s));
+        HIPCHK(launch_flow_solve(f->cells.as<float>(), 2 * cfs, f->D.as<float4>() + f->coff[l], f->cw[l], f->ch[l],
+                                 f->cuv.as<float2>(), cfs, m, s));
+        HIPCHK(launch_flow_cubic(CUBIC_ADD, up, m, s));
+      }
+    }
+    HIPCHK(launch_flow_to_remap(ua, ufs, w0, h0, (float2* const*)(dp + 3 * n + f0), rout_pitch, m, s));
+  }
+  f->prof.stop(s, SSG_PROF_ECCFLOW);
+  if (f->prof.on) {
+    // algorithmic bytes per frame: level-0 copy (8 N0) and the downscales (4 N_src + 4 N_l),
+    // per level and iteration the fused minimum 32 B/px (uv r+w, current, reference, Ix, Iy),
+    // level transitions (8 N_{l+1} + 8 N_l), the initial flow and the final remap
+    double b = 8.0 * n0;
+    for (int l = 1; l < L; ++l) b += 4.0 * f->lw[f->src[l]] * f->lh[f->src[l]] + 4.0 * f->lw[l] * f->lh[l];
+    for (int l = 0; l < L; ++l) {
+      const double nl = (double)f->lw[l] * f->lh[l];
+      b += f->o.max_iterations * 32.0 * nl;
+      if (l < L - 1) b += 8.0 * f->lw[l + 1] * f->lh[l + 1] + 8.0 * nl;
+    }
+    b += (rin ? 8.0 * n0 : 0.0) + 8.0 * f->lw[L - 1] * f->lh[L - 1] + 16.0 * n0;
+    f->prof.add_bytes(SSG_PROF_ECCFLOW, b * n);
+  }
+  return SSG_OK;
+}
+}  // namespace
+
+extern "C" int ssg_eccflow_compute_batch(ssg_eccflow* f, int n, const ssg_image* inputs, const ssg_image* input_masks,
+                                         const ssg_image* rmaps_in, ssg_image* rmaps_out) {
+  if (!f) return fail(SSG_E_INVALID_ARG, "null handle");
+  if (n <= 0) return SSG_OK;
+  if (!inputs || !rmaps_out) return fail(SSG_E_INVALID_ARG, "null inputs or rmaps_out");
+  if (!f->have_ref) return fail(SSG_E_NOT_READY, "no reference image set");
+  auto uniform = [&](const ssg_image* im, int type, int es, const char* what) -> int {
+    for (int i = 0; i < n; ++i) {
+      CHK(check_image(&im[i], what));
+      if (!im[i].on_device || im[i].type != type || im[i].width != f->w0 || im[i].height != f->h0 ||
+          im[i].step != im[0].step || im[i].step % es)
+        return fail(SSG_E_INVALID_ARG, "%s %d: device images of the reference size, one type and step", what, i);
+    }
+    return SSG_OK;
+  };
+  CHK(uniform(inputs, SSG_32FC1, 4, "input"));
+  if (input_masks) CHK(uniform(input_masks, SSG_8UC1, 1, "input mask"));
+  if (rmaps_in) CHK(uniform(rmaps_in, SSG_32FC2, 8, "rmap_in"));
+  CHK(uniform(rmaps_out, SSG_32FC2, 8, "rmap_out"));
+  std::vector<const float*> in(n);
+  std::vector<const uint8_t*> mk(input_masks ? n : 0);
+  std::vector<const float2*> ri(rmaps_in ? n : 0);
+  std::vector<float2*> ro(n);
+  for (int i = 0; i < n; ++i) {
+    in[i] = (const float*)inputs[i].data;
+    if (input_masks) mk[i] = (const uint8_t*)input_masks[i].data;
+    if (rmaps_in) ri[i] = (const float2*)rmaps_in[i].data;
+    ro[i] = (float2*)rmaps_out[i].data;
+  }
+  CHK(eccflow_run(f, n, in.data(), (int)(inputs[0].step / 4), input_masks ? mk.data() : nullptr,
+                  input_masks ? (int)input_masks[0].step : 0, rmaps_in ? ri.data() : nullptr,
+                  rmaps_in ? (int)(rmaps_in[0].step / 8) : 0, ro.data(), (int)(rmaps_out[0].step / 8)));
+  HIPCHK(hipStreamSynchronize(f->stream));
+  f->prof.collect();
+  return SSG_OK;
+}
+
+extern "C" int ssg_eccflow_compute(ssg_eccflow* f, const ssg_image* input, const ssg_image* input_mask,
+                                   const ssg_image* rmap_in, ssg_image* rmap_out) {
+  if (!f) return fail(SSG_E_INVALID_ARG, "null handle");
+  if (!f->have_ref) return fail(SSG_E_NOT_READY, "no reference image set");
+  CHK(check_image(input, "input"));
+  CHK(check_image(rmap_out, "rmap_out"));
+  auto same = [&](const ssg_image* im, int type, const char* what) -> int {
+    CHK(check_image(im, what));
+    if (im->type != type || im->width != f->w0 || im->height != f->h0)
+      return fail(SSG_E_INVALID_ARG, "%s: type %d of the reference size %dx%d expected", what, type, f->w0, f->h0);
+    if (im->on_device && im->step % elem_size(type)) return fail(SSG_E_INVALID_ARG, "%s: step", what);
+    return SSG_OK;
+  };
+  CHK(same(input, SSG_32FC1, "input"));
+  if (input_mask) CHK(same(input_mask, SSG_8UC1, "input mask"));
+  if (rmap_in) CHK(same(rmap_in, SSG_32FC2, "rmap_in"));
+  CHK(same(rmap_out, SSG_32FC2, "rmap_out"));
+  hipStream_t s = f->stream;
+  DevBuf ti, tm, tr, to;
+  const void *pi, *pm = nullptr, *pr = nullptr;
+  size_t bi, bm = 0, br = 0;
+  CHK(device_view(input, ti, s, &pi, &bi));
+  if (input_mask) CHK(device_view(input_mask, tm, s, &pm, &bm));
+  if (rmap_in) CHK(device_view(rmap_in, tr, s, &pr, &br));
+  float2* po;
+  size_t bo;
+  if (rmap_out->on_device) {
+    po = (float2*)rmap_out->data;
+    bo = (size_t)rmap_out->step;
+  } else {
+    bo = sizeof(float2) * f->w0;
+    CHK(to.ensure(bo * f->h0));
+    po = to.as<float2>();
+  }
+  const float* in1 = (const float*)pi;
+  const uint8_t* mk1 = (const uint8_t*)pm;
+  const float2* ri1 = (const float2*)pr;
+  CHK(eccflow_run(f, 1, &in1, (int)(bi / 4), input_mask ? &mk1 : nullptr, (int)bm, rmap_in ? &ri1 : nullptr,
+                  (int)(br / 8), &po, (int)(bo / 8)));
+  if (!rmap_out->on_device)
+    HIPCHK(hipMemcpy2DAsync(rmap_out->data, (size_t)rmap_out->step, po, bo, bo, f->h0, hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  f->prof.collect();
+  return SSG_OK;
+}
+
+extern "C" int ssg_eccflow_set_profiling(ssg_eccflow* f, int enable) {
+  if (!f) return fail(SSG_E_INVALID_ARG, "null handle");
+  f->prof.on = enable != 0;
+  return SSG_OK;
+}
+
+extern "C" int ssg_eccflow_get_profile(ssg_eccflow* f, ssg_profile* out, int reset) {
+  if (!f || !out) return fail(SSG_E_INVALID_ARG, "null argument");
+  *out = f->prof.acc;
+  if (reset) f->prof.acc = ssg_profile{};
+  return SSG_OK;
+}

@@ -260,6 +260,61 @@ hipError_t launch_pc_load(const void* const* frames, bool u16, int pitch, int bw
 hipError_t launch_pc_cross(float2* spec, const float2* ref, int M, int N, int nframes, hipStream_t s);
 hipError_t launch_pc_peak(const float2* r, int M, int N, int binning, int nframes, double* out, hipStream_t s);
 
+// dense ECC flow (ssg_eccflow.hip)
+struct AreaGeom {  // one cv::resize INTER_AREA geometry, sw x sh -> dw x dh
+  int sw, sh, dw, dh;
+  int fast, isx, isy;  // whole factors on both axes (resizeAreaFast_)
+  // otherwise the computeResizeAreaTab entries of output column dx are
+  // [xoff[dx], xoff[dx+1]) of (xsi, xal); rows likewise
+  const int* xoff;
+  const int* xsi;
+  const float* xal;
+  const int* yoff;
+  const int* ysi;
+  const float* yal;
+};
+struct FlowLevelArgs {
+  const float* cur;      // current level, frame-strided (cfs), pitch w
+  long long cfs;
+  const uint8_t* cmask;  // current mask level (frame-strided, mfs) or null
+  long long mfs;
+  const float* ref;      // reference level, pitch w
+  const float* Ix;
+  const float* Iy;
+  const uint8_t* rmask;  // reference mask level or null
+  int w, h;
+};
+enum { CUBIC_ADD = 0, CUBIC_SCALED = 1, CUBIC_INIT = 2 };
+struct FlowCubicArgs {
+  const float2* src;  // source flow, frame-strided (sfs), pitch sw
+  long long sfs;
+  int sw, sh;
+  const float2* const* maps;  // CUBIC_INIT: the caller's remaps (pitch map_pitch)
+  int map_pitch;
+  float2* dst;  // output, frame-strided (dfs), pitch dw
+  long long dfs;
+  int dw, dh;
+  const int* xs;  // per output column: floor of the source position; per output row likewise
+  const float4* xa;
+  const int* ys;
+  const float4* ya;
+  double rx, ry;
+};
+hipError_t launch_area(int cn, const float* src, long long sfs, int spitch, float* dst, long long dfs, int dpitch,
+                       const AreaGeom& g, int nframes, hipStream_t s);
+hipError_t launch_flow_products(const float* Ix, const float* Iy, int n, float2* P2, float* P1, hipStream_t s);
+hipError_t launch_flow_D(const float* A2, const float* A1, int cw, int ch, float um, float reg, float4* D,
+                         hipStream_t s);
+hipError_t launch_flow_residual(const FlowLevelArgs& a, const float2* uv, long long ufs, float2* itxy, long long ifs,
+                                int nframes, hipStream_t s);
+hipError_t launch_flow_solve(const float* cells, long long cfs, const float4* D, int cw, int ch, float2* cuv,
+                             long long ufs, int nframes, hipStream_t s);
+hipError_t launch_flow_cubic(int mode, const FlowCubicArgs& a, int nframes, hipStream_t s);
+hipError_t launch_flow_to_remap(const float2* uv, long long ufs, int w, int h, float2* const* maps, int pitch,
+                                int nframes, hipStream_t s);
+hipError_t launch_nearest_u8(const uint8_t* src, long long sfs, int sw, int sh, uint8_t* dst, long long dfs, int dw,
+                             int dh, int nframes, hipStream_t s);
+
 // synthetic frames (ssg_synth.hip)
 struct SynthFrame {
   double A[6];  // scene coordinate = A * (x, y, 1): maps frame pixels to the scene

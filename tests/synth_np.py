@@ -94,9 +94,14 @@ def blob_scene(w: int, h: int, seed: int, n: int = 60, smin: float = 3.0, smax: 
         yy, xx = np.mgrid[0:h, 0:w].astype(np.float64)
         u = T[0, 0] * xx + T[0, 1] * yy + T[0, 2]
         v = T[1, 0] * xx + T[1, 1] * yy + T[1, 2]
-        out = np.zeros((h, w))
+        return at(u, v)
+
+    def at(u, v):
+        """The scene at arbitrary float64 coordinates (arrays u, v)."""
+        out = np.zeros(np.shape(u))
         for x, y, s, a in zip(xs, ys, ss, aa):
             out += a * np.exp(-((u - x) ** 2 + (v - y) ** 2) / (2 * s * s))
         return out
 
+    render.at = at
     return render

@@ -1,0 +1,181 @@
+"""GPU dense ECC flow (c_eccflow, ecc2.cc:2103-2865) against its FP32 oracle
+(oracle/eccflow.py) through the C ABI: the reference pyramid planes (level images,
+Ix, Iy and the D cells) bit for bit, and the refined remaps of whole compute() calls
+for every downscale method, with and without an initial remap and masks, single
+and batched."""
+import numpy as np
+import pytest
+
+from tests import synth_np
+
+pytestmark = pytest.mark.gpu
+
+# The GPU runs the oracle's canonical arithmetic: planes and refined remaps are
+# bit-exact (a zero bound, in px).
+FLOW_ATOL = 0.0
+
+
+@pytest.fixture(scope="module")
+def ssg():
+    import torch  # noqa: F401
+
+    import serstacker_amd as S
+
+    S._lib.load()
+    return S
+
+
+def gpu(a):
+    import torch
+
+    return torch.from_numpy(np.ascontiguousarray(a)).cuda()
+
+
+def scene(w, h, seed):
+    return synth_np.blob_scene(w, h, seed, n=int(w * h / 50), smin=3.0, smax=6.0)
+
+
+def frames_pair(w, h, seed, shift=(1.3, -0.8), amp=1.0):
+    S = scene(w, h, seed)
+    yy, xx = np.mgrid[0:h, 0:w].astype(np.float64)
+    r = np.random.default_rng(seed)
+    p = r.uniform(0, 2 * np.pi, 2)
+    ex = shift[0] + amp * np.sin(2 * np.pi * xx / (1.3 * w) + p[0])
+    ey = shift[1] + amp * np.cos(2 * np.pi * yy / (1.1 * h) + p[1])
+    ref = S([1, 0, 0, 0, 1, 0]).astype(np.float32)
+    cur = S.at(xx - ex, yy - ey).astype(np.float32)
+    return ref, cur
+
+
+OPTS = {
+    "pyramid": dict(downscale=2, support_scale=3, max_iterations=3, min_image_size=-1, max_pyramid_level=-1),
+    "pyramid_ss2": dict(downscale=2, support_scale=2, max_iterations=2, min_image_size=-1, max_pyramid_level=3),
+    "recursive": dict(downscale=0, scale_factor=0.75, support_scale=3, max_iterations=2, min_image_size=8),
+    "full": dict(downscale=1, scale_factor=0.5, support_scale=2, max_iterations=2, min_image_size=4),
+}
+
+
+def make_pair(ssg, oracle32, name):
+    from oracle import eccflow as EF
+
+    o = OPTS[name]
+    g = ssg.c_eccflow(**o)
+    f = EF.EccFlow(oracle32, **o)
+    return g, f
+
+
+@pytest.mark.parametrize("name", list(OPTS))
+@pytest.mark.parametrize("shape", [(160, 128), (173, 97)])
+def test_reference_pyramid_is_bit_exact(ssg, oracle32, name, shape):
+    w, h = shape
+    ref, _ = frames_pair(w, h, 11)
+    r = np.random.default_rng(3)
+    mask = (r.uniform(0, 1, (h, w)) > 0.1).astype(np.uint8) * 255
+    g, f = make_pair(ssg, oracle32, name)
+    g.set_reference_image(gpu(ref), gpu(mask))
+    f.set_reference_image(ref, mask)
+    assert g.num_levels() == len(f.levels)
+    for lvl, e in enumerate(f.levels):
+        assert g.level_size(lvl) == f.sizes[lvl]
+        np.testing.assert_array_equal(g.reference_plane(lvl, "image"), e["ref"], err_msg=f"level {lvl} image")
+        for k in ("Ix", "Iy", "a00", "a01", "a11", "idet"):
+            np.testing.assert_array_equal(g.reference_plane(lvl, k), e[k], err_msg=f"level {lvl} {k}")
+
+
+@pytest.mark.parametrize("name", list(OPTS))
+@pytest.mark.parametrize("init", [False, True])
+def test_compute_matches_oracle(ssg, oracle32, name, init):
+    w, h = 160, 128
+    ref, cur = frames_pair(w, h, 21)
+    g, f = make_pair(ssg, oracle32, name)
+    g.set_reference_image(gpu(ref))
+    f.set_reference_image(ref)
+    rmap = None
+    if init:
+        yy, xx = np.mgrid[0:h, 0:w].astype(np.float32)
+        rmap = np.stack([xx + np.float32(1.1), yy - np.float32(0.6)], -1)
+    want = f.compute(cur, rmap)
+    got = g.compute(gpu(cur), None if rmap is None else gpu(rmap))
+    d = np.abs(got - want)
+    print(f"{name} init={init}: max |d| {d.max():.3g} px, exact {np.mean(d == 0):.4f}")
+    assert d.max() <= FLOW_ATOL
+
+
+def test_compute_with_masks_matches_oracle(ssg, oracle32):
+    w, h = 160, 128
+    ref, cur = frames_pair(w, h, 31)
+    r = np.random.default_rng(4)
+    rmask = np.full((h, w), 255, np.uint8)
+    rmask[40:60, 30:90] = 0
+    cmask = (r.uniform(0, 1, (h, w)) > 0.05).astype(np.uint8) * 255
+    g, f = make_pair(ssg, oracle32, "pyramid")
+    g.set_reference_image(gpu(ref), gpu(rmask))
+    f.set_reference_image(ref, rmask)
+    want = f.compute(cur, None, cmask)
+    got = g.compute(gpu(cur), None, gpu(cmask))
+    assert np.abs(got - want).max() <= FLOW_ATOL
+
+
+def test_host_images_and_in_place_rmap(ssg, oracle32):
+    w, h = 96, 80
+    ref, cur = frames_pair(w, h, 41)
+    g, f = make_pair(ssg, oracle32, "pyramid_ss2")
+    g.set_reference_image(ref)
+    f.set_reference_image(ref)
+    yy, xx = np.mgrid[0:h, 0:w].astype(np.float32)
+    rmap = np.stack([xx + np.float32(0.5), yy], -1)
+    want = f.compute(cur, rmap)
+    got = g.compute(cur, rmap)  # host views
+    assert np.abs(got - want).max() <= FLOW_ATOL
+    dev = gpu(rmap)
+    g.compute(gpu(cur), dev, out=dev)  # rmap_out aliasing rmap_in
+    assert np.abs(dev.cpu().numpy() - want).max() <= FLOW_ATOL
+
+
+def test_batch_equals_single_calls(ssg, oracle32):
+    import torch
+
+    w, h, n = 160, 128, 5
+    ref, _ = frames_pair(w, h, 51)
+    curs = [frames_pair(w, h, 51, shift=(0.3 * i, -0.2 * i))[1] for i in range(n)]
+    g, _ = make_pair(ssg, oracle32, "pyramid")
+    g.set_reference_image(gpu(ref))
+    yy, xx = np.mgrid[0:h, 0:w].astype(np.float32)
+    rmaps = [gpu(np.stack([xx + np.float32(0.1 * i), yy], -1)) for i in range(n)]
+    outs = [torch.zeros((h, w, 2), dtype=torch.float32, device="cuda") for _ in range(n)]
+    dev = [gpu(c) for c in curs]
+    g.compute_batch(dev, outs, rmaps=rmaps)
+    for i in range(n):
+        single = g.compute(dev[i], rmaps[i])
+        np.testing.assert_array_equal(outs[i].cpu().numpy(), single)
+
+
+def test_single_level_and_zero_support(ssg, oracle32):
+    from oracle import eccflow as EF
+
+    w, h = 64, 48
+    ref, cur = frames_pair(w, h, 61, shift=(0.4, 0.2), amp=0.2)
+    o = dict(downscale=2, support_scale=0, max_iterations=2, max_pyramid_level=0)
+    g = ssg.c_eccflow(**o)
+    f = EF.EccFlow(oracle32, **o)
+    g.set_reference_image(gpu(ref))
+    f.set_reference_image(ref)
+    assert g.num_levels() == 1
+    yy, xx = np.mgrid[0:h, 0:w].astype(np.float32)
+    rmap = np.stack([xx + np.float32(0.3), yy + np.float32(0.1)], -1)
+    assert np.abs(g.compute(gpu(cur), gpu(rmap)) - f.compute(cur, rmap)).max() <= FLOW_ATOL
+
+
+def test_argument_errors(ssg):
+    import torch
+
+    g = ssg.c_eccflow(downscale=2, support_scale=2)
+    with pytest.raises(ssg.SsgError):
+        g.compute(gpu(np.zeros((32, 32), np.float32)))  # no reference yet
+    g.set_reference_image(gpu(np.zeros((32, 32), np.float32)))
+    with pytest.raises(ssg.SsgError):
+        g.compute(gpu(np.zeros((32, 40), np.float32)))  # size mismatch
+    with pytest.raises(ssg.SsgError):
+        ssg.c_eccflow(downscale=7)
+    with pytest.raises(ssg.SsgError):
+        g.compute_batch([gpu(np.zeros((32, 32), np.float32))], [torch.zeros((32, 32), device="cuda")])
