@@ -182,11 +182,30 @@ int ssg_registration_register_frame(ssg_registration *r, const ssg_image *image,
 int ssg_registration_register_batch(ssg_registration *r, int n, const ssg_image *frames,
                                     const ssg_image *masks, ssg_registration_status *status);
 
-/* c_frame_registration::current_remap / c_image_transform::create_remap
- * (c_image_transform.h:54): writes the SSG_32FC2 remap field of a status's
- * transform at the reference frame size. */
+/* c_image_transform::create_remap (c_image_transform.h:54): writes the SSG_32FC2
+ * remap field of a status's transform at the reference frame size. */
 int ssg_registration_create_remap(ssg_registration *r, const ssg_registration_status *status,
                                   ssg_image *map);
+
+/* c_frame_registration::current_remap (c_frame_registration.h:292): the remap of the last frame
+ * passed to ssg_registration_register_frame, at the reference size (SSG_32FC2, host or device).
+ * With enable_eccflow_registration it is the ECC transform's map refined by the dense ECC flow
+ * (c_frame_registration.cc:883-917), the field c_frame_registration::remap warps with. */
+int ssg_registration_current_remap(ssg_registration *r, ssg_image *map);
+
+/* c_frame_registration::remap (c_frame_registration.cc:1405-1417): custom_remap of src (and
+ * src_mask) with the current remap of the last frame passed to ssg_registration_register_frame;
+ * interpolation / border_mode < 0 take the options' values (and border_value[0]). */
+int ssg_registration_remap(ssg_registration *r, const ssg_image *src, const ssg_image *src_mask, ssg_image *dst,
+                           ssg_image *dst_mask, int32_t interpolation, int32_t border_mode, double border_value);
+
+/* Batched register_frame that also returns each frame's current remap (no reference
+ * counterpart): remaps[i] (device SSG_32FC2, reference size, one step) receives the frame's
+ * transform map, refined by the dense ECC flow for the accepted frames when
+ * enable_eccflow_registration is set (then frames and masks are device images of one step). */
+int ssg_registration_register_batch_remaps(ssg_registration *r, int n, const ssg_image *frames,
+                                           const ssg_image *masks, ssg_registration_status *status,
+                                           ssg_image *remaps);
 
 /* c_frame_registration::custom_remap / base_remap (c_frame_registration.cc:1265-1417):
  * dst = remap(src, map) with the given interpolation/border; dst_mask (optional)

@@ -92,21 +92,31 @@ class c_frame_registration {
   }
 
   // c_frame_registration::current_remap (c_frame_registration.h:292): the remap
-  // field of the last registered frame at the reference size (SSG_32FC2).
+  // field of the last registered frame at the reference size (SSG_32FC2), refined
+  // by the dense ECC flow when enable_eccflow_registration is set.
   bool current_remap(image &map) const {
     if (!h_) return fail_state();
-    return ssg_registration_create_remap(h_, &status_, &map) == SSG_OK;
+    return ssg_registration_current_remap(h_, &map) == SSG_OK;
   }
 
-  // c_frame_registration::remap (c_frame_registration.cc:1265-1300): warps with the
-  // last registration; unknown interpolation/border (-1) take the options' values.
+  // Batched register_frame that also writes each frame's remap (device SSG_32FC2 maps).
+  bool register_batch_remaps(const std::vector<image> &frames, std::vector<registration_status> &out,
+                             std::vector<image> &remaps, const std::vector<image> *masks = nullptr) {
+    if (!h_ || remaps.size() != frames.size()) return fail_state();
+    out.resize(frames.size());
+    std::vector<ssg_image> f(frames.begin(), frames.end()), r(remaps.begin(), remaps.end()), m;
+    if (masks) m.assign(masks->begin(), masks->end());
+    return ssg_registration_register_batch_remaps(h_, (int)f.size(), f.data(), masks ? m.data() : nullptr,
+                                                  out.data(), r.data()) == SSG_OK;
+  }
+
+  // c_frame_registration::remap (c_frame_registration.cc:1405-1417): warps with the
+  // current remap; unknown interpolation/border (-1) take the options' values.
   bool remap(const image &src, image &dst, const image &src_mask = image(), image *dst_mask = nullptr,
              int interpolation = -1, int border_mode = -1, double border_value = 0) const {
-    if (!status_.ok) return fail_state();
-    return ssg_remap(&src, src_mask.empty() ? nullptr : &src_mask, nullptr, status_.matrix, opts_.motion_type,
-                     status_.params, &dst, dst_mask, interpolation < 0 ? opts_.interpolation : interpolation,
-                     border_mode < 0 ? opts_.border_mode : border_mode,
-                     border_mode < 0 ? opts_.border_value[0] : border_value) == SSG_OK;
+    if (!h_) return fail_state();
+    return ssg_registration_remap(h_, &src, src_mask.empty() ? nullptr : &src_mask, &dst, dst_mask, interpolation,
+                                  border_mode, border_value) == SSG_OK;
   }
 
   // c_frame_registration::custom_remap (c_frame_registration.cc:1302-1417)

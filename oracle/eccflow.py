@@ -53,44 +53,50 @@ RECURSIVE_RESIZE, FULL_RESIZE, PYRAMID = 0, 1, 2  # ECCFlowDownscaleMethod (ecc2
 # --------------------------------------------------------------------------------------------
 # cv::resize INTER_CUBIC
 # --------------------------------------------------------------------------------------------
-def cubic_coeffs(x):
-    """interpolateCubic (imgproc resize.cpp) in float, vectorised over x."""
-    x = np.asarray(x, F32)
-    A = F32(-0.75)
-    one = F32(1)
+def _dt(a):
+    """Working type of an image: float64 for the FP64 oracle, float32 otherwise."""
+    return np.float64 if np.asarray(a).dtype == np.float64 else F32
+
+
+def cubic_coeffs(x, dt=F32):
+    """interpolateCubic (imgproc resize.cpp) in float (or double), vectorised over x."""
+    x = np.asarray(x, dt)
+    A = dt(-0.75)
+    one = dt(1)
     x1 = x + one
-    c0 = ((A * x1 - F32(5) * A) * x1 + F32(8) * A) * x1 - F32(4) * A
-    c1 = ((A + F32(2)) * x - (A + F32(3))) * x * x + one
+    c0 = ((A * x1 - dt(5) * A) * x1 + dt(8) * A) * x1 - dt(4) * A
+    c1 = ((A + dt(2)) * x - (A + dt(3))) * x * x + one
     xr = one - x
-    c2 = ((A + F32(2)) * xr - (A + F32(3))) * xr * xr + one
+    c2 = ((A + dt(2)) * xr - (A + dt(3))) * xr * xr + one
     c3 = one - c0 - c1 - c2
     return c0, c1, c2, c3
 
 
-def cubic_axis(ssize, dsize):
-    """Source taps (clamped) and float coefficients of one axis."""
+def cubic_axis(ssize, dsize, dt=F32):
+    """Source taps (clamped) and coefficients of one axis."""
     scale = 1.0 / (dsize / ssize)
-    f = np.array([(d + 0.5) * scale - 0.5 for d in range(dsize)], np.float64).astype(F32)
+    f = np.array([(d + 0.5) * scale - 0.5 for d in range(dsize)], np.float64).astype(dt)
     s = np.floor(f).astype(np.int64)
-    f = (f - s.astype(F32)).astype(F32)
+    f = (f - s.astype(dt)).astype(dt)
     taps = np.clip(s[:, None] + np.arange(-1, 3)[None, :], 0, ssize - 1)
-    return taps, np.stack(cubic_coeffs(f), 1)
+    return taps, np.stack(cubic_coeffs(f, dt), 1)
 
 
 def _cubic_1ch(img, dw, dh):
     h, w = img.shape
-    xt, a = cubic_axis(w, dw)
-    yt, b = cubic_axis(h, dh)
+    dt = img.dtype.type
+    xt, a = cubic_axis(w, dw, dt)
+    yt, b = cubic_axis(h, dh, dt)
     rows = ((img[:, xt[:, 0]] * a[:, 0] + img[:, xt[:, 1]] * a[:, 1]) + img[:, xt[:, 2]] * a[:, 2]) \
         + img[:, xt[:, 3]] * a[:, 3]
     out = ((rows[yt[:, 0]] * b[:, 0:1] + rows[yt[:, 1]] * b[:, 1:2]) + rows[yt[:, 2]] * b[:, 2:3]) \
         + rows[yt[:, 3]] * b[:, 3:4]
-    return out.astype(F32)
+    return out.astype(dt)
 
 
 def resize_cubic(img, dw, dh):
     """cv::resize(img, (dw, dh), INTER_CUBIC) of a float image with 1 or more channels."""
-    img = np.asarray(img, F32)
+    img = np.asarray(img, _dt(img))
     if img.shape[1] == dw and img.shape[0] == dh:
         return img.copy()
     if img.ndim == 2:
@@ -109,7 +115,8 @@ def area_is_fast(ssize, dsize):
 
 
 def area_tab(ssize, dsize):
-    """computeResizeAreaTab: list of (di, si, alpha) in the reference order."""
+    """computeResizeAreaTab: list of (di, si, alpha) in the reference order (alpha in
+    double; the float path rounds it to float as the reference table does)."""
     scale = 1.0 / (dsize / ssize)
     tab = []
     for dx in range(dsize):
@@ -120,11 +127,11 @@ def area_tab(ssize, dsize):
         sx2 = min(sx2, ssize - 1)
         sx1 = min(sx1, sx2)
         if sx1 - fsx1 > 1e-3:
-            tab.append((dx, sx1 - 1, F32((sx1 - fsx1) / cell)))
+            tab.append((dx, sx1 - 1, (sx1 - fsx1) / cell))
         for sx in range(sx1, sx2):
-            tab.append((dx, sx, F32(1.0 / cell)))
+            tab.append((dx, sx, 1.0 / cell))
         if fsx2 - sx2 > 1e-3:
-            tab.append((dx, sx2, F32(min(min(fsx2 - sx2, 1.0), cell) / cell)))
+            tab.append((dx, sx2, min(min(fsx2 - sx2, 1.0), cell) / cell))
     return tab
 
 
@@ -135,7 +142,7 @@ def _padded(tab, dsize):
         per[di].append((si, al))
     m = max(len(p) for p in per)
     idx = np.zeros((dsize, m), np.int64)
-    alp = np.zeros((dsize, m), F32)
+    alp = np.zeros((dsize, m), np.float64)
     for d, p in enumerate(per):
         for j, (si, al) in enumerate(p):
             idx[d, j] = si
@@ -145,12 +152,13 @@ def _padded(tab, dsize):
 
 def _area_1ch(img, dw, dh):
     h, w = img.shape
+    dt = img.dtype.type
     fx, isx = area_is_fast(w, dw)
     fy, isy = area_is_fast(h, dh)
     if fx and fy:
         area = isx * isy
         cells = img[:dh * isy, :dw * isx].reshape(dh, isy, dw, isx).transpose(0, 2, 1, 3).reshape(dh, dw, area)
-        s = np.zeros((dh, dw), F32)
+        s = np.zeros((dh, dw), dt)
         k = 0
         while k <= area - 4:
             s = s + (((cells[..., k] + cells[..., k + 1]) + cells[..., k + 2]) + cells[..., k + 3])
@@ -158,21 +166,22 @@ def _area_1ch(img, dw, dh):
         while k < area:
             s = s + cells[..., k]
             k += 1
-        return (s * (F32(1) / F32(area))).astype(F32)
+        return (s * (dt(1) / dt(area))).astype(dt)
     xi, xa = _padded(area_tab(w, dw), dw)
     yi, ya = _padded(area_tab(h, dh), dh)
-    buf = np.zeros((h, dw), F32)
+    xa, ya = xa.astype(dt), ya.astype(dt)
+    buf = np.zeros((h, dw), dt)
     for j in range(xi.shape[1]):
         buf = buf + img[:, xi[:, j]] * xa[:, j]
     s = buf[yi[:, 0]] * ya[:, 0:1]
     for j in range(1, yi.shape[1]):
         s = s + buf[yi[:, j]] * ya[:, j:j + 1]
-    return s.astype(F32)
+    return s.astype(dt)
 
 
 def resize_area(img, dw, dh):
     """cv::resize(img, (dw, dh), INTER_AREA) for a shrinking (or equal) size."""
-    img = np.asarray(img, F32)
+    img = np.asarray(img, _dt(img))
     h, w = img.shape[:2]
     if w == dw and h == dh:
         return img.copy()
@@ -211,21 +220,22 @@ def remap_nearest_u8(m, rmap):
 def scale_flow(uv, rx, ry):
     """cv::multiply(uv, Scalar(rx, ry), uv): with a Scalar operand the product is taken in
     double and rounded to float."""
-    return (uv.astype(np.float64) * np.array([rx, ry], np.float64)).astype(F32)
+    return (uv.astype(np.float64) * np.array([rx, ry], np.float64)).astype(uv.dtype)
 
 
 class EccFlow:
-    """c_eccflow with c_eccflow_options (ecc2.h:515-527); defaults are those options'."""
+    """c_eccflow with c_eccflow_options (ecc2.h:515-527); defaults are those options'.
+    With the FP64 oracle every plane and flow is float64 (the warp-accuracy reference)."""
 
     def __init__(self, ora, update_multiplier=1.5, noise_level=-1.0, max_iterations=1, support_scale=5,
                  min_image_size=4, max_pyramid_level=-1, downscale=RECURSIVE_RESIZE, scale_factor=0.5):
-        assert ora.bits == 32
         self.o = ora
+        self.dt = np.float64 if ora.bits == 64 else F32
         self.um, self.noise = update_multiplier, noise_level
         self.iters, self.ss = max_iterations, support_scale
         self.min_size, self.max_level = min_image_size, max_pyramid_level
         self.method, self.sf = downscale, scale_factor
-        self.G3 = np.array([0.25, 0.5, 0.25], F32)
+        self.G3 = np.array([0.25, 0.5, 0.25], self.dt)
 
     # avgdown (ecc2.cc:2400-2418)
     def cell_size(self, w, h):
@@ -280,7 +290,7 @@ class EccFlow:
 
     def _pyramid(self, img, mask):
         h0, w0 = img.shape
-        imgs = [np.asarray(img, F32).copy()]
+        imgs = [np.asarray(img, self.dt).copy()]
         masks = [None if mask is None else np.asarray(mask, np.uint8).copy()]
         for size, src in zip(self.sizes[1:], self.srcs[1:]):
             d, m = self._downscale(imgs[src], masks[src], size)
@@ -297,16 +307,17 @@ class EccFlow:
         for level, (r, m) in enumerate(zip(imgs, masks)):
             Ix, Iy = self.o.ecc_differentiate(r)
             a00, a01, a11 = self.avgdown(Ix * Ix), self.avgdown(Ix * Iy), self.avgdown(Iy * Iy)
-            reg = F32((1e-5 * noise / (1 << level)) ** 4) if noise > 0 else F32(0)
+            dt = self.dt
+            reg = dt((1e-5 * noise / (1 << level)) ** 4) if noise > 0 else dt(0)
             det = np.abs(a00 * a11 - a01 * a01)
-            idet = (F32(self.um) / (det + reg)).astype(F32)
+            idet = (dt(self.um) / (det + reg)).astype(dt)
             self.levels.append(dict(ref=r, mask=m, Ix=Ix, Iy=Iy, a00=a00, a01=a01, a11=a11, idet=idet))
 
     def iterate(self, e, cur, cmask, uv):
         """One compute_uv(pyramid_entry) step (:2237-2398): returns the cell update
         resized to the level."""
         h, w = e["ref"].shape
-        yy, xx = np.mgrid[0:h, 0:w].astype(F32)
+        yy, xx = np.mgrid[0:h, 0:w].astype(self.dt)
         rmap = np.stack([uv[..., 0] + xx, uv[..., 1] + yy], -1)
         W = self.o.remap(cur, rmap, interp=1, border=1)
         M = None if cmask is None else remap_nearest_u8(cmask, rmap)
@@ -327,11 +338,13 @@ class EccFlow:
         cur, cmasks = self._pyramid(frame, mask)
         w0, h0 = self.sizes[0]
         wl, hl = self.sizes[-1]
+        dt = self.dt
         if rmap is None:
-            uv = np.zeros((hl, wl, 2), F32)
+            uv = np.zeros((hl, wl, 2), dt)
         else:
-            yy, xx = np.mgrid[0:h0, 0:w0].astype(F32)
-            flow = np.stack([rmap[..., 0] - xx, rmap[..., 1] - yy], -1).astype(F32)
+            yy, xx = np.mgrid[0:h0, 0:w0].astype(dt)
+            rmap = np.asarray(rmap, dt)
+            flow = np.stack([rmap[..., 0] - xx, rmap[..., 1] - yy], -1).astype(dt)
             uv = scale_flow(resize_cubic(flow, wl, hl), wl / w0, hl / h0)
         for i in range(L - 1, -1, -1):
             w, h = self.sizes[i]
@@ -346,5 +359,5 @@ class EccFlow:
         """Returns the refined remap (identity + uv) at the reference size."""
         uv = self.compute_uv(frame, rmap, mask)
         h0, w0 = uv.shape[:2]
-        yy, xx = np.mgrid[0:h0, 0:w0].astype(F32)
+        yy, xx = np.mgrid[0:h0, 0:w0].astype(self.dt)
         return np.stack([uv[..., 0] + xx, uv[..., 1] + yy], -1)

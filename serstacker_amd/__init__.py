@@ -124,16 +124,49 @@ class c_frame_registration:
             raise SsgError(self.last_error)
         return st
 
+    def register_batch_remaps(self, frames, remaps, masks=None):
+        """Batched registration that also writes each frame's remap (device tensors
+        (H, W, 2)), refined by the dense ECC flow when eccflow registration is enabled."""
+        views = image_views(frames)
+        mviews = _lib.images([image_view(m) for m in masks]) if masks is not None else None
+        rviews = image_views(remaps)
+        st = (_lib.RegistrationStatus * len(frames))()
+        rc = self._lib.ssg_registration_register_batch_remaps(self._h, len(frames), views, mviews, st, rviews)
+        if rc:
+            self._fail(rc, "register_batch_remaps")
+            raise SsgError(self.last_error)
+        return st
+
     def status(self) -> _lib.RegistrationStatus:
         return self._status
 
     def current_remap(self, status=None) -> np.ndarray:
-        st = status if status is not None else self._status
+        """The remap of a status's transform; without a status, c_frame_registration::
+        current_remap() -- the last registered frame's field, eccflow-refined if enabled."""
         w, h = self._ref_size
         m = np.zeros((h, w, 2), np.float32)
         mv = _lib.host_image(m)
-        _lib.check(self._lib.ssg_registration_create_remap(self._h, C.byref(st), C.byref(mv)), "create_remap")
+        if status is None:
+            _lib.check(self._lib.ssg_registration_current_remap(self._h, C.byref(mv)), "current_remap")
+        else:
+            _lib.check(self._lib.ssg_registration_create_remap(self._h, C.byref(status), C.byref(mv)),
+                       "create_remap")
         return m
+
+    def remap(self, src, src_mask=None, interpolation=-1, border_mode=-1, border_value=0.0, want_mask=True):
+        """c_frame_registration::remap (c_frame_registration.cc:1405-1417): warps src with the
+        current remap of the last registered frame.  Returns (dst, dst_mask) numpy arrays."""
+        w, h = self._ref_size
+        cn = 1 if np.ndim(src) == 2 else src.shape[2]
+        dst = np.zeros((h, w) if cn == 1 else (h, w, cn), np.float32)
+        dm = np.zeros((h, w), np.uint8) if want_mask else None
+        sv, smv = image_view(src), image_view(src_mask)
+        dv = _lib.host_image(dst)
+        dmv = _lib.host_image(dm) if want_mask else None
+        _lib.check(self._lib.ssg_registration_remap(self._h, C.byref(sv), C.byref(smv) if smv else None, C.byref(dv),
+                                                    C.byref(dmv) if dmv else None, interpolation, border_mode,
+                                                    border_value), "remap")
+        return dst, dm
 
     def custom_remap(self, src, src_mask=None, interpolation=None, border_mode=None, border_value=0.0,
                      status=None, rmap=None, want_mask=True):

@@ -173,6 +173,11 @@ _SIGS = [
     ("ssg_registration_register_frame", C.c_int, [C.c_void_p, _P(Image), _P(Image), _P(RegistrationStatus)]),
     ("ssg_registration_register_batch", C.c_int, [C.c_void_p, C.c_int, _P(Image), _P(Image), _P(RegistrationStatus)]),
     ("ssg_registration_create_remap", C.c_int, [C.c_void_p, _P(RegistrationStatus), _P(Image)]),
+    ("ssg_registration_current_remap", C.c_int, [C.c_void_p, _P(Image)]),
+    ("ssg_registration_remap", C.c_int, [C.c_void_p, _P(Image), _P(Image), _P(Image), _P(Image), C.c_int32,
+                                         C.c_int32, C.c_double]),
+    ("ssg_registration_register_batch_remaps", C.c_int, [C.c_void_p, C.c_int, _P(Image), _P(Image),
+                                                         _P(RegistrationStatus), _P(Image)]),
     ("ssg_remap", C.c_int, [_P(Image), _P(Image), _P(Image), _P(C.c_double), C.c_int32, _P(C.c_float), _P(Image),
                             _P(Image), C.c_int32, C.c_int32, C.c_double]),
     ("ssg_accumulation_create", C.c_int, [C.c_int32, C.c_int32, C.c_int32, C.c_int32, _P(C.c_void_p)]),

@@ -322,6 +322,11 @@ struct ssg_registration {
   int* pinned_count = nullptr;
   bool have_ref = false;
   Profiler prof;
+  // eccflow refinement (enable_eccflow_registration, c_frame_registration.cc:900-917)
+  ssg_eccflow* flow = nullptr;
+  DevBuf cur_map;  // the last registered frame's remap at the reference size (refined)
+  bool have_status = false;
+  ssg_registration_status last{};
 };
 
 extern "C" void ssg_default_image_registration_options(ssg_image_registration_options* o) {
@@ -390,7 +395,9 @@ extern "C" int ssg_registration_create(const ssg_image_registration_options* opt
   if (!(o.ecc.scale <= 0 || o.ecc.scale == 1 || fabs(o.ecc.scale - 0.5) < 1e-2)) return fail(SSG_E_UNSUPPORTED, "ecc.scale %g: only 1 and 0.5", o.ecc.scale);
   if (o.ecc.normalization_scale > 0 && o.ecc.normalization_noise > 0) return fail(SSG_E_UNSUPPORTED, "ecc normalization");
   if (o.ecc.replace_planetary_disk_with_mask) return fail(SSG_E_UNSUPPORTED, "replace_planetary_disk_with_mask");
-  if (o.enable_eccflow_registration) return fail(SSG_E_UNSUPPORTED, "eccflow refinement");
+  if (o.enable_eccflow_registration && o.eccflow.downscale_method != SSG_ECCFLOW_DOWNSCALE_PYRAMID &&
+      !(o.eccflow.scale_factor > 0 && o.eccflow.scale_factor < 1))
+    return fail(SSG_E_INVALID_ARG, "eccflow.scale_factor must be in (0, 1)");
   ssg_registration* r = new ssg_registration();
   r->o = o;
   hipError_t e = hipStreamCreateWithFlags(&r->stream, hipStreamNonBlocking);
@@ -405,7 +412,25 @@ extern "C" int ssg_registration_create(const ssg_image_registration_options* opt
     return fail(SSG_E_HIP, "hipHostMalloc: %s", hipGetErrorString(e));
   }
   int rc = ensure_tables(r->stream);
+  if (rc == SSG_OK && o.enable_eccflow_registration) {
+    // the reference sets every c_eccflow option from c_eccflow_registration_options
+    // (c_frame_registration.cc:640-650)
+    ssg_eccflow_options fo;
+    ssg_default_eccflow_options(&fo);
+    fo.update_multiplier = o.eccflow.update_multiplier;
+    fo.input_smooth_sigma = o.eccflow.input_smooth_sigma;
+    fo.reference_smooth_sigma = o.eccflow.reference_smooth_sigma;
+    fo.noise_level = o.eccflow.noise_level;
+    fo.scale_factor = o.eccflow.scale_factor;
+    fo.max_iterations = o.eccflow.max_iterations;
+    fo.support_scale = o.eccflow.support_scale;
+    fo.min_image_size = o.eccflow.min_image_size;
+    fo.max_pyramid_level = o.eccflow.max_pyramid_level;
+    fo.downscale = o.eccflow.downscale_method;
+    rc = ssg_eccflow_create(&fo, &r->flow);
+  }
   if (rc != SSG_OK) {
+    if (r->flow) ssg_eccflow_destroy(r->flow);
     (void)hipHostFree(r->pinned_count);
     (void)hipStreamDestroy(r->stream);
     delete r;
@@ -418,6 +443,7 @@ extern "C" int ssg_registration_create(const ssg_image_registration_options* opt
 extern "C" void ssg_registration_destroy(ssg_registration* r) {
   if (!r) return;
   if (r->stream) (void)hipStreamSynchronize(r->stream);
+  if (r->flow) ssg_eccflow_destroy(r->flow);
   if (r->pinned_count) (void)hipHostFree(r->pinned_count);
   if (r->stream) (void)hipStreamDestroy(r->stream);
   delete r;
@@ -882,14 +908,27 @@ extern "C" int ssg_registration_setup_reference_frame(ssg_registration* r, const
     r->ref_mean[l] = sum / ((double)r->lw[l] * r->lh[l]);
   }
   HIPCHK(hipStreamSynchronize(s));
+  // eccflow works on the full-resolution ECC image and mask (c_frame_registration.cc:637-660)
+  if (r->flow) CHK(ssg_eccflow_set_reference_image(r->flow, image, mask));
+  r->have_status = false;
   r->have_ref = true;
   return SSG_OK;
+}
+
+namespace {
+int refine_current_remap(ssg_registration* r, const ssg_image* image, const ssg_image* mask);
 }
 
 extern "C" int ssg_registration_register_frame(ssg_registration* r, const ssg_image* image, const ssg_image* mask,
                                                ssg_registration_status* status) {
   if (!r) return fail(SSG_E_INVALID_ARG, "null handle");
-  return register_n(r, 1, image, mask, status, true);
+  r->have_status = false;
+  CHK(register_n(r, 1, image, mask, status, true));
+  r->last = *status;
+  // create_remap(reference size) then the eccflow refine (c_frame_registration.cc:883-917)
+  if (r->flow && status->ok) CHK(refine_current_remap(r, image, mask));
+  r->have_status = true;
+  return SSG_OK;
 }
 
 extern "C" int ssg_registration_register_batch(ssg_registration* r, int n, const ssg_image* frames,
@@ -2754,4 +2793,105 @@ extern "C" int ssg_eccflow_get_profile(ssg_eccflow* f, ssg_profile* out, int res
   *out = f->prof.acc;
   if (reset) f->prof.acc = ssg_profile{};
   return SSG_OK;
+}
+
+// =====================================================================================================
+// registration with the eccflow refine
+// =====================================================================================================
+namespace {
+int refine_current_remap(ssg_registration* r, const ssg_image* image, const ssg_image* mask) {
+  const size_t pitch = sizeof(float2) * r->ref_w;
+  CHK(r->cur_map.ensure(pitch * r->ref_h));
+  const MapCoef mc = map_coef(r->o.motion_type, r->last.params, nullptr);
+  HIPCHK(launch_create_map(r->o.motion_type, mc, r->cur_map.as<float>(), (int)(pitch / 4), r->ref_w, r->ref_h,
+                           r->stream));
+  HIPCHK(hipStreamSynchronize(r->stream));
+  ssg_image m{r->cur_map.p, r->ref_w, r->ref_h, (int64_t)pitch, SSG_32FC2, 1};
+  return ssg_eccflow_compute(r->flow, image, mask, &m, &m);
+}
+}  // namespace
+
+extern "C" int ssg_registration_current_remap(ssg_registration* r, ssg_image* map) {
+  if (!r) return fail(SSG_E_INVALID_ARG, "null handle");
+  if (!r->have_status || !r->last.ok) return fail(SSG_E_NOT_READY, "no successfully registered frame");
+  if (!r->flow) return ssg_registration_create_remap(r, &r->last, map);
+  CHK(check_image(map, "map"));
+  if (map->type != SSG_32FC2 || map->width != r->ref_w || map->height != r->ref_h)
+    return fail(SSG_E_INVALID_ARG, "map must be SSG_32FC2 of the reference size %dx%d", r->ref_w, r->ref_h);
+  const size_t row = sizeof(float2) * r->ref_w;
+  HIPCHK(hipMemcpy2DAsync(map->data, (size_t)map->step, r->cur_map.p, row, row, r->ref_h,
+                          map->on_device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, r->stream));
+  HIPCHK(hipStreamSynchronize(r->stream));
+  return SSG_OK;
+}
+
+extern "C" int ssg_registration_register_batch_remaps(ssg_registration* r, int n, const ssg_image* frames,
+                                                      const ssg_image* masks, ssg_registration_status* status,
+                                                      ssg_image* remaps) {
+  if (!r) return fail(SSG_E_INVALID_ARG, "null handle");
+  if (n < 0) return fail(SSG_E_INVALID_ARG, "n < 0");
+  if (n == 0) return SSG_OK;
+  if (!remaps) return fail(SSG_E_INVALID_ARG, "null remaps");
+  for (int f = 0; f < n; ++f) {
+    CHK(check_image(&remaps[f], "remap"));
+    if (!remaps[f].on_device || remaps[f].type != SSG_32FC2 || remaps[f].width != r->ref_w ||
+        remaps[f].height != r->ref_h || remaps[f].step != remaps[0].step || remaps[f].step % 8)
+      return fail(SSG_E_INVALID_ARG, "remap %d: device SSG_32FC2 maps of the reference size and one step", f);
+  }
+  if (r->flow) {
+    for (int f = 0; f < n; ++f) {
+      if (!frames[f].on_device || frames[f].step != frames[0].step || frames[f].step % 4)
+        return fail(SSG_E_INVALID_ARG, "frame %d: the eccflow refine takes device frames of one step", f);
+      if (masks && (!masks[f].on_device || masks[f].step != masks[0].step))
+        return fail(SSG_E_INVALID_ARG, "mask %d: the eccflow refine takes device masks of one step", f);
+    }
+  }
+  r->have_status = false;
+  CHK(register_n(r, n, frames, masks, status, false));
+  hipStream_t s = r->stream;
+  const int mpitch = (int)(remaps[0].step / 4);
+  for (int f = 0; f < n; ++f) {
+    const MapCoef mc = map_coef(r->o.motion_type, status[f].params, nullptr);
+    HIPCHK(launch_create_map(r->o.motion_type, mc, (float*)remaps[f].data, mpitch, r->ref_w, r->ref_h, s));
+  }
+  HIPCHK(hipStreamSynchronize(s));
+  if (!r->flow) return SSG_OK;
+  // refine the maps of the accepted frames in place
+  std::vector<const float*> in;
+  std::vector<const uint8_t*> mk;
+  std::vector<const float2*> ri;
+  std::vector<float2*> ro;
+  for (int f = 0; f < n; ++f) {
+    if (!status[f].ok) continue;
+    in.push_back((const float*)frames[f].data);
+    if (masks) mk.push_back((const uint8_t*)masks[f].data);
+    ri.push_back((const float2*)remaps[f].data);
+    ro.push_back((float2*)remaps[f].data);
+  }
+  if (in.empty()) return SSG_OK;
+  CHK(eccflow_run(r->flow, (int)in.size(), in.data(), (int)(frames[0].step / 4), masks ? mk.data() : nullptr,
+                  masks ? (int)masks[0].step : 0, ri.data(), (int)(remaps[0].step / 8), ro.data(),
+                  (int)(remaps[0].step / 8)));
+  HIPCHK(hipStreamSynchronize(r->flow->stream));
+  r->flow->prof.collect();
+  return SSG_OK;
+}
+
+extern "C" int ssg_registration_remap(ssg_registration* r, const ssg_image* src, const ssg_image* src_mask,
+                                      ssg_image* dst, ssg_image* dst_mask, int32_t interpolation, int32_t border_mode,
+                                      double border_value) {
+  if (!r) return fail(SSG_E_INVALID_ARG, "null handle");
+  if (!r->have_status || !r->last.ok) return fail(SSG_E_NOT_READY, "no successfully registered frame");
+  if (interpolation < 0) interpolation = r->o.interpolation;
+  if (border_mode < 0) {
+    border_mode = r->o.border_mode;
+    border_value = r->o.border_value[0];
+  }
+  if (r->flow) {
+    ssg_image m{r->cur_map.p, r->ref_w, r->ref_h, (int64_t)(sizeof(float2) * r->ref_w), SSG_32FC2, 1};
+    return ssg_remap(src, src_mask, &m, nullptr, r->o.motion_type, nullptr, dst, dst_mask, interpolation, border_mode,
+                     border_value);
+  }
+  return ssg_remap(src, src_mask, nullptr, r->last.matrix, r->o.motion_type, r->last.params, dst, dst_mask,
+                   interpolation, border_mode, border_value);
 }

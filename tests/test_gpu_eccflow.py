@@ -179,3 +179,104 @@ def test_argument_errors(ssg):
         ssg.c_eccflow(downscale=7)
     with pytest.raises(ssg.SsgError):
         g.compute_batch([gpu(np.zeros((32, 32), np.float32))], [torch.zeros((32, 32), device="cuda")])
+
+
+# ---------------------------------------------------------------- registration with the eccflow refine
+def reg_options(ssg):
+    o = ssg.c_image_registration_options(motion_type=3, scale=1.0, eps=0.02, ecch_max_level=2, max_iterations=50)
+    o.enable_eccflow_registration = 1
+    o.eccflow.downscale_method = 2
+    o.eccflow.support_scale = 3
+    o.eccflow.max_iterations = 2
+    o.eccflow.min_image_size = -1
+    o.eccflow.max_pyramid_level = 3
+    return o
+
+
+def eccflow_oracle_for(oracle32, o):
+    from oracle import eccflow as EF
+
+    e = o.eccflow
+    return EF.EccFlow(oracle32, update_multiplier=e.update_multiplier, noise_level=e.noise_level,
+                      max_iterations=e.max_iterations, support_scale=e.support_scale, min_image_size=e.min_image_size,
+                      max_pyramid_level=e.max_pyramid_level, downscale=e.downscale_method, scale_factor=e.scale_factor)
+
+
+def registration_frames(w, h, n, seed):
+    S = scene(w, h, seed)
+    r = np.random.default_rng(seed)
+    yy, xx = np.mgrid[0:h, 0:w].astype(np.float64)
+    ref = S([1, 0, 0, 0, 1, 0]).astype(np.float32)
+    out = []
+    for _ in range(n):
+        a = np.array([1 + r.uniform(-0.004, 0.004), r.uniform(-0.01, 0.01), r.uniform(-3, 3),
+                      r.uniform(-0.01, 0.01), 1 + r.uniform(-0.004, 0.004), r.uniform(-3, 3)])
+        p = r.uniform(0, 2 * np.pi, 2)
+        u = a[0] * xx + a[1] * yy + a[2] + 0.6 * np.sin(2 * np.pi * xx / (1.2 * w) + p[0])
+        v = a[3] * xx + a[4] * yy + a[5] + 0.6 * np.cos(2 * np.pi * yy / (1.3 * h) + p[1])
+        out.append(S.at(u, v).astype(np.float32))
+    return ref, out
+
+
+def test_register_frame_refines_the_current_remap(ssg, oracle32):
+    w, h = 160, 128
+    ref, frames = registration_frames(w, h, 2, 71)
+    o = reg_options(ssg)
+    reg = ssg.c_frame_registration(o)
+    assert reg.setup_reference_frame(gpu(ref))
+    f = eccflow_oracle_for(oracle32, o)
+    f.set_reference_image(ref)
+    for fr in frames:
+        assert reg.register_frame(gpu(fr))
+        st = reg.status()
+        t = oracle32.xform(3, list(st.params))
+        want = f.compute(fr, oracle32.create_map(t, w, h))
+        got = reg.current_remap()
+        np.testing.assert_array_equal(got, want)
+        # c_frame_registration::remap warps with that field
+        d, m = reg.remap(gpu(fr))
+        cd, cm = reg.custom_remap(fr, rmap=want)
+        np.testing.assert_array_equal(d, cd)
+        np.testing.assert_array_equal(m, cm)
+
+
+def test_batched_remaps_equal_per_frame_registration(ssg):
+    import torch
+
+    w, h, n = 160, 128, 4
+    ref, frames = registration_frames(w, h, n, 81)
+    o = reg_options(ssg)
+    reg = ssg.c_frame_registration(o)
+    assert reg.setup_reference_frame(gpu(ref))
+    dev = [gpu(fr) for fr in frames]
+    maps = [torch.zeros((h, w, 2), dtype=torch.float32, device="cuda") for _ in range(n)]
+    st = reg.register_batch_remaps(dev, maps)
+    for i in range(n):
+        assert st[i].ok
+        assert reg.register_frame(dev[i])
+        np.testing.assert_array_equal(maps[i].cpu().numpy(), reg.current_remap())
+    # without the refine the maps are the transforms' fields
+    o.enable_eccflow_registration = 0
+    plain = ssg.c_frame_registration(o)
+    assert plain.setup_reference_frame(gpu(ref))
+    st2 = plain.register_batch_remaps(dev, maps)
+    for i in range(n):
+        np.testing.assert_array_equal(maps[i].cpu().numpy(), plain.current_remap(st2[i]))
+
+
+def test_flow_endpoint_error_vs_fp64_oracle(ssg, oracle64):
+    """Reported figure: endpoint error of the GPU (FP32) flow against the same algorithm in FP64."""
+    from oracle import eccflow as EF
+
+    w, h = 256, 256
+    ref, cur = frames_pair(w, h, 91, shift=(2.1, -1.4), amp=1.2)
+    o = dict(downscale=2, support_scale=3, max_iterations=3, min_image_size=-1, max_pyramid_level=-1)
+    g = ssg.c_eccflow(**o)
+    g.set_reference_image(gpu(ref))
+    got = g.compute(gpu(cur))
+    f = EF.EccFlow(oracle64, **o)
+    f.set_reference_image(ref.astype(np.float64))
+    want = f.compute(cur.astype(np.float64))
+    epe = np.hypot(got[..., 0] - want[..., 0], got[..., 1] - want[..., 1])
+    print(f"flow EPE vs FP64 oracle: mean {epe.mean():.2e} px, p99 {np.percentile(epe, 99):.2e}, max {epe.max():.2e}")
+    assert epe.mean() < 5e-3 and np.percentile(epe, 99) < 2e-2
