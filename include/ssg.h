@@ -300,6 +300,15 @@ int ssg_accumulation_add_registered_batch(ssg_accumulation *a, int n, const ssg_
                                           const float *frame_weights, const ssg_image *weight_maps,
                                           int32_t interpolation, int32_t border_mode, double border_value);
 
+/* The same fused remap -> mask -> weight -> accumulate with a dense remap field per frame
+ * instead of a transform: the frames warped by c_frame_registration::remap with the
+ * eccflow-refined current remap (c_frame_registration.cc:1405-1417), then accumulated
+ * (c_image_stacking_pipeline.cc:1644-1776).  rmaps: n device SSG_32FC2 fields of the
+ * accumulator size and one step; every frame counts (weights <= 0 excepted). */
+int ssg_accumulation_add_remapped_batch(ssg_accumulation *a, int n, const ssg_image *frames, const ssg_image *rmaps,
+                                        const float *frame_weights, const ssg_image *weight_maps,
+                                        int32_t interpolation, int32_t border_mode, double border_value);
+
 /* c_frame_accumulation::compute(avg, mask, dscale, ddepth) (c_frame_accumulation.cc:223-248):
  * avg = sum / weight (FP64 division), mask = weight > 0 (SSG_8UC1, optional).
  * Bayer (:1198-1238): avg SSG_32FC3 = acc / counter per channel (float), mask = any

@@ -193,6 +193,20 @@ class c_weigthed_average : public c_frame_accumulation {
                                                  border_value) == SSG_OK;
   }
 
+  // The same fused sequence with a dense remap field per frame (eccflow-refined remaps).
+  bool add_remapped_batch(const std::vector<image> &frames, const std::vector<image> &rmaps,
+                          const std::vector<float> *frame_weights = nullptr, int interpolation = SSG_INTER_LINEAR,
+                          int border_mode = SSG_BORDER_REFLECT101, double border_value = 0,
+                          const std::vector<image> *weight_maps = nullptr) {
+    if (!a_ || rmaps.size() != frames.size()) return false;
+    std::vector<ssg_image> f(frames.begin(), frames.end()), m(rmaps.begin(), rmaps.end()), wm;
+    if (weight_maps) wm.assign(weight_maps->begin(), weight_maps->end());
+    return ssg_accumulation_add_remapped_batch(a_, (int)f.size(), f.data(), m.data(),
+                                               frame_weights ? frame_weights->data() : nullptr,
+                                               weight_maps ? wm.data() : nullptr, interpolation, border_mode,
+                                               border_value) == SSG_OK;
+  }
+
   bool compute(image &avg, image *mask = nullptr, double dscale = 1.0) const override {
     return a_ && ssg_accumulation_compute(a_, &avg, mask, dscale) == SSG_OK;
   }

@@ -229,6 +229,17 @@ class c_weigthed_average:
                                                              wmv, interpolation, border_mode, border_value)
         _lib.check(rc, "add_registered_batch")
 
+    def add_remapped_batch(self, frames, rmaps, frame_weights=None, interpolation=_lib.INTER_LINEAR,
+                           border_mode=_lib.BORDER_REFLECT101, border_value=0.0, weight_maps=None):
+        """The fused warp + accumulate with a dense remap field per frame (device (H, W, 2))."""
+        views = image_views(frames)
+        mv = image_views(rmaps)
+        wts = (C.c_float * len(frames))(*frame_weights) if frame_weights is not None else None
+        wmv = image_views(weight_maps) if weight_maps is not None else None
+        rc = self._lib.ssg_accumulation_add_remapped_batch(self._h, len(frames), views, mv, wts, wmv, interpolation,
+                                                           border_mode, border_value)
+        _lib.check(rc, "add_remapped_batch")
+
     def compute(self, dscale: float = 1.0):
         avg = np.zeros((self.h, self.w) if self.cn == 1 else (self.h, self.w, self.cn), np.float32)
         mask = np.zeros((self.h, self.w), np.uint8)

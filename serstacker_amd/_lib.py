@@ -186,6 +186,8 @@ _SIGS = [
     ("ssg_accumulation_add_registered_batch", C.c_int, [C.c_void_p, C.c_int, _P(Image), _P(RegistrationStatus),
                                                         C.c_int32, _P(C.c_float), _P(Image), C.c_int32, C.c_int32,
                                                         C.c_double]),
+    ("ssg_accumulation_add_remapped_batch", C.c_int, [C.c_void_p, C.c_int, _P(Image), _P(Image), _P(C.c_float),
+                                                      _P(Image), C.c_int32, C.c_int32, C.c_double]),
     ("ssg_accumulation_compute", C.c_int, [C.c_void_p, _P(Image), _P(Image), C.c_double]),
     ("ssg_accumulation_get_acc_counters", C.c_int, [C.c_void_p, _P(Image)]),
     ("ssg_accumulation_get_counts", C.c_int, [C.c_void_p, _P(Image)]),

@@ -216,8 +216,12 @@ constexpr int AC_GROUP = 16;  // frames summed plainly before folding into the c
 // mask / 255, c_image_stacking_pipeline.cc:1704-1712, the mask being 0 or 255).
 template <int MOTION, int INTERP, int CN, bool WMAP>
 __global__ void __launch_bounds__(256) k_accumulate(AccumArgs a) {
+  // MOTION_MAP: the source positions come from a dense per-frame field, staged in LDS for the
+  // tile and its 2-px erosion halo (the whole tile takes the bordered path)
+  constexpr bool DENSE = MOTION == MOTION_MAP;
   __shared__ uint8_t mk[AC_TH + 4][AC_TW + 4];
   __shared__ uint8_t rmk[AC_TH + 4][AC_TW];
+  __shared__ float2 smap[DENSE ? (AC_TH + 4) * (AC_TW + 4) : 1];
   const int tid = threadIdx.x, lane = tid & (WAVE - 1), wv = tid / WAVE;
   const int x0 = blockIdx.x * AC_TW, y0 = blockIdx.y * AC_TH;
   const int chunk = blockIdx.z;
@@ -240,15 +244,18 @@ __global__ void __launch_bounds__(256) k_accumulate(AccumArgs a) {
   for (int k = kb; k < kend; ++k) {
     if (!a.valid[k]) continue;
     MapCoef mc;
+    if constexpr (!DENSE) {
 #pragma unroll
-    for (int i = 0; i < 8; ++i) mc.c[i] = a.maps[k].c[i];
+      for (int i = 0; i < 8; ++i) mc.c[i] = a.maps[k].c[i];
+    }
+    const float2* __restrict__ rm = DENSE ? a.rmaps[k] : nullptr;
     const float wk = a.weights[k];
     const float* __restrict__ img = a.frames[k];
     // uniform test: tile + erosion halo inside the sampling fast region
     const int ex0 = max(x0 - 2, 0), ey0 = max(y0 - 2, 0);
     const int ex1 = min(x0 + AC_TW + 1, a.dw - 1), ey1 = min(y0 + AC_TH + 1, a.dh - 1);
     float mnx = 1e30f, mxx = -1e30f, mny = 1e30f, mxy = -1e30f;
-    {
+    if constexpr (!DENSE) {
       const int cx[4] = {ex0, ex1, ex0, ex1}, cy[4] = {ey0, ey0, ey1, ey1};
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
@@ -257,7 +264,7 @@ __global__ void __launch_bounds__(256) k_accumulate(AccumArgs a) {
         mnx = fminf(mnx, sx); mxx = fmaxf(mxx, sx); mny = fminf(mny, sy); mxy = fmaxf(mxy, sy);
       }
     }
-    const bool interior = mnx >= lo_m && mny >= lo_m && mxx <= hi_x && mxy <= hi_y;
+    const bool interior = !DENSE && mnx >= lo_m && mny >= lo_m && mxx <= hi_x && mxy <= hi_y;
     if (!interior) {
       // warped mask for the tile + 2-px halo; outside the destination counts as 255
       for (int i = tid; i < (AC_TH + 4) * (AC_TW + 4); i += 256) {
@@ -266,7 +273,14 @@ __global__ void __launch_bounds__(256) k_accumulate(AccumArgs a) {
         uint8_t v = 1;
         if ((unsigned)px < (unsigned)a.dw && (unsigned)py < (unsigned)a.dh) {
           float sx, sy;
-          warp_xy<MOTION>(mc, px, py, sx, sy);
+          if constexpr (DENSE) {
+            const float2 m = rm[(size_t)py * a.rmap_pitch + px];
+            smap[i] = m;
+            sx = m.x;
+            sy = m.y;
+          } else {
+            warp_xy<MOTION>(mc, px, py, sx, sy);
+          }
           v = mask_value<INTERP>(nullptr, 0, a.sw, a.sh, quant32(sx), quant32(sy)) >= 255;
         }
         mk[ly][lx] = v;
@@ -286,7 +300,13 @@ __global__ void __launch_bounds__(256) k_accumulate(AccumArgs a) {
       if (!interior) m = rmk[ly][lane] & rmk[ly + 1][lane] & rmk[ly + 2][lane] & rmk[ly + 3][lane] & rmk[ly + 4][lane];
       if (!m) continue;
       float sx, sy;
-      warp_xy<MOTION>(mc, x, y, sx, sy);
+      if constexpr (DENSE) {
+        const float2 mv = smap[(ly + 2) * (AC_TW + 4) + lane + 2];
+        sx = mv.x;
+        sy = mv.y;
+      } else {
+        warp_xy<MOTION>(mc, x, y, sx, sy);
+      }
       const int X = quant32(sx), Y = quant32(sy);
       float v[CN];
       bool ok;
@@ -386,6 +406,7 @@ hipError_t launch_accumulate(int motion, const AccumArgs& a, hipStream_t s) {
     case MOTION_EUCLIDEAN:
     case MOTION_SCALED_EUCLIDEAN: SSG_ACC_IC(MOTION_EUCLIDEAN); break;
     case MOTION_AFFINE: SSG_ACC_IC(MOTION_AFFINE); break;
+    case MOTION_MAP: SSG_ACC_IC(MOTION_MAP); break;
     default: return hipErrorInvalidValue;
   }
 #undef SSG_ACC_IC

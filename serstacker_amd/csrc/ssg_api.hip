@@ -1728,24 +1728,9 @@ int bayer_batch(ssg_accumulation* a, int n, const ssg_image* frames, const ssg_r
 }
 }  // namespace
 
-extern "C" int ssg_accumulation_add_registered_batch(ssg_accumulation* a, int n, const ssg_image* frames,
-                                                     const ssg_registration_status* statuses, int32_t motion_type,
-                                                     const float* frame_weights, const ssg_image* weight_maps,
-                                                     int32_t interpolation, int32_t border_mode,
-                                                     double border_value) {
-  if (!a) return fail(SSG_E_INVALID_ARG, "null handle");
-  if (n <= 0) return SSG_OK;
-  if (!frames || !statuses) return fail(SSG_E_INVALID_ARG, "null frames or statuses");
-  if (interpolation != SSG_INTER_LINEAR && interpolation != SSG_INTER_CUBIC)
-    return fail(SSG_E_UNSUPPORTED, "interpolation %d", interpolation);
-  if (motion_type < SSG_MOTION_TRANSLATION || motion_type > SSG_MOTION_AFFINE)
-    return fail(SSG_E_INVALID_ARG, "motion_type %d", motion_type);
-  if (a->bayer()) {
-    if (weight_maps) return fail(SSG_E_INVALID_ARG, "bayer batches take the base_remap mask only (weight_maps NULL)");
-    return bayer_batch(a, n, frames, statuses, motion_type, frame_weights, interpolation);
-  }
-  if (frame_weights && weight_maps) return fail(SSG_E_INVALID_ARG, "pass frame_weights or weight_maps, not both");
-  if (a->focus()) return fail(SSG_E_UNSUPPORTED, "focus stacking adds frames one by one (ssg_accumulation_add)");
+namespace {
+// Shared validation of a fused batch: frames (device, one geometry) and optional weight maps.
+int check_batch(ssg_accumulation* a, int n, const ssg_image* frames, const ssg_image* weight_maps) {
   const int sw = frames[0].width, sh = frames[0].height;
   const int64_t step = frames[0].step;
   for (int f = 0; f < n; ++f) {
@@ -1764,33 +1749,49 @@ extern "C" int ssg_accumulation_add_registered_batch(ssg_accumulation* a, int n,
         return fail(SSG_E_INVALID_ARG, "weight map %d: SSG_32FC1 device images of the frame size and one step", f);
     }
   }
+  return SSG_OK;
+}
+
+// The fused remap -> mask -> weight -> accumulate of n frames.  motion is a parametric
+// motion type (maps[f] per frame) or MOTION_MAP (rmaps[f]: dense fields at the accumulator size).
+int accumulate_fused(ssg_accumulation* a, int n, const ssg_image* frames, int motion, const MapCoef* maps,
+                     const ssg_image* rmaps, const int* valid, const float* w, const ssg_image* weight_maps,
+                     int32_t interpolation, int32_t border_mode, double border_value) {
   hipStream_t s = a->stream;
-  std::vector<MapCoef> maps(n);
-  std::vector<float> w(n);
-  std::vector<int> valid(n);
+  const int sw = frames[0].width, sh = frames[0].height;
+  const int64_t step = frames[0].step;
   std::vector<const float*> ptrs(n);
   int nvalid = 0;
   for (int f = 0; f < n; ++f) {
-    maps[f] = map_coef(motion_type, statuses[f].params, nullptr);
-    w[f] = frame_weights ? frame_weights[f] : 1.f;
-    // a weight <= 0 contributes nothing (c_frame_accumulation.cc:112: w > 0)
-    valid[f] = statuses[f].ok && w[f] > 0.f;
-    nvalid += valid[f];
     ptrs[f] = (const float*)frames[f].data;
+    nvalid += valid[f];
   }
-  CHK(a->maps.ensure(sizeof(MapCoef) * n));
+  const int nptr = 1 + (weight_maps ? 1 : 0) + (rmaps ? 1 : 0);
   CHK(a->weights.ensure(sizeof(float) * n));
   CHK(a->valid.ensure(sizeof(int) * n));
-  CHK(a->frame_ptrs.ensure(sizeof(void*) * n * (weight_maps ? 2 : 1)));
+  CHK(a->frame_ptrs.ensure(sizeof(void*) * n * nptr));
+  const float** dptr = a->frame_ptrs.as<const float*>();
+  int slot = 1;
+  const float* const* wmaps = nullptr;
+  const float2* const* dmaps = nullptr;
   if (weight_maps) {
     std::vector<const float*> wp(n);
     for (int f = 0; f < n; ++f) wp[f] = (const float*)weight_maps[f].data;
-    HIPCHK(hipMemcpyAsync(a->frame_ptrs.as<const float*>() + n, wp.data(), sizeof(void*) * n, hipMemcpyHostToDevice,
-                          s));
+    HIPCHK(hipMemcpyAsync(dptr + slot * n, wp.data(), sizeof(void*) * n, hipMemcpyHostToDevice, s));
+    wmaps = dptr + slot * n;
+    ++slot;
   }
-  HIPCHK(hipMemcpyAsync(a->maps.p, maps.data(), sizeof(MapCoef) * n, hipMemcpyHostToDevice, s));
-  HIPCHK(hipMemcpyAsync(a->weights.p, w.data(), sizeof(float) * n, hipMemcpyHostToDevice, s));
-  HIPCHK(hipMemcpyAsync(a->valid.p, valid.data(), sizeof(int) * n, hipMemcpyHostToDevice, s));
+  if (rmaps) {
+    std::vector<const void*> mp(n);
+    for (int f = 0; f < n; ++f) mp[f] = rmaps[f].data;
+    HIPCHK(hipMemcpyAsync(dptr + slot * n, mp.data(), sizeof(void*) * n, hipMemcpyHostToDevice, s));
+    dmaps = (const float2* const*)(dptr + slot * n);
+  } else {
+    CHK(a->maps.ensure(sizeof(MapCoef) * n));
+    HIPCHK(hipMemcpyAsync(a->maps.p, maps, sizeof(MapCoef) * n, hipMemcpyHostToDevice, s));
+  }
+  HIPCHK(hipMemcpyAsync(a->weights.p, w, sizeof(float) * n, hipMemcpyHostToDevice, s));
+  HIPCHK(hipMemcpyAsync(a->valid.p, valid, sizeof(int) * n, hipMemcpyHostToDevice, s));
   HIPCHK(hipMemcpyAsync(a->frame_ptrs.p, ptrs.data(), sizeof(void*) * n, hipMemcpyHostToDevice, s));
   // chunks: enough blocks to fill the chip, each chunk a contiguous frame range
   const long long tiles = (long long)((a->w + 63) / 64) * ((a->h + 15) / 16);
@@ -1806,9 +1807,11 @@ extern "C" int ssg_accumulation_add_registered_batch(ssg_accumulation* a, int n,
   g.sw = sw;
   g.sh = sh;
   g.cn = a->cn;
-  g.maps = a->maps.as<MapCoef>();
+  g.maps = rmaps ? nullptr : a->maps.as<MapCoef>();
+  g.rmaps = dmaps;
+  g.rmap_pitch = rmaps ? (int)(rmaps[0].step / 8) : 0;
   g.weights = a->weights.as<float>();
-  g.wmaps = weight_maps ? a->frame_ptrs.as<const float*>() + n : nullptr;
+  g.wmaps = wmaps;
   g.wmap_pitch = weight_maps ? (int)(weight_maps[0].step / 4) : 0;
   g.valid = a->valid.as<int>();
   g.nframes = n;
@@ -1823,18 +1826,81 @@ extern "C" int ssg_accumulation_add_registered_batch(ssg_accumulation* a, int n,
   g.plane = a->plane;
   g.nchunks = nch;
   a->prof.start(s);
-  HIPCHK(launch_accumulate(motion_type, g, s));
+  HIPCHK(launch_accumulate(motion, g, s));
   a->prof.stop(s, SSG_PROF_ACCUMULATE);
   HIPCHK(launch_accum_merge(a->part.as<float>(), a->part_cnt.as<unsigned>(), nch, a->plane, a->cn, a->acc.as<float>(),
                             nullptr, a->cnt.as<unsigned>(), s));
   HIPCHK(hipStreamSynchronize(s));
-  // algorithmic bytes of the fused kernel: each registered frame is read once
-  // (4 B per channel and destination pixel); each chunk writes its partial planes
-  a->prof.add_bytes(SSG_PROF_ACCUMULATE, (double)nvalid * 4.0 * a->cn * a->plane +
+  // algorithmic bytes of the fused kernel: each frame read once (4 B per channel and destination
+  // pixel; +4 B for a weight map, +8 B for a dense field); each chunk writes its partial planes
+  const double per_px = 4.0 * a->cn + (weight_maps ? 4.0 : 0.0) + (rmaps ? 8.0 : 0.0);
+  a->prof.add_bytes(SSG_PROF_ACCUMULATE, (double)nvalid * per_px * a->plane +
                                              (double)nch * ((2.0 * a->cn + 2.0) * 4.0 + 4.0) * a->plane);
   a->prof.collect();
   a->accumulated += nvalid;
   return SSG_OK;
+}
+}  // namespace
+
+extern "C" int ssg_accumulation_add_registered_batch(ssg_accumulation* a, int n, const ssg_image* frames,
+                                                     const ssg_registration_status* statuses, int32_t motion_type,
+                                                     const float* frame_weights, const ssg_image* weight_maps,
+                                                     int32_t interpolation, int32_t border_mode,
+                                                     double border_value) {
+  if (!a) return fail(SSG_E_INVALID_ARG, "null handle");
+  if (n <= 0) return SSG_OK;
+  if (!frames || !statuses) return fail(SSG_E_INVALID_ARG, "null frames or statuses");
+  if (interpolation != SSG_INTER_LINEAR && interpolation != SSG_INTER_CUBIC)
+    return fail(SSG_E_UNSUPPORTED, "interpolation %d", interpolation);
+  if (motion_type < SSG_MOTION_TRANSLATION || motion_type > SSG_MOTION_AFFINE)
+    return fail(SSG_E_INVALID_ARG, "motion_type %d", motion_type);
+  if (a->bayer()) {
+    if (weight_maps) return fail(SSG_E_INVALID_ARG, "bayer batches take the base_remap mask only (weight_maps NULL)");
+    return bayer_batch(a, n, frames, statuses, motion_type, frame_weights, interpolation);
+  }
+  if (frame_weights && weight_maps) return fail(SSG_E_INVALID_ARG, "pass frame_weights or weight_maps, not both");
+  if (a->focus()) return fail(SSG_E_UNSUPPORTED, "focus stacking adds frames one by one (ssg_accumulation_add)");
+  CHK(check_batch(a, n, frames, weight_maps));
+  std::vector<MapCoef> maps(n);
+  std::vector<float> w(n);
+  std::vector<int> valid(n);
+  for (int f = 0; f < n; ++f) {
+    maps[f] = map_coef(motion_type, statuses[f].params, nullptr);
+    w[f] = frame_weights ? frame_weights[f] : 1.f;
+    // a weight <= 0 contributes nothing (c_frame_accumulation.cc:112: w > 0)
+    valid[f] = statuses[f].ok && w[f] > 0.f;
+  }
+  return accumulate_fused(a, n, frames, motion_type, maps.data(), nullptr, valid.data(), w.data(), weight_maps,
+                          interpolation, border_mode, border_value);
+}
+
+extern "C" int ssg_accumulation_add_remapped_batch(ssg_accumulation* a, int n, const ssg_image* frames,
+                                                   const ssg_image* rmaps, const float* frame_weights,
+                                                   const ssg_image* weight_maps, int32_t interpolation,
+                                                   int32_t border_mode, double border_value) {
+  if (!a) return fail(SSG_E_INVALID_ARG, "null handle");
+  if (n <= 0) return SSG_OK;
+  if (!frames || !rmaps) return fail(SSG_E_INVALID_ARG, "null frames or rmaps");
+  if (interpolation != SSG_INTER_LINEAR && interpolation != SSG_INTER_CUBIC)
+    return fail(SSG_E_UNSUPPORTED, "interpolation %d", interpolation);
+  if (a->bayer() || a->focus())
+    return fail(SSG_E_UNSUPPORTED, "dense-field batches go to the weighted average accumulator");
+  if (frame_weights && weight_maps) return fail(SSG_E_INVALID_ARG, "pass frame_weights or weight_maps, not both");
+  CHK(check_batch(a, n, frames, weight_maps));
+  for (int f = 0; f < n; ++f) {
+    CHK(check_image(&rmaps[f], "rmap"));
+    if (!rmaps[f].on_device || rmaps[f].type != SSG_32FC2 || rmaps[f].width != a->w || rmaps[f].height != a->h ||
+        rmaps[f].step != rmaps[0].step || rmaps[0].step % 8)
+      return fail(SSG_E_INVALID_ARG, "rmap %d: SSG_32FC2 device maps of the accumulator size and one step", f);
+  }
+  std::vector<float> w(n);
+  std::vector<int> valid(n);
+  for (int f = 0; f < n; ++f) {
+    w[f] = frame_weights ? frame_weights[f] : 1.f;
+    valid[f] = w[f] > 0.f;
+  }
+  return accumulate_fused(a, n, frames, MOTION_MAP, nullptr, rmaps, valid.data(), w.data(), weight_maps,
+                          interpolation, border_mode, border_value);
 }
 
 extern "C" int ssg_accumulation_compute(ssg_accumulation* a, ssg_image* avg, ssg_image* mask, double dscale) {

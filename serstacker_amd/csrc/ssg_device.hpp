@@ -16,6 +16,8 @@ enum : int { BORDER_CONSTANT = 0, BORDER_REPLICATE = 1, BORDER_REFLECT = 2, BORD
              BORDER_REFLECT101 = 4, BORDER_TRANSPARENT = 5 };
 enum : int { INTER_NEAREST = 0, INTER_LINEAR = 1, INTER_CUBIC = 2 };
 enum : int { MOTION_TRANSLATION = 0, MOTION_EUCLIDEAN = 1, MOTION_SCALED_EUCLIDEAN = 2, MOTION_AFFINE = 3 };
+// internal: a dense per-frame remap field (SSG_32FC2) instead of a parametric transform
+constexpr int MOTION_MAP = 4;
 
 constexpr int WAVE = 64;
 

@@ -124,7 +124,9 @@ struct LevelBeginArgs {
 struct AccumArgs {
   const float* const* frames;  // n device pointers
   int src_pitch, sw, sh, cn;   // all frames share the geometry
-  const MapCoef* maps;         // per frame
+  const MapCoef* maps;         // per frame (parametric motions)
+  const float2* const* rmaps;  // per frame dense remap fields (MOTION_MAP), pitch rmap_pitch float2
+  int rmap_pitch;
   const float* weights;        // per frame scalar weight (when wmaps is null)
   const float* const* wmaps;   // per frame weight image on the frame grid (f32) or null
   int wmap_pitch;              // elements per row of the weight images

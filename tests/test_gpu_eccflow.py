@@ -280,3 +280,80 @@ def test_flow_endpoint_error_vs_fp64_oracle(ssg, oracle64):
     epe = np.hypot(got[..., 0] - want[..., 0], got[..., 1] - want[..., 1])
     print(f"flow EPE vs FP64 oracle: mean {epe.mean():.2e} px, p99 {np.percentile(epe, 99):.2e}, max {epe.max():.2e}")
     assert epe.mean() < 5e-3 and np.percentile(epe, 99) < 2e-2
+
+
+# ---------------------------------------------------------------- dense-field warp + accumulate
+def rel_l2(a, b):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+def test_remapped_batch_equals_registered_batch_for_transform_fields(ssg):
+    w, h, n = 160, 128, 5
+    ref, frames = registration_frames(w, h, n, 101)
+    reg = ssg.c_frame_registration(ssg.c_image_registration_options(motion_type=3, scale=1.0, eps=0.02,
+                                                                     ecch_max_level=2, max_iterations=50))
+    assert reg.setup_reference_frame(gpu(ref))
+    dev = [gpu(f) for f in frames]
+    st = reg.register_batch(dev)
+    maps = [gpu(reg.current_remap(st[i])) for i in range(n)]
+    wts = [0.5 + 0.1 * i for i in range(n)]
+    for interp in (ssg.INTER_LINEAR, ssg.INTER_CUBIC):
+        a = ssg.c_weigthed_average(w, h, 1)
+        a.add_registered_batch(dev, st, 3, frame_weights=wts, interpolation=interp)
+        b = ssg.c_weigthed_average(w, h, 1)
+        b.add_remapped_batch(dev, maps, frame_weights=wts, interpolation=interp)
+        for x, y in zip(a.compute(), b.compute()):
+            np.testing.assert_array_equal(x, y)
+        np.testing.assert_array_equal(a.counts(), b.counts())
+        np.testing.assert_array_equal(a.get_acc_counters(), b.get_acc_counters())
+
+
+def test_turbulence_pipeline_phasecorr_eccflow_bicubic_stack(ssg, oracle32):
+    """Phase-correlation pre-alignment -> eccflow -> per-pixel bicubic remap -> average, against
+    the oracle chain fed with the GPU's translation estimates."""
+    from oracle import eccflow as EF
+
+    w, h, n = 192, 160, 4
+    S = scene(w, h, 111)
+    r = np.random.default_rng(111)
+    yy, xx = np.mgrid[0:h, 0:w].astype(np.float64)
+    ref = S([1, 0, 0, 0, 1, 0]).astype(np.float32)
+    frames = []
+    for i in range(n):
+        T = r.uniform(-6, 6, 2)
+        p = r.uniform(0, 2 * np.pi, 2)
+        ex = T[0] + 1.0 * np.sin(2 * np.pi * xx / (1.1 * w) + p[0])
+        ey = T[1] + 1.0 * np.cos(2 * np.pi * yy / (1.2 * h) + p[1])
+        frames.append(S.at(xx - ex, yy - ey).astype(np.float32))
+    dev = [gpu(f) for f in frames]
+    pc = ssg.phase_correlation(w, h, 1)
+    pc.set_reference(gpu(ref))
+    est = pc.estimate(dev)
+    o = dict(downscale=2, support_scale=3, max_iterations=3, min_image_size=-1, max_pyramid_level=3)
+    g = ssg.c_eccflow(**o)
+    g.set_reference_image(gpu(ref))
+    fx, fy = xx.astype(np.float32), yy.astype(np.float32)
+    init = [np.stack([fx + np.float32(est[i, 0]), fy + np.float32(est[i, 1])], -1) for i in range(n)]
+    maps = [gpu(m) for m in init]
+    g.compute_batch(dev, maps, rmaps=maps)
+    acc = ssg.c_weigthed_average(w, h, 1)
+    acc.add_remapped_batch(dev, maps, interpolation=ssg.INTER_CUBIC)
+    avg, mask = acc.compute()
+    f = EF.EccFlow(oracle32, **o)
+    f.set_reference_image(ref)
+    A = np.zeros((h, w), np.float32)
+    W = np.zeros((h, w), np.float32)
+    cnt = np.zeros((h, w), np.int64)
+    for i in range(n):
+        m = f.compute(frames[i], init[i])
+        np.testing.assert_array_equal(maps[i].cpu().numpy(), m)
+        d, dm = oracle32.base_remap(frames[i], m, interp=2, border=4)
+        oracle32.weighted_average_update(d, A, W, mask=dm)
+        cnt += dm > 0
+    np.testing.assert_array_equal(acc.counts(), cnt)
+    assert rel_l2(avg, A) <= 1e-5
+    # the flow brings the frames onto the reference
+    inner = (slice(24, -24), slice(24, -24))
+    assert rel_l2(avg[inner], ref[inner]) < 0.02
