@@ -222,6 +222,7 @@ __global__ void __launch_bounds__(256) k_accumulate(AccumArgs a) {
   __shared__ uint8_t mk[AC_TH + 4][AC_TW + 4];
   __shared__ uint8_t rmk[AC_TH + 4][AC_TW];
   __shared__ float2 smap[DENSE ? (AC_TH + 4) * (AC_TW + 4) : 1];
+  __shared__ float sred[DENSE ? AC_WAVES * 4 : 1];
   const int tid = threadIdx.x, lane = tid & (WAVE - 1), wv = tid / WAVE;
   const int x0 = blockIdx.x * AC_TW, y0 = blockIdx.y * AC_TH;
   const int chunk = blockIdx.z;
@@ -262,9 +263,44 @@ __global__ void __launch_bounds__(256) k_accumulate(AccumArgs a) {
         float sx, sy;
         warp_xy<MOTION>(mc, cx[i], cy[i], sx, sy);
         mnx = fminf(mnx, sx); mxx = fmaxf(mxx, sx); mny = fminf(mny, sy); mxy = fmaxf(mxy, sy);
+        if (sx != sx || sy != sy) mnx = -1e30f;  // NaN positions take the bordered path
       }
     }
-    const bool interior = !DENSE && mnx >= lo_m && mny >= lo_m && mxx <= hi_x && mxy <= hi_y;
+    if constexpr (DENSE) {
+      // stage the field of the tile + halo and reduce its extent over the block
+      for (int i = tid; i < (AC_TH + 4) * (AC_TW + 4); i += 256) {
+        const int ly = i / (AC_TW + 4), lx = i - ly * (AC_TW + 4);
+        const int px = x0 - 2 + lx, py = y0 - 2 + ly;
+        if ((unsigned)px < (unsigned)a.dw && (unsigned)py < (unsigned)a.dh) {
+          const float2 m = rm[(size_t)py * a.rmap_pitch + px];
+          smap[i] = m;
+          mnx = fminf(mnx, m.x); mxx = fmaxf(mxx, m.x); mny = fminf(mny, m.y); mxy = fmaxf(mxy, m.y);
+          if (m.x != m.x || m.y != m.y) mnx = -1e30f;  // a NaN position takes the bordered path
+        }
+      }
+#pragma unroll
+      for (int o = 32; o > 0; o >>= 1) {
+        mnx = fminf(mnx, __shfl_xor(mnx, o));
+        mxx = fmaxf(mxx, __shfl_xor(mxx, o));
+        mny = fminf(mny, __shfl_xor(mny, o));
+        mxy = fmaxf(mxy, __shfl_xor(mxy, o));
+      }
+      if (lane == 0) {
+        sred[wv * 4 + 0] = mnx;
+        sred[wv * 4 + 1] = mxx;
+        sred[wv * 4 + 2] = mny;
+        sred[wv * 4 + 3] = mxy;
+      }
+      __syncthreads();
+#pragma unroll
+      for (int q = 0; q < AC_WAVES; ++q) {
+        mnx = fminf(mnx, sred[q * 4 + 0]);
+        mxx = fmaxf(mxx, sred[q * 4 + 1]);
+        mny = fminf(mny, sred[q * 4 + 2]);
+        mxy = fmaxf(mxy, sred[q * 4 + 3]);
+      }
+    }
+    const bool interior = mnx >= lo_m && mny >= lo_m && mxx <= hi_x && mxy <= hi_y;
     if (!interior) {
       // warped mask for the tile + 2-px halo; outside the destination counts as 255
       for (int i = tid; i < (AC_TH + 4) * (AC_TW + 4); i += 256) {
@@ -274,8 +310,7 @@ __global__ void __launch_bounds__(256) k_accumulate(AccumArgs a) {
         if ((unsigned)px < (unsigned)a.dw && (unsigned)py < (unsigned)a.dh) {
           float sx, sy;
           if constexpr (DENSE) {
-            const float2 m = rm[(size_t)py * a.rmap_pitch + px];
-            smap[i] = m;
+            const float2 m = smap[i];
             sx = m.x;
             sy = m.y;
           } else {
@@ -340,7 +375,8 @@ __global__ void __launch_bounds__(256) k_accumulate(AccumArgs a) {
       wg[r] += wp;
       cnt[r] += 1;
     }
-    if (!interior) __syncthreads();
+    // the next frame rewrites the LDS tiles (masks, staged field)
+    if (DENSE || !interior) __syncthreads();
     if (++ingroup == AC_GROUP) {
       ingroup = 0;
 #pragma unroll

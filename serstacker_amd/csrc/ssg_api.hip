@@ -2403,6 +2403,7 @@ struct ssg_eccflow {
   hipStream_t stream = nullptr;
   int L = 0, w0 = 0, h0 = 0;
   std::vector<int> lw, lh, cw, ch, src;  // level and cell sizes, source level of each level
+  std::vector<int> fuse_k;               // cell size of levels that take the fused iteration, else 0
   std::vector<size_t> loff, coff;        // element offsets of the levels / cell grids when packed
   size_t lsum = 0, csum = 0;
   bool have_ref = false, ref_mask = false;
@@ -2547,6 +2548,15 @@ extern "C" int ssg_eccflow_set_reference_image(ssg_eccflow* f, const ssg_image* 
     f->lvl_tab.push_back(l + 1 < f->L ? cubic_host(f->lw[l + 1], f->lh[l + 1], f->lw[l], f->lh[l], tb) : CubicHost{});
   }
   f->init_tab = cubic_host(w0, h0, f->lw[f->L - 1], f->lh[f->L - 1], tb);
+  // the fused iteration covers whole-factor cells of k = 4..32 px (a power of two) that tile the level
+  f->fuse_k.assign(f->L, 0);
+  for (int l = 0; l < f->L; ++l) {
+    const AreaGeom& g = f->cell_geom[l].g;
+    const int k = g.isx;
+    if (g.fast && g.isx == g.isy && k >= 4 && k <= 32 && (k & (k - 1)) == 0 && f->lw[l] == f->cw[l] * k &&
+        f->lh[l] == f->ch[l] * k)
+      f->fuse_k[l] = k;
+  }
   CHK(f->tabs.ensure(tb.bytes.size()));
   HIPCHK(hipMemcpyAsync(f->tabs.p, tb.bytes.data(), tb.bytes.size(), hipMemcpyHostToDevice, s));
   const char* base = (const char*)f->tabs.p;
@@ -2691,6 +2701,7 @@ int eccflow_run(ssg_eccflow* f, int n, const float* const* in, int in_pitch, con
     if (hm)
       HIPCHK(launch_mask_prepare((const uint8_t* const*)(dp + n + f0), mask_pitch, cm, fs, w0, w0, h0, w0, h0, m, s));
     for (int l = 1; l < L; ++l) CHK(eccflow_downscale(f, cur, fs, cm, l, m));
+    bool mapped = false;
     // initial flow at the coarsest level
     float2* ua = f->uvA.as<float2>();
     float2* ub = f->uvB.as<float2>();
@@ -2735,6 +2746,34 @@ int eccflow_run(ssg_eccflow* f, int n, const float* const* in, int in_pitch, con
       up.sfs = cfs;
       up.dst = ua;
       up.dfs = ufs;
+      if (f->fuse_k[l] && f->o.max_iterations > 0) {
+        FlowIterArgs ia{};
+        ia.L = la;
+        ia.uv = ua;
+        ia.ufs = ufs;
+        ia.cuv = f->cuv.as<float2>();
+        ia.cfs = cfs;
+        ia.cw = f->cw[l];
+        ia.ch = f->ch[l];
+        ia.k = f->fuse_k[l];
+        ia.xs = up.xs;
+        ia.xa = up.xa;
+        ia.ys = up.ys;
+        ia.ya = up.ya;
+        ia.cells = f->cells.as<float>();
+        ia.cellfs = 2 * cfs;
+        ia.maps = (float2* const*)(dp + 3 * n + f0);
+        ia.map_pitch = rout_pitch;
+        for (int it = 0; it < f->o.max_iterations; ++it) {
+          HIPCHK(launch_flow_iter(it > 0, true, false, ia, m, s));
+          HIPCHK(launch_flow_solve(f->cells.as<float>(), 2 * cfs, f->D.as<float4>() + f->coff[l], f->cw[l], f->ch[l],
+                                   f->cuv.as<float2>(), cfs, m, s));
+        }
+        // the last update; at level 0 it goes straight into the caller's maps
+        HIPCHK(launch_flow_iter(true, false, l == 0, ia, m, s));
+        if (l == 0) mapped = true;
+        continue;
+      }
       for (int it = 0; it < f->o.max_iterations; ++it) {
         HIPCHK(launch_flow_residual(la, ua, ufs, f->itxy.as<float2>(), ufs, m, s));
         HIPCHK(launch_area(2, f->itxy.as<float>(), 2 * ufs, 2 * w, f->cells.as<float>(), 2 * cfs, 2 * f->cw[l],
@@ -2744,7 +2783,7 @@ int eccflow_run(ssg_eccflow* f, int n, const float* const* in, int in_pitch, con
         HIPCHK(launch_flow_cubic(CUBIC_ADD, up, m, s));
       }
     }
-    HIPCHK(launch_flow_to_remap(ua, ufs, w0, h0, (float2* const*)(dp + 3 * n + f0), rout_pitch, m, s));
+    if (!mapped) HIPCHK(launch_flow_to_remap(ua, ufs, w0, h0, (float2* const*)(dp + 3 * n + f0), rout_pitch, m, s));
   }
   f->prof.stop(s, SSG_PROF_ECCFLOW);
   if (f->prof.on) {

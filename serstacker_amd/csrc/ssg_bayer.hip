@@ -150,6 +150,7 @@ __global__ void __launch_bounds__(256) k_bayer_batch(BayerBatchArgs a) {
         float sx, sy;
         warp_xy<MOTION>(mc, cx[i], cy[i], sx, sy);
         mnx = fminf(mnx, sx); mxx = fmaxf(mxx, sx); mny = fminf(mny, sy); mxy = fmaxf(mxy, sy);
+        if (sx != sx || sy != sy) mnx = -1e30f;  // NaN positions take the bordered path
       }
     }
     const bool interior = mnx >= 0.25f && mny >= 0.25f && mxx <= (float)st.w - 1.25f && mxy <= (float)st.h - 1.25f;

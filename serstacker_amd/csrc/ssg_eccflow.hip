@@ -268,6 +268,121 @@ hipError_t launch_flow_cubic(int mode, const FlowCubicArgs& a, int nframes, hipS
   return hipGetLastError();
 }
 
+// ---------------------------------------------------------------------------------------------
+// Fused iteration for k x k cells, k = 4..32 a power of two dividing the level (the whole-factor
+// INTER_AREA and an exact 1/k cubic scale).  One wave owns 64 columns (lane = column) of one
+// cell row (k pixel rows):
+//   ADD    : uv += cubic(cuv): the lane filters the five cell rows j-2..j+2 horizontally once
+//            (rows kj..kj+k-1 take their four vertical taps from them), then per pixel row
+//            forms the vertical sum and adds it (cv::add(uv, cuv) of ecc2.cc:2830);
+//   RESID  : the residual products of k_flow_residual at the updated uv, and their cell sums in
+//            resizeAreaFast_ order -- each group of four columns of a row summed
+//            ((a+b)+c)+d across lanes, then the groups added to the cell sum row by row, left
+//            to right, at the cell's first lane;
+//   TO_MAP : (with ADD, without RESID) the updated flow goes out as identity + uv into the
+//            caller's maps (ecc_flow_to_remap) instead of back into uv.
+// ---------------------------------------------------------------------------------------------
+template <bool ADD, bool RESID, bool TO_MAP>
+__global__ void __launch_bounds__(256) k_flow_iter(FlowIterArgs a) {
+  const int lane = threadIdx.x & (WAVE - 1);
+  const int j = blockIdx.y * 4 + (threadIdx.x >> 6);
+  if (j >= a.ch) return;  // whole waves leave; no block barrier below
+  const int w = a.L.w, k = a.k, f = blockIdx.z;
+  const int x = blockIdx.x * WAVE + lane;
+  const bool colok = x < w;
+  const int xc = colok ? x : w - 1;
+  const int y0 = j * k;
+  float2* U = a.uv + f * a.ufs;
+  // horizontal cubic of cell rows j-2..j+2 at this column
+  float hx[5], hy[5];
+  if (ADD) {
+    const int sx = a.xs[xc];
+    const float4 al = a.xa[xc];
+    int cx[4];
+#pragma unroll
+    for (int t = 0; t < 4; ++t) cx[t] = min(max(sx - 1 + t, 0), a.cw - 1);
+    const float2* C = a.cuv + f * a.cfs;
+#pragma unroll
+    for (int rr = 0; rr < 5; ++rr) {
+      const int row = min(max(j - 2 + rr, 0), a.ch - 1);
+      const float2* R = C + (size_t)row * a.cw;
+      const float2 v0 = R[cx[0]], v1 = R[cx[1]], v2 = R[cx[2]], v3 = R[cx[3]];
+      hx[rr] = ((v0.x * al.x + v1.x * al.y) + v2.x * al.z) + v3.x * al.w;
+      hy[rr] = ((v0.y * al.x + v1.y * al.y) + v2.y * al.z) + v3.y * al.w;
+    }
+  }
+  const float* Cur = a.L.cur + f * a.L.cfs;
+  float accx = 0.f, accy = 0.f;
+  const int gper = k >> 2;
+  const bool base = (lane & (k - 1)) == 0;
+  for (int r = 0; r < k; ++r) {
+    const int y = y0 + r;
+    const size_t p = (size_t)y * w + xc;
+    float2 u = U[p];
+    if (ADD) {
+      // rows ys[y]-1 .. ys[y]+2 are hx[o..o+3] with o = ys[y] - (j - 1) in {0, 1}
+      const bool o = a.ys[y] != j - 1;
+      const float4 be = a.ya[y];
+      const float h0x = o ? hx[1] : hx[0], h1x = o ? hx[2] : hx[1], h2x = o ? hx[3] : hx[2], h3x = o ? hx[4] : hx[3];
+      const float h0y = o ? hy[1] : hy[0], h1y = o ? hy[2] : hy[1], h2y = o ? hy[3] : hy[2], h3y = o ? hy[4] : hy[3];
+      const float vx = ((h0x * be.x + h1x * be.y) + h2x * be.z) + h3x * be.w;
+      const float vy = ((h0y * be.x + h1y * be.y) + h2y * be.z) + h3y * be.w;
+      u.x = u.x + vx;
+      u.y = u.y + vy;
+      if (colok) {
+        if (TO_MAP) a.maps[f][(size_t)y * a.map_pitch + x] = make_float2(u.x + (float)x, u.y + (float)y);
+        else U[p] = u;
+      }
+    }
+    if (RESID) {
+      const float mx = u.x + (float)xc, my = u.y + (float)y;
+      const float Wv = sample_linear_replicate(Cur, w, w, a.L.h, quant32(mx), quant32(my));
+      int m = colok ? 255 : 0;
+      if (a.L.cmask) {
+        const int ix = round_coord(mx), iy = round_coord(my);
+        m = ((unsigned)ix < (unsigned)w && (unsigned)iy < (unsigned)a.L.h)
+                ? (m & (int)a.L.cmask[f * a.L.mfs + (size_t)iy * w + ix])
+                : 0;
+      }
+      if (a.L.rmask) m &= (int)a.L.rmask[p];
+      float p0 = 0.f, p1 = 0.f;
+      if (m) {
+        const float It = a.L.ref[p] - Wv;
+        p0 = It * a.L.Ix[p];
+        p1 = It * a.L.Iy[p];
+      }
+      // ((v_l + v_l+1) + v_l+2) + v_l+3 at lanes l = 0 mod 4
+      float gx = p0 + __shfl_down(p0, 1);
+      float gy = p1 + __shfl_down(p1, 1);
+      gx = gx + __shfl_down(p0, 2);
+      gy = gy + __shfl_down(p1, 2);
+      gx = gx + __shfl_down(p0, 3);
+      gy = gy + __shfl_down(p1, 3);
+      for (int q = 0; q < gper; ++q) {
+        const float tx = __shfl(gx, lane + 4 * q), ty = __shfl(gy, lane + 4 * q);
+        accx = accx + tx;
+        accy = accy + ty;
+      }
+    }
+  }
+  if (RESID && base && colok) {
+    const float scale = 1.f / (float)(k * k);
+    float* D = a.cells + f * a.cellfs + (size_t)j * 2 * a.cw + 2 * (x / k);
+    D[0] = accx * scale;
+    D[1] = accy * scale;
+  }
+}
+
+hipError_t launch_flow_iter(bool add, bool resid, bool to_map, const FlowIterArgs& a, int nframes, hipStream_t s) {
+  dim3 grid((a.L.w + WAVE - 1) / WAVE, (a.ch + 3) / 4, nframes), block(256);
+  if (add && resid) hipLaunchKernelGGL((k_flow_iter<true, true, false>), grid, block, 0, s, a);
+  else if (resid) hipLaunchKernelGGL((k_flow_iter<false, true, false>), grid, block, 0, s, a);
+  else if (add && to_map) hipLaunchKernelGGL((k_flow_iter<true, false, true>), grid, block, 0, s, a);
+  else if (add) hipLaunchKernelGGL((k_flow_iter<true, false, false>), grid, block, 0, s, a);
+  else return hipErrorInvalidValue;
+  return hipGetLastError();
+}
+
 // rmap = uv + identity (ecc_flow_to_remap, ecc2.cc:452-481), into the caller's maps
 __global__ void k_flow_to_remap(const float2* __restrict__ uv, long long ufs, int w, int h, float2* const* maps,
                                 int pitch) {

@@ -302,6 +302,26 @@ struct FlowCubicArgs {
   const float4* ya;
   double rx, ry;
 };
+// One fused eccflow iteration over a level whose cells are k x k pixels (k = 4..32, a power of
+// two dividing the level size): optionally adds the pending cell update (cubic-upsampled) to uv,
+// then warps, forms the residual products and sums them per cell in the INTER_AREA order.
+struct FlowIterArgs {
+  FlowLevelArgs L;
+  float2* uv;  // level flow, frame-strided (ufs), pitch L.w
+  long long ufs;
+  const float2* cuv;  // pending cell update (pitch cw), frame-strided (cfs)
+  long long cfs;
+  int cw, ch, k;
+  const int* xs;  // cells -> level INTER_CUBIC tables
+  const float4* xa;
+  const int* ys;
+  const float4* ya;
+  float* cells;  // cell sums, float2 interleaved (pitch 2 cw floats), frame-strided (cellfs floats)
+  long long cellfs;
+  float2* const* maps;  // final pass at level 0: identity + uv into the caller's maps
+  int map_pitch;
+};
+hipError_t launch_flow_iter(bool add, bool resid, bool to_map, const FlowIterArgs& a, int nframes, hipStream_t s);
 hipError_t launch_area(int cn, const float* src, long long sfs, int spitch, float* dst, long long dfs, int dpitch,
                        const AreaGeom& g, int nframes, hipStream_t s);
 hipError_t launch_flow_products(const float* Ix, const float* Iy, int n, float2* P2, float* P1, hipStream_t s);

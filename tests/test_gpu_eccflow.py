@@ -50,6 +50,8 @@ def frames_pair(w, h, seed, shift=(1.3, -0.8), amp=1.0):
 OPTS = {
     "pyramid": dict(downscale=2, support_scale=3, max_iterations=3, min_image_size=-1, max_pyramid_level=-1),
     "pyramid_ss2": dict(downscale=2, support_scale=2, max_iterations=2, min_image_size=-1, max_pyramid_level=3),
+    "pyramid_ss4": dict(downscale=2, support_scale=4, max_iterations=2, min_image_size=-1, max_pyramid_level=2),
+    "pyramid_ss5": dict(downscale=2, support_scale=5, max_iterations=2, min_image_size=-1, max_pyramid_level=1),
     "recursive": dict(downscale=0, scale_factor=0.75, support_scale=3, max_iterations=2, min_image_size=8),
     "full": dict(downscale=1, scale_factor=0.5, support_scale=2, max_iterations=2, min_image_size=4),
 }
