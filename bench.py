@@ -65,9 +65,10 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="the CPU sample stops after this much time")
     ap.add_argument("--oracle-frames", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--workload", choices=["affine", "turbulence"], default="affine",
+    ap.add_argument("--workload", choices=["affine", "turbulence", "sweep"], default="affine",
                     help="affine: batched affine ECC + weighted stack of 1080p frames (the headline metric); "
-                         "turbulence: phase correlation + eccflow + bicubic remap + average of 1024x1024 frames")
+                         "turbulence: phase correlation + eccflow + bicubic remap + average of 1024x1024 frames; "
+                         "sweep: each remaining kernel family at its configuration's frame size")
     ap.add_argument("--copy-calib-mib", type=int, default=0,
                     help="after the timed region, copy a buffer of this size (a known-size reference for "
                          "rocprofv3 FETCH_SIZE/WRITE_SIZE)")
@@ -90,6 +91,8 @@ def main():
     dev = torch.device("cuda:0")
     if a.workload == "turbulence":
         return bench_turbulence(a, S, _lib, torch)
+    if a.workload == "sweep":
+        return bench_sweep(a, S, _lib, torch)
     n, W, H = a.frames, a.width, a.height
 
     # ---- inputs resident in HBM (not timed)
@@ -467,6 +470,80 @@ def bench_turbulence(a, S, _lib, torch):
                     for k, v in kern.items()},
     }
     print(json.dumps(out))
+
+
+def bench_sweep(a, S, _lib, torch):
+    """Per-family throughput of the kernels outside the headline workload, at the frame sizes of
+    their configurations, on synthetic device-resident inputs.  Each entry: mean wall time per call
+    (the calls return after their stream finished) and algorithmic bytes per call / that time."""
+    lib = _lib.load()
+    dev = torch.device("cuda:0")
+    g = torch.Generator(device="cpu").manual_seed(7)
+    reps = max(1, a.steps)
+    res = {}
+
+    def timed(name, fn, nbytes, unit_count, unit):
+        for _ in range(max(1, a.warmup)):
+            fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            fn()
+        torch.cuda.synchronize()
+        dt = (time.perf_counter() - t0) / reps
+        res[name] = {"ms_per_call": round(dt * 1e3, 3), f"{unit}_per_s": round(unit_count / dt, 1),
+                     "algorithmic_bytes": nbytes, "gbs": round(nbytes / dt / 1e9, 1),
+                     "frac": round(nbytes / dt / 1e9 / HBM_PEAK_GBS, 4)}
+        print(f"[sweep] {name}: {dt * 1e3:.2f} ms/call, {nbytes / dt / 1e9:.0f} GB/s", file=sys.stderr, flush=True)
+
+    # Bayer average: 64 u16 RGGB frames of 4096x3000 (12-bit), even-step translations + subpixel
+    W, H, n = 4096, 3000, 64
+    raw = (torch.rand((n, H, W), generator=g) * 4095).to(torch.int16).to(dev)
+    st = (_lib.RegistrationStatus * n)()
+    for i in range(n):
+        st[i].ok, st[i].nparams = 1, 2
+        st[i].params[0] = float(2 * int(torch.randint(-7, 8, (1,), generator=g)) + 0.3)
+        st[i].params[1] = float(2 * int(torch.randint(-7, 8, (1,), generator=g)) - 0.2)
+    views = _lib.images([S.image_view(raw[i]) for i in range(n)])
+    bay = S.c_bayer_average(W, H, bpp=12)
+    torch.cuda.synchronize()
+    # u16 frames read once; the BGR sum, weight and count planes (3 x 12 B/px) read and written once
+    timed("bayer_batch_64x4096x3000_u16", lambda: bay.add_registered_batch(views, st, S.MOTION_TRANSLATION),
+          n * 2 * W * H + 2 * 36 * W * H, n, "frames")
+    # phase correlation on 2x2-binned luminance of the same frames: padded 2048x2048 spectra
+    pc = S.phase_correlation(W, H, 2)
+    pc.set_reference(raw[0])
+    M = 2048 * 2048
+    timed("phasecorr_64x4096x3000_binned2", lambda: pc.estimate(views), n * (2 * W * H + 17 * 8 * M), n, "frames")
+    del raw, views
+    # focus stacking: 16 RGB f32 frames of 2048x1536 per stack (add + compute)
+    W, H, n = 2048, 1536, 16
+    fr = torch.rand((n, H, W, 3), generator=g).to(dev)
+    for pol in (_lib.FOCUS_SELECT_MAX_ENERGY, _lib.FOCUS_LPG_WEIGHTS):
+        def stack(pol=pol):
+            fs = S.c_laplacian_pyramid_focus_stacking(W, H, 3, fusing_policy=pol)
+            for i in range(n):
+                fs.add(fr[i])
+            fs.compute()
+        timed(f"focus_stack_{'lpg' if pol == _lib.FOCUS_LPG_WEIGHTS else 'max_energy'}_16x2048x1536_rgb", stack,
+              n * 12 * W * H, n, "frames")
+    # lpg sharpness map of one RGB frame; local-variance weight map of a mono 1920x1080 frame
+    out = torch.empty((H, W), dtype=torch.float32, device=dev)
+    sv, ov = S.image_view(fr[0]), S.image_view(out)
+    sh = C.c_double()
+    timed("lpg_2048x1536_rgb", lambda: _lib.check(lib.ssg_lpg(C.byref(sv), C.byref(ov), 6.0, 2.0, C.byref(sh))),
+          (12 + 4) * W * H, 1, "frames")
+    W, H = 1920, 1080
+    mono = torch.rand((H, W), generator=g).to(dev)
+    mout = torch.empty((H, W), dtype=torch.float32, device=dev)
+    mv, mo = S.image_view(mono), S.image_view(mout)
+    q = C.c_double()
+    timed("local_variance_map_1920x1080", lambda: _lib.check(lib.ssg_local_variance_map(C.byref(mv), 1, 1,
+                                                                                         C.byref(mo), C.byref(q))),
+          (4 + 4) * W * H, 1, "frames")
+    out_line = {"metric": "per-family kernel throughput (sweep)", "unit": "GB/s", "n_gpus": 1, "data": "synthetic",
+                "dtype": "f32", "config": {"workload": "kernel sweep"}, "families": res}
+    print(json.dumps(out_line))
 
 
 if __name__ == "__main__":

@@ -4,8 +4,12 @@
  * interfaces, so pipeline code binds to it one call for one call:
  *
  *   ssg::c_frame_registration   core/proc/image_registration/c_frame_registration.h:210-354
+ *   ssg::c_eccflow              core/proc/image_registration/ecc2.h:548-662
  *   ssg::c_frame_accumulation   core/average/c_frame_accumulation.h:14-37
  *   ssg::c_weigthed_average     core/average/c_frame_accumulation.h:39-63
+ *   ssg::c_bayer_average        core/average/c_frame_accumulation.h:222-262
+ *   ssg::c_laplacian_pyramid_focus_stacking  core/average/c_frame_accumulation.h:141-184
+ *   ssg::phase_correlation      (no reference counterpart: the FFT pre-alignment)
  *
  * Images are ssg::image views (pointer, size, byte stride, type, host/device)
  * instead of cv::InputArray; the caller owns the memory.  Methods return bool
@@ -228,6 +232,168 @@ class c_weigthed_average : public c_frame_accumulation {
   int w_, h_, cn_;
   ssg_accumulation *a_ = nullptr;
 };
+
+// Accumulators other than the weighted average share one handle type.
+class accumulation_handle : public c_frame_accumulation {
+ public:
+  accumulation_handle(int32_t method, int width, int height, int channels) : w_(width), h_(height) {
+    ssg_accumulation_create(method, width, height, channels, &a_);
+  }
+  ~accumulation_handle() override {
+    if (a_) ssg_accumulation_destroy(a_);
+  }
+  accumulation_handle(const accumulation_handle &) = delete;
+  accumulation_handle &operator=(const accumulation_handle &) = delete;
+  bool valid() const { return a_ != nullptr; }
+  bool add(const image &src, const image &weights = image()) override {
+    return a_ && ssg_accumulation_add(a_, &src, weights.empty() ? nullptr : &weights) == SSG_OK;
+  }
+  bool compute(image &avg, image *mask = nullptr, double dscale = 1.0) const override {
+    return a_ && ssg_accumulation_compute(a_, &avg, mask, dscale) == SSG_OK;
+  }
+  bool get_acc_counters(image &accw) const override {
+    return a_ && ssg_accumulation_get_acc_counters(a_, &accw) == SSG_OK;
+  }
+  void clear() override {
+    if (a_) ssg_accumulation_clear(a_);
+  }
+  std::pair<int, int> accumulator_size() const override { return {w_, h_}; }
+  int accumulated_frames() const override {
+    int32_t n = 0;
+    if (a_) ssg_accumulation_accumulated_frames(a_, &n);
+    return n;
+  }
+  ssg_accumulation *handle() const { return a_; }
+
+ protected:
+  int w_, h_;
+  ssg_accumulation *a_ = nullptr;
+};
+
+// c_bayer_average (c_frame_accumulation.h:222-262, c_frame_accumulation.cc:988-1334): raw
+// Bayer frames (SSG_32FC1 or SSG_16UC1) into a 3-channel BGR accumulator.
+class c_bayer_average : public accumulation_handle {
+ public:
+  c_bayer_average(int width, int height) : accumulation_handle(SSG_ACC_BAYER_AVERAGE, width, height, 1) {}
+  // set_bayer_pattern (c_frame_accumulation.cc:1136-1142): SSG_COLORID_BAYER_*
+  bool set_bayer_pattern(int colorid) { return a_ && ssg_accumulation_set_bayer_pattern(a_, colorid) == SSG_OK; }
+  // set_remap (c_frame_accumulation.cc:1149-1152); an empty image clears it
+  bool set_remap(const image &rmap) {
+    return a_ && ssg_accumulation_set_remap(a_, rmap.empty() ? nullptr : &rmap) == SSG_OK;
+  }
+  bool set_input_bpp(int bpp) { return a_ && ssg_accumulation_set_input_bpp(a_, bpp) == SSG_OK; }
+  // integer per-cell sample counts (SSG_32SC3)
+  bool get_counts(image &counts) const { return a_ && ssg_accumulation_get_counts(a_, &counts) == SSG_OK; }
+  // set_remap(current_remap) + add(raw, mask) for n registered device frames, in order
+  bool add_registered_batch(const std::vector<image> &frames, const std::vector<registration_status> &st,
+                            int motion_type) {
+    if (!a_ || st.size() != frames.size()) return false;
+    std::vector<ssg_image> f(frames.begin(), frames.end());
+    return ssg_accumulation_add_registered_batch(a_, (int)f.size(), f.data(), st.data(), motion_type, nullptr,
+                                                 nullptr, SSG_INTER_LINEAR, SSG_BORDER_REFLECT101, 0) == SSG_OK;
+  }
+};
+
+using focus_stacking_options = ssg_focus_stacking_options;
+inline focus_stacking_options default_focus_stacking_options() {
+  focus_stacking_options o;
+  ssg_default_focus_stacking_options(&o);
+  return o;
+}
+
+// c_laplacian_pyramid_focus_stacking (c_frame_accumulation.h:141-184, c_frame_accumulation.cc:449-700)
+class c_laplacian_pyramid_focus_stacking : public accumulation_handle {
+ public:
+  c_laplacian_pyramid_focus_stacking(int width, int height, int channels,
+                                     const focus_stacking_options &o = default_focus_stacking_options())
+      : accumulation_handle(SSG_ACC_FOCUS_STACK, width, height, channels) {
+    if (a_ && ssg_accumulation_set_focus_options(a_, &o) != SSG_OK) {
+      ssg_accumulation_destroy(a_);
+      a_ = nullptr;
+    }
+  }
+};
+
+using eccflow_options = ssg_eccflow_options;
+inline eccflow_options default_eccflow_options() {
+  eccflow_options o;
+  ssg_default_eccflow_options(&o);
+  return o;
+}
+
+// c_eccflow (ecc2.h:548-662, ecc2.cc:2103-2865): dense coarse-to-fine ECC flow
+class c_eccflow {
+ public:
+  explicit c_eccflow(const eccflow_options &o = default_eccflow_options()) { ssg_eccflow_create(&o, &h_); }
+  ~c_eccflow() {
+    if (h_) ssg_eccflow_destroy(h_);
+  }
+  c_eccflow(const c_eccflow &) = delete;
+  c_eccflow &operator=(const c_eccflow &) = delete;
+  bool valid() const { return h_ != nullptr; }
+  // set_reference_image (ecc2.cc:2489-2672)
+  bool set_reference_image(const image &img, const image &mask = image()) {
+    return h_ && ssg_eccflow_set_reference_image(h_, &img, mask.empty() ? nullptr : &mask) == SSG_OK;
+  }
+  // compute(input, rmap, mask) (ecc2.cc:2856-2865): rmap is the initial remap (empty: the zero
+  // flow) and receives identity + flow
+  bool compute(const image &input, image &rmap, const image &mask = image(), bool rmap_is_initial = true) {
+    return h_ && ssg_eccflow_compute(h_, &input, mask.empty() ? nullptr : &mask, rmap_is_initial ? &rmap : nullptr,
+                                     &rmap) == SSG_OK;
+  }
+  // batched compute over device frames; initial remaps optional (then the zero flow)
+  bool compute_batch(const std::vector<image> &inputs, std::vector<image> &rmaps, bool rmaps_are_initial = true,
+                     const std::vector<image> *masks = nullptr) {
+    if (!h_ || rmaps.size() != inputs.size()) return false;
+    std::vector<ssg_image> in(inputs.begin(), inputs.end()), r(rmaps.begin(), rmaps.end()), m;
+    if (masks) m.assign(masks->begin(), masks->end());
+    return ssg_eccflow_compute_batch(h_, (int)in.size(), in.data(), masks ? m.data() : nullptr,
+                                     rmaps_are_initial ? r.data() : nullptr, r.data()) == SSG_OK;
+  }
+  int num_levels() const {
+    int32_t n = 0;
+    if (h_) ssg_eccflow_num_levels(h_, &n);
+    return n;
+  }
+  ssg_eccflow *handle() const { return h_; }
+
+ private:
+  ssg_eccflow *h_ = nullptr;
+};
+
+// FFT phase-correlation translation estimate (the pre-alignment of the turbulence and Bayer
+// configurations; the reference has none on this path).
+class phase_correlation {
+ public:
+  phase_correlation(int width, int height, int binning = 1) { ssg_phasecorr_create(width, height, binning, &h_); }
+  ~phase_correlation() {
+    if (h_) ssg_phasecorr_destroy(h_);
+  }
+  phase_correlation(const phase_correlation &) = delete;
+  phase_correlation &operator=(const phase_correlation &) = delete;
+  bool valid() const { return h_ != nullptr; }
+  bool set_reference(const image &ref) { return h_ && ssg_phasecorr_set_reference(h_, &ref) == SSG_OK; }
+  // per frame (Tx, Ty, peak) with ref(x) = frame(x + T)
+  bool estimate(const std::vector<image> &frames, std::vector<double> &out) {
+    if (!h_) return false;
+    out.resize(3 * frames.size());
+    std::vector<ssg_image> f(frames.begin(), frames.end());
+    return ssg_phasecorr_estimate(h_, (int)f.size(), f.data(), out.data()) == SSG_OK;
+  }
+
+ private:
+  ssg_phasecorr *h_ = nullptr;
+};
+
+// lpg() sharpness map (core/proc/lpg.cc:277-344) and its mean
+inline bool lpg(const image &src, image &dst, double k = 6, double p = 2, double *sharpness = nullptr) {
+  return ssg_lpg(&src, &dst, k, p, sharpness) == SSG_OK;
+}
+
+// compute_local_variance_map (c_local_variance_sharpness_measure.cc:193-249)
+inline bool local_variance_map(const image &src, image &map, int kradius = 1, int dscale = 1, double *q = nullptr) {
+  return ssg_local_variance_map(&src, kradius, dscale, &map, q) == SSG_OK;
+}
 
 // Per-frame scalar weights of the C1 stacking configuration (mean squared laplacian).
 inline bool frame_sharpness(const std::vector<image> &frames, std::vector<float> &weights) {

@@ -2,7 +2,8 @@
 // written the way the stacking pipeline calls c_frame_registration and
 // c_weigthed_average (c_image_stacking_pipeline.cc:1539-1776): set the
 // reference, register every frame (per frame, then batched), warp + stack,
-// compute.  Prints one JSON line; the GPU test compares it with the Python
+// compute; then the same frames refined by c_eccflow and stacked with their
+// dense remap fields.  Prints one JSON line; the GPU test compares it with the Python
 // binding's results on the same seeded frames.
 //
 //   host_api_demo <width> <height> <frames>
@@ -119,11 +120,42 @@ int main(int argc, char** argv) {
     covered += mask[i] != 0;
   }
 
+  // dense refinement: the transform fields of the batch, refined by c_eccflow, then the fused
+  // warp + stack with the per-frame fields
+  float* maps = nullptr;
+  CHECK(hipMalloc((void**)&maps, sizeof(float) * 2 * plane * N) == hipSuccess);
+  std::vector<ssg::image> rmaps;
+  for (int i = 0; i < N; ++i) rmaps.push_back(ssg::image::map2f(maps + 2 * plane * i, W, H, true));
+  std::vector<ssg::registration_status> st2;
+  CHECK(reg.register_batch_remaps(frames, st2, rmaps));
+  ssg::eccflow_options fo = ssg::default_eccflow_options();
+  fo.downscale = SSG_ECCFLOW_DOWNSCALE_PYRAMID;
+  fo.support_scale = 3;
+  fo.max_iterations = 2;
+  fo.min_image_size = -1;
+  fo.max_pyramid_level = 3;
+  ssg::c_eccflow flow(fo);
+  CHECK(flow.valid());
+  CHECK(flow.set_reference_image(ref));
+  CHECK(flow.compute_batch(frames, rmaps));
+  ssg::c_weigthed_average acc2(W, H, 1);
+  CHECK(acc2.add_remapped_batch(frames, rmaps, &wts));
+  CHECK(acc2.compute(aview, &kview));
+  double fsum = 0;
+  long fcovered = 0;
+  for (size_t i = 0; i < plane; ++i) {
+    fsum += avg[i];
+    fcovered += mask[i] != 0;
+  }
+  hipFree(maps);
+
   std::printf("{\"device\": \"%s\", \"frames\": %d, \"plain_equals_batch\": %d, \"accumulated\": %d, "
-              "\"map_err\": %.3g, \"avg_sum\": %.17g, \"covered\": %ld, \"params0\": [%.9g, %.9g, %.9g, %.9g, %.9g, "
+              "\"map_err\": %.3g, \"avg_sum\": %.17g, \"covered\": %ld, \"flow_levels\": %d, "
+              "\"flow_avg_sum\": %.17g, \"flow_covered\": %ld, \"params0\": [%.9g, %.9g, %.9g, %.9g, %.9g, "
               "%.9g], \"iterations\": [",
-              name, N, same, acc.accumulated_frames(), map_err, sum, covered, batch[0].params[0], batch[0].params[1],
-              batch[0].params[2], batch[0].params[3], batch[0].params[4], batch[0].params[5]);
+              name, N, same, acc.accumulated_frames(), map_err, sum, covered, flow.num_levels(), fsum, fcovered,
+              batch[0].params[0], batch[0].params[1], batch[0].params[2], batch[0].params[3], batch[0].params[4],
+              batch[0].params[5]);
   for (int i = 0; i < N; ++i) std::printf("%s%d", i ? ", " : "", batch[i].iterations);
   std::printf("]}\n");
   hipFree(dev);

@@ -49,3 +49,16 @@ def test_cpp_host_api_matches_python_binding():
     avg = np.asarray(avg, np.float64)
     assert abs(avg.sum() - out["avg_sum"]) <= 1e-12 * max(1.0, abs(out["avg_sum"]))
     assert int((np.asarray(mask) != 0).sum()) == out["covered"]
+    # eccflow refinement + dense-field stack
+    maps = [torch.zeros((H, W, 2), dtype=torch.float32, device="cuda") for _ in range(N)]
+    reg.register_batch_remaps(frames, maps)
+    flow = S.c_eccflow(downscale=2, support_scale=3, max_iterations=2, min_image_size=-1, max_pyramid_level=3)
+    flow.set_reference_image(ref)
+    assert flow.num_levels() == out["flow_levels"]
+    flow.compute_batch(frames, maps, rmaps=maps)
+    acc2 = S.c_weigthed_average(W, H, 1)
+    acc2.add_remapped_batch(frames, maps, frame_weights=wts)
+    favg, fmask = acc2.compute()
+    favg = np.asarray(favg, np.float64)
+    assert abs(favg.sum() - out["flow_avg_sum"]) <= 1e-12 * max(1.0, abs(out["flow_avg_sum"]))
+    assert int((np.asarray(fmask) != 0).sum()) == out["flow_covered"]
