@@ -301,6 +301,145 @@ __global__ void __launch_bounds__(256) k_accumulate(AccumArgs a) {
       }
     }
     const bool interior = mnx >= lo_m && mny >= lo_m && mxx <= hi_x && mxy <= hi_y;
+    // Interior tiles with bilinear sampling take a phased path: every lane's source positions
+    // first, then all loads (raw buffer loads; lanes outside the destination sample a clamped
+    // pixel, whose position lies inside the tile's hull), then the weighted updates.  Two
+    // interior frames with scalar weights go together, so both frames' samples are in flight at
+    // once; per pixel the updates stay in frame order and the compensated fold keeps its
+    // 16-frame boundaries, so the sums are those of the one-frame path.
+    if constexpr (INTERP == INTER_LINEAR) {
+      if (interior) {
+        int k2 = kend;
+        MapCoef mc2;
+        if constexpr (!DENSE && !WMAP) {
+          if (ingroup + 2 <= AC_GROUP) {
+            int kk = k + 1;
+            while (kk < kend && !a.valid[kk]) ++kk;
+            if (kk < kend) {
+#pragma unroll
+              for (int i = 0; i < 8; ++i) mc2.c[i] = a.maps[kk].c[i];
+              float n0 = 1e30f, n1 = -1e30f, n2 = 1e30f, n3 = -1e30f;
+              const int cx[4] = {ex0, ex1, ex0, ex1}, cy[4] = {ey0, ey0, ey1, ey1};
+#pragma unroll
+              for (int i = 0; i < 4; ++i) {
+                float sx, sy;
+                warp_xy<MOTION>(mc2, cx[i], cy[i], sx, sy);
+                n0 = fminf(n0, sx); n1 = fmaxf(n1, sx); n2 = fminf(n2, sy); n3 = fmaxf(n3, sy);
+                if (sx != sx || sy != sy) n0 = -1e30f;
+              }
+              if (n0 >= lo_m && n2 >= lo_m && n1 <= hi_x && n3 <= hi_y) k2 = kk;
+            }
+          }
+        }
+        const bool pair = k2 < kend;
+        const int xc = min(x, a.dw - 1);
+        const uint32_t pb = (uint32_t)a.src_pitch * 4u;
+        int X1[AC_RPT], Y1[AC_RPT], X2[AC_RPT], Y2[AC_RPT];
+#pragma unroll
+        for (int r = 0; r < AC_RPT; ++r) {
+          const int yc = min(y0 + wv * AC_RPT + r, a.dh - 1);
+          float sx, sy;
+          if constexpr (DENSE) {
+            const float2 q = smap[(yc - y0 + 2) * (AC_TW + 4) + (xc - x0) + 2];
+            sx = q.x;
+            sy = q.y;
+          } else {
+            warp_xy<MOTION>(mc, xc, yc, sx, sy);
+          }
+          X1[r] = quant32(sx);
+          Y1[r] = quant32(sy);
+          if (pair) {
+            warp_xy<MOTION>(mc2, xc, yc, sx, sy);
+            X2[r] = quant32(sx);
+            Y2[r] = quant32(sy);
+          }
+        }
+        const rsrc_t r1 = make_rsrc(img, (uint32_t)a.sh * pb);
+        const rsrc_t r2 = make_rsrc(pair ? a.frames[k2] : img, (uint32_t)a.sh * pb);
+        float t1[AC_RPT][4][CN], t2[AC_RPT][4][CN];
+#pragma unroll
+        for (int r = 0; r < AC_RPT; ++r) {
+          const uint32_t o = (uint32_t)(Y1[r] >> 5) * pb + (uint32_t)(X1[r] >> 5) * (4u * CN);
+#pragma unroll
+          for (int c = 0; c < CN; ++c) {
+            t1[r][0][c] = bload_f32(r1, o + 4u * c, 0u);
+            t1[r][1][c] = bload_f32(r1, o + 4u * (CN + c), 0u);
+            t1[r][2][c] = bload_f32(r1, o + pb + 4u * c, 0u);
+            t1[r][3][c] = bload_f32(r1, o + pb + 4u * (CN + c), 0u);
+          }
+        }
+        if (pair) {
+#pragma unroll
+          for (int r = 0; r < AC_RPT; ++r) {
+            const uint32_t o = (uint32_t)(Y2[r] >> 5) * pb + (uint32_t)(X2[r] >> 5) * (4u * CN);
+#pragma unroll
+            for (int c = 0; c < CN; ++c) {
+              t2[r][0][c] = bload_f32(r2, o + 4u * c, 0u);
+              t2[r][1][c] = bload_f32(r2, o + 4u * (CN + c), 0u);
+              t2[r][2][c] = bload_f32(r2, o + pb + 4u * c, 0u);
+              t2[r][3][c] = bload_f32(r2, o + pb + 4u * (CN + c), 0u);
+            }
+          }
+        }
+        const float wk2 = pair ? a.weights[k2] : 0.f;
+        const float* __restrict__ wimg = WMAP ? a.wmaps[k] : nullptr;
+#pragma unroll
+        for (int r = 0; r < AC_RPT; ++r) {
+          const int y = y0 + wv * AC_RPT + r;
+          if (x >= a.dw || y >= a.dh) continue;
+          float w0, w1, w2, w3;
+          bilinear_w(X1[r] & 31, Y1[r] & 31, w0, w1, w2, w3);
+          float wp;
+          if (WMAP) {
+            float wv1[1];
+            lin_fast<1>(wimg, a.wmap_pitch, X1[r], Y1[r], wv1);
+            wp = wv1[0];
+          } else {
+            wp = const_sample<INTERP>(wk, a.sw, a.sh, X1[r], Y1[r]);
+          }
+#pragma unroll
+          for (int c = 0; c < CN; ++c)
+            sg[r][c] = fmaf(wp, bilinear_sum(t1[r][0][c], t1[r][1][c], t1[r][2][c], t1[r][3][c], w0, w1, w2, w3),
+                            sg[r][c]);
+          wg[r] += wp;
+          cnt[r] += 1;
+          if (pair) {
+            bilinear_w(X2[r] & 31, Y2[r] & 31, w0, w1, w2, w3);
+            const float wp2 = const_sample<INTERP>(wk2, a.sw, a.sh, X2[r], Y2[r]);
+#pragma unroll
+            for (int c = 0; c < CN; ++c)
+              sg[r][c] = fmaf(wp2, bilinear_sum(t2[r][0][c], t2[r][1][c], t2[r][2][c], t2[r][3][c], w0, w1, w2, w3),
+                              sg[r][c]);
+            wg[r] += wp2;
+            cnt[r] += 1;
+          }
+        }
+        if (DENSE) __syncthreads();  // the next frame rewrites the staged field
+        const int done = pair ? 2 : 1;
+        if (pair) k = k2;
+        ingroup += done;
+        if (ingroup == AC_GROUP) {
+          ingroup = 0;
+#pragma unroll
+          for (int r = 0; r < AC_RPT; ++r) {
+#pragma unroll
+            for (int c = 0; c < CN; ++c) {
+              float s_, e_;
+              two_sum(shi[r][c], sg[r][c], s_, e_);
+              shi[r][c] = s_;
+              slo[r][c] += e_;
+              sg[r][c] = 0.f;
+            }
+            float s_, e_;
+            two_sum(whi[r], wg[r], s_, e_);
+            whi[r] = s_;
+            wlo[r] += e_;
+            wg[r] = 0.f;
+          }
+        }
+        continue;
+      }
+    }
     if (!interior) {
       // warped mask for the tile + 2-px halo; outside the destination counts as 255
       for (int i = tid; i < (AC_TH + 4) * (AC_TW + 4); i += 256) {

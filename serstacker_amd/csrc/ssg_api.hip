@@ -1793,7 +1793,12 @@ int accumulate_fused(ssg_accumulation* a, int n, const ssg_image* frames, int mo
   HIPCHK(hipMemcpyAsync(a->weights.p, w, sizeof(float) * n, hipMemcpyHostToDevice, s));
   HIPCHK(hipMemcpyAsync(a->valid.p, valid, sizeof(int) * n, hipMemcpyHostToDevice, s));
   HIPCHK(hipMemcpyAsync(a->frame_ptrs.p, ptrs.data(), sizeof(void*) * n, hipMemcpyHostToDevice, s));
-  // chunks: enough blocks to fill the chip, each chunk a contiguous frame range
+  AccumArgs g{};
+  g.cn = a->cn;
+  g.interp = interpolation == SSG_INTER_CUBIC ? INTER_CUBIC : INTER_LINEAR;
+  g.wmaps = wmaps;
+  // chunks: about 4096 blocks over the tiles, each chunk a contiguous frame range (measured:
+  // more chunks of fewer frames each run slower)
   const long long tiles = (long long)((a->w + 63) / 64) * ((a->h + 15) / 16);
   int nch = (int)std::max(1LL, std::min<long long>(16, (4096 + tiles - 1) / tiles));
   nch = std::min(nch, n);
@@ -1801,24 +1806,20 @@ int accumulate_fused(ssg_accumulation* a, int n, const ssg_image* frames, int mo
   nch = (n + per - 1) / per;
   CHK(a->part.ensure(sizeof(float) * a->plane * (2 * a->cn + 2) * nch));
   CHK(a->part_cnt.ensure(sizeof(unsigned) * a->plane * nch));
-  AccumArgs g{};
   g.frames = a->frame_ptrs.as<const float*>();
   g.src_pitch = (int)(step / 4);
   g.sw = sw;
   g.sh = sh;
-  g.cn = a->cn;
   g.maps = rmaps ? nullptr : a->maps.as<MapCoef>();
   g.rmaps = dmaps;
   g.rmap_pitch = rmaps ? (int)(rmaps[0].step / 8) : 0;
   g.weights = a->weights.as<float>();
-  g.wmaps = wmaps;
   g.wmap_pitch = weight_maps ? (int)(weight_maps[0].step / 4) : 0;
   g.valid = a->valid.as<int>();
   g.nframes = n;
   g.per_chunk = per;
   g.dw = a->w;
   g.dh = a->h;
-  g.interp = interpolation == SSG_INTER_CUBIC ? INTER_CUBIC : INTER_LINEAR;
   g.border = border_mode;
   g.cval = (float)border_value;
   g.part = a->part.as<float>();
