@@ -2049,7 +2049,7 @@ extern "C" int ssg_accumulation_get_profile(ssg_accumulation* a, ssg_profile* ou
 struct ssg_phasecorr {
   int w = 0, h = 0, binning = 1, bw = 0, bh = 0, M = 0, N = 0;
   hipStream_t stream = nullptr;
-  DevBuf wx, wy, tw_m, tw_n, ref_spec, buf, tmp, frame_ptrs, out, mean;
+  DevBuf wx, wy, tw_m, tw_n, ref_spec, buf, tmp, frame_ptrs, out, mean, part, pv, pi;
   int chunk = 1;
   bool have_ref = false;
   Profiler prof;
@@ -2072,8 +2072,9 @@ int pow2_at_least(int n) {
 int pc_spectra(ssg_phasecorr* pc, int n, const void* const* dev_ptrs, bool u16, int pitch) {
   hipStream_t s = pc->stream;
   CHK(pc->mean.ensure(sizeof(float) * n));
-  HIPCHK(launch_pc_load(dev_ptrs, u16, pitch, pc->bw, pc->bh, pc->binning, pc->mean.as<float>(), pc->wx.as<float>(),
-                        pc->wy.as<float>(), pc->buf.as<float2>(), pc->M, pc->N, n, s));
+  CHK(pc->part.ensure(sizeof(double) * PC_PARTS_MAX * n));
+  HIPCHK(launch_pc_load(dev_ptrs, u16, pitch, pc->bw, pc->bh, pc->binning, pc->mean.as<float>(), pc->part.as<double>(),
+                        pc->wx.as<float>(), pc->wy.as<float>(), pc->buf.as<float2>(), pc->M, pc->N, n, s));
   HIPCHK(launch_fft2(pc->buf.as<float2>(), pc->tmp.as<float2>(), pc->M, pc->N, n, pc->tw_m.as<float2>(),
                      pc->tw_n.as<float2>(), false, s));
   return SSG_OK;
@@ -2184,7 +2185,10 @@ extern "C" int ssg_phasecorr_estimate(ssg_phasecorr* pc, int n, const ssg_image*
     HIPCHK(launch_pc_cross(pc->tmp.as<float2>(), pc->ref_spec.as<float2>(), pc->M, pc->N, m, s));
     HIPCHK(launch_fft2(pc->tmp.as<float2>(), pc->buf.as<float2>(), pc->M, pc->N, m, pc->tw_m.as<float2>(),
                        pc->tw_n.as<float2>(), true, s));
-    HIPCHK(launch_pc_peak(pc->buf.as<float2>(), pc->M, pc->N, pc->binning, m, pc->out.as<double>() + 3 * f0, s));
+    CHK(pc->pv.ensure(sizeof(float) * PC_PARTS_MAX * m));
+    CHK(pc->pi.ensure(sizeof(int) * PC_PARTS_MAX * m));
+    HIPCHK(launch_pc_peak(pc->buf.as<float2>(), pc->M, pc->N, pc->binning, m, pc->pv.as<float>(), pc->pi.as<int>(),
+                          pc->out.as<double>() + 3 * f0, s));
   }
   pc->prof.stop(s, SSG_PROF_FFT);
   HIPCHK(hipMemcpyAsync(out, pc->out.p, sizeof(double) * 3 * n, hipMemcpyDeviceToHost, s));

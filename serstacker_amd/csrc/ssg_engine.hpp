@@ -257,10 +257,13 @@ hipError_t launch_resize_linear(const float* src, int sw, int sh, float* dst, in
 // FFT phase correlation (ssg_fft.hip)
 hipError_t launch_fft2(float2* buf, float2* tmp, int M, int N, int nframes, const float2* tw_m, const float2* tw_n,
                        bool inverse, hipStream_t s);
+constexpr int PC_PARTS_MAX = 64;  // blocks per frame of the phase-correlation reductions
 hipError_t launch_pc_load(const void* const* frames, bool u16, int pitch, int bw, int bh, int binning, float* mean,
-                          const float* wx, const float* wy, float2* out, int M, int N, int nframes, hipStream_t s);
+                          double* part, const float* wx, const float* wy, float2* out, int M, int N, int nframes,
+                          hipStream_t s);
 hipError_t launch_pc_cross(float2* spec, const float2* ref, int M, int N, int nframes, hipStream_t s);
-hipError_t launch_pc_peak(const float2* r, int M, int N, int binning, int nframes, double* out, hipStream_t s);
+hipError_t launch_pc_peak(const float2* r, int M, int N, int binning, int nframes, float* pv, int* pi, double* out,
+                          hipStream_t s);
 
 // dense ECC flow (ssg_eccflow.hip)
 struct AreaGeom {  // one cv::resize INTER_AREA geometry, sw x sh -> dw x dh

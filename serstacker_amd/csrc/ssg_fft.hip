@@ -80,21 +80,33 @@ __device__ __forceinline__ float binned(const void* src, int pitch, int binning,
 }
 
 // mean of the binned image of frame blockIdx.x (double accumulation)
+constexpr int PC_PARTS = PC_PARTS_MAX;  // blocks per frame of the two-stage reductions
+
+// stage 1: double sums of the binned frame over row bands, one partial per block (PC_PARTS per frame)
 template <bool U16>
-__global__ void __launch_bounds__(1024) k_pc_mean(const void* const* frames, int pitch, int bw, int bh, int binning,
-                                                  float* mean) {
-  __shared__ double red[1024];
-  const void* src = frames[blockIdx.x];
+__global__ void __launch_bounds__(256) k_pc_mean_part(const void* const* frames, int pitch, int bw, int bh,
+                                                      int binning, double* part) {
+  __shared__ double red[256];
+  const void* src = frames[blockIdx.y];
+  const int y0 = (int)((long long)bh * blockIdx.x / PC_PARTS), y1 = (int)((long long)bh * (blockIdx.x + 1) / PC_PARTS);
   double s = 0;
-  for (int y = threadIdx.x >> 6; y < bh; y += 16)
+  for (int y = y0 + (threadIdx.x >> 6); y < y1; y += 4)
     for (int x = threadIdx.x & 63; x < bw; x += 64) s += (double)binned<U16>(src, pitch, binning, x, y);
   red[threadIdx.x] = s;
   __syncthreads();
-  for (int k = 512; k > 0; k >>= 1) {
+  for (int k = 128; k > 0; k >>= 1) {
     if (threadIdx.x < k) red[threadIdx.x] += red[threadIdx.x + k];
     __syncthreads();
   }
-  if (threadIdx.x == 0) mean[blockIdx.x] = (float)(red[0] / ((double)bw * bh));
+  if (threadIdx.x == 0) part[(size_t)blockIdx.y * PC_PARTS + blockIdx.x] = red[0];
+}
+
+// stage 2: the partials of a frame in order, then the mean
+__global__ void k_pc_mean_fin(const double* part, int bw, int bh, float* mean) {
+  if (threadIdx.x != 0) return;
+  double s = 0;
+  for (int i = 0; i < PC_PARTS; ++i) s += part[(size_t)blockIdx.x * PC_PARTS + i];
+  mean[blockIdx.x] = (float)(s / ((double)bw * bh));
 }
 
 // binning, mean removal, apodisation and zero padding into the complex buffer [N][M]
@@ -128,15 +140,16 @@ __global__ void __launch_bounds__(256) k_pc_cross(float2* spec, const float2* re
   *s = mag > 0.f ? make_float2(re / mag * wgt, im / mag * wgt) : make_float2(0.f, 0.f);
 }
 
-// per frame: argmax of the real part (first index on ties), parabolic refinement in FP64
-__global__ void __launch_bounds__(1024) k_pc_peak(const float2* r, int M, int N, int binning, double* out) {
-  __shared__ float sv[1024];
-  __shared__ int si[1024];
-  const float2* p = r + (size_t)blockIdx.x * M * N;
+// stage 1: argmax of the real part over a band of the frame (first index on ties)
+__global__ void __launch_bounds__(256) k_pc_peak_part(const float2* r, int M, int N, float* pv, int* pi) {
+  __shared__ float sv[256];
+  __shared__ int si[256];
+  const float2* p = r + (size_t)blockIdx.y * M * N;
   const int total = M * N;
+  const int i0 = (int)((long long)total * blockIdx.x / PC_PARTS), i1 = (int)((long long)total * (blockIdx.x + 1) / PC_PARTS);
   float best = -3.4e38f;
-  int bi = 0;
-  for (int i = threadIdx.x; i < total; i += 1024) {
+  int bi = i0;
+  for (int i = i0 + threadIdx.x; i < i1; i += 256) {
     const float v = p[i].x;
     if (v > best) {
       best = v;
@@ -146,7 +159,7 @@ __global__ void __launch_bounds__(1024) k_pc_peak(const float2* r, int M, int N,
   sv[threadIdx.x] = best;
   si[threadIdx.x] = bi;
   __syncthreads();
-  for (int s = 512; s > 0; s >>= 1) {
+  for (int s = 128; s > 0; s >>= 1) {
     if (threadIdx.x < s) {
       const float v = sv[threadIdx.x + s];
       const int j = si[threadIdx.x + s];
@@ -158,7 +171,27 @@ __global__ void __launch_bounds__(1024) k_pc_peak(const float2* r, int M, int N,
     __syncthreads();
   }
   if (threadIdx.x == 0) {
-    const int i = si[0], py = i / M, px = i - py * M;
+    pv[(size_t)blockIdx.y * PC_PARTS + blockIdx.x] = sv[0];
+    pi[(size_t)blockIdx.y * PC_PARTS + blockIdx.x] = si[0];
+  }
+}
+
+// stage 2, per frame: the band maxima (first index on ties), then the 3-point fits in FP64
+__global__ void k_pc_peak(const float2* r, int M, int N, int binning, const float* pv, const int* pi,
+                          double* out) {
+  const float2* p = r + (size_t)blockIdx.x * M * N;
+  if (threadIdx.x == 0) {
+    float best = pv[(size_t)blockIdx.x * PC_PARTS];
+    int bi = pi[(size_t)blockIdx.x * PC_PARTS];
+    for (int b = 1; b < PC_PARTS; ++b) {
+      const float v = pv[(size_t)blockIdx.x * PC_PARTS + b];
+      const int j = pi[(size_t)blockIdx.x * PC_PARTS + b];
+      if (v > best || (v == best && j < bi)) {
+        best = v;
+        bi = j;
+      }
+    }
+    const int i = bi, py = i / M, px = i - py * M;
     const double r0 = p[i].x;
     const double rxm = p[(size_t)py * M + (px + M - 1) % M].x, rxp = p[(size_t)py * M + (px + 1) % M].x;
     const double rym = p[(size_t)((py + N - 1) % N) * M + px].x, ryp = p[(size_t)((py + 1) % N) * M + px].x;
@@ -179,6 +212,7 @@ __global__ void __launch_bounds__(1024) k_pc_peak(const float2* r, int M, int N,
     out[3 * blockIdx.x + 2] = r0;
   }
 }
+
 
 int ilog2(int n) {
   int l = 0;
@@ -211,13 +245,18 @@ hipError_t launch_fft2(float2* buf, float2* tmp, int M, int N, int nframes, cons
 }
 
 hipError_t launch_pc_load(const void* const* frames, bool u16, int pitch, int bw, int bh, int binning, float* mean,
-                          const float* wx, const float* wy, float2* out, int M, int N, int nframes, hipStream_t s) {
+                          double* part, const float* wx, const float* wy, float2* out, int M, int N, int nframes,
+                          hipStream_t s) {
   dim3 grid((M + 63) / 64, (N + 3) / 4, nframes);
   if (u16) {
-    hipLaunchKernelGGL(k_pc_mean<true>, dim3(nframes), dim3(1024), 0, s, frames, pitch, bw, bh, binning, mean);
+    hipLaunchKernelGGL(k_pc_mean_part<true>, dim3(PC_PARTS, nframes), dim3(256), 0, s, frames, pitch, bw, bh, binning,
+                       part);
+    hipLaunchKernelGGL(k_pc_mean_fin, dim3(nframes), dim3(64), 0, s, part, bw, bh, mean);
     hipLaunchKernelGGL(k_pc_load<true>, grid, dim3(256), 0, s, frames, pitch, bw, bh, binning, mean, wx, wy, out, M, N);
   } else {
-    hipLaunchKernelGGL(k_pc_mean<false>, dim3(nframes), dim3(1024), 0, s, frames, pitch, bw, bh, binning, mean);
+    hipLaunchKernelGGL(k_pc_mean_part<false>, dim3(PC_PARTS, nframes), dim3(256), 0, s, frames, pitch, bw, bh, binning,
+                       part);
+    hipLaunchKernelGGL(k_pc_mean_fin, dim3(nframes), dim3(64), 0, s, part, bw, bh, mean);
     hipLaunchKernelGGL(k_pc_load<false>, grid, dim3(256), 0, s, frames, pitch, bw, bh, binning, mean, wx, wy, out, M, N);
   }
   return hipGetLastError();
@@ -229,8 +268,10 @@ hipError_t launch_pc_cross(float2* spec, const float2* ref, int M, int N, int nf
   return hipGetLastError();
 }
 
-hipError_t launch_pc_peak(const float2* r, int M, int N, int binning, int nframes, double* out, hipStream_t s) {
-  hipLaunchKernelGGL(k_pc_peak, dim3(nframes), dim3(1024), 0, s, r, M, N, binning, out);
+hipError_t launch_pc_peak(const float2* r, int M, int N, int binning, int nframes, float* pv, int* pi, double* out,
+                          hipStream_t s) {
+  hipLaunchKernelGGL(k_pc_peak_part, dim3(PC_PARTS, nframes), dim3(256), 0, s, r, M, N, pv, pi);
+  hipLaunchKernelGGL(k_pc_peak, dim3(nframes), dim3(64), 0, s, r, M, N, binning, pv, pi, out);
   return hipGetLastError();
 }
 
