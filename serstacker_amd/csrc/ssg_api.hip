@@ -2725,14 +2725,15 @@ int eccflow_run(ssg_eccflow* f, int n, const float* const* in, int in_pitch, con
     for (int l = L - 1; l >= 0; --l) {
       const int w = f->lw[l], h = f->lh[l];
       if (l < L - 1) {
+        // cv::multiply(uv, size_ratio, uv) at the coarser level, then the INTER_CUBIC upscale
+        HIPCHK(launch_flow_scale(ua, ufs, f->lw[l + 1], f->lh[l + 1], (double)w / (double)f->lw[l + 1],
+                                 (double)h / (double)f->lh[l + 1], m, s));
         FlowCubicArgs a = cubic_args(f->lvl_tab[l], base);
         a.src = ua;
         a.sfs = ufs;
         a.dst = ub;
         a.dfs = ufs;
-        a.rx = (double)w / (double)f->lw[l + 1];
-        a.ry = (double)h / (double)f->lh[l + 1];
-        HIPCHK(launch_flow_cubic(CUBIC_SCALED, a, m, s));
+        HIPCHK(launch_flow_cubic(CUBIC_PLAIN, a, m, s));
         std::swap(ua, ub);
       }
       FlowLevelArgs la{};

@@ -208,7 +208,7 @@ hipError_t launch_flow_solve(const float* cells, long long cfs, const float4* D,
 // h = ((S0 a0 + S1 a1) + S2 a2) + S3 a3, then vertically in the same order.  The taps and
 // float coefficients of each axis come from host tables (CubicAxis).  Modes:
 //   CUBIC_ADD   : out = base + cubic(src)                         (cv::add(uv, cuv, uv))
-//   CUBIC_SCALED: out = cubic((float)((double)src * ratio))        (multiply, then upscale)
+//   CUBIC_PLAIN : out = cubic(src)                                (the level upscale, after k_flow_scale)
 //   CUBIC_INIT  : out = (float)((double)cubic(map - identity) * ratio)  (initial flow)
 // ---------------------------------------------------------------------------------------------
 template <int MODE>
@@ -235,9 +235,6 @@ __global__ void __launch_bounds__(256) k_flow_cubic(FlowCubicArgs a) {
       if (MODE == CUBIC_INIT) {
         t.x = t.x - (float)cx[k];
         t.y = t.y - (float)ry;
-      } else if (MODE == CUBIC_SCALED) {
-        t.x = (float)((double)t.x * a.rx);
-        t.y = (float)((double)t.y * a.ry);
       }
       v[k] = t;
     }
@@ -262,7 +259,7 @@ hipError_t launch_flow_cubic(int mode, const FlowCubicArgs& a, int nframes, hipS
   dim3 grid((a.dw + 63) / 64, (a.dh + 3) / 4, nframes), block(64, 4);
   switch (mode) {
     case CUBIC_ADD: hipLaunchKernelGGL(k_flow_cubic<CUBIC_ADD>, grid, block, 0, s, a); break;
-    case CUBIC_SCALED: hipLaunchKernelGGL(k_flow_cubic<CUBIC_SCALED>, grid, block, 0, s, a); break;
+    case CUBIC_PLAIN: hipLaunchKernelGGL(k_flow_cubic<CUBIC_PLAIN>, grid, block, 0, s, a); break;
     default: hipLaunchKernelGGL(k_flow_cubic<CUBIC_INIT>, grid, block, 0, s, a); break;
   }
   return hipGetLastError();
@@ -380,6 +377,21 @@ hipError_t launch_flow_iter(bool add, bool resid, bool to_map, const FlowIterArg
   else if (add && to_map) hipLaunchKernelGGL((k_flow_iter<true, false, true>), grid, block, 0, s, a);
   else if (add) hipLaunchKernelGGL((k_flow_iter<true, false, false>), grid, block, 0, s, a);
   else return hipErrorInvalidValue;
+  return hipGetLastError();
+}
+
+// cv::multiply(uv, Scalar(rx, ry), uv) in place: each product taken in double, rounded to float
+__global__ void k_flow_scale(float2* __restrict__ uv, long long ufs, int w, int h, double rx, double ry) {
+  const int x = blockIdx.x * 64 + threadIdx.x, y = blockIdx.y * 4 + threadIdx.y;
+  if (x >= w || y >= h) return;
+  float2* p = uv + blockIdx.z * ufs + (size_t)y * w + x;
+  const float2 u = *p;
+  *p = make_float2((float)((double)u.x * rx), (float)((double)u.y * ry));
+}
+
+hipError_t launch_flow_scale(float2* uv, long long ufs, int w, int h, double rx, double ry, int nframes, hipStream_t s) {
+  dim3 grid((w + 63) / 64, (h + 3) / 4, nframes), block(64, 4);
+  hipLaunchKernelGGL(k_flow_scale, grid, block, 0, s, uv, ufs, w, h, rx, ry);
   return hipGetLastError();
 }
 

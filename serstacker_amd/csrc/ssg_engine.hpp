@@ -289,7 +289,7 @@ struct FlowLevelArgs {
   const uint8_t* rmask;  // reference mask level or null
   int w, h;
 };
-enum { CUBIC_ADD = 0, CUBIC_SCALED = 1, CUBIC_INIT = 2 };
+enum { CUBIC_ADD = 0, CUBIC_PLAIN = 1, CUBIC_INIT = 2 };
 struct FlowCubicArgs {
   const float2* src;  // source flow, frame-strided (sfs), pitch sw
   long long sfs;
@@ -335,6 +335,7 @@ hipError_t launch_flow_residual(const FlowLevelArgs& a, const float2* uv, long l
 hipError_t launch_flow_solve(const float* cells, long long cfs, const float4* D, int cw, int ch, float2* cuv,
                              long long ufs, int nframes, hipStream_t s);
 hipError_t launch_flow_cubic(int mode, const FlowCubicArgs& a, int nframes, hipStream_t s);
+hipError_t launch_flow_scale(float2* uv, long long ufs, int w, int h, double rx, double ry, int nframes, hipStream_t s);
 hipError_t launch_flow_to_remap(const float2* uv, long long ufs, int w, int h, float2* const* maps, int pitch,
                                 int nframes, hipStream_t s);
 hipError_t launch_nearest_u8(const uint8_t* src, long long sfs, int sw, int sh, uint8_t* dst, long long dfs, int dw,
