@@ -519,14 +519,21 @@ def bench_sweep(a, S, _lib, torch):
     # focus stacking: 16 RGB f32 frames of 2048x1536 per stack (add + compute)
     W, H, n = 2048, 1536, 16
     fr = torch.rand((n, H, W, 3), generator=g).to(dev)
-    for pol in (_lib.FOCUS_SELECT_MAX_ENERGY, _lib.FOCUS_LPG_WEIGHTS):
-        def stack(pol=pol):
-            fs = S.c_laplacian_pyramid_focus_stacking(W, H, 3, fusing_policy=pol)
-            for i in range(n):
-                fs.add(fr[i])
-            fs.compute()
-        timed(f"focus_stack_{'lpg' if pol == _lib.FOCUS_LPG_WEIGHTS else 'max_energy'}_16x2048x1536_rgb", stack,
-              n * 12 * W * H, n, "frames")
+    fused = torch.empty((H, W, 3), dtype=torch.float32, device=dev)
+    for pol in (_lib.FOCUS_SELECT_MAX_ENERGY, _lib.FOCUS_WEIGHTED_AVERAGE, _lib.FOCUS_LPG_WEIGHTS):
+        for batched in (False, True):
+            def stack(pol=pol, batched=batched):
+                fs = S.c_laplacian_pyramid_focus_stacking(W, H, 3, fusing_policy=pol)
+                if batched:
+                    fs.add_frames([fr[i] for i in range(n)])
+                else:
+                    for i in range(n):
+                        fs.add(fr[i])
+                fs.compute(fused)
+            pname = {_lib.FOCUS_SELECT_MAX_ENERGY: "max_energy", _lib.FOCUS_WEIGHTED_AVERAGE: "weighted_average",
+                     _lib.FOCUS_LPG_WEIGHTS: "lpg"}[pol]
+            timed(f"focus_stack_{pname}{'_batched' if batched else ''}_16x2048x1536_rgb", stack, n * 12 * W * H, n,
+                  "frames")
     # lpg sharpness map of one RGB frame; local-variance weight map of a mono 1920x1080 frame
     out = torch.empty((H, W), dtype=torch.float32, device=dev)
     sv, ov = S.image_view(fr[0]), S.image_view(out)

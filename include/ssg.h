@@ -282,6 +282,12 @@ int ssg_local_variance_map(const ssg_image *image, int32_t kradius, int32_t dsca
  * sources are SSG_32FC1 or SSG_16UC1 raw frames of the accumulator size. */
 int ssg_accumulation_add(ssg_accumulation *a, const ssg_image *src, const ssg_image *weights);
 
+/* n successive add(frames[i]) calls without weights (no reference counterpart: a batch of the
+ * reference's per-frame adds, c_image_stacking_pipeline.cc:1644-1776).  Focus stacks build the
+ * laplacian pyramids of a batch together and fuse each level frame after frame per pixel in one
+ * launch; the results equal those of the single adds. */
+int ssg_accumulation_add_frames(ssg_accumulation *a, int n, const ssg_image *frames);
+
 /* Fused remap + base_remap mask + weighted add of n registered frames (the
  * per-frame sequence custom_remap -> weights*mask/255 -> add of
  * c_image_stacking_pipeline.cc:1644-1776 for every frame at once).

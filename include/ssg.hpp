@@ -251,6 +251,11 @@ class accumulation_handle : public c_frame_accumulation {
   bool compute(image &avg, image *mask = nullptr, double dscale = 1.0) const override {
     return a_ && ssg_accumulation_compute(a_, &avg, mask, dscale) == SSG_OK;
   }
+  // add() of each frame in order, batched (focus stacks build the batch's pyramids together)
+  bool add_frames(const std::vector<image> &frames) {
+    std::vector<ssg_image> f(frames.begin(), frames.end());
+    return a_ && ssg_accumulation_add_frames(a_, (int)f.size(), f.data()) == SSG_OK;
+  }
   bool get_acc_counters(image &accw) const override {
     return a_ && ssg_accumulation_get_acc_counters(a_, &accw) == SSG_OK;
   }

@@ -496,9 +496,16 @@ class c_laplacian_pyramid_focus_stacking:
             return False
         return True
 
-    def compute(self):
-        avg = np.zeros((self.h, self.w) if self.cn == 1 else (self.h, self.w, self.cn), np.float32)
-        av = _lib.host_image(avg)
+    def add_frames(self, frames):
+        """n successive add() calls, batched: pyramids built together, levels fused per pixel."""
+        views = image_views(frames)
+        _lib.check(self._lib.ssg_accumulation_add_frames(self._h, len(views), views), "add_frames")
+
+    def compute(self, out=None):
+        """The fused image; into `out` (host array or device tensor of the frame shape) when given."""
+        avg = out if out is not None else np.zeros((self.h, self.w) if self.cn == 1 else (self.h, self.w, self.cn),
+                                                   np.float32)
+        av = image_view(avg)
         _lib.check(self._lib.ssg_accumulation_compute(self._h, C.byref(av), None, 1.0), "compute")
         return avg
 

@@ -183,6 +183,7 @@ _SIGS = [
     ("ssg_accumulation_create", C.c_int, [C.c_int32, C.c_int32, C.c_int32, C.c_int32, _P(C.c_void_p)]),
     ("ssg_accumulation_destroy", None, [C.c_void_p]),
     ("ssg_accumulation_add", C.c_int, [C.c_void_p, _P(Image), _P(Image)]),
+    ("ssg_accumulation_add_frames", C.c_int, [C.c_void_p, C.c_int, _P(Image)]),
     ("ssg_accumulation_add_registered_batch", C.c_int, [C.c_void_p, C.c_int, _P(Image), _P(RegistrationStatus),
                                                         C.c_int32, _P(C.c_float), _P(Image), C.c_int32, C.c_int32,
                                                         C.c_double]),

@@ -1081,6 +1081,7 @@ struct ssg_accumulation {
   long long P = 0;  // elements of one single-plane pyramid
   int L = 0, wc = 1;
   DevBuf facc, fw, fp, fup, ft0, ft1, ft2, ft3, fgray, fm, fW, fptrs, fstd;
+  DevBuf fbpyr, fbup, fbW;  // batch planes of add_frames, kept for the next batch
   std::vector<const float*> hptrs;
   int slot = 0;
   SepKernel G{}, Wk{};
@@ -1468,6 +1469,91 @@ int focus_add(ssg_accumulation* a, const ssg_image* src) {
   return SSG_OK;
 }
 
+// n successive add() calls of a focus stack in batches of m frames: the laplacian pyramids of a
+// batch are built together (level planes [frame][channel][pixel], one launch per level and
+// step), and each level is fused frame after frame per pixel by one kernel; the results are
+// those of n single adds.  Smoothed energies (kradius/ksigma) couple neighbouring pixels between
+// frames, so that configuration adds frame by frame.
+int focus_add_frames(ssg_accumulation* a, int n, const ssg_image* frames) {
+  const int cn = a->cn;
+  for (int f = 0; f < n; ++f) {
+    CHK(check_image(&frames[f], "frame"));
+    if (frames[f].type != (cn == 3 ? SSG_32FC3 : SSG_32FC1) || frames[f].width != a->w || frames[f].height != a->h)
+      return fail(SSG_E_INVALID_ARG, "frame %d: f32 with the accumulator's size and %d channels", f, cn);
+  }
+  if (a->accumulated == 0) CHK(focus_prepare(a));
+  const ssg_focus_stacking_options& o = a->fo;
+  if (a->hasG && o.fusing_policy != SSG_FOCUS_LPG_WEIGHTS) {
+    for (int f = 0; f < n; ++f) CHK(focus_add(a, &frames[f]));
+    return SSG_OK;
+  }
+  hipStream_t s = a->stream;
+  const size_t n0 = (size_t)a->w * a->h;
+  const bool lpgw = o.fusing_policy == SSG_FOCUS_LPG_WEIGHTS;
+  const size_t per = sizeof(float) * ((size_t)cn * a->P + (size_t)cn * n0 + (lpgw ? a->P : 0));
+  const int chunk = (int)std::max<size_t>(1, std::min<size_t>({(size_t)n, (size_t)64, ((size_t)2 << 30) / per}));
+  DevBuf &pyr = a->fbpyr, &up = a->fbup, &W = a->fbW;
+  CHK(pyr.ensure(sizeof(float) * (size_t)cn * a->P * chunk));
+  CHK(up.ensure(sizeof(float) * (size_t)cn * n0 * chunk));
+  if (lpgw) CHK(W.ensure(sizeof(float) * a->P * chunk));
+  auto nl = [&](int i) { return (size_t)a->lw[i] * a->lh[i]; };
+  float* acc = a->facc.as<float>();
+  const int top = a->L - 1;
+  for (int f0 = 0; f0 < n; f0 += chunk) {
+    const int m = std::min(chunk, n - f0);
+    const bool first = a->accumulated == 0;
+    // level i of the batch: m * cn planes of nl(i) at pyr + m * cn * loff[i]
+    auto lv = [&](int i) { return pyr.as<float>() + (size_t)m * cn * a->loff[i]; };
+    auto lw1 = [&](int i) { return W.as<float>() + (size_t)m * a->loff[i]; };
+    a->slot = 0;
+    for (int k = 0; k < m; ++k) {
+      DevBuf ts;
+      const void* sp;
+      size_t spb;
+      CHK(device_view(&frames[f0 + k], ts, s, &sp, &spb));
+      float* planes = lv(0) + (size_t)k * cn * n0;
+      HIPCHK(launch_split((const float*)sp, (int)(spb / 4), a->w, a->h, cn, planes, s));
+      if (lpgw) {
+        // lpg(gray) of frame k into plane k of the pyrup scratch, blurred below for the whole batch
+        const double kk = o.lpg_k;
+        const float l2 = (100.f / 4.f) * (100.f / 4.f), g2 = (100.f / 36.f) * (100.f / 36.f);
+        const float alpha = (float)(kk / (kk + 1)) * l2, beta = (float)(1. / (kk + 1)) * g2;
+        HIPCHK(launch_gray(planes, n0, cn, a->fgray.as<float>(), s));
+        HIPCHK(launch_lpg(a->fgray.as<float>(), a->w, a->h, alpha, beta, 1e-9f, (float)o.lpg_p,
+                          up.as<float>() + (size_t)k * n0, s));
+      }
+      if (!frames[f0 + k].on_device) HIPCHK(hipStreamSynchronize(s));  // the staging copy goes out of scope
+    }
+    if (lpgw) CHK(blur_planes(a, up.as<float>(), lw1(0), a->w, a->h, m, a->Wk));
+    for (int i = 0; i + 1 < a->L; ++i) {
+      if (lpgw)
+        HIPCHK(launch_pyrdown(lw1(i), (long long)nl(i), a->lw[i], a->lw[i], a->lh[i], lw1(i + 1), (long long)nl(i + 1),
+                              a->lw[i + 1], a->lw[i + 1], a->lh[i + 1], m, SSG_BORDER_REFLECT101, s));
+      HIPCHK(launch_pyrdown(lv(i), (long long)nl(i), a->lw[i], a->lw[i], a->lh[i], lv(i + 1), (long long)nl(i + 1),
+                            a->lw[i + 1], a->lw[i + 1], a->lh[i + 1], m * cn, SSG_BORDER_REFLECT101, s));
+      HIPCHK(launch_pyrup(lv(i + 1), (long long)nl(i + 1), a->lw[i + 1], a->lh[i + 1], up.as<float>(), (long long)nl(i),
+                          a->lw[i], a->lh[i], m * cn, s));
+      HIPCHK(launch_ew(0, lv(i), up.as<float>(), lv(i), (size_t)m * cn * nl(i), s));
+    }
+    for (int i = 0; i < a->L; ++i) {
+      float* A = acc + (size_t)cn * a->loff[i];
+      if (lpgw) {
+        HIPCHK(launch_wacc_frames(A, a->fw.as<float>() + a->loff[i], lw1(i), lv(i), nl(i), cn, m, s));
+      } else if (i == top) {
+        HIPCHK(launch_top_frames(A, lv(i), nl(i), cn, m, first, s));
+      } else if (o.fusing_policy == SSG_FOCUS_SELECT_MAX_ENERGY) {
+        HIPCHK(launch_select_frames(A, lv(i), (size_t)cn * nl(i), (size_t)cn * nl(i), m, first, s));
+      } else {
+        HIPCHK(launch_wavg_frames(A, a->fw.as<float>() + (size_t)a->wc * a->loff[i], lv(i), nl(i), cn, o.avgchannel,
+                                  a->wc, m, first, s));
+      }
+    }
+    HIPCHK(hipStreamSynchronize(s));
+    a->accumulated += m;
+  }
+  return SSG_OK;
+}
+
 // c_laplacian_pyramid_focus_stacking::compute (:631-662) + reconstruct_laplacian_pyramid (:90-127):
 // the reconstructed image is left in level 0 of fp
 int focus_compute(ssg_accumulation* a) {
@@ -1633,6 +1719,15 @@ extern "C" int ssg_lpg(const ssg_image* src, ssg_image* dst, double k, double p,
   HIPCHK(hipMemcpy2DAsync(dst->data, (size_t)dst->step, m.p, (size_t)w * 4, (size_t)w * 4, h,
                           dst->on_device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, s));
   HIPCHK(hipStreamSynchronize(s));
+  return SSG_OK;
+}
+
+extern "C" int ssg_accumulation_add_frames(ssg_accumulation* a, int n, const ssg_image* frames) {
+  if (!a) return fail(SSG_E_INVALID_ARG, "null handle");
+  if (n <= 0) return SSG_OK;
+  if (!frames) return fail(SSG_E_INVALID_ARG, "null frames");
+  if (a->focus()) return focus_add_frames(a, n, frames);
+  for (int f = 0; f < n; ++f) CHK(ssg_accumulation_add(a, &frames[f], nullptr));
   return SSG_OK;
 }
 

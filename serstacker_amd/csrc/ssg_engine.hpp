@@ -244,6 +244,13 @@ hipError_t launch_energy(const float* lap, float* w, size_t n, int cn, int avgc,
 hipError_t launch_mulw(float* lap, const float* w, int wc, size_t n, int cn, hipStream_t s);
 hipError_t launch_blend(float* a, const float* b, float alpha, float beta, size_t n, hipStream_t s);
 hipError_t launch_plane_stdev(const float* planes, size_t n, int nplanes, double* out, hipStream_t s);
+hipError_t launch_select_frames(float* acc, const float* L, size_t ne, size_t fstride, int m, bool first,
+                                hipStream_t s);
+hipError_t launch_wavg_frames(float* acc, float* wsum, const float* L, size_t n, int cn, int avgc, int wc, int m,
+                              bool first, hipStream_t s);
+hipError_t launch_wacc_frames(float* acc, float* wsum, const float* W, const float* L, size_t n, int cn, int m,
+                              hipStream_t s);
+hipError_t launch_top_frames(float* acc, const float* L, size_t n, int cn, int m, bool first, hipStream_t s);
 
 // local-variance sharpness map (ssg_lvm.hip)
 hipError_t launch_bgr2gray(const float* src, int pitch, int w, int h, float* g, hipStream_t s);

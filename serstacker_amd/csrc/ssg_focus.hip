@@ -215,6 +215,129 @@ hipError_t launch_pyrup(const float* src, long long src_fstride, int sw, int sh,
                      dst_fstride, dw, dh);
   return hipGetLastError();
 }
+// ---- batches of frames: the per-pixel fusion of m frames in frame order, one launch per level ----
+// Level planes of a batch are [frame][channel][pixel]; every loop below applies the one-frame
+// update of the kernels above frame after frame, so a batch gives the sums of m single adds.
+
+// select_max_energy without smoothing: acc = |L_k| > |acc| ? L_k : acc, frame after frame
+__global__ void __launch_bounds__(256) k_select_frames(float* acc, const float* L, size_t ne, size_t fstride, int m,
+                                                       int first) {
+  const size_t e = gidx();
+  if (e >= ne) return;
+  float A = acc[e];
+  int k = 0;
+  if (first) {
+    A = L[e];
+    k = 1;
+  }
+  for (; k < m; ++k) {
+    const float v = L[(size_t)k * fstride + e];
+    if (fabsf(v) > fabsf(A)) A = v;
+  }
+  acc[e] = A;
+}
+
+// weighted_average without smoothing: w = compute_energy(L_k), acc += L_k w, wsum += w
+__global__ void __launch_bounds__(256) k_wavg_frames(float* acc, float* wsum, const float* L, size_t n, int cn,
+                                                     int avgc, int wc, int m, int first) {
+  const size_t i = gidx();
+  if (i >= n) return;
+  for (int k = 0; k < m; ++k) {
+    const float* Lk = L + (size_t)k * cn * n;
+    float w[3];
+    if (!avgc || cn == 1) {
+      for (int c = 0; c < cn; ++c) w[c] = Lk[(size_t)c * n + i] * Lk[(size_t)c * n + i];
+    } else {
+      const float sm = (Lk[i] + Lk[n + i]) + Lk[2 * n + i];
+      w[0] = sm * sm;
+    }
+    const bool init = first && k == 0;
+    for (int c = 0; c < cn; ++c) {
+      const float lw = Lk[(size_t)c * n + i] * w[wc > 1 ? c : 0];
+      acc[(size_t)c * n + i] = init ? lw : lw + acc[(size_t)c * n + i];
+    }
+    for (int c = 0; c < wc; ++c) wsum[(size_t)c * n + i] = init ? w[c] : w[c] + wsum[(size_t)c * n + i];
+  }
+}
+
+// lpg weights: acc += W_k L_k, wsum += W_k (k_wacc per frame)
+__global__ void __launch_bounds__(256) k_wacc_frames(float* acc, float* wsum, const float* W, const float* L, size_t n,
+                                                     int cn, int m) {
+  const size_t i = gidx();
+  if (i >= n) return;
+  for (int k = 0; k < m; ++k) {
+    const float w = W[(size_t)k * n + i];
+    const float* Lk = L + (size_t)k * cn * n;
+    for (int c = 0; c < cn; ++c) acc[(size_t)c * n + i] = acc[(size_t)c * n + i] + w * Lk[(size_t)c * n + i];
+    wsum[i] = wsum[i] + w;
+  }
+}
+
+// the top level of select / weighted average, frame after frame in one block: the blend weighted
+// by the channel-summed standard deviations of the accumulator and the frame (k_plane_stdev and
+// k_blend with the weights the host would form)
+__global__ void __launch_bounds__(256) k_top_frames(float* acc, const float* L, size_t n, int cn, int m, int first) {
+  __shared__ double sd[6];
+  __shared__ float ab[2];
+  int k = 0;
+  if (first) {
+    for (size_t e = threadIdx.x; e < (size_t)cn * n; e += 256) acc[e] = L[e];
+    __syncthreads();
+    k = 1;
+  }
+  for (; k < m; ++k) {
+    const float* Lk = L + (size_t)k * cn * n;
+    const int t = threadIdx.x;
+    if (t < 2 * cn) {
+      const float* pl = t < cn ? acc + (size_t)t * n : Lk + (size_t)(t - cn) * n;
+      double sm = 0, sq = 0;
+      for (size_t i = 0; i < n; ++i) {
+        const double v = pl[i];
+        sm += v;
+        sq += v * v;
+      }
+      const double scale = 1.0 / (double)n;
+      sm *= scale;
+      const double var = sq * scale - sm * sm;
+      sd[t] = sqrt(var > 0 ? var : 0.0);
+    }
+    __syncthreads();
+    if (t == 0) {
+      double s0 = 0, s1 = 0;
+      for (int c = 0; c < cn; ++c) {
+        s0 += sd[c];
+        s1 += sd[cn + c];
+      }
+      ab[0] = (float)(s0 / (s0 + s1));
+      ab[1] = (float)(s1 / (s0 + s1));
+    }
+    __syncthreads();
+    const float alpha = ab[0], beta = ab[1];
+    for (size_t e = t; e < (size_t)cn * n; e += 256) acc[e] = acc[e] * alpha + Lk[e] * beta;
+    __syncthreads();
+  }
+}
+
+hipError_t launch_select_frames(float* acc, const float* L, size_t ne, size_t fstride, int m, bool first,
+                                hipStream_t s) {
+  hipLaunchKernelGGL(k_select_frames, g1(ne), dim3(256), 0, s, acc, L, ne, fstride, m, first ? 1 : 0);
+  return hipGetLastError();
+}
+hipError_t launch_wavg_frames(float* acc, float* wsum, const float* L, size_t n, int cn, int avgc, int wc, int m,
+                              bool first, hipStream_t s) {
+  hipLaunchKernelGGL(k_wavg_frames, g1(n), dim3(256), 0, s, acc, wsum, L, n, cn, avgc, wc, m, first ? 1 : 0);
+  return hipGetLastError();
+}
+hipError_t launch_wacc_frames(float* acc, float* wsum, const float* W, const float* L, size_t n, int cn, int m,
+                              hipStream_t s) {
+  hipLaunchKernelGGL(k_wacc_frames, g1(n), dim3(256), 0, s, acc, wsum, W, L, n, cn, m);
+  return hipGetLastError();
+}
+hipError_t launch_top_frames(float* acc, const float* L, size_t n, int cn, int m, bool first, hipStream_t s) {
+  hipLaunchKernelGGL(k_top_frames, dim3(1), dim3(256), 0, s, acc, L, n, cn, m, first ? 1 : 0);
+  return hipGetLastError();
+}
+
 hipError_t launch_ew(int op, const float* a, const float* b, float* d, size_t n, hipStream_t s) {
   switch (op) {
     case 0: hipLaunchKernelGGL(k_sub, g1(n), dim3(256), 0, s, a, b, d, n); break;

@@ -96,3 +96,37 @@ def test_focus_lpg_weights_large_rgb(ssg, oracle32):
         assert acc.add(gpu(f))
         fs.add(planes(f))
     np.testing.assert_array_equal(acc.compute(), np.stack(fs.compute(), -1))
+
+
+@pytest.mark.parametrize("cn", [1, 3])
+@pytest.mark.parametrize("policy,avgc,kradius,ksigma", [
+    (0, 1, 0, 0.0), (0, 1, 2, 1.0), (1, 1, 0, 0.0), (1, 0, 0, 0.0), (2, 1, 0, 0.0)])
+def test_focus_add_frames_equals_single_adds(ssg, cn, policy, avgc, kradius, ksigma):
+    """add_frames (pyramids of a batch built together, levels fused frame after frame in one
+    launch) gives the stack of the same frames added one by one, bit for bit: a first batch,
+    a single add between, and a second batch that continues the stack."""
+    S = ssg
+    w, h = 130, 98
+    frames = focus_frames(7, w, h, cn, seed=23 + policy)
+    print(f"[focus batch] cn {cn} policy {policy} avgc {avgc} kradius {kradius}", flush=True)
+    kw = dict(fusing_policy=policy, avgchannel=avgc, kradius=kradius, ksigma=ksigma, lpg_sigma=1.5)
+    one = S.c_laplacian_pyramid_focus_stacking(w, h, cn, **kw)
+    for f in frames:
+        assert one.add(gpu(f))
+    bat = S.c_laplacian_pyramid_focus_stacking(w, h, cn, **kw)
+    bat.add_frames([gpu(f) for f in frames[:3]])
+    assert bat.add(gpu(frames[3]))
+    bat.add_frames(frames[4:])  # host frames
+    assert bat.accumulated_frames() == 7
+    np.testing.assert_array_equal(bat.compute(), one.compute())
+
+
+def test_focus_add_frames_errors(ssg):
+    S = ssg
+    acc = S.c_laplacian_pyramid_focus_stacking(64, 48, 1)
+    with pytest.raises(RuntimeError):
+        acc.add_frames([np.zeros((48, 64, 3), np.float32)])
+    with pytest.raises(RuntimeError):
+        acc.add_frames([np.zeros((40, 64), np.float32)])
+    acc.add_frames([])
+    assert acc.accumulated_frames() == 0
