@@ -18,6 +18,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 
 MOTION_TRANSLATION, MOTION_EUCLIDEAN, MOTION_SCALED_EUCLIDEAN, MOTION_AFFINE = 0, 1, 2, 3
+ECC_FORWARD_ADDITIVE, ECC_INVERSE_COMPOSITIONAL, ECC_LM, ECC_INVERSE_COMPOSITIONAL_LM = 0, 1, 2, 3
 BORDER_CONSTANT, BORDER_REPLICATE, BORDER_REFLECT, BORDER_WRAP, BORDER_REFLECT101, BORDER_TRANSPARENT = range(6)
 INTER_NEAREST, INTER_LINEAR, INTER_CUBIC, INTER_AREA = 0, 1, 2, 3
 
@@ -38,6 +39,7 @@ class RegOptions(C.Structure):
         ("ecch_max_level", C.c_int32),
         ("ecch_minimum_image_size", C.c_int32),
         ("ecch_estimate_translation_first", C.c_int32),
+        ("ecc_method", C.c_int32),
     ]
 
 
@@ -79,6 +81,7 @@ def reg_options(**kw) -> RegOptions:
     o.ecch_max_level = 0
     o.ecch_minimum_image_size = 16
     o.ecch_estimate_translation_first = 1
+    o.ecc_method = ECC_FORWARD_ADDITIVE
     for k, v in kw.items():
         setattr(o, k, v)
     return o
@@ -156,6 +159,14 @@ class Oracle:
         L.ora_ecc_fa_align.argtypes = [rp, u8p, C.c_int, C.c_int, rp, u8p, C.c_int, C.c_int, xp,
                                        C.c_int, C.c_int, C.c_double, C.c_double, C.POINTER(LevelResult)]
         L.ora_ecc_fa_align.restype = C.c_int
+        L.ora_ecc_lm_align.argtypes = [rp, u8p, C.c_int, C.c_int, rp, u8p, C.c_int, C.c_int, xp,
+                                       C.c_int, C.c_double, C.c_double, C.POINTER(LevelResult)]
+        L.ora_ecc_lm_align.restype = C.c_int
+        for fn in (L.ora_ecc_ic_align, L.ora_ecc_iclm_align):
+            fn.argtypes = [rp, u8p, C.c_int, C.c_int, rp, u8p, C.c_int, C.c_int, xp, xp,
+                           C.c_int, C.c_double, C.c_double, C.POINTER(LevelResult)]
+            fn.restype = C.c_int
+        L.ora_xform_invert_and_compose.argtypes = [xp, rp, rp]
         L.ora_compute_correlation.argtypes = [rp, u8p, C.c_int, C.c_int, rp, u8p, C.c_int, C.c_int, rp]
         L.ora_compute_correlation.restype = C.c_double
         L.ora_reg_create.argtypes = [C.POINTER(RegOptions)]
@@ -323,6 +334,42 @@ class Oracle:
         ok = self.lib.ora_ecc_fa_align(rp_, rmp, ref.shape[1], ref.shape[0], cp, cmp_, cur.shape[1], cur.shape[0],
                                        C.byref(t), interp, max_iterations, max_eps, update_step_scale, C.byref(res))
         return bool(ok), res
+
+    def ecc_lm_align(self, ref, cur, t, ref_mask=None, cur_mask=None, max_iterations=50, max_eps=0.2,
+                     update_step_scale=1.5):
+        """c_ecclm::align on one level (ref_mask: the eroded reference mask)."""
+        ref, rp_ = self._r(ref)
+        cur, cp = self._r(cur)
+        rm, rmp = self._u8(ref_mask)
+        cm, cmp_ = self._u8(cur_mask)
+        res = LevelResult()
+        ok = self.lib.ora_ecc_lm_align(rp_, rmp, ref.shape[1], ref.shape[0], cp, cmp_, cur.shape[1], cur.shape[0],
+                                       C.byref(t), max_iterations, max_eps, update_step_scale, C.byref(res))
+        return bool(ok), res
+
+    def ecc_ic_align(self, ref, cur, t, ref_mask=None, cur_mask=None, max_iterations=50, max_eps=0.2,
+                     update_step_scale=1.5, lm=False, jx=None):
+        """c_ecc_inverse_compositional::align, or c_ecclm_inverse_compositional::align
+        with lm=True; J at the parameters of jx (default: t as passed)."""
+        ref, rp_ = self._r(ref)
+        cur, cp = self._r(cur)
+        rm, rmp = self._u8(ref_mask)
+        cm, cmp_ = self._u8(cur_mask)
+        if jx is None:
+            jx = self.XForm()
+            C.pointer(jx)[0] = t
+        res = LevelResult()
+        fn = self.lib.ora_ecc_iclm_align if lm else self.lib.ora_ecc_ic_align
+        ok = fn(rp_, rmp, ref.shape[1], ref.shape[0], cp, cmp_, cur.shape[1], cur.shape[0], C.byref(t),
+                C.byref(jx), max_iterations, max_eps, update_step_scale, C.byref(res))
+        return bool(ok), res
+
+    def invert_and_compose(self, t, dp):
+        dp, dpp = self._r(np.asarray(dp, dtype=self.dt))
+        out = np.zeros(6, self.dt)
+        out[:t.nparams] = np.asarray(t.p[:t.nparams])
+        self.lib.ora_xform_invert_and_compose(C.byref(t), dpp, out.ctypes.data_as(self._rp))
+        return out[:t.nparams]
 
     def compute_correlation(self, cur, ref, rmap, cur_mask=None, ref_mask=None):
         cur, cp = self._r(cur)

@@ -18,10 +18,16 @@
 #define RFMA fma
 #define RSQRT sqrt
 #define RLRINT lrint
+#define RSIN sin
+#define RCOS cos
+#define RATAN2 atan2
 #else
 #define RFMA fmaf
 #define RSQRT sqrtf
 #define RLRINT lrintf
+#define RSIN sinf
+#define RCOS cosf
+#define RATAN2 atan2f
 #endif
 
 typedef ora_real real;
@@ -971,6 +977,389 @@ int ora_ecc_fa_align(const real *ref, const uint8_t *ref_mask, int w, int h, con
   return !failed;
 }
 
+/* ------------------------------------------------------------------------ */
+/* invert_and_compose.  translation: p - dp (c_image_transform.h:164-167);   */
+/* affine: inv(inv(M(p)) + M(dp)) (c_image_transform.h:312-318); euclidean:  */
+/* M(p) * inv(M(dp)) with the increment's scale 1 + ds (1 when the scale is  */
+/* fixed), parameters read back from the product (c_image_transform.cc:      */
+/* 736-833).  cv::invertAffineTransform works in double on the stored        */
+/* entries and stores its result; Matx products sum k = 0..2 in order.       */
+/* ------------------------------------------------------------------------ */
+static void inv_affine23(const real *M, real *iM) {
+  double D = (double)M[0] * M[4] - (double)M[1] * M[3];
+  D = D != 0 ? 1. / D : 0.;
+  const double A11 = M[4] * D, A22 = M[0] * D, A12 = -M[1] * D, A21 = -M[3] * D;
+  const double b1 = -A11 * M[2] - A12 * M[5];
+  const double b2 = -A21 * M[2] - A22 * M[5];
+  iM[0] = (real)A11; iM[1] = (real)A12; iM[2] = (real)b1;
+  iM[3] = (real)A21; iM[4] = (real)A22; iM[5] = (real)b2;
+}
+
+static void euclid_mat(real tX, real tY, real ang, real scl, real cX, real cY, real *M) {
+  const real sa = RSIN(ang), ca = RCOS(ang);
+  M[0] = scl * ca; M[1] = -scl * sa; M[2] = tX - scl * ca * cX + scl * sa * cY;
+  M[3] = scl * sa; M[4] = scl * ca;  M[5] = tY - scl * sa * cX - scl * ca * cY;
+  M[6] = 0; M[7] = 0; M[8] = 1;
+}
+
+void ora_xform_invert_and_compose(const ora_xform *t, const real *dp, real *out) {
+  if (t->motion == ORA_MOTION_TRANSLATION) {
+    out[0] = t->p[0] - dp[0];
+    out[1] = t->p[1] - dp[1];
+    return;
+  }
+  if (t->motion == ORA_MOTION_AFFINE) {
+    real a[6];
+    inv_affine23(t->p, a);
+    for (int i = 0; i < 6; i++) a[i] = a[i] + dp[i];
+    inv_affine23(a, out);
+    return;
+  }
+  real Tx, Ty, ang, s;
+  euclid_params(t, t->p, &Tx, &Ty, &ang, &s);
+  const real Cx = t->C[0], Cy = t->C[1];
+  const int fix_scale = t->motion != ORA_MOTION_SCALED_EUCLIDEAN;
+  const real sdp = fix_scale ? (real)1 : (real)1 + dp[3];
+  real Mp[9], Md[9], Mi[9], R[9];
+  euclid_mat(Tx, Ty, ang, s, Cx, Cy, Mp);
+  euclid_mat(dp[0], dp[1], dp[2], sdp, Cx, Cy, Md);
+  inv_affine23(Md, Mi);
+  Mi[6] = 0; Mi[7] = 0; Mi[8] = 1;
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 3; j++) {
+      real acc = 0;
+      for (int k = 0; k < 3; k++) acc = acc + Mp[i * 3 + k] * Mi[k * 3 + j];
+      R[i * 3 + j] = acc;
+    }
+  const real m00 = R[0], m10 = R[3];
+  const real rs = fix_scale ? s : RSQRT(m00 * m00 + m10 * m10);
+  const real ra = RATAN2(m10, m00);
+  const real rca = RCOS(ra), rsa = RSIN(ra);
+  out[0] = R[2] + rs * rca * Cx - rs * rsa * Cy;
+  out[1] = R[5] + rs * rsa * Cx + rs * rca * Cy;
+  out[2] = ra;
+  if (!fix_scale) out[3] = rs;
+}
+
+/* cv::solve(H, v, DECOMP_CHOLESKY) of the normal equations: the failure rule
+ * of ora_cholesky_invert (determinant for n <= 3, float Cholesky pivots
+ * otherwise); a failed solve yields dp = 0.  Returns 1 on success. */
+static int solve_normal(const real *H, const real *v, int n, real *dp) {
+  real A[36];
+  memcpy(A, H, sizeof(real) * n * n);
+  if (!ora_cholesky_invert(A, n)) {
+    for (int i = 0; i < n; i++) dp[i] = 0;
+    return 0;
+  }
+  for (int i = 0; i < n; i++) {
+    double s = 0;
+    for (int k = 0; k < n; k++) s += (double)A[i * n + k] * v[k];
+    dp[i] = (real)s;
+  }
+  return 1;
+}
+
+static void xform_with(const ora_xform *t, const real *p, ora_xform *out) {
+  *out = *t;
+  memcpy(out->p, p, sizeof(real) * 6);
+}
+
+/* H = J^T J (lower triangle mirrored) and v = J^T rhs from n SD images */
+static void normal_eqs(real **J, const real *rhs, int n, int w, int h, real *H, real *v) {
+  for (int i = 0; i < n; i++) {
+    if (v) v[i] = (real)dot_rows(J[i], rhs, w, h);
+    for (int j = 0; j <= i; j++) H[i * n + j] = (real)dot_rows(J[i], J[j], w, h);
+  }
+  for (int i = 0; i < n; i++)
+    for (int j = i + 1; j < n; j++) H[i * n + j] = H[j * n + i];
+}
+
+/* the gemm(Hp, deltap, -1, v, 2) of the LM step quality: dS = dp.(2v - Hp dp) */
+static double lm_predicted(const real *Hp, const real *dp, const real *v, int n) {
+  double dS = 0;
+  for (int i = 0; i < n; i++) {
+    double s = 0;
+    for (int k = 0; k < n; k++) s += (double)Hp[i * n + k] * dp[k];
+    const real td = (real)(-s + 2.0 * (double)v[i]);
+    dS += (double)dp[i] * td;
+  }
+  return dS;
+}
+
+/* ------------------------------------------------------------------------ */
+/* c_ecclm (ecc2.cc:1371-1650): forward-additive Levenberg-Marquardt.        */
+/* compute_remap (:1440-1473): g = remap(cur, LINEAR, REPLICATE), mask =     */
+/* remap(cur mask or 255, LINEAR, CONSTANT 0) >= 250 (ecc2.cc:178-213) and    */
+/* the 5x5-eroded reference mask; rhs = g - f, 0 off the mask.               */
+/* compute_jac (:1480-1523): gradients of g zeroed off the mask, J at the    */
+/* evaluated parameters, H = J^T J, v = J^T rhs.                             */
+/* ------------------------------------------------------------------------ */
+typedef struct lm_ws {
+  real *map, *gw, *rhs, *gx, *gy;
+  uint8_t *m, *all;
+  real *J[6];
+} lm_ws;
+
+static void lm_ws_init(lm_ws *ws, size_t N, int np, size_t cn) {
+  ws->map = (real *)xcalloc(2 * N, sizeof(real));
+  ws->gw = (real *)xcalloc(N, sizeof(real));
+  ws->rhs = (real *)xcalloc(N, sizeof(real));
+  ws->gx = (real *)xcalloc(N, sizeof(real));
+  ws->gy = (real *)xcalloc(N, sizeof(real));
+  ws->m = (uint8_t *)xcalloc(N, 1);
+  ws->all = (uint8_t *)xcalloc(cn, 1);
+  memset(ws->all, 255, cn);
+  for (int i = 0; i < 6; i++) ws->J[i] = i < np ? (real *)xcalloc(N, sizeof(real)) : NULL;
+}
+
+static void lm_ws_free(lm_ws *ws) {
+  free(ws->map); free(ws->gw); free(ws->rhs); free(ws->gx); free(ws->gy); free(ws->m); free(ws->all);
+  for (int i = 0; i < 6; i++) free(ws->J[i]);
+}
+
+static double lm_remap(const ora_xform *t, const real *params, const real *ref, const uint8_t *ref_mask, int w,
+                       int h, const real *cur, const uint8_t *cur_mask, int cw, int chh, lm_ws *ws) {
+  const size_t N = (size_t)w * h;
+  ora_xform tp;
+  xform_with(t, params, &tp);
+  ora_xform_create_map(&tp, w, h, ws->map);
+  ora_remap(cur, cw, chh, ws->map, w, h, ws->gw, ORA_INTER_LINEAR, ORA_BORDER_REPLICATE, 0);
+  ora_remap_u8(cur_mask ? cur_mask : ws->all, cw, chh, ws->map, w, h, ws->m, ORA_BORDER_CONSTANT, 0);
+  for (size_t i = 0; i < N; i++) {
+    const int ok = ws->m[i] >= 250 && (!ref_mask || ref_mask[i]);
+    ws->m[i] = ok ? 255 : 0;
+    ws->rhs[i] = ok ? ws->gw[i] - ref[i] : 0;
+  }
+  return dot_rows(ws->rhs, ws->rhs, w, h);
+}
+
+int ora_ecc_lm_align(const real *ref, const uint8_t *ref_mask, int w, int h, const real *cur,
+                     const uint8_t *cur_mask, int cw, int chh, ora_xform *t, int max_iterations,
+                     double max_eps, double update_step_scale, ora_ecc_level_result *res) {
+  const size_t N = (size_t)w * h;
+  const int M = t->nparams;
+  const double epse = 1e-4; /* c_ecclm::_max_epse (ecc2.h:145) */
+  lm_ws ws;
+  lm_ws_init(&ws, N, M, (size_t)cw * chh);
+  real params[6], newp[6], H[36], Hp[36], v[6], dp[6];
+  memcpy(params, t->p, sizeof(params));
+  double lambda = 0.1, rms = 0, eps = FLT_MAX;
+  int iteration = 0, converged = 0, recompute = 1;
+  while (iteration < max_iterations) {
+    if (recompute) rms = lm_remap(t, params, ref, ref_mask, w, h, cur, cur_mask, cw, chh, &ws);
+    ora_ecc_differentiate(ws.gw, w, h, ws.gx, ws.gy);
+    for (size_t i = 0; i < N; i++)
+      if (!ws.m[i]) ws.gx[i] = ws.gy[i] = 0;
+    ora_xform tp;
+    xform_with(t, params, &tp);
+    ora_xform_sd_images(&tp, ws.gx, ws.gy, w, h, ws.J);
+    normal_eqs(ws.J, ws.rhs, M, w, h, H, v);
+    const double err = rms;
+    if (err < 1) {
+      converged = 1;
+      break;
+    }
+    memcpy(Hp, H, sizeof(H));
+    while (iteration++ < max_iterations) {
+      recompute = 1;
+      for (int i = 0; i < M; i++) H[i * M + i] = (real)((1 + lambda) * Hp[i * M + i]);
+      solve_normal(H, v, M, dp);
+      for (int i = 0; i < M; i++) newp[i] = RFMA(dp[i], (real)(-update_step_scale), params[i]);
+      for (int i = M; i < 6; i++) newp[i] = params[i];
+      eps = ora_xform_eps(t, dp, w, h);
+      if (eps <= max_eps) {
+        memcpy(params, newp, sizeof(params));
+        converged = 1;
+        break;
+      }
+      const double newerr = lm_remap(t, newp, ref, ref_mask, w, h, cur, cur_mask, cw, chh, &ws);
+      if (newerr > err) {
+        if (lambda > 1e6) break;
+        lambda *= 10.0f;
+        continue;
+      }
+      memcpy(params, newp, sizeof(params));
+      recompute = 0;
+      rms = newerr;
+      const double diff = err - newerr;
+      if (diff < err * epse) {
+        converged = 1;
+        break;
+      }
+      const double dS = lm_predicted(Hp, dp, v, M);
+      const double rho = fabs(dS) > 1e-9f ? diff / fabs(dS) : diff;
+      if (rho > 0.25) lambda = fmax(1e-8, 0.2 * lambda);
+      else if (rho < 0.1) lambda = lambda < 1.0 ? 1.0 : lambda * 10.0;
+      break;
+    }
+    if (converged) break;
+  }
+  memcpy(t->p, params, sizeof(params));
+  if (res) {
+    res->num_iterations = iteration;
+    res->failed = !converged;
+    res->eps = eps;
+  }
+  lm_ws_free(&ws);
+  return converged;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Inverse-compositional ECC.  Both variants precompute J on the reference  */
+/* (gradients zeroed where the reference mask is 0, J at the parameters of  */
+/* the transform `jx` holds when the precompute runs) and H = J^T J.         */
+/* ------------------------------------------------------------------------ */
+static void ic_precompute(const real *ref, const uint8_t *ref_mask, int w, int h, const ora_xform *jx, lm_ws *ws,
+                          real *Hp) {
+  const size_t N = (size_t)w * h;
+  ora_ecc_differentiate(ref, w, h, ws->gx, ws->gy);
+  if (ref_mask)
+    for (size_t i = 0; i < N; i++)
+      if (!ref_mask[i]) ws->gx[i] = ws->gy[i] = 0;
+  ora_xform_sd_images(jx, ws->gx, ws->gy, w, h, ws->J);
+  normal_eqs(ws->J, NULL, jx->nparams, w, h, Hp, NULL);
+}
+
+static double count_nonzero(const uint8_t *m, size_t n) {
+  double c = 0;
+  for (size_t i = 0; i < n; i++) c += m[i] != 0;
+  return c;
+}
+
+/* c_ecc_inverse_compositional::align (ecc2.cc:1690-1792): rhs = remap(cur,
+ * LINEAR, REPLICATE) - f, zeroed off remap(cur mask or 255, LINEAR,
+ * CONSTANT 0) >= 250; rms = |rhs|^2 RMA^2/CMA^2; dp = H^-1 J^T rhs RMA/CMA;
+ * stop when rms stops decreasing or eps(dp) < max_eps; p = p o (s dp)^-1. */
+int ora_ecc_ic_align(const real *ref, const uint8_t *ref_mask, int w, int h, const real *cur,
+                     const uint8_t *cur_mask, int cw, int chh, ora_xform *t, const ora_xform *jx,
+                     int max_iterations, double max_eps, double update_step_scale,
+                     ora_ecc_level_result *res) {
+  const size_t N = (size_t)w * h;
+  const int M = t->nparams;
+  lm_ws ws;
+  lm_ws_init(&ws, N, M, (size_t)cw * chh);
+  real H[36], v[6], dp[6], sdp[6], newp[6];
+  ic_precompute(ref, ref_mask, w, h, jx, &ws, H);
+  const double RMA = ref_mask ? count_nonzero(ref_mask, N) : (double)N;
+  double rmsold = FLT_MAX, eps = FLT_MAX;
+  int it = 0;
+  while (it++ < max_iterations) {
+    ora_xform_create_map(t, w, h, ws.map);
+    ora_remap(cur, cw, chh, ws.map, w, h, ws.gw, ORA_INTER_LINEAR, ORA_BORDER_REPLICATE, 0);
+    ora_remap_u8(cur_mask ? cur_mask : ws.all, cw, chh, ws.map, w, h, ws.m, ORA_BORDER_CONSTANT, 0);
+    double CMA = 0;
+    for (size_t i = 0; i < N; i++) {
+      const int ok = ws.m[i] >= 250;
+      ws.rhs[i] = ok ? ws.gw[i] - ref[i] : 0;
+      CMA += ok;
+    }
+    const double rmsnew = dot_rows(ws.rhs, ws.rhs, w, h) * (RMA * RMA) / (CMA * CMA);
+    for (int i = 0; i < M; i++) v[i] = (real)((real)dot_rows(ws.J[i], ws.rhs, w, h) * (RMA / CMA));
+    solve_normal(H, v, M, dp);
+    if (rmsnew >= rmsold) break;
+    for (int i = 0; i < M; i++) sdp[i] = (real)(update_step_scale * dp[i]);
+    ora_xform_invert_and_compose(t, sdp, newp);
+    rmsold = rmsnew;
+    memcpy(t->p, newp, sizeof(real) * M);
+    eps = ora_xform_eps(t, dp, w, h);
+    if (eps < max_eps) break;
+  }
+  if (res) {
+    res->num_iterations = it;
+    res->failed = 0;
+    res->eps = eps;
+  }
+  lm_ws_free(&ws);
+  return 1;
+}
+
+/* c_ecclm_inverse_compositional (ecc2.cc:1796-2086).  compute_rhs
+ * (:1892-1917): bad = remap(~cur mask, NEAREST, CONSTANT 255) | ~ref mask,
+ * rhs = g - f zeroed at bad, CMA = area - bad, rms = |rhs|^2 RMA^2/CMA^2;
+ * compute_v (:1919-1924): v = J^T rhs RMA/CMA. */
+static double iclm_rhs(const ora_xform *t, const real *params, const real *ref, const uint8_t *ref_mask, int w, int h,
+                       const real *cur, const uint8_t *cur_mask, int cw, int chh, double RMA, lm_ws *ws,
+                       double *CMA) {
+  const size_t N = (size_t)w * h;
+  ora_xform tp;
+  xform_with(t, params, &tp);
+  ora_xform_create_map(&tp, w, h, ws->map);
+  ora_remap(cur, cw, chh, ws->map, w, h, ws->gw, ORA_INTER_LINEAR, ORA_BORDER_REPLICATE, 0);
+  double bad = 0;
+  for (size_t i = 0; i < N; i++) {
+    long X = RLRINT(ws->map[2 * i]), Y = RLRINT(ws->map[2 * i + 1]);
+    X = X < SHRT_MIN ? SHRT_MIN : (X > SHRT_MAX ? SHRT_MAX : X);
+    Y = Y < SHRT_MIN ? SHRT_MIN : (Y > SHRT_MAX ? SHRT_MAX : Y);
+    int b = (X < 0 || X >= cw || Y < 0 || Y >= chh) ? 1 : (cur_mask && !cur_mask[(size_t)Y * cw + X]);
+    if (ref_mask && !ref_mask[i]) b = 1;
+    ws->m[i] = b ? 0 : 255;
+    ws->rhs[i] = b ? 0 : ws->gw[i] - ref[i];
+    bad += b;
+  }
+  *CMA = (double)N - bad;
+  return dot_rows(ws->rhs, ws->rhs, w, h) * (RMA * RMA) / (*CMA * *CMA);
+}
+
+int ora_ecc_iclm_align(const real *ref, const uint8_t *ref_mask, int w, int h, const real *cur,
+                       const uint8_t *cur_mask, int cw, int chh, ora_xform *t, const ora_xform *jx,
+                       int max_iterations, double max_eps, double update_step_scale,
+                       ora_ecc_level_result *res) {
+  const size_t N = (size_t)w * h;
+  const int M = t->nparams;
+  lm_ws ws;
+  lm_ws_init(&ws, N, M, (size_t)cw * chh);
+  real Hp[36], H[36], v[6], dp[6], sdp[6], params[6], newp[6];
+  ic_precompute(ref, ref_mask, w, h, jx, &ws, Hp);
+  const double RMA = ref_mask ? count_nonzero(ref_mask, N) : (double)N;
+  memcpy(params, t->p, sizeof(params));
+  double lambda = 0.001, dpe = 0, last_rms = 0, CMA = 0, err = 0, newerr = 0;
+  int it = 0, recompute = 1;
+  while (it < max_iterations) {
+    if (recompute) last_rms = iclm_rhs(t, params, ref, ref_mask, w, h, cur, cur_mask, cw, chh, RMA, &ws, &CMA);
+    for (int i = 0; i < M; i++) v[i] = (real)((real)dot_rows(ws.J[i], ws.rhs, w, h) * (RMA / CMA));
+    memcpy(H, Hp, sizeof(H));
+    err = last_rms;
+    do {
+      ++it;
+      recompute = 1;
+      for (int i = 0; i < M; i++) H[i * M + i] = (real)((1 + lambda) * Hp[i * M + i]);
+      solve_normal(H, v, M, dp);
+      for (int i = 0; i < M; i++) sdp[i] = (real)(update_step_scale * dp[i]);
+      ora_xform tp;
+      xform_with(t, params, &tp);
+      ora_xform_invert_and_compose(&tp, sdp, newp);
+      for (int i = M; i < 6; i++) newp[i] = params[i];
+      newerr = last_rms = iclm_rhs(t, newp, ref, ref_mask, w, h, cur, cur_mask, cw, chh, RMA, &ws, &CMA);
+      if ((dpe = ora_xform_eps(t, dp, w, h)) < max_eps) break;
+      const double dS = lm_predicted(Hp, dp, v, M);
+      const double rho = (err - newerr) / (fabs(dS) > DBL_EPSILON ? dS : 1);
+      if (rho > 0.25) {
+        if (lambda > 1e-6) lambda = fmax(1e-6, lambda / 5);
+      } else if (rho > 0.1) {
+      } else if (lambda < 1) {
+        lambda = 1;
+      } else {
+        lambda *= 10;
+      }
+      if (newerr < err) break;
+    } while (it < max_iterations);
+    if (newerr < err) {
+      err = newerr;
+      recompute = 0;
+      memcpy(params, newp, sizeof(params));
+    }
+    if (dpe < max_eps) break;
+  }
+  memcpy(t->p, params, sizeof(params));
+  if (res) {
+    res->num_iterations = it;
+    res->failed = 0;
+    res->eps = dpe;
+  }
+  lm_ws_free(&ws);
+  return 1;
+}
+
 /* compute_correlation (ecc2.cc:65-137): remap (LINEAR, CONSTANT 0), mask =
  * remap(mask) >= 254 & reference mask, Pearson correlation over the mask */
 double ora_compute_correlation(const real *cur, const uint8_t *cur_mask, int cw, int chh,
@@ -1025,6 +1414,10 @@ struct ora_reg {
   int nlevels;
   int ref_w, ref_h;  /* full-resolution reference size */
   level_img ref[ORA_MAX_LEVELS];
+  /* inverse-compositional J cache per level: the transform state of the
+   * precompute, kept until the reference or the aligner's transform changes */
+  ora_xform jx[ORA_MAX_LEVELS];
+  int jvalid[ORA_MAX_LEVELS];
 };
 
 static void next_size(int w, int h, int *nw, int *nh) {
@@ -1057,7 +1450,11 @@ static int count_all(const uint8_t *m, size_t n) {
  * eroded (ecc2.cc:1214-1234).  The mask passed down the pyramid is the
  * un-eroded one resized INTER_NEAREST (ecc2.cc:972-982). */
 static void build_levels(const real *img0, const uint8_t *mask0, int w, int h, int nlev,
-                         double sigma, int is_ref, level_img *L) {
+                         double sigma, int is_ref, int method, level_img *L) {
+  /* erosion of partial masks: forward additive erodes both (ecc2.cc:1187-1234),
+   * c_ecclm only the reference (ecc2.cc:1387-1400), the inverse-compositional
+   * aligners neither (base c_ecc_align::set_*_image, ecc2.cc:585-630) */
+  const int erode = method == ORA_ECC_FORWARD_ADDITIVE || (method == ORA_ECC_LM && is_ref);
   real *img = (real *)xcalloc((size_t)w * h, sizeof(real));
   memcpy(img, img0, sizeof(real) * w * h);
   uint8_t *mask = NULL;
@@ -1076,7 +1473,7 @@ static void build_levels(const real *img0, const uint8_t *mask0, int w, int h, i
     const size_t n = (size_t)cw * chh;
     if (mask && !(is_ref && count_all(mask, n))) {
       L[l].mask = (uint8_t *)xcalloc(n, 1);
-      if (count_all(mask, n)) {
+      if (count_all(mask, n) || !erode) {
         memcpy(L[l].mask, mask, n);
       } else {
         ora_erode_rect_u8(mask, cw, chh, 5, 5, L[l].mask, ORA_BORDER_REPLICATE, 255);
@@ -1168,7 +1565,8 @@ int ora_reg_setup_reference(ora_reg *r, const real *img, const uint8_t *mask, in
     if (lv >= ORA_MAX_LEVELS) break;
   }
   r->nlevels = lv;
-  build_levels(eimg, emask, ew, eh, lv, r->o.reference_smooth_sigma, 1, r->ref);
+  build_levels(eimg, emask, ew, eh, lv, r->o.reference_smooth_sigma, 1, r->o.ecc_method, r->ref);
+  memset(r->jvalid, 0, sizeof(r->jvalid));
   free(simg);
   free(smask);
   return 1;
@@ -1205,10 +1603,31 @@ static void ecch_align(ora_reg *r, level_img *cur, ora_xform *t, int *iters, int
   for (; lvl >= 0; --lvl) {
     ora_ecc_level_result res;
     double max_eps = r->o.eps * (double)(1 << lvl);
-    int ok = ora_ecc_fa_align(r->ref[lvl].img, r->ref[lvl].mask, r->ref[lvl].w, r->ref[lvl].h,
-                              cur[lvl].img, cur[lvl].mask, cur[lvl].w, cur[lvl].h, t,
-                              r->o.interpolation, r->o.max_iterations, max_eps,
-                              r->o.update_step_scale, &res);
+    const level_img *R = &r->ref[lvl], *Cu = &cur[lvl];
+    int ok;
+    switch (r->o.ecc_method) {
+      case ORA_ECC_FORWARD_ADDITIVE:
+        ok = ora_ecc_fa_align(R->img, R->mask, R->w, R->h, Cu->img, Cu->mask, Cu->w, Cu->h, t, r->o.interpolation,
+                              r->o.max_iterations, max_eps, r->o.update_step_scale, &res);
+        break;
+      case ORA_ECC_INVERSE_COMPOSITIONAL:
+      case ORA_ECC_INVERSE_COMPOSITIONAL_LM:
+        if (!r->jvalid[lvl]) {
+          r->jx[lvl] = *t;
+          r->jvalid[lvl] = 1;
+        }
+        if (r->o.ecc_method == ORA_ECC_INVERSE_COMPOSITIONAL)
+          ok = ora_ecc_ic_align(R->img, R->mask, R->w, R->h, Cu->img, Cu->mask, Cu->w, Cu->h, t, &r->jx[lvl],
+                                r->o.max_iterations, max_eps, r->o.update_step_scale, &res);
+        else
+          ok = ora_ecc_iclm_align(R->img, R->mask, R->w, R->h, Cu->img, Cu->mask, Cu->w, Cu->h, t, &r->jx[lvl],
+                                  r->o.max_iterations, max_eps, r->o.update_step_scale, &res);
+        break;
+      default:
+        ok = ora_ecc_lm_align(R->img, R->mask, R->w, R->h, Cu->img, Cu->mask, Cu->w, Cu->h, t, r->o.max_iterations,
+                              max_eps, r->o.update_step_scale, &res);
+        break;
+    }
     if (!ok) {
       (*failed_levels)++;
     } else if (lvl > 0) {
@@ -1235,7 +1654,7 @@ int ora_reg_register(ora_reg *r, const real *img, const uint8_t *mask, int w, in
   }
   level_img cur[ORA_MAX_LEVELS];
   memset(cur, 0, sizeof(cur));
-  build_levels(eimg, emask, ew, eh, r->nlevels, r->o.input_smooth_sigma, 0, cur);
+  build_levels(eimg, emask, ew, eh, r->nlevels, r->o.input_smooth_sigma, 0, r->o.ecc_method, cur);
   /* forward-additive always holds a current mask (all 255 when none given) */
   ora_xform t;
   ora_xform_reset(&t, r->o.motion_type);
@@ -1249,7 +1668,11 @@ int ora_reg_register(ora_reg *r, const real *img, const uint8_t *mask, int w, in
     real T[2];
     ora_xform_translation(&t, T);
     ora_xform_set_translation(&tt, T);
+    /* c_ecch::set_image_transform to the translation transform and back
+     * (c_frame_registration.cc:823,844) drops the inverse-compositional J */
+    memset(r->jvalid, 0, sizeof(r->jvalid));
     ecch_align(r, cur, &tt, &st->iterations_translation, &st->levels_failed, NULL);
+    memset(r->jvalid, 0, sizeof(r->jvalid));
     ora_xform_create_map(&tt, r->ref[0].w, r->ref[0].h, map);
     st->rho_translation = ora_compute_correlation(cur[0].img, cur[0].mask, cur[0].w, cur[0].h,
                                                   r->ref[0].img, r->ref[0].mask, r->ref[0].w,

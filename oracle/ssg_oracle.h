@@ -40,6 +40,9 @@ enum { ORA_INTER_NEAREST = 0, ORA_INTER_LINEAR = 1, ORA_INTER_CUBIC = 2, ORA_INT
 /* values follow the reference's IMAGE_MOTION_TYPE (image_transform.h:15-26) */
 enum { ORA_MOTION_TRANSLATION = 0, ORA_MOTION_EUCLIDEAN = 1, ORA_MOTION_SCALED_EUCLIDEAN = 2,
        ORA_MOTION_AFFINE = 3 };
+/* values follow the reference's ECC_ALIGN_METHOD (ecc2.h:59-64) */
+enum { ORA_ECC_FORWARD_ADDITIVE = 0, ORA_ECC_INVERSE_COMPOSITIONAL = 1, ORA_ECC_LM = 2,
+       ORA_ECC_INVERSE_COMPOSITIONAL_LM = 3 };
 
 int  ora_real_size(void);
 void ora_set_threads(int n);
@@ -85,6 +88,8 @@ void     ora_xform_set_translation(ora_xform *t, const ora_real T[2]);
 void     ora_xform_sd_images(const ora_xform *t, const ora_real *gx, const ora_real *gy,
                              int w, int h, ora_real **J);
 void     ora_xform_matrix(const ora_xform *t, double m[6]);
+/* invert_and_compose(p, dp) of the inverse-compositional aligners */
+void     ora_xform_invert_and_compose(const ora_xform *t, const ora_real *dp, ora_real *out);
 
 /* ---- ECC (ecc2.cc) ---- */
 typedef struct ora_ecc_level_result {
@@ -98,6 +103,21 @@ int ora_ecc_fa_align(const ora_real *ref, const uint8_t *ref_mask, int w, int h,
                      ora_xform *t, int interp, int max_iterations, double max_eps,
                      double update_step_scale, ora_ecc_level_result *res);
 
+/* c_ecclm::align (ecc2.cc:1526-1650); ref_mask already eroded */
+int ora_ecc_lm_align(const ora_real *ref, const uint8_t *ref_mask, int w, int h,
+                     const ora_real *cur, const uint8_t *cur_mask, int cw, int ch,
+                     ora_xform *t, int max_iterations, double max_eps,
+                     double update_step_scale, ora_ecc_level_result *res);
+/* c_ecc_inverse_compositional::align (ecc2.cc:1690-1792); J from jx */
+int ora_ecc_ic_align(const ora_real *ref, const uint8_t *ref_mask, int w, int h,
+                     const ora_real *cur, const uint8_t *cur_mask, int cw, int ch,
+                     ora_xform *t, const ora_xform *jx, int max_iterations, double max_eps,
+                     double update_step_scale, ora_ecc_level_result *res);
+/* c_ecclm_inverse_compositional::align (ecc2.cc:1926-2086); J from jx */
+int ora_ecc_iclm_align(const ora_real *ref, const uint8_t *ref_mask, int w, int h,
+                       const ora_real *cur, const uint8_t *cur_mask, int cw, int ch,
+                       ora_xform *t, const ora_xform *jx, int max_iterations, double max_eps,
+                       double update_step_scale, ora_ecc_level_result *res);
 double ora_compute_correlation(const ora_real *cur, const uint8_t *cur_mask, int cw, int ch,
                                const ora_real *ref, const uint8_t *ref_mask, int w, int h,
                                const ora_real *map);
@@ -118,6 +138,7 @@ typedef struct ora_reg_options {
   int32_t ecch_max_level;
   int32_t ecch_minimum_image_size;
   int32_t ecch_estimate_translation_first;
+  int32_t ecc_method;
 } ora_reg_options;
 
 typedef struct ora_reg_status {
