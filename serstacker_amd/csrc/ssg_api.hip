@@ -314,10 +314,14 @@ struct ssg_registration {
   DevBuf ref_pyr, ref_mask_pyr;
   bool ref_has_mask[MAX_LEVELS] = {};
   double ref_mean[MAX_LEVELS] = {};
+  // inverse-compositional aligners: reference gradients (2 planes per level, pyramid
+  // layout), their Hessian moments, reference-mask pixel counts, the J record
+  DevBuf ref_grad, ref_hm, ref_part, jrec;
+  double rma[MAX_LEVELS] = {};
   // batch workspace
   int cap = 0;
   DevBuf cur_pyr, cur_mask_pyr, half_buf;
-  DevBuf xf_t, xf_m, res, partials, active[2], active_count[2];
+  DevBuf xf_t, xf_m, res, partials, active[2], active_count[2], lm;
   DevBuf frame_ptrs, mask_ptrs, zero_count, frame_list, tmp_in, tmp_mask, sum_tmp;
   int* pinned_count = nullptr;
   bool have_ref = false;
@@ -390,7 +394,9 @@ extern "C" int ssg_registration_create(const ssg_image_registration_options* opt
   else ssg_default_image_registration_options(&o);
   if (!o.enable_ecc_registration) return fail(SSG_E_UNSUPPORTED, "only ECC registration is on this path (enable_ecc_registration=0)");
   if (o.enable_feature_registration) return fail(SSG_E_UNSUPPORTED, "sparse feature registration is not on this path");
-  if (o.ecc.ecc_method != SSG_ECC_ALIGN_FORWARD_ADDITIVE) return fail(SSG_E_UNSUPPORTED, "ecc_method %d: only ECC_ALIGN_FORWARD_ADDITIVE", o.ecc.ecc_method);
+  if (o.ecc.ecc_method < SSG_ECC_ALIGN_FORWARD_ADDITIVE || o.ecc.ecc_method > SSG_ECC_ALIGN_INVERSE_COMPOSITIONAL_LM)
+    return fail(SSG_E_INVALID_ARG, "ecc_method %d", o.ecc.ecc_method);
+  if (o.ecc.max_iterations < 1) return fail(SSG_E_INVALID_ARG, "ecc.max_iterations %d", o.ecc.max_iterations);
   if (o.motion_type < SSG_MOTION_TRANSLATION || o.motion_type > SSG_MOTION_AFFINE) return fail(SSG_E_UNSUPPORTED, "motion_type %d", o.motion_type);
   if (!(o.ecc.scale <= 0 || o.ecc.scale == 1 || fabs(o.ecc.scale - 0.5) < 1e-2)) return fail(SSG_E_UNSUPPORTED, "ecc.scale %g: only 1 and 0.5", o.ecc.scale);
   if (o.ecc.normalization_scale > 0 && o.ecc.normalization_noise > 0) return fail(SSG_E_UNSUPPORTED, "ecc normalization");
@@ -477,7 +483,12 @@ int build_pyramids(ssg_registration* r, const float* const* src_ptrs, int src_pi
 // has_mask[l*n + f] receives whether frame f keeps a mask at level l
 // (reference masks without zeros are dropped).
 int build_mask_pyramids(ssg_registration* r, const uint8_t* const* src_ptrs, int src_pitch, int n, uint8_t* mpyr,
-                        bool is_ref, std::vector<char>& keep) {
+                        bool is_ref, std::vector<char>& keep, unsigned* zeros_out = nullptr) {
+  // partial masks are eroded by the forward-additive aligner (reference and current,
+  // ecc2.cc:1187-1234) and by c_ecclm (reference only, ecc2.cc:1387-1400); the
+  // inverse-compositional aligners keep them as given (ecc2.cc:585-630)
+  const int method = r->o.ecc.ecc_method;
+  const bool erode = method == SSG_ECC_ALIGN_FORWARD_ADDITIVE || (method == SSG_ECC_ALIGN_LM && is_ref);
   hipStream_t s = r->stream;
   // raw (un-eroded) chain into tmp_mask, one level at a time, then erode into mpyr
   CHK(r->tmp_mask.ensure((size_t)r->mask_elems * n));
@@ -506,12 +517,14 @@ int build_mask_pyramids(ssg_registration* r, const uint8_t* const* src_ptrs, int
     for (int f = 0; f < n; ++f) {
       uint8_t* src = raw + (size_t)f * r->mask_elems + r->loff[l];
       uint8_t* dst = mpyr + (size_t)f * r->mask_elems + r->loff[l];
+      if (zeros_out && f == 0) zeros_out[l] = zc[0];
       if (zc[f] == 0) {
         keep[(size_t)l * n + f] = is_ref ? 0 : 1;
         if (!is_ref) HIPCHK(hipMemcpy2DAsync(dst, lp, src, lp, r->lw[l], r->lh[l], hipMemcpyDeviceToDevice, s));
       } else {
         keep[(size_t)l * n + f] = 1;
-        HIPCHK(launch_erode5(src, 0, lp, dst, 0, lp, r->lw[l], r->lh[l], 1, BORDER_REPLICATE, 255, s));
+        if (erode) HIPCHK(launch_erode5(src, 0, lp, dst, 0, lp, r->lw[l], r->lh[l], 1, BORDER_REPLICATE, 255, s));
+        else HIPCHK(hipMemcpy2DAsync(dst, lp, src, lp, r->lw[l], r->lh[l], hipMemcpyDeviceToDevice, s));
       }
     }
   }
@@ -526,6 +539,7 @@ int ensure_workspace(ssg_registration* r, int n) {
   CHK(r->xf_m.ensure(sizeof(FrameXf) * n));
   CHK(r->res.ensure(sizeof(FrameRes) * n));
   CHK(r->partials.ensure(sizeof(double) * (size_t)n * tx0 * ty0 * NMOM_FULL));
+  if (r->o.ecc.ecc_method != SSG_ECC_ALIGN_FORWARD_ADDITIVE) CHK(r->lm.ensure(sizeof(FrameLm) * n));
   for (int i = 0; i < 2; ++i) {
     CHK(r->active[i].ensure(sizeof(int) * n));
     CHK(r->active_count[i].ensure(sizeof(int) * 4));
@@ -547,6 +561,10 @@ int ensure_workspace(ssg_registration* r, int n) {
 int run_ecc(ssg_registration* r, int n, int pass, bool has_cmask, bool plain) {
   hipStream_t s = r->stream;
   const int motion = pass == 0 ? SSG_MOTION_TRANSLATION : r->o.motion_type;
+  const int method = r->o.ecc.ecc_method;
+  const bool tfirst = r->o.motion_type != SSG_MOTION_TRANSLATION && r->o.ecc.ecch_estimate_translation_first &&
+                      r->o.ecc.ecch_max_level != 0;
+  const bool ic = method == ECC_IC || method == ECC_ICLM;
   const int L = r->nlevels;
   const int grid = ecc_persistent_grid();
   const bool translation_first_ran = pass == 1 && r->o.motion_type != SSG_MOTION_TRANSLATION &&
@@ -567,6 +585,11 @@ int run_ecc(ssg_registration* r, int n, int pass, bool has_cmask, bool plain) {
     lb.cf = (float)r->ref_mean[l];
     lb.active_out = r->active[0].as<int>();
     lb.active_count_out = r->active_count[0].as<int>();
+    lb.method = method;
+    lb.lm = method != ECC_FA ? r->lm.as<FrameLm>() : nullptr;
+    // the inverse-compositional J is dropped whenever the translation-first pass swaps
+    // the aligner's transform (c_frame_registration.cc:823,844); otherwise it persists
+    lb.jrec = (method == ECC_IC || method == ECC_ICLM) && !tfirst ? r->jrec.as<JRec>() : nullptr;
     HIPCHK(launch_level_begin(lb, s));
 
     EccPassArgs a{};
@@ -596,23 +619,41 @@ int run_ecc(ssg_registration* r, int n, int pass, bool has_cmask, bool plain) {
     a.level = l;
     a.max_iterations = r->o.ecc.max_iterations;
     double max_eps = r->o.ecc.eps * (double)(1 << l);
-    if (max_eps <= 0) max_eps = 1e-3;
+    // only the forward-additive aligner replaces a non-positive max_eps (ecc2.cc:1274-1276)
+    if (max_eps <= 0 && method == ECC_FA) max_eps = 1e-3;
     a.max_eps = (float)max_eps;
     a.update_step_scale = r->o.ecc.update_step_scale;
+    a.method = method;
+    a.lm = method != ECC_FA ? r->lm.as<FrameLm>() : nullptr;
+    if (ic) {
+      a.rgx = r->ref_grad.as<float>() + r->loff[l];
+      a.rgy = r->ref_grad.as<float>() + r->pyr_elems + r->loff[l];
+      a.ref_h = r->ref_hm.as<double>() + (size_t)NMOM_REF * l;
+      a.rma = r->rma[l];
+      a.tiles_x = (a.w + CORR_TW - 1) / CORR_TW;
+      a.tiles_y = (a.h + CORR_TH - 1) / CORR_TH;
+      a.ntiles = a.tiles_x * a.tiles_y;
+    }
     const int g = std::min(grid, std::max(8, n * a.ntiles));
-    for (int it = 0; it < r->o.ecc.max_iterations; ++it) {
+    // the Levenberg-Marquardt aligners evaluate the starting parameters before their
+    // first counted iteration: one pass more
+    const int max_passes = r->o.ecc.max_iterations + (method == ECC_LM || method == ECC_ICLM ? 1 : 0);
+    for (int it = 0; it < max_passes; ++it) {
       a.active_in = r->active[it & 1].as<int>();
       a.active_count_in = r->active_count[it & 1].as<int>();
       a.active_out = r->active[(it + 1) & 1].as<int>();
       a.active_count_out = r->active_count[(it + 1) & 1].as<int>();
       r->prof.start(s);
-      HIPCHK(launch_ecc_pass(motion, a, g, s));
+      if (method == ECC_FA) HIPCHK(launch_ecc_pass(motion, a, g, s));
+      else if (method == ECC_LM) HIPCHK(launch_lm_pass(motion, a, g, s));
+      else HIPCHK(launch_ic_pass(motion, a, g, s));
       r->prof.stop(s, pass == 0 ? SSG_PROF_ECC_TRANSLATION : SSG_PROF_ECC_MAIN);
       r->prof.start(s);
-      HIPCHK(launch_ecc_solve(motion, a, s));
+      if (method == ECC_FA) HIPCHK(launch_ecc_solve(motion, a, s));
+      else HIPCHK(launch_lm_solve(motion, a, s));
       r->prof.stop(s, SSG_PROF_ECC_SOLVE);
       const bool check = plain || ((it & 3) == 3);
-      if (check && it + 1 < r->o.ecc.max_iterations) {
+      if (check && it + 1 < max_passes) {
         HIPCHK(hipMemcpyAsync(r->pinned_count, a.active_count_out, sizeof(int), hipMemcpyDeviceToHost, s));
         HIPCHK(hipStreamSynchronize(s));
         if (*r->pinned_count == 0) break;
@@ -895,8 +936,32 @@ extern "C" int ssg_registration_setup_reference_frame(ssg_registration* r, const
     CHK(r->mask_ptrs.ensure(sizeof(void*)));
     HIPCHK(hipMemcpyAsync(r->mask_ptrs.p, &mptr, sizeof(void*), hipMemcpyHostToDevice, s));
     std::vector<char> keep;
-    CHK(build_mask_pyramids(r, r->mask_ptrs.as<const uint8_t*>(), (int)mpitch, 1, r->ref_mask_pyr.as<uint8_t>(), true, keep));
-    for (int l = 0; l < lv; ++l) r->ref_has_mask[l] = keep[l] != 0;
+    unsigned zeros[MAX_LEVELS] = {};
+    CHK(build_mask_pyramids(r, r->mask_ptrs.as<const uint8_t*>(), (int)mpitch, 1, r->ref_mask_pyr.as<uint8_t>(), true, keep,
+                            zeros));
+    for (int l = 0; l < lv; ++l) {
+      r->ref_has_mask[l] = keep[l] != 0;
+      r->rma[l] = (double)r->lw[l] * r->lh[l] - zeros[l];
+    }
+  } else {
+    for (int l = 0; l < lv; ++l) r->rma[l] = (double)r->lw[l] * r->lh[l];
+  }
+  const int method = r->o.ecc.ecc_method;
+  if (method == ECC_IC || method == ECC_ICLM) {
+    // J precompute on the reference (ecc2.cc:1733-1738, 1985-1991): gradients zeroed off
+    // the reference mask and their Hessian moments, per level
+    CHK(r->ref_grad.ensure(sizeof(float) * 2 * (size_t)off));
+    CHK(r->ref_hm.ensure(sizeof(double) * NMOM_REF * MAX_LEVELS));
+    const size_t tiles0 = (size_t)((r->lw[0] + 63) / 64) * ((r->lh[0] + 63) / 64);
+    CHK(r->ref_part.ensure(sizeof(double) * NMOM_REF * tiles0));
+    for (int l = 0; l < lv; ++l) {
+      const uint8_t* m = r->ref_has_mask[l] ? r->ref_mask_pyr.as<uint8_t>() + r->loff[l] : nullptr;
+      HIPCHK(launch_ic_reference(r->ref_pyr.as<float>() + r->loff[l], r->lpitch[l], r->lw[l], r->lh[l], m, r->lpitch[l],
+                                 r->ref_grad.as<float>() + r->loff[l], r->ref_grad.as<float>() + off + r->loff[l],
+                                 r->ref_part.as<double>(), r->ref_hm.as<double>() + (size_t)NMOM_REF * l, s));
+    }
+    CHK(r->jrec.ensure(sizeof(JRec) * MAX_LEVELS));
+    HIPCHK(hipMemsetAsync(r->jrec.p, 0, sizeof(JRec) * MAX_LEVELS, s));
   }
   // level means (centring constants)
   CHK(r->sum_tmp.ensure(sizeof(double) * (r->lh[0] + 1)));

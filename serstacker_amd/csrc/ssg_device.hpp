@@ -64,6 +64,10 @@ __device__ __forceinline__ int quant32(float v) {
 
 __device__ __forceinline__ int sat_short(int v) { return min(max(v, -32768), 32767); }
 
+// INTER_NEAREST position of a float map coordinate: saturate_cast<short>(float),
+// round to nearest even; out-of-range values (NaN included) land outside any image
+__device__ __forceinline__ int round_coord(float v) { return __float2int_rn(fminf(fmaxf(v, -1e9f), 1e9f)); }
+
 // bilinear float weights at fraction indices (fx, fy) in 1/32 px
 __device__ __forceinline__ void bilinear_w(int fx, int fy, float& w0, float& w1, float& w2, float& w3) {
   const float tx = (float)fx * (1.f / 32.f), ty = (float)fy * (1.f / 32.f);

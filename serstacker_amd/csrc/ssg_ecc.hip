@@ -451,8 +451,37 @@ __global__ void __launch_bounds__(1024) k_level_begin(LevelBeginArgs a, int use_
     X.cg = a.cf;
     X.eps = FLT_MAX;
     update_map(mmot, X);
+    if (a.lm) {
+      // the aligner's local state at the start of its align() (ecc2.cc:1546-1560, 1731-1747, 1980-2000)
+      FrameLm& S = a.lm[f];
+      S.lambda = a.method == ECC_LM ? 0.1 : 0.001;
+      S.err = 0;
+      S.rmsold = (double)FLT_MAX;
+      S.phase = 0;
+      S.iteration = a.method == ECC_IC ? 1 : 0;  // the plain loop counts its first check before the first pass
+      S.dpe = 0.f;
+      // J of the inverse-compositional aligners at the parameters the level starts from
+      S.jc[0] = X.mc.c[0]; S.jc[1] = X.mc.c[1]; S.jc[2] = X.mc.c[2];
+      S.jc[3] = X.C[0]; S.jc[4] = X.C[1];
+    }
   }
   __syncthreads();
+  if (a.lm && a.jrec && mmot != MOTION_TRANSLATION && mmot != MOTION_AFFINE) {
+    // J kept across frames: the first frame aligned after the reference was set fixes it
+    JRec& J = a.jrec[a.level];
+    if (threadIdx.x == 0 && !J.valid) {
+      for (int f = 0; f < a.nframes; ++f) {
+        if (a.res[f].rejected) continue;
+        for (int i = 0; i < 5; ++i) J.c[i] = a.lm[f].jc[i];
+        J.valid = 1;
+        break;
+      }
+    }
+    __syncthreads();
+    for (int f = threadIdx.x; f < a.nframes; f += blockDim.x)
+      for (int i = 0; i < 5; ++i) a.lm[f].jc[i] = J.c[i];
+    __syncthreads();
+  }
   compact_active(a.pass == 0 ? a.xf_t : a.xf_m, a.nframes, a.active_out, a.active_count_out);
 }
 
@@ -463,6 +492,8 @@ hipError_t launch_level_begin(const LevelBeginArgs& a, hipStream_t s) {
   hipLaunchKernelGGL(k_level_begin, dim3(1), dim3(1024), 0, s, b, use_t);
   return hipGetLastError();
 }
+
+#include "ssg_ecc_lm.inc"
 
 // ------------------------------------------------------------------------------------------------
 // compute_correlation (ecc2.cc:65-137) for a batch: remap(cur, INTER_LINEAR,

@@ -141,10 +141,6 @@ hipError_t launch_flow_D(const float* A2, const float* A1, int cw, int ch, float
 // remap(current mask, INTER_NEAREST, BORDER_CONSTANT) & reference mask, It = ref - W,
 // Itxy = (It Ix, It Iy), zero where the mask is zero.
 // ---------------------------------------------------------------------------------------------
-__device__ __forceinline__ int round_coord(float v) {
-  // saturate_cast<short>(float): round to nearest even, saturated (NaN lands outside)
-  return __float2int_rn(fminf(fmaxf(v, -1e9f), 1e9f));
-}
 
 __global__ void __launch_bounds__(256) k_flow_residual(FlowLevelArgs a, const float2* __restrict__ uv, long long ufs,
                                                        float2* __restrict__ itxy, long long ifs) {

@@ -26,6 +26,35 @@ struct FrameXf {
 
 constexpr int MAX_LEVELS = 8;
 
+// ECC_ALIGN_METHOD values (ecc2.h:59-64)
+enum : int { ECC_FA = 0, ECC_IC = 1, ECC_LM = 2, ECC_ICLM = 3 };
+
+// Per-frame state of the Levenberg-Marquardt and inverse-compositional aligners
+// (c_ecclm, c_ecc_inverse_compositional, c_ecclm_inverse_compositional).  A pass
+// evaluates the map in FrameXf::mc: the current parameters (phase 0) or the
+// trial parameters newp (phase 1).
+struct FrameLm {
+  double lambda;
+  double err;       // error at the current parameters
+  double rmsold;    // plain inverse compositional: the previous iteration's error
+  double Hp[36];    // Hessian without damping
+  double v[6];      // J^T rhs at the current parameters
+  double dp[6];     // increment of the pending trial
+  float newp[6];    // trial parameters
+  float jc[5];      // inverse compositional: sa, ca, s, Cx, Cy of the J precompute
+  int phase;
+  int iteration;
+  float dpe;        // eps of the last increment
+  int pad;
+};
+
+// The J parameters of the inverse-compositional aligners when the reference
+// keeps one J per level across frames (no translation-first pass)
+struct JRec {
+  int valid;
+  float c[5];
+};
+
 struct FrameRes {
   int level_iters[2][MAX_LEVELS];  // [pass: 0 translation-first, 1 main][level]
   int level_failed[2][MAX_LEVELS];
@@ -38,6 +67,10 @@ struct FrameRes {
 constexpr int NMOM_FULL = 41;   // affine moments (see ssg_ecc.hip)
 constexpr int NMOM_TRANS = 14;  // translation moments
 constexpr int NMOM_CORR = 6;    // correlation moments
+constexpr int NMOM_LM_FULL = 25;  // c_ecclm pass: 18 Hessian + 6 J^T rhs moments + |rhs|^2
+constexpr int NMOM_LM_TRANS = 6;  // gx^2, gx gy, gy^2, gx rhs, gy rhs, |rhs|^2
+constexpr int NMOM_IC = 8;        // inverse-compositional pass: 6 J^T rhs moments, |rhs|^2, good pixels
+constexpr int NMOM_REF = 18;      // reference-gradient Hessian moments of a level
 
 constexpr int ECC_LW = 64;                    // LDS row width: one column per lane
 constexpr int ECC_TW = ECC_LW - 4;            // output columns per tile (2-px gradient halo each side)
@@ -83,6 +116,13 @@ struct EccPassArgs {
   int max_iterations;
   float max_eps;
   double update_step_scale;
+  // Levenberg-Marquardt / inverse-compositional aligners
+  int method;              // ECC_*
+  FrameLm* lm;
+  const float* rgx;        // reference gradients of the level (ref_pitch), zero off the reference mask
+  const float* rgy;
+  const double* ref_h;     // NMOM_REF moments of the reference gradients of the level
+  double rma;              // reference-mask pixels of the level (the area without a mask)
 };
 
 struct CorrPassArgs {
@@ -119,6 +159,9 @@ struct LevelBeginArgs {
   float cf;
   int* active_out;
   int* active_count_out;
+  int method;              // ECC_*
+  FrameLm* lm;
+  JRec* jrec;              // per-level record when J persists across frames, else null
 };
 
 struct AccumArgs {
@@ -195,6 +238,15 @@ hipError_t launch_u16_to_f32(const uint16_t* src, int src_pitch, float* dst, int
 // ECC (ssg_ecc.hip)
 hipError_t launch_level_begin(const LevelBeginArgs& a, hipStream_t s);
 hipError_t launch_ecc_pass(int motion, const EccPassArgs& a, int grid, hipStream_t s);
+// c_ecclm pass (gradients of the warped image, J at the evaluated parameters) and the
+// inverse-compositional pass (reference gradients); then the per-frame state machines
+hipError_t launch_lm_pass(int motion, const EccPassArgs& a, int grid, hipStream_t s);
+hipError_t launch_ic_pass(int motion, const EccPassArgs& a, int grid, hipStream_t s);
+hipError_t launch_lm_solve(int motion, const EccPassArgs& a, hipStream_t s);
+// reference gradients of a level (ecc_differentiate, zeroed where mask == 0) and their
+// Hessian moments (deterministic tile order); part: scratch of ntiles * NMOM_REF doubles
+hipError_t launch_ic_reference(const float* ref, int pitch, int w, int h, const uint8_t* mask, int mpitch, float* gx,
+                               float* gy, double* part, double* out, hipStream_t s);
 hipError_t launch_ecc_solve(int motion, const EccPassArgs& a, hipStream_t s);
 hipError_t launch_corr_pass(int motion, const CorrPassArgs& a, int grid, hipStream_t s);
 int ecc_persistent_grid();
