@@ -47,9 +47,14 @@ if ROOT not in sys.path:
 HBM_PEAK_GBS = 8000.0
 
 
-def c2_options(S):
+ECC_METHODS = {"fa": 0, "ic": 1, "lm": 2, "iclm": 3}
+ECC_METHOD_NAMES = {"fa": "forward-additive", "ic": "inverse-compositional", "lm": "Levenberg-Marquardt",
+                    "iclm": "inverse-compositional LM"}
+
+
+def c2_options(S, method="fa"):
     return S.c_image_registration_options(motion_type=S.MOTION_AFFINE, scale=1.0, eps=0.02, ecch_max_level=4,
-                                          max_iterations=50, min_rho=0.8)
+                                          max_iterations=50, min_rho=0.8, ecc_method=ECC_METHODS[method])
 
 
 def main():
@@ -69,6 +74,8 @@ def main():
                     help="affine: batched affine ECC + weighted stack of 1080p frames (the headline metric); "
                          "turbulence: phase correlation + eccflow + bicubic remap + average of 1024x1024 frames; "
                          "sweep: each remaining kernel family at its configuration's frame size")
+    ap.add_argument("--ecc-method", choices=list(ECC_METHODS), default="fa",
+                    help="ECC_ALIGN_METHOD of the affine workload (the headline is forward additive)")
     ap.add_argument("--copy-calib-mib", type=int, default=0,
                     help="after the timed region, copy a buffer of this size (a known-size reference for "
                          "rocprofv3 FETCH_SIZE/WRITE_SIZE)")
@@ -107,7 +114,7 @@ def main():
     t_synth = time.time() - t0
     print(f"[bench] synthesized {n} frames {W}x{H} in {t_synth:.1f}s", file=sys.stderr, flush=True)
 
-    opts = c2_options(S)
+    opts = c2_options(S, a.ecc_method)
     reg = S.c_frame_registration(opts)
     assert reg.setup_reference_frame(ref)
     acc = S.c_weigthed_average(W, H, 1)
@@ -205,7 +212,8 @@ def main():
         from oracle.oracle import Oracle, reg_options
 
         threads = min(16, os.cpu_count() or 1)
-        kw = dict(motion_type=3, scale=1.0, eps=0.02, ecch_max_level=4, max_iterations=50, min_rho=0.8)
+        kw = dict(motion_type=3, scale=1.0, eps=0.02, ecch_max_level=4, max_iterations=50, min_rho=0.8,
+                  ecc_method=ECC_METHODS[a.ecc_method])
         o32 = Oracle(32)
         o32.set_threads(threads)
         ref_h = ref.cpu().numpy()
@@ -263,7 +271,8 @@ def main():
         "dtype": "f32",
         "data": "synthetic",
         "config": {"workload": f"batched affine ECC + weighted stack, {n} f32 mono frames {W}x{H}, 4-level "
-                               f"pyramid, translation-first, eps 0.02 px, <=50 iterations",
+                               f"pyramid, translation-first, eps 0.02 px, <=50 iterations"
+                               + ("" if a.ecc_method == "fa" else f", {ECC_METHOD_NAMES[a.ecc_method]} ECC"),
                    "frames": n, "width": W, "height": H},
         "roofline": {"bound": "hbm", "kernel": dom_name, "achieved": round(dom["gbs"], 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(dom["frac"], 4), "traffic": traffic,
