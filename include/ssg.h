@@ -60,10 +60,17 @@ enum { SSG_INTER_NEAREST = 0, SSG_INTER_LINEAR = 1, SSG_INTER_CUBIC = 2, SSG_INT
 enum { SSG_BORDER_CONSTANT = 0, SSG_BORDER_REPLICATE = 1, SSG_BORDER_REFLECT = 2, SSG_BORDER_WRAP = 3,
        SSG_BORDER_REFLECT101 = 4, SSG_BORDER_TRANSPARENT = 5 };
 /* image element types (OpenCV type codes) */
-enum { SSG_8UC1 = 0, SSG_16UC1 = 2, SSG_32SC1 = 4, SSG_32FC1 = 5, SSG_32FC2 = 13, SSG_32SC3 = 20, SSG_32FC3 = 21 };
-/* COLORID of the Bayer patterns, core/io/debayer.h:22-25 */
-enum { SSG_COLORID_BAYER_RGGB = 8, SSG_COLORID_BAYER_GRBG = 9, SSG_COLORID_BAYER_GBRG = 10,
-       SSG_COLORID_BAYER_BGGR = 11 };
+enum { SSG_8UC1 = 0, SSG_16UC1 = 2, SSG_32SC1 = 4, SSG_32FC1 = 5, SSG_64FC1 = 6, SSG_32FC2 = 13, SSG_8UC3 = 16,
+       SSG_16UC3 = 18, SSG_32SC3 = 20, SSG_32FC3 = 21, SSG_64FC3 = 22 };
+/* COLORID, core/io/debayer.h:19-35 */
+enum { SSG_COLORID_MONO = 0, SSG_COLORID_BAYER_RGGB = 8, SSG_COLORID_BAYER_GRBG = 9, SSG_COLORID_BAYER_GBRG = 10,
+       SSG_COLORID_BAYER_BGGR = 11, SSG_COLORID_BAYER_CYYM = 16, SSG_COLORID_BAYER_YCMY = 17,
+       SSG_COLORID_BAYER_YMCY = 18, SSG_COLORID_BAYER_MYYC = 19, SSG_COLORID_RGB = 100, SSG_COLORID_BGR = 101 };
+/* DEBAYER_ALGORITHM, core/io/debayer.h:38-56 */
+enum { SSG_DEBAYER_DEFAULT = -1, SSG_DEBAYER_DISABLE = 0, SSG_DEBAYER_NN = 1, SSG_DEBAYER_NN2 = 2,
+       SSG_DEBAYER_VNG = 3, SSG_DEBAYER_EA = 4, SSG_DEBAYER_SP = 5, SSG_DEBAYER_MATRIX = 6,
+       SSG_DEBAYER_PLANE_0 = 7, SSG_DEBAYER_PLANE_1 = 8, SSG_DEBAYER_PLANE_2 = 9, SSG_DEBAYER_PLANE_3 = 10,
+       SSG_DEBAYER_AVGBP = 11 };
 /* frame_accumulation_method, core/pipeline/c_image_stacking_pipeline/c_image_stacking_pipeline.h:26-34 */
 enum { SSG_ACC_AVERAGE = 0, SSG_ACC_WEIGHTED_AVERAGE = 1, SSG_ACC_FOCUS_STACK = 2, SSG_ACC_FFT = 3,
        SSG_ACC_BAYER_AVERAGE = 4 };
@@ -448,6 +455,71 @@ typedef struct ssg_synth_params {
  * and returns each frame's true map (reference grid -> frame) in truth[6*i].
  * Frame index -1 renders the un-warped reference (identity map, mean blur). */
 int ssg_synth_render(const ssg_synth_params *p, int first, int n, const ssg_image *frames, double *truth);
+
+/* ---- input stage: SER sequences and raw-frame conversion ------------------------------------ */
+
+/* c_ser_file::file_header (core/io/c_ser_file.h:38-53): the 178-byte SER header.
+ * is_little_endian holds the reader's interpretation (c_ser_file.cc:279: the
+ * stored field is 0 for little-endian frames). */
+#pragma pack(push, 1)
+typedef struct ssg_ser_header {
+  char file_id[14];
+  int32_t luid;
+  int32_t color_id;         /* SSG_COLORID_* */
+  int32_t is_little_endian;
+  int32_t image_width;
+  int32_t image_height;
+  int32_t bits_per_plane;   /* 1..8 u8, 9..16 u16, 17..32 s32, -32 f32, -64 f64 */
+  int32_t frames_count;
+  char observer[40];
+  char instrument[40];
+  char telescope[40];
+  uint64_t date_time;
+  uint64_t date_time_utc;
+} ssg_ser_header;
+#pragma pack(pop)
+
+typedef struct ssg_ser_reader ssg_ser_reader;
+
+/* c_ser_reader::open (c_ser_file.cc:246-353): header checks, the trailing timestamp
+ * array when the file holds one; the read position is 0. */
+int ssg_ser_open(const char *path, ssg_ser_reader **out);
+/* c_ser_reader::close */
+void ssg_ser_close(ssg_ser_reader *r);
+/* c_ser_file::header(); *frame_type = cvtype() as SSG_* (channels 3 for RGB/BGR) */
+int ssg_ser_get_header(const ssg_ser_reader *r, ssg_ser_header *out, int32_t *frame_type);
+/* c_ser_file::timestamps(): writes min(n, count) values, *count = timestamps in the file (0 or frames_count) */
+int ssg_ser_timestamps(const ssg_ser_reader *r, uint64_t *ts, int32_t n, int32_t *count);
+/* c_ser_reader::seek / curpos (c_ser_file.cc:355-386) */
+int ssg_ser_seek(ssg_ser_reader *r, int32_t frame_index);
+int ssg_ser_curpos(const ssg_ser_reader *r, int32_t *pos);
+/* c_ser_reader::read (c_ser_file.cc:388-450): the frame at the read position into a host or
+ * device image of the file's type and size (byte order as the reference handles it); advances
+ * the position. */
+int ssg_ser_read(ssg_ser_reader *r, ssg_image *image);
+/* n successive read() calls into device images (no reference counterpart: the batch form of
+ * the pipeline's frame loop, c_image_stacking_pipeline.cc:1358-1390).  File reads go through two
+ * pinned buffers, each frame's host-to-device copy overlapping the next frame's read. */
+int ssg_ser_read_batch(ssg_ser_reader *r, int32_t n, ssg_image *images);
+
+/* debayer() (core/io/debayer.cc:1197-1355) on the GPU; src and dst device or host images.
+ * src: one channel, SSG_8UC1, SSG_16UC1 or SSG_32FC1, even size for the Bayer methods.  dst has
+ * the source depth: 3 channels for NN2 / MATRIX / SP (SP at half size), 1 channel for DISABLE
+ * (a copy), PLANE_k and AVGBP (half size).  NN, EA and VNG on 8/16-bit sources call OpenCV's
+ * demosaicing in the reference and return SSG_E_UNSUPPORTED here; on f32 sources (and VNG on
+ * 16-bit) they fall back to NN2 as the reference does.  DEFAULT is NN2. */
+int ssg_debayer(const ssg_image *src, int32_t colorid, int32_t algorithm, ssg_image *dst);
+
+/* The conversion of c_image_stacking_pipeline_base::read_input_frame
+ * (c_image_stacking_pipeline_base.cc:124-279) for n raw frames of one sequence: with a
+ * darkbayer or flatbayer (SSG_32FC1 of the raw size, or NULL) the frame is converted to f32
+ * (x / 2^bpp), the dark subtracted and the flat divided; Bayer frames are debayered with
+ * `algorithm` (in the source type when no dark/flat is set, with the reference's integer
+ * rounding); the result is converted to f32 (x / 2^bpp) when not yet.  raw: SSG_8UC1 /
+ * SSG_16UC1 / SSG_32FC1 (or 3-channel RGB/BGR frames, converted only); out: device or host
+ * SSG_32FC1 / SSG_32FC3 of the debayered size.  Batches of device frames run in one launch. */
+int ssg_read_input_frames(int32_t n, const ssg_image *raw, int32_t colorid, int32_t bpp, int32_t algorithm,
+                          const ssg_image *darkbayer, const ssg_image *flatbayer, ssg_image *out);
 
 #ifdef __cplusplus
 }

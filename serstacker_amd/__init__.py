@@ -13,6 +13,7 @@ reference, operations return False and keep an error message on failure.
 from __future__ import annotations
 
 import ctypes as C
+import os
 import logging
 
 import numpy as np
@@ -27,7 +28,7 @@ from ._lib import (  # noqa: F401
 log = logging.getLogger("serstacker_amd")
 
 __all__ = ["c_image_registration_options", "c_frame_registration", "c_weigthed_average", "image_view",
-           "c_eccflow", "synth_params", "synth_render", "SsgError"]
+           "c_eccflow", "synth_params", "synth_render", "SsgError", "c_ser_reader", "debayer", "read_input_frames"]
 
 
 def _is_torch(x) -> bool:
@@ -581,3 +582,124 @@ def synth_render(p: _lib.SynthParams, first: int, frames) -> np.ndarray:
     rc = lib.ssg_synth_render(C.byref(p), first, len(frames), views, truth.ctypes.data_as(C.POINTER(C.c_double)))
     _lib.check(rc, "ssg_synth_render")
     return truth
+
+
+class c_ser_reader:
+    """c_ser_reader (core/io/c_ser_file.h:136-180): SER frames read into host arrays or
+    device tensors, in the file's pixel type, without debayer."""
+
+    def __init__(self, filename=None):
+        self._lib = _lib.load()
+        self._h = None
+        if filename is not None:
+            self.open(filename)
+
+    def open(self, filename) -> bool:
+        self.close()
+        h = C.c_void_p()
+        rc = self._lib.ssg_ser_open(os.fsencode(filename), C.byref(h))
+        if rc:
+            log.error("open: %s", self._lib.ssg_last_error().decode())
+            return False
+        self._h = h
+        self._hdr = _lib.SerHeader()
+        t = C.c_int32()
+        _lib.check(self._lib.ssg_ser_get_header(self._h, C.byref(self._hdr), C.byref(t)), "header")
+        self.frame_type = t.value
+        return True
+
+    def is_open(self) -> bool:
+        return bool(self._h)
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.ssg_ser_close(self._h)
+        self._h = None
+
+    __del__ = close
+
+    def header(self) -> _lib.SerHeader:
+        return self._hdr
+
+    def image_width(self) -> int:
+        return self._hdr.image_width
+
+    def image_height(self) -> int:
+        return self._hdr.image_height
+
+    def color_id(self) -> int:
+        return self._hdr.color_id
+
+    def bits_per_plane(self) -> int:
+        return self._hdr.bits_per_plane
+
+    def num_frames(self) -> int:
+        return self._hdr.frames_count
+
+    def timestamps(self):
+        n = C.c_int32()
+        _lib.check(self._lib.ssg_ser_timestamps(self._h, None, 0, C.byref(n)), "timestamps")
+        ts = (C.c_uint64 * max(1, n.value))()
+        _lib.check(self._lib.ssg_ser_timestamps(self._h, ts, n.value, C.byref(n)), "timestamps")
+        return np.array(ts[:n.value], np.uint64)
+
+    def curpos(self) -> int:
+        p = C.c_int32()
+        _lib.check(self._lib.ssg_ser_curpos(self._h, C.byref(p)), "curpos")
+        return p.value
+
+    def seek(self, frame_index: int) -> bool:
+        rc = self._lib.ssg_ser_seek(self._h, frame_index)
+        if rc:
+            log.error("seek: %s", self._lib.ssg_last_error().decode())
+        return rc == 0
+
+    def _shape_dtype(self):
+        dt, cn = _lib.TYPE_NP[self.frame_type]
+        shape = (self._hdr.image_height, self._hdr.image_width) + ((cn,) if cn > 1 else ())
+        return shape, dt
+
+    def read(self, out=None):
+        """The next frame into `out` (host array or device tensor) or a new host array."""
+        if out is None:
+            shape, dt = self._shape_dtype()
+            out = np.empty(shape, dt)
+        v = image_view(out)
+        _lib.check(self._lib.ssg_ser_read(self._h, C.byref(v)), "read")
+        return out
+
+    def read_batch(self, outs):
+        """len(outs) successive frames into device tensors."""
+        views = image_views(outs)
+        _lib.check(self._lib.ssg_ser_read_batch(self._h, len(views), views), "read_batch")
+        return outs
+
+
+def debayer(src, colorid: int, algorithm: int = -1, out=None):
+    """debayer() (core/io/debayer.cc:1197-1355) on the GPU; `out` of the result's shape and
+    the source dtype (host array or device tensor), allocated on the host when omitted."""
+    if out is None:
+        h, w = src.shape[:2]
+        half = algorithm in (_lib.DEBAYER_SP, _lib.DEBAYER_AVGBP) or \
+            _lib.DEBAYER_PLANE_0 <= algorithm <= _lib.DEBAYER_PLANE_3
+        cn = 3 if algorithm in (-1, _lib.DEBAYER_NN, _lib.DEBAYER_NN2, _lib.DEBAYER_VNG, _lib.DEBAYER_EA,
+                                _lib.DEBAYER_SP, _lib.DEBAYER_MATRIX) else 1
+        shape = ((h // 2, w // 2) if half else (h, w)) + ((3,) if cn == 3 else ())
+        dt = src.cpu().numpy().dtype if _is_torch(src) else np.asarray(src).dtype
+        out = np.empty(shape, dt)
+    sv, ov = image_view(src), image_view(out)
+    _lib.check(_lib.load().ssg_debayer(C.byref(sv), colorid, algorithm, C.byref(ov)), "debayer")
+    return out
+
+
+def read_input_frames(raws, colorid: int, bpp: int, algorithm: int = -1, dark=None, flat=None, outs=None):
+    """The raw-frame conversion of read_input_frame (c_image_stacking_pipeline_base.cc:124-279)
+    for a list of frames; outs: f32 host arrays or device tensors of the debayered shape."""
+    rv = image_views(raws)
+    ov = image_views(outs)
+    dv, fv = image_view(dark), image_view(flat)
+    _lib.check(_lib.load().ssg_read_input_frames(len(rv), rv, colorid, bpp, algorithm,
+                                                 C.byref(dv) if dv else None, C.byref(fv) if fv else None, ov),
+               "read_input_frames")
+    return outs
+

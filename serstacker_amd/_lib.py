@@ -16,7 +16,13 @@ LIB_PATH = os.path.join(HERE, "libssg.so")
 
 SSG_OK = 0
 SSG_8UC1, SSG_16UC1, SSG_32SC1, SSG_32FC1, SSG_32FC2, SSG_32SC3, SSG_32FC3 = 0, 2, 4, 5, 13, 20, 21
+SSG_64FC1, SSG_8UC3, SSG_16UC3, SSG_64FC3 = 6, 16, 18, 22
+COLORID_MONO, COLORID_RGB, COLORID_BGR = 0, 100, 101
 COLORID_BAYER_RGGB, COLORID_BAYER_GRBG, COLORID_BAYER_GBRG, COLORID_BAYER_BGGR = 8, 9, 10, 11
+COLORID_BAYER_CYYM, COLORID_BAYER_YCMY, COLORID_BAYER_YMCY, COLORID_BAYER_MYYC = 16, 17, 18, 19
+(DEBAYER_DISABLE, DEBAYER_NN, DEBAYER_NN2, DEBAYER_VNG, DEBAYER_EA, DEBAYER_SP, DEBAYER_MATRIX, DEBAYER_PLANE_0,
+ DEBAYER_PLANE_1, DEBAYER_PLANE_2, DEBAYER_PLANE_3, DEBAYER_AVGBP) = range(12)
+DEBAYER_DEFAULT = -1
 FOCUS_SELECT_MAX_ENERGY, FOCUS_WEIGHTED_AVERAGE, FOCUS_LPG_WEIGHTS = 0, 1, 2
 MOTION_TRANSLATION, MOTION_EUCLIDEAN, MOTION_SCALED_EUCLIDEAN, MOTION_AFFINE = 0, 1, 2, 3
 ECC_ALIGN_FORWARD_ADDITIVE, ECC_ALIGN_INVERSE_COMPOSITIONAL, ECC_ALIGN_LM, ECC_ALIGN_INVERSE_COMPOSITIONAL_LM = 0, 1, 2, 3
@@ -28,6 +34,16 @@ REG_OK, REG_NOT_RUN, REG_LEVEL_FAILED, REG_RHO_TRANSLATION, REG_RHO = range(5)
 
 class SsgError(RuntimeError):
     pass
+
+
+class SerHeader(C.Structure):
+    """ssg_ser_header: c_ser_file::file_header (core/io/c_ser_file.h:38-53)."""
+    _pack_ = 1
+    _fields_ = [("file_id", C.c_char * 14), ("luid", C.c_int32), ("color_id", C.c_int32),
+                ("is_little_endian", C.c_int32), ("image_width", C.c_int32), ("image_height", C.c_int32),
+                ("bits_per_plane", C.c_int32), ("frames_count", C.c_int32), ("observer", C.c_char * 40),
+                ("instrument", C.c_char * 40), ("telescope", C.c_char * 40), ("date_time", C.c_uint64),
+                ("date_time_utc", C.c_uint64)]
 
 
 class Image(C.Structure):
@@ -203,6 +219,17 @@ _SIGS = [
     ("ssg_default_focus_stacking_options", None, [_P(FocusStackingOptions)]),
     ("ssg_accumulation_set_focus_options", C.c_int, [C.c_void_p, _P(FocusStackingOptions)]),
     ("ssg_lpg", C.c_int, [_P(Image), _P(Image), C.c_double, C.c_double, _P(C.c_double)]),
+    ("ssg_ser_open", C.c_int, [C.c_char_p, _P(C.c_void_p)]),
+    ("ssg_ser_close", None, [C.c_void_p]),
+    ("ssg_ser_get_header", C.c_int, [C.c_void_p, _P(SerHeader), _P(C.c_int32)]),
+    ("ssg_ser_timestamps", C.c_int, [C.c_void_p, _P(C.c_uint64), C.c_int32, _P(C.c_int32)]),
+    ("ssg_ser_seek", C.c_int, [C.c_void_p, C.c_int32]),
+    ("ssg_ser_curpos", C.c_int, [C.c_void_p, _P(C.c_int32)]),
+    ("ssg_ser_read", C.c_int, [C.c_void_p, _P(Image)]),
+    ("ssg_ser_read_batch", C.c_int, [C.c_void_p, C.c_int32, _P(Image)]),
+    ("ssg_debayer", C.c_int, [_P(Image), C.c_int32, C.c_int32, _P(Image)]),
+    ("ssg_read_input_frames", C.c_int, [C.c_int32, _P(Image), C.c_int32, C.c_int32, C.c_int32, _P(Image), _P(Image),
+                                        _P(Image)]),
     ("ssg_local_variance_map", C.c_int, [_P(Image), C.c_int32, C.c_int32, _P(Image), _P(C.c_double)]),
     ("ssg_accumulation_set_bayer_pattern", C.c_int, [C.c_void_p, C.c_int32]),
     ("ssg_accumulation_set_remap", C.c_int, [C.c_void_p, _P(Image)]),
@@ -257,7 +284,10 @@ def check(rc: int, what: str = ""):
 _NP_TYPE = {(np.dtype(np.uint8), 1): SSG_8UC1, (np.dtype(np.uint16), 1): SSG_16UC1,
             (np.dtype(np.int32), 1): SSG_32SC1, (np.dtype(np.int32), 3): SSG_32SC3,
             (np.dtype(np.float32), 1): SSG_32FC1,
-            (np.dtype(np.float32), 2): SSG_32FC2, (np.dtype(np.float32), 3): SSG_32FC3}
+            (np.dtype(np.float32), 2): SSG_32FC2, (np.dtype(np.float32), 3): SSG_32FC3,
+            (np.dtype(np.float64), 1): SSG_64FC1, (np.dtype(np.float64), 3): SSG_64FC3,
+            (np.dtype(np.uint8), 3): SSG_8UC3, (np.dtype(np.uint16), 3): SSG_16UC3}
+TYPE_NP = {v: k for k, v in _NP_TYPE.items()}
 
 
 def host_image(a: np.ndarray) -> Image:
@@ -276,7 +306,8 @@ def device_image(t) -> Image:
     if not t.is_cuda:
         raise ValueError("tensor must be on the GPU")
     cn = 1 if t.dim() == 2 else t.shape[2]
-    dts = {torch.uint8: np.uint8, torch.int16: np.uint16, torch.int32: np.int32, torch.float32: np.float32}
+    dts = {torch.uint8: np.uint8, torch.int16: np.uint16, torch.int32: np.int32, torch.float32: np.float32,
+           torch.float64: np.float64}
     if hasattr(torch, "uint16"):
         dts[torch.uint16] = np.uint16
     dt = dts[t.dtype]

@@ -86,6 +86,10 @@ int elem_size(int type) {
     case SSG_32FC2: return 8;
     case SSG_32FC3: return 12;
     case SSG_32SC3: return 12;
+    case SSG_64FC1: return 8;
+    case SSG_8UC3: return 3;
+    case SSG_16UC3: return 6;
+    case SSG_64FC3: return 24;
   }
   return 0;
 }
@@ -3166,3 +3170,5 @@ extern "C" int ssg_registration_remap(ssg_registration* r, const ssg_image* src,
   return ssg_remap(src, src_mask, nullptr, r->last.matrix, r->o.motion_type, r->last.params, dst, dst_mask,
                    interpolation, border_mode, border_value);
 }
+
+#include "ssg_input_api.inc"

@@ -164,6 +164,32 @@ struct LevelBeginArgs {
   JRec* jrec;              // per-level record when J persists across frames, else null
 };
 
+// ---- input stage (raw frames -> pipeline f32 images, debayer) ----------------------------------
+enum : int { INPUT_COPY = 0, INPUT_NN2 = 1, INPUT_MATRIX = 2, INPUT_SP = 3, INPUT_AVGBP = 4, INPUT_PLANE0 = 5 };
+
+struct InputArgs {
+  const void* src;         // frame f at src + f * src_fstride (bytes), rows of src_pitch bytes
+  long long src_fstride;
+  int src_pitch;
+  int w, h;                // source size
+  int ow, oh;              // output size
+  int pattern;             // Bayer layout: 0 RGGB, 1 GRBG, 2 GBRG, 3 BGGR
+  int method;              // INPUT_*
+  int convert;             // integer source converted to f32: multiply by scale
+  float scale;             // 1 / 2^bpp
+  const float* dark;       // darkbayer (floats, dark_pitch per row) or null
+  int dark_pitch;
+  const float* flat;       // flatbayer or null
+  int flat_pitch;
+  void* dst;               // frame f at dst + f * dst_fstride (bytes)
+  long long dst_fstride;
+  int dst_pitch;
+};
+
+// in_depth: 0 u8, 2 u16, 5 f32 (OpenCV depth codes); out_f32: f32 output (else the
+// source depth); float_work: the pipeline's float samples (dark/flat or f32 sources)
+hipError_t launch_input(const InputArgs& a, int in_depth, bool out_f32, bool float_work, int nframes, hipStream_t s);
+
 struct AccumArgs {
   const float* const* frames;  // n device pointers
   int src_pitch, sw, sh, cn;   // all frames share the geometry

@@ -74,6 +74,7 @@ def test_struct_layout_matches_header(tmp_path):
         "ssg_registration_status": _lib.RegistrationStatus,
         "ssg_profile": _lib.Profile,
         "ssg_synth_params": _lib.SynthParams,
+        "ssg_ser_header": _lib.SerHeader,
     }
     lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "ssg.h"', "int main(void) {"]
     for cname, cls in structs.items():
