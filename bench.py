@@ -543,6 +543,36 @@ def bench_sweep(a, S, _lib, torch):
                      _lib.FOCUS_LPG_WEIGHTS: "lpg"}[pol]
             timed(f"focus_stack_{pname}{'_batched' if batched else ''}_16x2048x1536_rgb", stack, n * 12 * W * H, n,
                   "frames")
+    # input stage: 12-bit RGGB frames of 4096x3000 -> f32 BGR (read_input_frame's conversion with
+    # the default NN2 debayer), one launch for the batch; and SER frames from a file to HBM
+    Wi, Hi, ni = 4096, 3000, 32
+    raw16 = (torch.rand((ni, Hi, Wi), generator=g) * 4095).to(torch.int16).to(dev)
+    bgr = torch.empty((ni, Hi, Wi, 3), dtype=torch.float32, device=dev)
+    rin = _lib.images([S.image_view(raw16[i]) for i in range(ni)])
+    rout = _lib.images([S.image_view(bgr[i]) for i in range(ni)])
+    timed(f"input_nn2_{ni}x{Wi}x{Hi}_u16_to_f32bgr",
+          lambda: _lib.check(lib.ssg_read_input_frames(ni, rin, _lib.COLORID_BAYER_RGGB, 12, _lib.DEBAYER_NN2, None,
+                                                       None, rout), "read_input_frames"),
+          ni * (2 + 12) * Wi * Hi, ni, "frames")
+    del bgr
+    import struct
+    import tempfile
+    ns = 8
+    with tempfile.TemporaryDirectory() as td:
+        path = os.path.join(td, "bench.ser")
+        host = raw16[:ns].cpu().numpy().view(np.uint16)
+        with open(path, "wb") as fh:  # SER v3 header (178 bytes, little-endian) + frames
+            fh.write(struct.pack("<14s7i40s40s40sQQ", b"LUCAM-RECORDER", 0, _lib.COLORID_BAYER_RGGB, 0, Wi, Hi, 12, ns,
+                                 b"", b"", b"", 0, 0))
+            fh.write(np.ascontiguousarray(host).tobytes())
+        rd = S.c_ser_reader(path)
+
+        def ser_read():
+            rd.seek(0)
+            rd.read_batch([raw16[i] for i in range(ns)])
+        timed(f"ser_read_batch_{ns}x{Wi}x{Hi}_u16", ser_read, ns * 2 * Wi * Hi, ns, "frames")
+        rd.close()
+    del raw16
     # lpg sharpness map of one RGB frame; local-variance weight map of a mono 1920x1080 frame
     out = torch.empty((H, W), dtype=torch.float32, device=dev)
     sv, ov = S.image_view(fr[0]), S.image_view(out)
