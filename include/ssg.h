@@ -521,6 +521,20 @@ int ssg_debayer(const ssg_image *src, int32_t colorid, int32_t algorithm, ssg_im
 int ssg_read_input_frames(int32_t n, const ssg_image *raw, int32_t colorid, int32_t bpp, int32_t algorithm,
                           const ssg_image *darkbayer, const ssg_image *flatbayer, ssg_image *out);
 
+/* ---- finish of a stack ------------------------------------------------------------------------ */
+
+/* linear_interpolation_inpaint(image, mask) (core/proc/inpaint/linear_interpolation_inpaint.cc:
+ * 335-361), in place: holes (mask == 0) are filled from the linear interpolations along their
+ * row and their column, weighted by distance, repeated while holes get filled.  image: SSG_32FC1
+ * or SSG_32FC3, host or device; mask: SSG_8UC1 of the image size (not modified). */
+int ssg_inpaint_linear(ssg_image *image, const ssg_image *mask);
+
+/* unsharp_mask(src, dst, sigma, alpha) (core/proc/unsharp_mask.cc:72-115) for f32 images: the
+ * low-pass image of create_lpass_image (a gaussian with BORDER_REFLECT, through a pyrDown /
+ * pyrUp chain for sigma > 2), then dst = src / (1 - alpha) - lpass * alpha / (1 - alpha).
+ * src, dst: SSG_32FC1 or SSG_32FC3 of one size, host or device (dst may be src). */
+int ssg_unsharp_mask(const ssg_image *src, ssg_image *dst, double sigma, double alpha);
+
 #ifdef __cplusplus
 }
 #endif

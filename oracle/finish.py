@@ -1,0 +1,149 @@
+"""CPU oracle of the stack finish (test infrastructure only): linear-interpolation
+inpainting (core/proc/inpaint/linear_interpolation_inpaint.cc) and unsharp_mask
+(core/proc/unsharp_mask.cc:15-115).
+
+Only tests/, ``__graft_entry__.smoke()`` and bench.py's CPU leg use this module.
+The inpainting follows the reference's run loops literally (pure Python over
+rows and columns, float32 scalars), so it is meant for small images.
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+
+f32 = np.float32
+
+
+def _runs(valid):
+    """(start, end) of the runs of invalid entries, as the reference's while loops find them."""
+    n = len(valid)
+    start = 0
+    while start < n:
+        while start < n and valid[start]:
+            start += 1
+        if start >= n:
+            break
+        end = start + 1
+        while end < n and not valid[end]:
+            end += 1
+        yield start, end
+        start = end
+
+
+def _fill_line(vals, valid, out, dist):
+    """One row / column of _interpolate_holes_h2 / _v2 (linear_interpolation_inpaint.cc:14-227)."""
+    n = len(valid)
+    for start, end in _runs(valid):
+        if start > 0 and end < n:
+            s, e = start - 1, end
+            scale = f32(1.0) / f32(end - start)
+            kk = (vals[e] - vals[s]) * scale
+            for x in range(start, e):
+                out[x] = vals[s] + f32(x - s) * kk
+                dist[x] = max(x - s, e - x)
+        elif start > 0:
+            s = start - 1
+            for x in range(start, n):
+                out[x] = vals[s]
+                dist[x] = x - s
+        elif end < n:
+            for x in range(start, end):
+                out[x] = vals[end]
+                dist[x] = end - x
+
+
+def inpaint_linear(image: np.ndarray, mask: np.ndarray) -> np.ndarray:
+    """linear_interpolation_inpaint(image, mask) (linear_interpolation_inpaint.cc:335-361)."""
+    img = np.array(image, dtype=np.float32, copy=True)
+    if img.ndim == 2:
+        img = img[..., None]
+    h, w, cn = img.shape
+    m = np.array(mask, dtype=np.uint8, copy=True)
+    total = int((m == 0).sum())
+    while total > 0:
+        ih = np.zeros_like(img)
+        iv = np.zeros_like(img)
+        hd = np.zeros((h, w), np.float32)
+        vd = np.zeros((h, w), np.float32)
+        valid = m != 0
+        for y in range(h):
+            _fill_line(img[y], valid[y], ih[y], hd[y])
+        for x in range(w):
+            col_out = iv[:, x]
+            col_d = vd[:, x]
+            _fill_line(img[:, x], valid[:, x], col_out, col_d)
+        filled = 0
+        for y in range(h):
+            for x in range(w):
+                if m[y, x]:
+                    continue
+                dh, dv = hd[y, x], vd[y, x]
+                if dh > 0 and dv > 0:
+                    dd = f32(1.0) / (dh + dv)
+                    img[y, x] = dd * (ih[y, x] * dv + iv[y, x] * dh)
+                elif dv > 0:
+                    img[y, x] = iv[y, x]
+                elif dh > 0:
+                    img[y, x] = ih[y, x]
+                else:
+                    continue
+                m[y, x] = 255
+                filled += 1
+        if filled < 1:
+            break
+        total -= filled
+    return img[..., 0] if image.ndim == 2 else img
+
+
+def lpass_plan(sigma: float, w: int, h: int):
+    """create_lpass_image's pyramid choice (unsharp_mask.cc:15-48): (levels, Ci, delta)."""
+    level, Ci = 0, 0
+    if sigma > 2:
+        min_size, imax = min(w, h), 0
+        while True:
+            min_size >>= 1
+            if not min_size:
+                break
+            imax += 1
+        C = int(sigma * sigma / 2)
+        while level < imax and (1 + 4 * Ci) <= C:
+            Ci = 1 + 4 * Ci
+            level += 1
+    delta = math.sqrt(sigma * sigma - 2 * Ci) / (1 << level) if level else sigma
+    return level, Ci, delta
+
+
+def _gaussian(ora, plane, sigma):
+    n = 2 * max(1, int(sigma * 5)) + 1
+    k = ora.gaussian_kernel(n, sigma)
+    return ora.sep_filter(plane, k, k, 2)  # BORDER_REFLECT
+
+
+def lpass(ora, plane: np.ndarray, sigma: float) -> np.ndarray:
+    """create_lpass_image (unsharp_mask.cc:15-70) of one f32 plane, over the C oracle's
+    sepFilter2D / pyrDown / pyrUp."""
+    h, w = plane.shape
+    level, Ci, delta = lpass_plan(sigma, w, h)
+    if level < 1:
+        return _gaussian(ora, plane, sigma)
+    sizes = [(w, h)]
+    lp = ora.pyr_down(plane, border=2)
+    for _ in range(1, level):
+        sizes.append((lp.shape[1], lp.shape[0]))
+        lp = ora.pyr_down(lp, border=2)
+    if delta > 0:
+        lp = _gaussian(ora, lp, delta)
+    for j in range(len(sizes) - 1, -1, -1):
+        lp = ora.pyr_up(lp, sizes[j])
+    return lp
+
+
+def unsharp_mask(ora, image: np.ndarray, sigma: float, alpha: float) -> np.ndarray:
+    """unsharp_mask(src, dst, sigma, alpha) (unsharp_mask.cc:72-115) for f32 images."""
+    if sigma <= 0 or alpha <= 0:
+        return image.copy()
+    planes = [image] if image.ndim == 2 else [np.ascontiguousarray(image[..., c]) for c in range(image.shape[2])]
+    a, b = f32(1.0 / (1.0 - alpha)), f32(-alpha / (1.0 - alpha))
+    out = [(p * a + lpass(ora, p, sigma).astype(np.float32) * b).astype(np.float32) for p in planes]
+    return out[0] if image.ndim == 2 else np.stack(out, -1)

@@ -28,7 +28,8 @@ from ._lib import (  # noqa: F401
 log = logging.getLogger("serstacker_amd")
 
 __all__ = ["c_image_registration_options", "c_frame_registration", "c_weigthed_average", "image_view",
-           "c_eccflow", "synth_params", "synth_render", "SsgError", "c_ser_reader", "debayer", "read_input_frames"]
+           "c_eccflow", "synth_params", "synth_render", "SsgError", "c_ser_reader", "debayer", "read_input_frames",
+           "inpaint_linear", "unsharp_mask"]
 
 
 def _is_torch(x) -> bool:
@@ -702,4 +703,22 @@ def read_input_frames(raws, colorid: int, bpp: int, algorithm: int = -1, dark=No
                                                  C.byref(dv) if dv else None, C.byref(fv) if fv else None, ov),
                "read_input_frames")
     return outs
+
+
+def inpaint_linear(image, mask):
+    """linear_interpolation_inpaint(image, mask) (core/proc/inpaint/linear_interpolation_inpaint.cc:
+    335-361): a device tensor is filled in place; a host array is returned filled (a copy)."""
+    img = image if _is_torch(image) else np.array(image, dtype=np.float32, copy=True)
+    iv, mv = image_view(img), image_view(mask)
+    _lib.check(_lib.load().ssg_inpaint_linear(C.byref(iv), C.byref(mv)), "inpaint_linear")
+    return img
+
+
+def unsharp_mask(src, sigma: float, alpha: float, out=None):
+    """unsharp_mask(src, dst, sigma, alpha) (core/proc/unsharp_mask.cc:72-115) of an f32 image."""
+    if out is None:
+        out = np.empty(tuple(src.shape), np.float32) if not _is_torch(src) else src.clone()
+    sv, ov = image_view(src), image_view(out)
+    _lib.check(_lib.load().ssg_unsharp_mask(C.byref(sv), C.byref(ov), sigma, alpha), "unsharp_mask")
+    return out
 

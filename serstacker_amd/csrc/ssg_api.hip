@@ -3172,3 +3172,4 @@ extern "C" int ssg_registration_remap(ssg_registration* r, const ssg_image* src,
 }
 
 #include "ssg_input_api.inc"
+#include "ssg_finish_api.inc"

@@ -190,6 +190,23 @@ struct InputArgs {
 // source depth); float_work: the pipeline's float samples (dark/flat or f32 sources)
 hipError_t launch_input(const InputArgs& a, int in_depth, bool out_f32, bool float_work, int nframes, hipStream_t s);
 
+// ---- finish of a stack: linear-interpolation inpainting, unsharp mask -------------------------
+struct InpaintArgs {
+  float* img;              // interleaved f32 image, pitch floats per row
+  int pitch, w, h, cn;
+  uint8_t* mask;           // 0 = hole; filled holes become 255
+  int mpitch;
+  float* ih;               // horizontal / vertical fills [h][w][cn]
+  float* iv;
+  float* hd;               // their distances [h][w], 0 where the row / column has no data
+  float* vd;
+  int* s_tmp;              // [h][w] scratch of the row scan
+  unsigned* filled;        // holes filled by the pass
+};
+hipError_t launch_inpaint_pass(const InpaintArgs& a, hipStream_t s);
+hipError_t launch_unsharp_blend(const float* src, int spitch, const float* lp, int lpitch, float* dst, int dpitch,
+                                int wcn, int h, float alpha, float beta, hipStream_t s);
+
 struct AccumArgs {
   const float* const* frames;  // n device pointers
   int src_pitch, sw, sh, cn;   // all frames share the geometry

@@ -1,0 +1,77 @@
+"""Pins of the stack-finish oracle (oracle/finish.py): linear-interpolation inpainting
+(linear_interpolation_inpaint.cc) and unsharp_mask (unsharp_mask.cc) by closed forms."""
+import math
+
+import numpy as np
+import pytest
+
+from oracle import finish as F
+
+
+def ramp(h, w, cn=1):
+    y, x = np.mgrid[0:h, 0:w].astype(np.float64)
+    planes = [0.25 + 0.01 * x - 0.02 * y + 0.5 * c for c in range(cn)]
+    img = np.stack(planes, -1) if cn > 1 else planes[0]
+    return img.astype(np.float32)
+
+
+@pytest.mark.parametrize("cn", [1, 3])
+def test_inpaint_row_formula(cn):
+    """A one-row image (no column data): a run of holes [start, end) between the valid
+    pixels s = start - 1 and e = end takes sv + (x - s) * (ev - sv) / (end - start), the
+    reference's slope (linear_interpolation_inpaint.cc:50-63), which reaches ev at the last
+    hole rather than at e."""
+    img = ramp(1, 12, cn)
+    mask = np.full((1, 12), 255, np.uint8)
+    mask[0, 3:7] = 0
+    holed = img.copy()
+    holed[mask == 0] = 9.0
+    out = F.inpaint_linear(holed, mask)
+    sv, ev = img[0, 2], img[0, 7]
+    kk = (ev - sv) * (np.float32(1) / np.float32(4))
+    for x in range(3, 7):
+        np.testing.assert_array_equal(out[0, x], sv + np.float32(x - 2) * kk)
+    np.testing.assert_array_equal(out[0, 6], ev)
+
+
+def test_inpaint_edges_copy_and_constant():
+    img = np.full((6, 8), 3.5, np.float32)
+    mask = np.full((6, 8), 255, np.uint8)
+    mask[:, :2] = 0  # holes touching the left edge: the first valid pixel of each row
+    mask[0, :] = 0   # a whole row: filled from the columns
+    holed = img.copy()
+    holed[mask == 0] = -1
+    np.testing.assert_array_equal(F.inpaint_linear(holed, mask), img)
+    # no data at all: nothing is filled, and the loop ends
+    empty = np.zeros((4, 5), np.uint8)
+    src = np.arange(20, dtype=np.float32).reshape(4, 5)
+    np.testing.assert_array_equal(F.inpaint_linear(src, empty), src)
+
+
+def test_inpaint_single_hole_closed_form():
+    """One hole at (2, 2) with row neighbours a, b and column neighbours c, d: the run slope
+    (b - a) / 1 makes the row fill b and the column fill d, both at distance max(1, 1) = 1,
+    so the hole takes (1 / 2) * (b + d)."""
+    img = np.zeros((5, 5), np.float32)
+    img[2, 1], img[2, 3], img[1, 2], img[3, 2] = 1.0, 3.0, 10.0, 20.0
+    mask = np.full((5, 5), 255, np.uint8)
+    mask[2, 2] = 0
+    out = F.inpaint_linear(img, mask)
+    assert out[2, 2] == np.float32(0.5) * (np.float32(3.0) + np.float32(20.0))
+
+
+def test_lpass_plan():
+    assert F.lpass_plan(1.5, 100, 100) == (0, 0, 1.5)
+    lv, ci, d = F.lpass_plan(3.0, 100, 100)
+    assert (lv, ci) == (1, 1) and math.isclose(d, math.sqrt(7) / 2)
+    lv, ci, d = F.lpass_plan(10.0, 400, 300)
+    assert (lv, ci) == (3, 21) and math.isclose(d, math.sqrt(58) / 8)
+    assert F.lpass_plan(10.0, 6, 6)[0] == 2  # limited by the image size (imax = 2)
+
+
+@pytest.mark.parametrize("sigma", [1.0, 3.0, 8.0])
+def test_unsharp_of_a_constant_is_the_constant(oracle32, sigma):
+    img = np.full((40, 50), 0.25, np.float32)
+    out = F.unsharp_mask(oracle32, img, sigma, 0.6)
+    np.testing.assert_allclose(out, img, atol=1e-6)
+    np.testing.assert_array_equal(F.unsharp_mask(oracle32, img, 0, 0.6), img)
