@@ -521,6 +521,25 @@ int ssg_debayer(const ssg_image *src, int32_t colorid, int32_t algorithm, ssg_im
 int ssg_read_input_frames(int32_t n, const ssg_image *raw, int32_t colorid, int32_t bpp, int32_t algorithm,
                           const ssg_image *darkbayer, const ssg_image *flatbayer, ssg_image *out);
 
+/* ---- master frame (c_image_stacking_pipeline_base.cc:282-404, c_image_stacking_pipeline.cc:1112-1312) */
+
+/* The quality metric of select_master_frame's best-of scan (c_image_stacking_pipeline_base.cc:
+ * 350-359): average_bayer_planes of Bayer frames (colorid a Bayer pattern), then
+ * compute_local_variance_map's Q (c_local_variance_sharpness_measure.cc:193-249) with
+ * kradius / dscale of the options' quality_estimation.  frames: raw SSG_8UC1 / SSG_16UC1 /
+ * SSG_32FC1 (or SSG_32FC3 for colour), host or device; quality[i] receives frame i's Q. */
+int ssg_master_frame_quality(int32_t n, const ssg_image *frames, int32_t colorid, int32_t kradius, int32_t dscale,
+                             double *quality);
+/* The scan window of master_frame_best_of_100_in_middle (c_image_stacking_pipeline_base.cc:312-326). */
+int ssg_master_scan_window(int32_t size, int32_t max_frames_to_scan, int32_t *start_pos, int32_t *end_pos);
+/* The selection rule (c_image_stacking_pipeline_base.cc:369-395): the first frame whose quality
+ * exceeds every earlier one (and 0); *index = its position + start_pos. */
+int ssg_select_master_frame(int32_t n, const double *quality, int32_t start_pos, int32_t *index);
+/* create_reference_frame's stacking window around the master frame
+ * (c_image_stacking_pipeline.cc:1219-1232). */
+int ssg_master_stack_window(int32_t size, int32_t start_frame_index, int32_t master_frame_pos,
+                            int32_t max_frames_to_stack, int32_t *startpos, int32_t *endpos);
+
 /* ---- finish of a stack ------------------------------------------------------------------------ */
 
 /* linear_interpolation_inpaint(image, mask) (core/proc/inpaint/linear_interpolation_inpaint.cc:

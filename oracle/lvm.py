@@ -12,6 +12,10 @@ oracle's pyrDown and numpy float32 elementwise arithmetic:
      low bits depend on the scheduling),
   5. map = G^3 * float(depthScale^3) + float(0.05 Q), depthScale = 20 for float images,
   6. cv::resize INTER_LINEAR back to the input size (half-pixel centres, clamped taps).
+
+8-bit and 16-bit single-channel images stay integer through the pyramid (cv::pyrDown
+on integer depths sums the 5x5 taps in int and rounds with (sum + 128) >> 8) and use
+depthScale = 20 / maxval(depth) (getMaxValForPixelDepth: 255, 65535).
 """
 from __future__ import annotations
 
@@ -27,19 +31,36 @@ def gray(img):
     return (b * F32(0.114) + g * F32(0.587)) + r * F32(0.299)
 
 
+def pyr_down_int(m):
+    """cv::pyrDown of an 8/16-bit image: [1 4 6 4 1] taps per axis summed in int,
+    (sum + 128) >> 8, BORDER_REFLECT_101, dst size ((w + 1) / 2, (h + 1) / 2)."""
+    h, w = m.shape
+    dh, dw = (h + 1) // 2, (w + 1) // 2
+    p = np.pad(m.astype(np.int64), 2, mode="reflect")  # numpy's "reflect" is REFLECT_101
+    taps = (1, 4, 6, 4, 1)
+    cols = 2 * np.arange(dw)
+    rows = sum(t * p[:, cols + k] for k, t in enumerate(taps))
+    ys = 2 * np.arange(dh)
+    tot = sum(t * rows[ys + k] for k, t in enumerate(taps))
+    return np.clip((tot + 128) >> 8, 0, np.iinfo(m.dtype).max).astype(m.dtype)
+
+
 def pdownscale(ora, m, level):
+    down = pyr_down_int if m.dtype.kind == "u" else ora.pyr_down
     if min(m.shape) < 4:
         return m
-    m = ora.pyr_down(m)
+    m = down(m)
     if min(m.shape) >= 4:
         for _ in range(1, level):
-            m = ora.pyr_down(m)
+            m = down(m)
             if min(m.shape) < 4:
                 break
     return m
 
 
 def morph_gradient(m, r):
+    """dilate - erode with a (2r+1)^2 square, BORDER_REPLICATE; in the image's depth, then
+    as float32 (exact for 8/16-bit)."""
     k = 2 * max(1, r) + 1
     p = np.pad(m, k // 2, mode="edge")
     h, w = m.shape
@@ -75,14 +96,15 @@ def resize_linear(m, dw, dh):
 def local_variance_map(ora, img, kradius=1, dscale=1):
     """Returns (map at the input size, Q); (zeros, 0) when the gradient vanishes."""
     h, w = img.shape[:2]
-    M = gray(img)
+    integral = img.dtype in (np.uint8, np.uint16)
+    M = img if integral else gray(img)
     if dscale > 0:
         M = pdownscale(ora, M, dscale)
     G = morph_gradient(M, kradius)
     W = float(np.abs(G.astype(np.float64)).sum())
     if not W > 0:
         return np.zeros((h, w), F32), 0.0
-    ds = 20.0
+    ds = 20.0 / float(np.iinfo(img.dtype).max) if integral else 20.0
     g64 = G.astype(np.float64)
     Q = ds ** 3 / W * float((g64 ** 4).sum())
     m = ((G * G) * G) * F32(ds ** 3)

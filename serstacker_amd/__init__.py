@@ -29,7 +29,8 @@ log = logging.getLogger("serstacker_amd")
 
 __all__ = ["c_image_registration_options", "c_frame_registration", "c_weigthed_average", "image_view",
            "c_eccflow", "synth_params", "synth_render", "SsgError", "c_ser_reader", "debayer", "read_input_frames",
-           "inpaint_linear", "unsharp_mask"]
+           "inpaint_linear", "unsharp_mask", "local_variance_map", "master_frame_quality", "master_scan_window",
+           "select_master_frame", "master_stack_window"]
 
 
 def _is_torch(x) -> bool:
@@ -539,6 +540,46 @@ def local_variance_map(image, kradius: int = 1, dscale: int = 1):
     _lib.check(lib.ssg_local_variance_map(C.byref(iv), kradius, dscale, C.byref(ov), C.byref(q)),
                "ssg_local_variance_map")
     return out, q.value
+
+
+def master_frame_quality(frames, colorid: int = 0, kradius: int = 1, dscale: int = 1) -> np.ndarray:
+    """The per-frame metric of select_master_frame's best-of scan
+    (c_image_stacking_pipeline_base.cc:350-359): average_bayer_planes of Bayer frames, then
+    compute_local_variance_map's Q.  frames: raw 8/16-bit or f32 arrays / device tensors."""
+    lib = _lib.load()
+    views = image_views(frames)
+    q = np.zeros(len(frames), np.float64)
+    _lib.check(lib.ssg_master_frame_quality(len(frames), views, colorid, kradius, dscale,
+                                            q.ctypes.data_as(C.POINTER(C.c_double))), "ssg_master_frame_quality")
+    return q
+
+
+def master_scan_window(size: int, max_frames_to_scan: int = 200):
+    """(start_pos, end_pos) of master_frame_best_of_100_in_middle
+    (c_image_stacking_pipeline_base.cc:312-326)."""
+    a, b = C.c_int32(), C.c_int32()
+    _lib.check(_lib.load().ssg_master_scan_window(size, max_frames_to_scan, C.byref(a), C.byref(b)),
+               "ssg_master_scan_window")
+    return a.value, b.value
+
+
+def select_master_frame(quality, start_pos: int = 0) -> int:
+    """The best-of rule (c_image_stacking_pipeline_base.cc:369-395): the global index of the first
+    frame whose quality exceeds all earlier ones."""
+    q = np.ascontiguousarray(quality, dtype=np.float64)
+    idx = C.c_int32()
+    _lib.check(_lib.load().ssg_select_master_frame(len(q), q.ctypes.data_as(C.POINTER(C.c_double)), start_pos,
+                                                   C.byref(idx)), "ssg_select_master_frame")
+    return idx.value
+
+
+def master_stack_window(size: int, start_frame_index: int, master_frame_pos: int, max_frames_to_stack: int):
+    """(startpos, endpos) of the frames create_reference_frame stacks around the master frame
+    (c_image_stacking_pipeline.cc:1219-1232)."""
+    a, b = C.c_int32(), C.c_int32()
+    _lib.check(_lib.load().ssg_master_stack_window(size, start_frame_index, master_frame_pos, max_frames_to_stack,
+                                                   C.byref(a), C.byref(b)), "ssg_master_stack_window")
+    return a.value, b.value
 
 
 def frame_sharpness(frames) -> np.ndarray:

@@ -233,6 +233,10 @@ _SIGS = [
     ("ssg_read_input_frames", C.c_int, [C.c_int32, _P(Image), C.c_int32, C.c_int32, C.c_int32, _P(Image), _P(Image),
                                         _P(Image)]),
     ("ssg_local_variance_map", C.c_int, [_P(Image), C.c_int32, C.c_int32, _P(Image), _P(C.c_double)]),
+    ("ssg_master_frame_quality", C.c_int, [C.c_int32, _P(Image), C.c_int32, C.c_int32, C.c_int32, _P(C.c_double)]),
+    ("ssg_master_scan_window", C.c_int, [C.c_int32, C.c_int32, _P(C.c_int32), _P(C.c_int32)]),
+    ("ssg_select_master_frame", C.c_int, [C.c_int32, _P(C.c_double), C.c_int32, _P(C.c_int32)]),
+    ("ssg_master_stack_window", C.c_int, [C.c_int32, C.c_int32, C.c_int32, C.c_int32, _P(C.c_int32), _P(C.c_int32)]),
     ("ssg_accumulation_set_bayer_pattern", C.c_int, [C.c_void_p, C.c_int32]),
     ("ssg_accumulation_set_remap", C.c_int, [C.c_void_p, _P(Image)]),
     ("ssg_accumulation_set_input_bpp", C.c_int, [C.c_void_p, C.c_int32]),

@@ -11,6 +11,7 @@
 #include <algorithm>
 #include <mutex>
 #include <string>
+#include <cstring>
 #include <vector>
 
 #include <cmath>
@@ -1660,7 +1661,9 @@ int focus_compute(ssg_accumulation* a) {
 extern "C" int ssg_local_variance_map(const ssg_image* image, int32_t kradius, int32_t dscale, ssg_image* map,
                                       double* q) {
   CHK(check_image(image, "image"));
-  if (image->type != SSG_32FC1 && image->type != SSG_32FC3) return fail(SSG_E_INVALID_ARG, "image must be SSG_32FC1/3");
+  if (image->type != SSG_32FC1 && image->type != SSG_32FC3 && image->type != SSG_8UC1 && image->type != SSG_16UC1)
+    return fail(SSG_E_INVALID_ARG, "image must be SSG_32FC1/3, SSG_8UC1 or SSG_16UC1");
+  const bool integral = image->type == SSG_8UC1 || image->type == SSG_16UC1;
   if (map) {
     CHK(check_image(map, "map"));
     if (map->type != SSG_32FC1 || map->width != image->width || map->height != image->height)
@@ -1687,6 +1690,8 @@ extern "C" int ssg_local_variance_map(const ssg_image* image, int32_t kradius, i
   // gray, dense
   if (image->type == SSG_32FC3) {
     HIPCHK(launch_bgr2gray((const float*)sp, (int)(spb / 4), w, h, a.as<float>(), s));
+  } else if (integral) {
+    HIPCHK(launch_int_to_f32(sp, (int)spb, w, h, image->type == SSG_16UC1, a.as<float>(), s));
   } else {
     HIPCHK(hipMemcpy2DAsync(a.p, (size_t)w * 4, sp, spb, (size_t)w * 4, h, hipMemcpyDeviceToDevice, s));
   }
@@ -1698,6 +1703,7 @@ extern "C" int ssg_local_variance_map(const ssg_image* image, int32_t kradius, i
     for (int l = 0; l < dscale; ++l) {
       const int nw = (cw + 1) / 2, nh = (ch + 1) / 2;
       HIPCHK(launch_pyrdown(cur, 0, cw, cw, ch, oth, 0, nw, nw, nh, 1, SSG_BORDER_REFLECT101, s));
+      if (integral) HIPCHK(launch_round_half_up(oth, (size_t)nw * nh, s));
       std::swap(cur, oth);
       cw = nw;
       ch = nh;
@@ -1718,13 +1724,18 @@ extern "C" int ssg_local_variance_map(const ssg_image* image, int32_t kradius, i
     W += hp[2 * i];
     S4 += hp[2 * i + 1];
   }
-  const double ds = 20.0;  // 20 * maxval(CV_32F) / maxval(CV_32F)
+  // depthScale = 20 * maxval(CV_32F) / maxval(depth) (getMaxValForPixelDepth: 1, 255, 65535)
+  const double ds = 20.0 / (image->type == SSG_8UC1 ? 255.0 : (image->type == SSG_16UC1 ? 65535.0 : 1.0));
   const double Q = W > 0 ? ds * ds * ds / W * S4 : 0.0;
   if (q) *q = Q;
   if (!map) return SSG_OK;
   if (!(W > 0)) {
-    HIPCHK(hipMemset2DAsync(map->data, (size_t)map->step, 0, (size_t)w * 4, h, s));
-    HIPCHK(hipStreamSynchronize(s));
+    if (map->on_device) {
+      HIPCHK(hipMemset2DAsync(map->data, (size_t)map->step, 0, (size_t)w * 4, h, s));
+      HIPCHK(hipStreamSynchronize(s));
+    } else {
+      for (int y = 0; y < h; ++y) std::memset((char*)map->data + (size_t)y * map->step, 0, (size_t)w * 4);
+    }
     return SSG_OK;
   }
   HIPCHK(launch_lvm_map(G, nl, (float)(ds * ds * ds), (float)(0.05 * Q), cur, s));

@@ -350,6 +350,8 @@ hipError_t launch_top_frames(float* acc, const float* L, size_t n, int cn, int m
 // local-variance sharpness map (ssg_lvm.hip)
 hipError_t launch_bgr2gray(const float* src, int pitch, int w, int h, float* g, hipStream_t s);
 hipError_t launch_morph_grad(const float* m, int w, int h, int r, float* g, hipStream_t s);
+hipError_t launch_int_to_f32(const void* src, int pitch, int w, int h, bool u16, float* dst, hipStream_t s);
+hipError_t launch_round_half_up(float* v, size_t n, hipStream_t s);
 int lvm_sum_blocks(size_t n);
 hipError_t launch_lvm_sums(const float* g, size_t n, double* partial, hipStream_t s);
 hipError_t launch_lvm_map(const float* g, size_t n, float map_scale, float add, float* m, hipStream_t s);

@@ -107,6 +107,33 @@ __global__ void __launch_bounds__(256) k_resize_linear(const float* src, int sw,
 
 }  // namespace
 
+namespace {
+// an 8/16-bit image as dense f32 (exact)
+__global__ void __launch_bounds__(256) k_int_to_f32(const uint8_t* src, int pitch, int w, int h, int u16, float* d) {
+  const size_t i = gid();
+  if (i >= (size_t)w * h) return;
+  const int x = (int)(i % w), y = (int)(i / w);
+  const uint8_t* row = src + (size_t)y * pitch;
+  d[i] = u16 ? (float)((const uint16_t*)row)[x] : (float)row[x];
+}
+// cv::pyrDown's integer rounding on 8/16-bit data, (sum + 128) >> 8, applied to the float
+// pyramid level (whose values are exact multiples of 1/256 for integer inputs)
+__global__ void __launch_bounds__(256) k_round_half_up(float* v, size_t n) {
+  const size_t i = gid();
+  if (i < n) v[i] = floorf(v[i] + 0.5f);
+}
+}  // namespace
+
+hipError_t launch_int_to_f32(const void* src, int pitch, int w, int h, bool u16, float* dst, hipStream_t s) {
+  hipLaunchKernelGGL(k_int_to_f32, dim3((unsigned)(((size_t)w * h + 255) / 256)), dim3(256), 0, s,
+                     (const uint8_t*)src, pitch, w, h, u16 ? 1 : 0, dst);
+  return hipGetLastError();
+}
+hipError_t launch_round_half_up(float* v, size_t n, hipStream_t s) {
+  hipLaunchKernelGGL(k_round_half_up, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, v, n);
+  return hipGetLastError();
+}
+
 hipError_t launch_bgr2gray(const float* src, int pitch, int w, int h, float* g, hipStream_t s) {
   hipLaunchKernelGGL(k_bgr2gray, dim3((unsigned)(((size_t)w * h + 255) / 256)), dim3(256), 0, s, src, pitch, w, h, g);
   return hipGetLastError();
