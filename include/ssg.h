@@ -548,6 +548,15 @@ int ssg_master_stack_window(int32_t size, int32_t start_frame_index, int32_t mas
  * or SSG_32FC3, host or device; mask: SSG_8UC1 of the image size (not modified). */
 int ssg_inpaint_linear(ssg_image *image, const ssg_image *mask);
 
+/* average_pyramid_inpaint(src, mask, dst, dstmask, max_levels)
+ * (core/proc/inpaint/average_pyramid_inpaint.cc:96-126), the pipeline's fill of the accumulated
+ * image (c_image_stacking_pipeline.cc:763-767, max_levels 100): a pyramid of 3x3 box averages
+ * over the valid pixels (BORDER_REPLICATE; downstrike_even / upject_even keep the odd rows and
+ * columns), each level falling back to its own valid pixels.  src / dst: SSG_32FC1 or
+ * SSG_32FC3 of one size (dst may be src); mask: SSG_8UC1, 0 = hole, or NULL (then a copy);
+ * dstmask: SSG_8UC1 or NULL.  Box sums are exact double sums rounded to float. */
+int ssg_average_pyramid_inpaint(const ssg_image *src, const ssg_image *mask, ssg_image *dst, ssg_image *dstmask,
+                                int32_t max_levels);
 /* unsharp_mask(src, dst, sigma, alpha) (core/proc/unsharp_mask.cc:72-115) for f32 images: the
  * low-pass image of create_lpass_image (a gaussian with BORDER_REFLECT, through a pyrDown /
  * pyrUp chain for sigma > 2), then dst = src / (1 - alpha) - lpass * alpha / (1 - alpha).

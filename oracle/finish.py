@@ -147,3 +147,77 @@ def unsharp_mask(ora, image: np.ndarray, sigma: float, alpha: float) -> np.ndarr
     a, b = f32(1.0 / (1.0 - alpha)), f32(-alpha / (1.0 - alpha))
     out = [(p * a + lpass(ora, p, sigma).astype(np.float32) * b).astype(np.float32) for p in planes]
     return out[0] if image.ndim == 2 else np.stack(out, -1)
+
+
+# ---- average_pyramid_inpaint (core/proc/inpaint/average_pyramid_inpaint.cc) ------------------------
+
+def _box3(a):
+    """cv::boxFilter(ksize 3x3, normalize=false, BORDER_REPLICATE) of an (h, w[, cn]) float32
+    array: the 9-term sum in float64 (OpenCV's sum type for float data), rounded to float32."""
+    h, w = a.shape[:2]
+    pad = ((1, 1), (1, 1)) + ((0, 0),) * (a.ndim - 2)
+    p = np.pad(a.astype(np.float64), pad, mode="edge")
+    s = np.zeros(a.shape, np.float64)
+    for dy in range(3):
+        for dx in range(3):
+            s += p[dy:dy + h, dx:dx + w]
+    return s.astype(np.float32)
+
+
+def _filter2(src, srcmask, fb_src, fb_mask):
+    """_average_pyramid_filter2 (average_pyramid_inpaint.cc:17-58): src (h, w, cn), masks (h, w)."""
+    dst = _box3(src)
+    dmask = _box3(srcmask)
+    fb = fb_mask != 0
+    div = ~fb & (dmask != 0)
+    scale = np.where(div, f32(1.0) / np.where(div, dmask, f32(1.0)), f32(1.0)).astype(np.float32)
+    dst = np.where(fb[..., None], fb_src, np.where(div[..., None], (dst * scale[..., None]).astype(np.float32), dst))
+    dmask = np.where(fb | div, f32(1.0), dmask).astype(np.float32)
+    return dst.astype(np.float32), dmask
+
+
+def _downstrike_even(a):
+    """downstrike_even (downstrike.cc:34-75): rows and columns min(2i + 1, n - 1)."""
+    h, w = a.shape[:2]
+    ys = np.minimum(2 * np.arange((h + 1) // 2) + 1, h - 1)
+    xs = np.minimum(2 * np.arange((w + 1) // 2) + 1, w - 1)
+    return a[ys][:, xs]
+
+
+def _upject_even(a, h, w):
+    """upject_even (downstrike.cc:254-367) to (h, w): values and a float mask of ones at the
+    odd-odd positions, zeros elsewhere."""
+    out = np.zeros((h, w) + a.shape[2:], a.dtype)
+    z = np.zeros((h, w), np.float32)
+    ny, nx = len(range(1, h, 2)), len(range(1, w, 2))
+    out[1::2, 1::2] = a[:ny, :nx]
+    z[1::2, 1::2] = 1
+    return out, z
+
+
+def _recurse(image, mask, max_levels):
+    """average_pyramid_recurse (average_pyramid_inpaint.cc:69-94)."""
+    h, w = image.shape[:2]
+    if min(w, h) > 1 and max_levels > 0:
+        fi, fm = _filter2(image, mask, image, mask)
+        fi, fm = _downstrike_even(fi), _downstrike_even(fm)
+        if np.count_nonzero(fm) < fm.size:
+            fi, fm = _recurse(fi, fm, max_levels - 1)
+        fi, fm = _upject_even(fi, h, w)
+        fi, fm = _filter2(fi, fm, image, mask)
+        return fi, fm
+    return image, mask
+
+
+def average_pyramid_inpaint(src, mask, max_levels=100):
+    """average_pyramid_inpaint (average_pyramid_inpaint.cc:96-126) of an f32 image (h, w[, cn])
+    and u8 mask; returns (image, mask)."""
+    src = np.asarray(src, np.float32)
+    if mask is None or np.count_nonzero(mask) == mask.size:
+        return src.copy(), (None if mask is None else mask.copy())
+    img = src if src.ndim == 3 else src[..., None]
+    img = np.where((mask != 0)[..., None], img, f32(0)).astype(np.float32)
+    msk = (mask.astype(np.float32) * f32(1.0 / 255.0)).astype(np.float32)
+    img, msk = _recurse(img, msk, max_levels)
+    out_mask = np.clip(np.rint(msk * f32(255.0)), 0, 255).astype(np.uint8)
+    return (img if src.ndim == 3 else img[..., 0]), out_mask

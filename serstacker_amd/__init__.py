@@ -29,8 +29,8 @@ log = logging.getLogger("serstacker_amd")
 
 __all__ = ["c_image_registration_options", "c_frame_registration", "c_weigthed_average", "image_view",
            "c_eccflow", "synth_params", "synth_render", "SsgError", "c_ser_reader", "debayer", "read_input_frames",
-           "inpaint_linear", "unsharp_mask", "local_variance_map", "master_frame_quality", "master_scan_window",
-           "select_master_frame", "master_stack_window"]
+           "inpaint_linear", "average_pyramid_inpaint", "unsharp_mask", "local_variance_map",
+           "master_frame_quality", "master_scan_window", "select_master_frame", "master_stack_window"]
 
 
 def _is_torch(x) -> bool:
@@ -753,6 +753,23 @@ def inpaint_linear(image, mask):
     iv, mv = image_view(img), image_view(mask)
     _lib.check(_lib.load().ssg_inpaint_linear(C.byref(iv), C.byref(mv)), "inpaint_linear")
     return img
+
+
+def average_pyramid_inpaint(src, mask, max_levels: int = 100, out=None, out_mask=None):
+    """average_pyramid_inpaint(src, mask, dst, dstmask, max_levels)
+    (core/proc/inpaint/average_pyramid_inpaint.cc:96-126): returns (image, mask); host arrays are
+    allocated for the outputs that are not given."""
+    h, w = src.shape[:2]
+    if out is None:
+        out = np.empty(tuple(src.shape), np.float32)
+    if out_mask is None:
+        out_mask = np.empty((h, w), np.uint8)
+    sv, ov, dmv = image_view(src), image_view(out), image_view(out_mask)
+    mv = image_view(mask) if mask is not None else None
+    _lib.check(_lib.load().ssg_average_pyramid_inpaint(C.byref(sv), C.byref(mv) if mv is not None else None,
+                                                       C.byref(ov), C.byref(dmv), max_levels),
+               "average_pyramid_inpaint")
+    return out, out_mask
 
 
 def unsharp_mask(src, sigma: float, alpha: float, out=None):

@@ -230,6 +230,7 @@ _SIGS = [
     ("ssg_debayer", C.c_int, [_P(Image), C.c_int32, C.c_int32, _P(Image)]),
     ("ssg_inpaint_linear", C.c_int, [_P(Image), _P(Image)]),
     ("ssg_unsharp_mask", C.c_int, [_P(Image), _P(Image), C.c_double, C.c_double]),
+    ("ssg_average_pyramid_inpaint", C.c_int, [_P(Image), _P(Image), _P(Image), _P(Image), C.c_int32]),
     ("ssg_read_input_frames", C.c_int, [C.c_int32, _P(Image), C.c_int32, C.c_int32, C.c_int32, _P(Image), _P(Image),
                                         _P(Image)]),
     ("ssg_local_variance_map", C.c_int, [_P(Image), C.c_int32, C.c_int32, _P(Image), _P(C.c_double)]),

@@ -204,6 +204,12 @@ struct InpaintArgs {
   unsigned* filled;        // holes filled by the pass
 };
 hipError_t launch_inpaint_pass(const InpaintArgs& a, hipStream_t s);
+hipError_t launch_apr_init(const float* src, int spitch, const uint8_t* mask, int mpitch, int w, int h, int cn,
+                           float* img, float* msk, unsigned* nz, hipStream_t s);
+hipError_t launch_apr_down(const float* img, const float* msk, int w, int h, int cn, float* dimg, float* dmsk, int dw,
+                           int dh, unsigned* nz, hipStream_t s);
+hipError_t launch_apr_up(float* img, float* msk, int w, int h, int cn, const float* simg, int sw, hipStream_t s);
+hipError_t launch_apr_mask_out(const float* msk, int w, int h, uint8_t* dst, int dpitch, hipStream_t s);
 hipError_t launch_unsharp_blend(const float* src, int spitch, const float* lp, int lpitch, float* dst, int dpitch,
                                 int wcn, int h, float alpha, float beta, hipStream_t s);
 

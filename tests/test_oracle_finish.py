@@ -75,3 +75,64 @@ def test_unsharp_of_a_constant_is_the_constant(oracle32, sigma):
     out = F.unsharp_mask(oracle32, img, sigma, 0.6)
     np.testing.assert_allclose(out, img, atol=1e-6)
     np.testing.assert_array_equal(F.unsharp_mask(oracle32, img, 0, 0.6), img)
+
+
+# ---- average_pyramid_inpaint: closed forms worked from the reference's rules ----------------------
+
+def _holes(h, w, pts):
+    m = np.full((h, w), 255, np.uint8)
+    for y, x in pts:
+        m[y, x] = 0
+    return m
+
+
+def test_average_pyramid_single_holes_closed_forms():
+    """Isolated interior holes: the odd-odd sampling of downstrike_even and the odd-odd placement of
+    upject_even give an even-even hole the mean of its 4 diagonal neighbours, an odd-odd hole the
+    mean of its 8 neighbours, and a mixed hole the mean of its 2 neighbours along the odd axis."""
+    from oracle import finish
+
+    r = np.random.default_rng(3)
+    img = r.uniform(0, 1, (11, 13)).astype(np.float32)
+    pts = [(2, 2), (3, 7), (6, 3), (5, 10), (8, 8)]
+    out, om = finish.average_pyramid_inpaint(img, _holes(11, 13, pts))
+    I = img.astype(np.float64)
+    assert out[2, 2] == np.float32((I[1, 1] + I[1, 3] + I[3, 1] + I[3, 3])) * np.float32(0.25)
+    assert out[3, 7] == np.float32(I[2:5, 6:9].sum() - I[3, 7]) * np.float32(0.125)
+    assert out[6, 3] == np.float32(I[5, 3] + I[7, 3]) * np.float32(0.5)
+    assert out[5, 10] == np.float32(I[5, 9] + I[5, 11]) * np.float32(0.5)
+    assert out[8, 8] == np.float32((I[7, 7] + I[7, 9] + I[9, 7] + I[9, 9])) * np.float32(0.25)
+    keep = _holes(11, 13, pts) != 0
+    np.testing.assert_array_equal(out[keep], img[keep])
+    assert (om == 255).all()
+
+
+def test_average_pyramid_replicate_corners():
+    """BORDER_REPLICATE: a hole at (0, 0) takes pixel (1, 1); a hole at the last odd-odd corner
+    takes its replicated 3x3 average, (a + 2b + 2c) / 5."""
+    from oracle import finish
+
+    r = np.random.default_rng(4)
+    img = r.uniform(0, 1, (8, 10)).astype(np.float32)
+    out, _ = finish.average_pyramid_inpaint(img, _holes(8, 10, [(0, 0), (7, 9)]))
+    I = img.astype(np.float64)
+    assert out[0, 0] == img[1, 1]
+    s = np.float32(I[6, 8] + 2 * I[6, 9] + 2 * I[7, 8])
+    assert out[7, 9] == s * (np.float32(1.0) / np.float32(5.0))
+
+
+def test_average_pyramid_constant_and_copies():
+    from oracle import finish
+
+    # a power-of-two constant survives every scaling k * (1 / k) of the averages, at every level
+    img = np.full((40, 52), 4.0, np.float32)
+    m = np.full((40, 52), 255, np.uint8)
+    m[5:33, 8:47] = 0
+    out, om = finish.average_pyramid_inpaint(img, m)
+    assert (out == 4.0).all() and (om == 255).all()
+    # nothing to fill: a copy; no levels: the masked copy and the mask unchanged
+    out, om = finish.average_pyramid_inpaint(img, np.full_like(m, 255))
+    assert (out == img).all() and (om == 255).all()
+    out, om = finish.average_pyramid_inpaint(img, m, max_levels=0)
+    assert (out[m == 0] == 0).all() and (out[m != 0] == 4.0).all()
+    np.testing.assert_array_equal(om, m)
