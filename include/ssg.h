@@ -548,6 +548,22 @@ int ssg_master_stack_window(int32_t size, int32_t start_frame_index, int32_t mas
  * or SSG_32FC3, host or device; mask: SSG_8UC1 of the image size (not modified). */
 int ssg_inpaint_linear(ssg_image *image, const ssg_image *mask);
 
+/* ---- stacking at a larger size: upscale of the registration remaps ---------------------------- */
+
+/* frame_upscale_option (c_image_stacking_pipeline.h:43-49). */
+enum { SSG_UPSCALE_NONE = 0, SSG_UPSCALE_X2 = 1, SSG_UPSCALE_X15 = 2, SSG_UPSCALE_X3 = 3 };
+
+/* The upscaled size: x2 (pyrUp) 2w x 2h, x1.5 (w*3/2) x (h*3/2), x3 3w x 3h, none w x h. */
+int ssg_upscale_size(int32_t option, int32_t w, int32_t h, int32_t *dw, int32_t *dh);
+/* upscale_remap (c_image_stacking_pipeline.cc:1869-1905) of n remaps, the
+ * frame_upscale_after_align step of the frame loop (:1633-1641): cv::resize INTER_LINEAR for
+ * x1.5 and x3 (float data: INTER_LINEAR_EXACT is INTER_LINEAR), cv::pyrUp for x2, a copy for
+ * none; the values (source positions) are not scaled.  src: SSG_32FC2 of one size; dst:
+ * SSG_32FC2 of ssg_upscale_size, 8-byte aligned steps.  Device maps with one step per side
+ * go in one launch; host maps are staged.  The upscaled remaps feed
+ * ssg_accumulation_add_remapped_batch with an accumulator of the upscaled size. */
+int ssg_upscale_remaps(int32_t option, int32_t n, const ssg_image *src, ssg_image *dst);
+
 /* average_pyramid_inpaint(src, mask, dst, dstmask, max_levels)
  * (core/proc/inpaint/average_pyramid_inpaint.cc:96-126), the pipeline's fill of the accumulated
  * image (c_image_stacking_pipeline.cc:763-767, max_levels 100): a pyramid of 3x3 box averages

@@ -221,3 +221,29 @@ def average_pyramid_inpaint(src, mask, max_levels=100):
     img, msk = _recurse(img, msk, max_levels)
     out_mask = np.clip(np.rint(msk * f32(255.0)), 0, 255).astype(np.uint8)
     return (img if src.ndim == 3 else img[..., 0]), out_mask
+
+
+# ---- upscale of registration remaps (c_image_stacking_pipeline.cc:1869-1905) -----------------------
+
+UPSCALE_NONE, UPSCALE_X2, UPSCALE_X15, UPSCALE_X3 = 0, 1, 2, 3
+
+
+def upscale_size(option, w, h):
+    return {UPSCALE_X2: (2 * w, 2 * h), UPSCALE_X15: (w * 3 // 2, h * 3 // 2), UPSCALE_X3: (3 * w, 3 * h)}.get(
+        option, (w, h))
+
+
+def upscale_map(ora, m, option):
+    """upscale_remap of an (h, w, 2) f32 map: cv::resize INTER_LINEAR per channel for x1.5 and x3
+    (INTER_LINEAR_EXACT on float data is INTER_LINEAR), cv::pyrUp per channel for x2."""
+    from oracle import lvm
+
+    h, w = m.shape[:2]
+    if option == UPSCALE_NONE:
+        return m.copy()
+    dw, dh = upscale_size(option, w, h)
+    if option == UPSCALE_X2:
+        chans = [ora.pyr_up(np.ascontiguousarray(m[..., c])) for c in range(2)]
+    else:
+        chans = [lvm.resize_linear(np.ascontiguousarray(m[..., c]), dw, dh) for c in range(2)]
+    return np.stack(chans, -1).astype(np.float32)

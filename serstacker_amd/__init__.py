@@ -29,7 +29,8 @@ log = logging.getLogger("serstacker_amd")
 
 __all__ = ["c_image_registration_options", "c_frame_registration", "c_weigthed_average", "image_view",
            "c_eccflow", "synth_params", "synth_render", "SsgError", "c_ser_reader", "debayer", "read_input_frames",
-           "inpaint_linear", "average_pyramid_inpaint", "unsharp_mask", "local_variance_map",
+           "inpaint_linear", "average_pyramid_inpaint", "unsharp_mask", "local_variance_map", "upscale_size",
+           "upscale_remaps",
            "master_frame_quality", "master_scan_window", "select_master_frame", "master_stack_window"]
 
 
@@ -770,6 +771,25 @@ def average_pyramid_inpaint(src, mask, max_levels: int = 100, out=None, out_mask
                                                        C.byref(ov), C.byref(dmv), max_levels),
                "average_pyramid_inpaint")
     return out, out_mask
+
+
+def upscale_size(option: int, w: int, h: int):
+    """(dw, dh) of a frame_upscale_option (c_image_stacking_pipeline.cc:1869-1905)."""
+    a, b = C.c_int32(), C.c_int32()
+    _lib.check(_lib.load().ssg_upscale_size(option, w, h, C.byref(a), C.byref(b)), "ssg_upscale_size")
+    return a.value, b.value
+
+
+def upscale_remaps(option: int, maps, out=None):
+    """upscale_remap (c_image_stacking_pipeline.cc:1869-1905) of (H, W, 2) f32 remaps: host
+    arrays or device tensors; `out` a list of the upscaled size, host arrays when omitted."""
+    h, w = maps[0].shape[:2]
+    dw, dh = upscale_size(option, w, h)
+    if out is None:
+        out = [np.empty((dh, dw, 2), np.float32) for _ in maps]
+    sv, dv = image_views(maps), image_views(out)
+    _lib.check(_lib.load().ssg_upscale_remaps(option, len(maps), sv, dv), "ssg_upscale_remaps")
+    return out
 
 
 def unsharp_mask(src, sigma: float, alpha: float, out=None):

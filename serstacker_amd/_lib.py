@@ -17,6 +17,7 @@ LIB_PATH = os.path.join(HERE, "libssg.so")
 SSG_OK = 0
 SSG_8UC1, SSG_16UC1, SSG_32SC1, SSG_32FC1, SSG_32FC2, SSG_32SC3, SSG_32FC3 = 0, 2, 4, 5, 13, 20, 21
 SSG_64FC1, SSG_8UC3, SSG_16UC3, SSG_64FC3 = 6, 16, 18, 22
+UPSCALE_NONE, UPSCALE_X2, UPSCALE_X15, UPSCALE_X3 = 0, 1, 2, 3
 COLORID_MONO, COLORID_RGB, COLORID_BGR = 0, 100, 101
 COLORID_BAYER_RGGB, COLORID_BAYER_GRBG, COLORID_BAYER_GBRG, COLORID_BAYER_BGGR = 8, 9, 10, 11
 COLORID_BAYER_CYYM, COLORID_BAYER_YCMY, COLORID_BAYER_YMCY, COLORID_BAYER_MYYC = 16, 17, 18, 19
@@ -231,6 +232,8 @@ _SIGS = [
     ("ssg_inpaint_linear", C.c_int, [_P(Image), _P(Image)]),
     ("ssg_unsharp_mask", C.c_int, [_P(Image), _P(Image), C.c_double, C.c_double]),
     ("ssg_average_pyramid_inpaint", C.c_int, [_P(Image), _P(Image), _P(Image), _P(Image), C.c_int32]),
+    ("ssg_upscale_size", C.c_int, [C.c_int32, C.c_int32, C.c_int32, _P(C.c_int32), _P(C.c_int32)]),
+    ("ssg_upscale_remaps", C.c_int, [C.c_int32, C.c_int32, _P(Image), _P(Image)]),
     ("ssg_read_input_frames", C.c_int, [C.c_int32, _P(Image), C.c_int32, C.c_int32, C.c_int32, _P(Image), _P(Image),
                                         _P(Image)]),
     ("ssg_local_variance_map", C.c_int, [_P(Image), C.c_int32, C.c_int32, _P(Image), _P(C.c_double)]),

@@ -68,6 +68,43 @@ __device__ __forceinline__ int sat_short(int v) { return min(max(v, -32768), 327
 // round to nearest even; out-of-range values (NaN included) land outside any image
 __device__ __forceinline__ int round_coord(float v) { return __float2int_rn(fminf(fmaxf(v, -1e9f), 1e9f)); }
 
+// cv::resize INTER_LINEAR tap of one axis (resize.cpp resize(): fx = (d+0.5)*scale - 0.5, the
+// taps clamped to the source)
+__device__ __forceinline__ void lin_tap(int d, double scale, int ssize, int& s0, int& s1, float& a0, float& a1) {
+  float f = (float)(((double)d + 0.5) * scale - 0.5);
+  int s = (int)floorf(f);
+  f -= (float)s;
+  if (s < 0) {
+    f = 0.f;
+    s = 0;
+  }
+  if (s >= ssize - 1) {
+    f = 0.f;
+    s = ssize - 1;
+  }
+  s0 = s;
+  s1 = min(s + 1, ssize - 1);
+  a0 = 1.f - f;
+  a1 = f;
+}
+
+// cv::pyrUp (float), horizontal pass: the upsampled value at output column xo of source row S
+// (ST floats between pixels): [1 6 1] at even columns, [4 4] at odd ones, left REFLECT101, right
+// replicated; an odd extra column repeats the last one
+template <int ST = 1>
+__device__ __forceinline__ float hup(const float* S, int sw, int xo) {
+  if (sw == 1) return S[0] * 8.f;
+  if (xo >= 2 * sw) xo = 2 * sw - 1;
+  const int x = xo >> 1;
+  if (xo & 1) {
+    if (x == sw - 1) return S[(sw - 1) * ST] * 8.f;
+    return (S[x * ST] + S[(x + 1) * ST]) * 4.f;
+  }
+  if (x == 0) return fmaf(S[0], 6.f, S[ST] + S[ST]);
+  if (x == sw - 1) return fmaf(S[(sw - 1) * ST], 6.f, S[(sw - 2) * ST] + S[(sw - 1) * ST]);
+  return fmaf(S[x * ST], 6.f, S[(x - 1) * ST] + S[(x + 1) * ST]);
+}
+
 // bilinear float weights at fraction indices (fx, fy) in 1/32 px
 __device__ __forceinline__ void bilinear_w(int fx, int fy, float& w0, float& w1, float& w2, float& w3) {
   const float tx = (float)fx * (1.f / 32.f), ty = (float)fy * (1.f / 32.f);

@@ -204,6 +204,12 @@ struct InpaintArgs {
   unsigned* filled;        // holes filled by the pass
 };
 hipError_t launch_inpaint_pass(const InpaintArgs& a, hipStream_t s);
+// frame_upscale_option (c_image_stacking_pipeline.h:43-49)
+enum { UPSCALE_NONE = 0, UPSCALE_X2 = 1, UPSCALE_X15 = 2, UPSCALE_X3 = 3 };
+void upscale_size(int option, int w, int h, int* dw, int* dh);
+// n interleaved 2-channel f32 fields (pitches in floats) upscaled by the option (not NONE)
+hipError_t launch_upscale_maps(int option, int n, const float* const* src, int spitch, int sw, int sh,
+                               float* const* dst, int dpitch, hipStream_t s);
 hipError_t launch_apr_init(const float* src, int spitch, const uint8_t* mask, int mpitch, int w, int h, int cn,
                            float* img, float* msk, unsigned* nz, hipStream_t s);
 hipError_t launch_apr_down(const float* img, const float* msk, int w, int h, int cn, float* dimg, float* dmsk, int dw,

@@ -73,20 +73,6 @@ __global__ void __launch_bounds__(256) k_lpg(const float* g, int w, int h, float
 // cv::pyrUp (float) of nplanes planes: the oracle's canonical form -- horizontally
 // upsampled rows (left REFLECT101, right replicated), then the vertical pass; an odd
 // extra column repeats the last one, an odd extra row repeats row 2*sh-2
-__device__ __forceinline__ float hup(const float* S, int sw, int xo) {
-  // horizontally upsampled value at output column xo of source row S
-  if (sw == 1) return S[0] * 8.f;
-  if (xo >= 2 * sw) xo = 2 * sw - 1;
-  const int x = xo >> 1;
-  if (xo & 1) {
-    if (x == sw - 1) return S[sw - 1] * 8.f;
-    return (S[x] + S[x + 1]) * 4.f;
-  }
-  if (x == 0) return fmaf(S[0], 6.f, S[1] + S[1]);
-  if (x == sw - 1) return fmaf(S[sw - 1], 6.f, S[sw - 2] + S[sw - 1]);
-  return fmaf(S[x], 6.f, S[x - 1] + S[x + 1]);
-}
-
 __global__ void __launch_bounds__(256) k_pyrup(const float* src, long long src_fstride, int sw, int sh, float* dst,
                                                long long dst_fstride, int dw, int dh) {
   const size_t i = gidx();

@@ -71,25 +71,6 @@ __global__ void __launch_bounds__(256) k_lvm_map(const float* g, size_t n, float
   m[i] = ((v * v) * v) * map_scale + add;
 }
 
-// cv::resize INTER_LINEAR tap of one axis (resize.cpp resize(): fx = (d+0.5)*scale - 0.5)
-__device__ __forceinline__ void lin_tap(int d, double scale, int ssize, int& s0, int& s1, float& a0, float& a1) {
-  float f = (float)(((double)d + 0.5) * scale - 0.5);
-  int s = (int)floorf(f);
-  f -= (float)s;
-  if (s < 0) {
-    f = 0.f;
-    s = 0;
-  }
-  if (s >= ssize - 1) {
-    f = 0.f;
-    s = ssize - 1;
-  }
-  s0 = s;
-  s1 = min(s + 1, ssize - 1);
-  a0 = 1.f - f;
-  a1 = f;
-}
-
 __global__ void __launch_bounds__(256) k_resize_linear(const float* src, int sw, int sh, float* dst, int dst_pitch,
                                                        int dw, int dh, double scale_x, double scale_y) {
   const size_t i = gid();

@@ -136,3 +136,24 @@ def test_average_pyramid_constant_and_copies():
     out, om = finish.average_pyramid_inpaint(img, m, max_levels=0)
     assert (out[m == 0] == 0).all() and (out[m != 0] == 4.0).all()
     np.testing.assert_array_equal(om, m)
+
+
+def test_upscale_map_closed_forms(oracle32):
+    """The identity remap upscaled: x1.5 / x3 reproduce the positions (X + 0.5) / s - 0.5 inside
+    (clamped at the borders), x2 (pyrUp) reproduces X / 2 at every interior output pixel."""
+    from oracle import finish
+
+    h, w = 12, 16
+    yy, xx = np.mgrid[0:h, 0:w].astype(np.float32)
+    ident = np.stack([xx, yy], -1)
+    for opt, s in ((finish.UPSCALE_X15, 1.5), (finish.UPSCALE_X3, 3.0)):
+        up = finish.upscale_map(oracle32, ident, opt)
+        dw, dh = finish.upscale_size(opt, w, h)
+        assert up.shape == (dh, dw, 2)
+        want = np.clip((np.arange(dw) + 0.5) / s - 0.5, 0, w - 1)
+        np.testing.assert_allclose(up[dh // 2, :, 0], want, atol=2e-6)
+    up = finish.upscale_map(oracle32, ident, finish.UPSCALE_X2)
+    assert up.shape == (2 * h, 2 * w, 2)
+    np.testing.assert_array_equal(up[4:-4, 2:-2, 0], np.tile(np.arange(2, 2 * w - 2) / 2, (2 * h - 8, 1)))
+    np.testing.assert_array_equal(up[2:-2, 4:-4, 1], np.tile((np.arange(2, 2 * h - 2) / 2)[:, None], (1, 2 * w - 8)))
+    assert finish.upscale_size(finish.UPSCALE_X15, 33, 21) == (49, 31)
