@@ -587,6 +587,49 @@ def bench_sweep(a, S, _lib, torch):
     timed("local_variance_map_1920x1080", lambda: _lib.check(lib.ssg_local_variance_map(C.byref(mv), 1, 1,
                                                                                          C.byref(mo), C.byref(q))),
           (4 + 4) * W * H, 1, "frames")
+    # master-frame scan: average_bayer_planes + local-variance Q of 32 raw 12-bit RGGB 4096x3000 frames
+    Wm, Hm, nm = 4096, 3000, 32
+    rawm = (torch.rand((nm, Hm, Wm), generator=g) * 4095).to(torch.int16).to(dev)
+    mviews = _lib.images([S.image_view(rawm[i]) for i in range(nm)])
+    qs = (C.c_double * nm)()
+    timed(f"master_frame_quality_{nm}x{Wm}x{Hm}_u16_rggb",
+          lambda: _lib.check(lib.ssg_master_frame_quality(nm, mviews, _lib.COLORID_BAYER_RGGB, 1, 1, qs),
+                             "master_frame_quality"), nm * 2 * Wm * Hm, nm, "frames")
+    del rawm
+    # upscale after align: 64 registration remaps of 1920x1080 to x1.5 (2880x1620)
+    nu = 64
+    dwu, dhu = S.upscale_size(_lib.UPSCALE_X15, W, H)
+    src_maps = torch.rand((nu, H, W, 2), generator=g).to(dev) * 1000
+    dst_maps = torch.empty((nu, dhu, dwu, 2), dtype=torch.float32, device=dev)
+    uin = _lib.images([S.image_view(src_maps[i]) for i in range(nu)])
+    uout = _lib.images([S.image_view(dst_maps[i]) for i in range(nu)])
+    timed(f"upscale_remaps_x1.5_{nu}x{W}x{H}",
+          lambda: _lib.check(lib.ssg_upscale_remaps(_lib.UPSCALE_X15, nu, uin, uout), "upscale_remaps"),
+          nu * 8 * (W * H + dwu * dhu), nu, "maps")
+    del src_maps, dst_maps
+    # finish of a x1.5 stack (2880x1620 RGB) with 5% scattered holes and a band of missing rows
+    img = torch.rand((dhu, dwu, 3), generator=g).to(dev)
+    hole = torch.rand((dhu, dwu), generator=g) < 0.05
+    hole[dhu // 3:dhu // 3 + 40, :] = True
+    msk = torch.where(hole, 0, 255).to(torch.uint8).to(dev)
+    fimg = torch.empty_like(img)
+    fmsk = torch.empty_like(msk)
+    iv, mv2, fv, fm = S.image_view(img), S.image_view(msk), S.image_view(fimg), S.image_view(fmsk)
+    timed(f"average_pyramid_inpaint_{dwu}x{dhu}_rgb",
+          lambda: _lib.check(lib.ssg_average_pyramid_inpaint(C.byref(iv), C.byref(mv2), C.byref(fv), C.byref(fm), 100),
+                             "average_pyramid_inpaint"), 2 * 13 * dwu * dhu, 1, "images")
+    lin_img = torch.empty_like(img)
+    lin_msk = torch.empty_like(msk)
+    lv, lm = S.image_view(lin_img), S.image_view(lin_msk)
+
+    def lin():
+        lin_img.copy_(img)
+        lin_msk.copy_(msk)
+        _lib.check(lib.ssg_inpaint_linear(C.byref(lv), C.byref(lm)), "inpaint_linear")
+    timed(f"inpaint_linear_{dwu}x{dhu}_rgb", lin, 2 * 13 * dwu * dhu, 1, "images")
+    timed(f"unsharp_mask_sigma2_{dwu}x{dhu}_rgb",
+          lambda: _lib.check(lib.ssg_unsharp_mask(C.byref(iv), C.byref(fv), 2.0, 0.5), "unsharp_mask"),
+          2 * 12 * dwu * dhu, 1, "images")
     out_line = {"metric": "per-family kernel throughput (sweep)", "unit": "GB/s", "n_gpus": 1, "data": "synthetic",
                 "dtype": "f32", "config": {"workload": "kernel sweep"}, "families": res}
     print(json.dumps(out_line))
