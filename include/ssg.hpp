@@ -10,6 +10,15 @@
  *   ssg::c_bayer_average        core/average/c_frame_accumulation.h:222-262
  *   ssg::c_laplacian_pyramid_focus_stacking  core/average/c_frame_accumulation.h:141-184
  *   ssg::phase_correlation      (no reference counterpart: the FFT pre-alignment)
+ *   ssg::c_ser_reader           core/io/c_ser_file.h:136-215
+ *   ssg::debayer, ssg::read_input_frames   core/io/debayer.cc:1197-1355,
+ *                               c_image_stacking_pipeline_base.cc:124-279
+ *   ssg::master_frame_quality and the master-frame windows
+ *                               c_image_stacking_pipeline_base.cc:282-404,
+ *                               c_image_stacking_pipeline.cc:1219-1232
+ *   ssg::linear_interpolation_inpaint, ssg::average_pyramid_inpaint, ssg::unsharp_mask
+ *                               core/proc/inpaint/, core/proc/unsharp_mask.cc
+ *   ssg::upscale_remap          c_image_stacking_pipeline.cc:1869-1905
  *
  * Images are ssg::image views (pointer, size, byte stride, type, host/device)
  * instead of cv::InputArray; the caller owns the memory.  Methods return bool
@@ -405,6 +414,124 @@ inline bool frame_sharpness(const std::vector<image> &frames, std::vector<float>
   weights.resize(frames.size());
   std::vector<ssg_image> f(frames.begin(), frames.end());
   return ssg_frame_sharpness((int)f.size(), f.data(), weights.data()) == SSG_OK;
+}
+
+// c_ser_reader (core/io/c_ser_file.h:136-215): header, seek, per-frame reads to host or device
+// memory, batched reads to device memory.
+class c_ser_reader {
+ public:
+  c_ser_reader() = default;
+  explicit c_ser_reader(const std::string &path) { open(path); }
+  ~c_ser_reader() { close(); }
+  c_ser_reader(const c_ser_reader &) = delete;
+  c_ser_reader &operator=(const c_ser_reader &) = delete;
+  bool open(const std::string &path) {
+    close();
+    return ssg_ser_open(path.c_str(), &h_) == SSG_OK;
+  }
+  void close() {
+    if (h_) ssg_ser_close(h_);
+    h_ = nullptr;
+  }
+  bool is_open() const { return h_ != nullptr; }
+  // the header as stored (is_little_endian as the reference interprets it) and the frame type
+  bool header(ssg_ser_header &hdr, int *frame_type = nullptr) const {
+    int32_t t = 0;
+    const bool ok = ssg_ser_get_header(h_, &hdr, &t) == SSG_OK;
+    if (frame_type) *frame_type = t;
+    return ok;
+  }
+  int num_frames() const {
+    ssg_ser_header hdr{};
+    return header(hdr) ? hdr.frames_count : 0;
+  }
+  bool seek(int index) { return ssg_ser_seek(h_, index) == SSG_OK; }
+  int curpos() const {
+    int32_t p = -1;
+    return ssg_ser_curpos(h_, &p) == SSG_OK ? p : -1;
+  }
+  bool read(image &frame) { return ssg_ser_read(h_, &frame) == SSG_OK; }
+  bool read(std::vector<image> &frames) {
+    std::vector<ssg_image> v(frames.begin(), frames.end());
+    return ssg_ser_read_batch(h_, (int)v.size(), v.data()) == SSG_OK;
+  }
+  bool timestamps(std::vector<uint64_t> &ts) const {
+    ts.resize((size_t)num_frames());
+    int32_t count = 0;
+    if (ssg_ser_timestamps(h_, ts.data(), (int32_t)ts.size(), &count) != SSG_OK) return false;
+    ts.resize((size_t)count);
+    return true;
+  }
+
+ private:
+  ssg_ser_reader *h_ = nullptr;
+};
+
+// debayer(src, dst, colorid, algo) (core/io/debayer.cc:1197-1355)
+inline bool debayer(const image &src, image &dst, int colorid, int algorithm = SSG_DEBAYER_DEFAULT) {
+  return ssg_debayer(&src, colorid, algorithm, &dst) == SSG_OK;
+}
+
+// read_input_frame's calibration, debayer and conversion of raw frames
+// (c_image_stacking_pipeline_base.cc:124-279), a batch at a time
+inline bool read_input_frames(const std::vector<image> &raw, std::vector<image> &out, int colorid, int bpp,
+                              int algorithm = SSG_DEBAYER_DEFAULT, const image *darkbayer = nullptr,
+                              const image *flatbayer = nullptr) {
+  std::vector<ssg_image> r(raw.begin(), raw.end()), o(out.begin(), out.end());
+  return ssg_read_input_frames((int)r.size(), r.data(), colorid, bpp, algorithm, darkbayer, flatbayer, o.data()) ==
+         SSG_OK;
+}
+
+// select_master_frame's per-frame metric (c_image_stacking_pipeline_base.cc:350-359)
+inline bool master_frame_quality(const std::vector<image> &frames, int colorid, int kradius, int dscale,
+                                 std::vector<double> &quality) {
+  quality.resize(frames.size());
+  std::vector<ssg_image> f(frames.begin(), frames.end());
+  return ssg_master_frame_quality((int)f.size(), f.data(), colorid, kradius, dscale, quality.data()) == SSG_OK;
+}
+// master_frame_best_of_100_in_middle's scan window (c_image_stacking_pipeline_base.cc:312-326)
+inline std::pair<int, int> master_scan_window(int size, int max_frames_to_scan) {
+  int32_t a = 0, b = 0;
+  ssg_master_scan_window(size, max_frames_to_scan, &a, &b);
+  return {a, b};
+}
+// the selection rule (c_image_stacking_pipeline_base.cc:369-395): a global frame index
+inline int select_master_frame(const std::vector<double> &quality, int start_pos) {
+  int32_t i = start_pos;
+  ssg_select_master_frame((int)quality.size(), quality.data(), start_pos, &i);
+  return i;
+}
+// create_reference_frame's stacking window (c_image_stacking_pipeline.cc:1219-1232)
+inline std::pair<int, int> master_stack_window(int size, int start_frame_index, int master_frame_pos,
+                                               int max_frames_to_stack) {
+  int32_t a = 0, b = 0;
+  ssg_master_stack_window(size, start_frame_index, master_frame_pos, max_frames_to_stack, &a, &b);
+  return {a, b};
+}
+
+// linear_interpolation_inpaint(image, mask, image) (linear_interpolation_inpaint.cc:335-361), in place
+inline bool linear_interpolation_inpaint(image &img, const image &mask) {
+  return ssg_inpaint_linear(&img, &mask) == SSG_OK;
+}
+// average_pyramid_inpaint(src, mask, dst, dstmask, max_levels) (average_pyramid_inpaint.cc:96-126)
+inline bool average_pyramid_inpaint(const image &src, const image *mask, image &dst, image *dstmask,
+                                    int max_levels = 100) {
+  return ssg_average_pyramid_inpaint(&src, mask, &dst, dstmask, max_levels) == SSG_OK;
+}
+// unsharp_mask(src, dst, sigma, alpha) (core/proc/unsharp_mask.cc:72-115)
+inline bool unsharp_mask(const image &src, image &dst, double sigma, double alpha) {
+  return ssg_unsharp_mask(&src, &dst, sigma, alpha) == SSG_OK;
+}
+
+// upscale_remap(option, srcmap, dstmap) (c_image_stacking_pipeline.cc:1869-1905) of a batch of remaps
+inline std::pair<int, int> upscale_size(int option, int w, int h) {
+  int32_t a = w, b = h;
+  ssg_upscale_size(option, w, h, &a, &b);
+  return {a, b};
+}
+inline bool upscale_remap(int option, const std::vector<image> &src, std::vector<image> &dst) {
+  std::vector<ssg_image> s(src.begin(), src.end()), d(dst.begin(), dst.end());
+  return ssg_upscale_remaps(option, (int)s.size(), s.data(), d.data()) == SSG_OK;
 }
 
 }  // namespace ssg

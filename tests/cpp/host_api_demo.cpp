@@ -147,9 +147,52 @@ int main(int argc, char** argv) {
     fsum += avg[i];
     fcovered += mask[i] != 0;
   }
+
+  // stacking at x1.5 after align: the refined remaps upscaled, the frames warped onto the larger grid
+  const auto usz = ssg::upscale_size(SSG_UPSCALE_X15, W, H);
+  const size_t uplane = (size_t)usz.first * usz.second;
+  float* umaps = nullptr;
+  CHECK(hipMalloc((void**)&umaps, sizeof(float) * 2 * uplane * N) == hipSuccess);
+  std::vector<ssg::image> urmaps;
+  for (int i = 0; i < N; ++i) urmaps.push_back(ssg::image::map2f(umaps + 2 * uplane * i, usz.first, usz.second, true));
+  CHECK(ssg::upscale_remap(SSG_UPSCALE_X15, rmaps, urmaps));
+  ssg::c_weigthed_average acc3(usz.first, usz.second, 1);
+  CHECK(acc3.add_remapped_batch(frames, urmaps, &wts));
+  std::vector<float> uavg(uplane);
+  std::vector<uint8_t> umask(uplane);
+  ssg::image uav = ssg::image::f32(uavg.data(), usz.first, usz.second, false);
+  ssg::image ukv = ssg::image::u8(umask.data(), usz.first, usz.second, false);
+  CHECK(acc3.compute(uav, &ukv));
+  double usum = 0;
+  long ucovered = 0;
+  for (size_t i = 0; i < uplane; ++i) {
+    usum += uavg[i];
+    ucovered += umask[i] != 0;
+  }
+  hipFree(umaps);
   hipFree(maps);
 
-  std::printf("{\"device\": \"%s\", \"frames\": %d, \"plain_equals_batch\": %d, \"accumulated\": %d, "
+  // the finish: average_pyramid_inpaint of the x1.5 stack with its mask
+  std::vector<float> fin(uplane);
+  std::vector<uint8_t> finm(uplane);
+  ssg::image fv = ssg::image::f32(fin.data(), usz.first, usz.second, false);
+  ssg::image fmv = ssg::image::u8(finm.data(), usz.first, usz.second, false);
+  CHECK(ssg::average_pyramid_inpaint(uav, &ukv, fv, &fmv, 100));
+  double apr_sum = 0;
+  for (size_t i = 0; i < uplane; ++i) apr_sum += fin[i];
+
+  // master-frame scan over the frames (mono): quality and selection
+  std::vector<double> quality;
+  CHECK(ssg::master_frame_quality(frames, SSG_COLORID_MONO, 1, 1, quality));
+  const auto win = ssg::master_scan_window(N, 3);
+  const int master = ssg::select_master_frame(quality, 0);
+
+  std::printf("{\"up_avg_sum\": %.17g, \"up_covered\": %ld, \"apr_sum\": %.17g, \"master\": %d, "
+              "\"scan\": [%d, %d], \"quality\": [",
+              usum, ucovered, apr_sum, master, win.first, win.second);
+  for (int i = 0; i < N; ++i) std::printf("%s%.17g", i ? ", " : "", quality[i]);
+  std::printf("], ");
+  std::printf("\"device\": \"%s\", \"frames\": %d, \"plain_equals_batch\": %d, \"accumulated\": %d, "
               "\"map_err\": %.3g, \"avg_sum\": %.17g, \"covered\": %ld, \"flow_levels\": %d, "
               "\"flow_avg_sum\": %.17g, \"flow_covered\": %ld, \"params0\": [%.9g, %.9g, %.9g, %.9g, %.9g, "
               "%.9g], \"iterations\": [",

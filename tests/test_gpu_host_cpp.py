@@ -62,3 +62,18 @@ def test_cpp_host_api_matches_python_binding():
     favg = np.asarray(favg, np.float64)
     assert abs(favg.sum() - out["flow_avg_sum"]) <= 1e-12 * max(1.0, abs(out["flow_avg_sum"]))
     assert int((np.asarray(fmask) != 0).sum()) == out["flow_covered"]
+    # x1.5 upscale of the refined remaps + stack, pyramid inpaint of that stack, master-frame scan
+    dw, dh = S.upscale_size(S._lib.UPSCALE_X15, W, H)
+    up = [torch.empty((dh, dw, 2), dtype=torch.float32, device="cuda") for _ in range(N)]
+    S.upscale_remaps(S._lib.UPSCALE_X15, maps, out=up)
+    acc3 = S.c_weigthed_average(dw, dh, 1)
+    acc3.add_remapped_batch(frames, up, frame_weights=wts)
+    uavg, umask = acc3.compute()
+    assert abs(np.asarray(uavg, np.float64).sum() - out["up_avg_sum"]) <= 1e-12 * max(1.0, abs(out["up_avg_sum"]))
+    assert int((np.asarray(umask) != 0).sum()) == out["up_covered"]
+    fin, _ = S.average_pyramid_inpaint(uavg, umask, 100)
+    assert abs(np.asarray(fin, np.float64).sum() - out["apr_sum"]) <= 1e-12 * max(1.0, abs(out["apr_sum"]))
+    q = S.master_frame_quality(frames, 0, 1, 1)
+    np.testing.assert_array_equal(q, np.asarray(out["quality"]))
+    assert S.select_master_frame(q, 0) == out["master"]
+    assert list(S.master_scan_window(N, 3)) == out["scan"]
