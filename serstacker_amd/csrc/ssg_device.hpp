@@ -105,6 +105,22 @@ __device__ __forceinline__ float hup(const float* S, int sw, int xo) {
   return fmaf(S[x * ST], 6.f, S[(x - 1) * ST] + S[(x + 1) * ST]);
 }
 
+// sum of p[t * elem_stride] for t = t0, t0 + stride, ... < n, added in that order, with eight
+// loads in flight ahead of the additions (the same sum as the plain loop, bit for bit)
+__device__ __forceinline__ double ordered_strided_sum(const double* p, int t0, int n, int stride, int elem_stride) {
+  double s = 0;
+  int t = t0;
+  for (; t + 7 * stride < n; t += 8 * stride) {
+    double v[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) v[u] = p[(size_t)(t + u * stride) * elem_stride];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) s += v[u];
+  }
+  for (; t < n; t += stride) s += p[(size_t)t * elem_stride];
+  return s;
+}
+
 // bilinear float weights at fraction indices (fx, fy) in 1/32 px
 __device__ __forceinline__ void bilinear_w(int fx, int fy, float& w0, float& w1, float& w2, float& w3) {
   const float tx = (float)fx * (1.f / 32.f), ty = (float)fy * (1.f / 32.f);

@@ -311,7 +311,7 @@ __global__ void __launch_bounds__(256) k_ecc_solve(EccPassArgs a) {
     double s = 0;
     if (k < NM) {
       const double* base = reinterpret_cast<const double*>(a.partials) + (size_t)f * a.ntiles * NM;
-      for (int t = j; t < a.ntiles; t += ECC_WAVES) s += base[(size_t)t * NM + k];
+      s = ordered_strided_sum(base + k, j, a.ntiles, ECC_WAVES, NM);
     }
     dred[j][k] = s;
   }
