@@ -4,6 +4,8 @@
 // All kernels are LDS-tiled: a block loads its tile plus halo once with
 // coalesced reads, filters rows into a second LDS buffer and writes the
 // column-filtered output with coalesced stores.
+#include <type_traits>
+
 #include "ssg_engine.hpp"
 
 namespace ssg {
@@ -143,28 +145,40 @@ __global__ void __launch_bounds__(256) k_sepfilter_t(const float* const* __restr
   float* __restrict__ D = dst + f * dst_fstride;
   const int x0 = blockIdx.x * ST_TW, y0 = blockIdx.y * ST_TH;
   {
-    constexpr int NS = TW * TH, PER = (NS + 255) / 256, FB = 4;
+    // tiles whose halo lies inside the image skip the border mapping (the same samples)
+    constexpr int NS = TW * TH, PER = (NS + 255) / 256, FB = 8;
     float* sf = &sm[0][0];
+    auto load_tile = [&](auto inside_c) {
+      constexpr bool INSIDE = decltype(inside_c)::value;
 #pragma nounroll
-    for (int b0 = 0; b0 < PER; b0 += FB) {
-      int off[FB], dsti[FB];
+      for (int b0 = 0; b0 < PER; b0 += FB) {
+        int off[FB], dsti[FB];
 #pragma unroll
-      for (int j = 0; j < FB; ++j) {
-        const int i0 = tid + (b0 + j) * 256;
-        const bool ok = b0 + j < PER && i0 < NS;
-        const int i = ok ? i0 : 0;
-        const int ly = i / TW, lx = i - ly * TW;
-        const int bx = border_interp(x0 - R + lx, w, border), by = border_interp(y0 - R + ly, h, border);
-        off[j] = (bx < 0 || by < 0) ? -1 : by * src_pitch + bx;
-        dsti[j] = ok ? i : -1;
+        for (int j = 0; j < FB; ++j) {
+          const int i0 = tid + (b0 + j) * 256;
+          const bool ok = b0 + j < PER && i0 < NS;
+          const int i = ok ? i0 : 0;
+          const int ly = i / TW, lx = i - ly * TW;
+          if (INSIDE) {
+            off[j] = (y0 - R + ly) * src_pitch + (x0 - R + lx);
+          } else {
+            const int bx = border_interp(x0 - R + lx, w, border), by = border_interp(y0 - R + ly, h, border);
+            off[j] = (bx < 0 || by < 0) ? -1 : by * src_pitch + bx;
+          }
+          dsti[j] = ok ? i : -1;
+        }
+        float v[FB];
+#pragma unroll
+        for (int j = 0; j < FB; ++j) v[j] = (INSIDE || off[j] >= 0) ? S[off[j]] : 0.f;
+#pragma unroll
+        for (int j = 0; j < FB; ++j)
+          if (dsti[j] >= 0) sf[dsti[j]] = v[j];
       }
-      float v[FB];
-#pragma unroll
-      for (int j = 0; j < FB; ++j) v[j] = off[j] >= 0 ? S[off[j]] : 0.f;
-#pragma unroll
-      for (int j = 0; j < FB; ++j)
-        if (dsti[j] >= 0) sf[dsti[j]] = v[j];
-    }
+    };
+    if (x0 - R >= 0 && y0 - R >= 0 && x0 + ST_TW + R <= w && y0 + ST_TH + R <= h)
+      load_tile(std::true_type{});
+    else
+      load_tile(std::false_type{});
   }
   __syncthreads();
   for (int i = tid; i < TH * ST_TW; i += 256) {
@@ -230,27 +244,39 @@ __global__ void __launch_bounds__(256) k_pyrdown(const float* __restrict__ src, 
   const int x0 = blockIdx.x * PD_TW, y0 = blockIdx.y * PD_TH;
   const int sx0 = 2 * x0 - 2, sy0 = 2 * y0 - 2;
   {
-    constexpr int NS = PD_SW * PD_SH, PER = (NS + 255) / 256, FB = 4;
+    // tiles whose source box lies inside the level skip the border mapping (the same samples)
+    constexpr int NS = PD_SW * PD_SH, PER = (NS + 255) / 256, FB = 8;
+    auto load_tile = [&](auto inside_c) {
+      constexpr bool INSIDE = decltype(inside_c)::value;
 #pragma nounroll
-    for (int b0 = 0; b0 < PER; b0 += FB) {
-      int off[FB], ly[FB], lx[FB];
+      for (int b0 = 0; b0 < PER; b0 += FB) {
+        int off[FB], ly[FB], lx[FB];
 #pragma unroll
-      for (int j = 0; j < FB; ++j) {
-        const int i0 = tid + (b0 + j) * 256;
-        const bool ok = b0 + j < PER && i0 < NS;
-        const int i = ok ? i0 : 0;
-        ly[j] = ok ? i / PD_SW : -1;
-        lx[j] = i - (i / PD_SW) * PD_SW;
-        const int bx = border_interp(sx0 + lx[j], sw, border), by = border_interp(sy0 + i / PD_SW, sh, border);
-        off[j] = (bx < 0 || by < 0) ? -1 : by * src_pitch + bx;
+        for (int j = 0; j < FB; ++j) {
+          const int i0 = tid + (b0 + j) * 256;
+          const bool ok = b0 + j < PER && i0 < NS;
+          const int i = ok ? i0 : 0;
+          ly[j] = ok ? i / PD_SW : -1;
+          lx[j] = i - (i / PD_SW) * PD_SW;
+          if (INSIDE) {
+            off[j] = (sy0 + i / PD_SW) * src_pitch + (sx0 + lx[j]);
+          } else {
+            const int bx = border_interp(sx0 + lx[j], sw, border), by = border_interp(sy0 + i / PD_SW, sh, border);
+            off[j] = (bx < 0 || by < 0) ? -1 : by * src_pitch + bx;
+          }
+        }
+        float v[FB];
+#pragma unroll
+        for (int j = 0; j < FB; ++j) v[j] = (INSIDE || off[j] >= 0) ? S[off[j]] : 0.f;
+#pragma unroll
+        for (int j = 0; j < FB; ++j)
+          if (ly[j] >= 0) sm[ly[j]][lx[j]] = v[j];
       }
-      float v[FB];
-#pragma unroll
-      for (int j = 0; j < FB; ++j) v[j] = off[j] >= 0 ? S[off[j]] : 0.f;
-#pragma unroll
-      for (int j = 0; j < FB; ++j)
-        if (ly[j] >= 0) sm[ly[j]][lx[j]] = v[j];
-    }
+    };
+    if (sx0 >= 0 && sy0 >= 0 && sx0 + PD_SW <= sw && sy0 + PD_SH <= sh)
+      load_tile(std::true_type{});
+    else
+      load_tile(std::false_type{});
   }
   __syncthreads();
   for (int i = threadIdx.x; i < PD_SH * PD_TW; i += blockDim.x) {
