@@ -274,6 +274,13 @@ int ssg_accumulation_set_focus_options(ssg_accumulation *a, const ssg_focus_stac
  * same size (host or device); sharpness (optional): the map's mean. */
 int ssg_lpg(const ssg_image *src, ssg_image *dst, double k, double p, double *sharpness);
 
+/* ecc_normalize(src, mask, dst, lvl) (ecc2.cc:385-397), the local-mean removal that registration
+ * applies when ecc.normalization_scale > 0 and ecc.normalization_noise > 0
+ * (c_frame_registration.cc:624-628, 807-812): dst = src - the pyrUp chain of
+ * pyrDown^lvl(src, BORDER_REPLICATE) back to the source size, then 0 where mask (optional,
+ * SSG_8UC1) is not 255.  src / dst: SSG_32FC1 of one size, host or device. */
+int ssg_ecc_normalize(const ssg_image *src, const ssg_image *mask, ssg_image *dst, int32_t lvl);
+
 /* compute_local_variance_map (core/proc/sharpness_measure/c_local_variance_sharpness_measure.cc:193-249),
  * the per-pixel weight of weighted-average stacking (c_image_stacking_pipeline.cc:2013-2020), for the
  * gray channel with uscale 0 (the pipeline's settings, c_image_stacking_pipeline.h:94-99: dscale 1,

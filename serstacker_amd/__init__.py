@@ -31,7 +31,8 @@ __all__ = ["c_image_registration_options", "c_frame_registration", "c_weigthed_a
            "c_eccflow", "synth_params", "synth_render", "SsgError", "c_ser_reader", "debayer", "read_input_frames",
            "inpaint_linear", "average_pyramid_inpaint", "unsharp_mask", "local_variance_map", "upscale_size",
            "upscale_remaps",
-           "master_frame_quality", "master_scan_window", "select_master_frame", "master_stack_window"]
+           "master_frame_quality", "master_scan_window", "select_master_frame", "master_stack_window",
+           "ecc_normalize"]
 
 
 def _is_torch(x) -> bool:
@@ -541,6 +542,17 @@ def local_variance_map(image, kradius: int = 1, dscale: int = 1):
     _lib.check(lib.ssg_local_variance_map(C.byref(iv), kradius, dscale, C.byref(ov), C.byref(q)),
                "ssg_local_variance_map")
     return out, q.value
+
+
+def ecc_normalize(src, mask=None, lvl: int = 1, out=None):
+    """ecc_normalize(src, mask, dst, lvl) (ecc2.cc:385-397) of an f32 image."""
+    if out is None:
+        out = np.empty(tuple(src.shape), np.float32)
+    sv, ov = image_view(src), image_view(out)
+    mv = image_view(mask) if mask is not None else None
+    _lib.check(_lib.load().ssg_ecc_normalize(C.byref(sv), C.byref(mv) if mv is not None else None, C.byref(ov), lvl),
+               "ssg_ecc_normalize")
+    return out
 
 
 def master_frame_quality(frames, colorid: int = 0, kradius: int = 1, dscale: int = 1) -> np.ndarray:

@@ -237,6 +237,7 @@ _SIGS = [
     ("ssg_read_input_frames", C.c_int, [C.c_int32, _P(Image), C.c_int32, C.c_int32, C.c_int32, _P(Image), _P(Image),
                                         _P(Image)]),
     ("ssg_local_variance_map", C.c_int, [_P(Image), C.c_int32, C.c_int32, _P(Image), _P(C.c_double)]),
+    ("ssg_ecc_normalize", C.c_int, [_P(Image), _P(Image), _P(Image), C.c_int32]),
     ("ssg_master_frame_quality", C.c_int, [C.c_int32, _P(Image), C.c_int32, C.c_int32, C.c_int32, _P(C.c_double)]),
     ("ssg_master_scan_window", C.c_int, [C.c_int32, C.c_int32, _P(C.c_int32), _P(C.c_int32)]),
     ("ssg_select_master_frame", C.c_int, [C.c_int32, _P(C.c_double), C.c_int32, _P(C.c_int32)]),

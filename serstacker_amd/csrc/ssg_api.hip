@@ -325,7 +325,7 @@ struct ssg_registration {
   double rma[MAX_LEVELS] = {};
   // batch workspace
   int cap = 0;
-  DevBuf cur_pyr, cur_mask_pyr, half_buf;
+  DevBuf cur_pyr, cur_mask_pyr, half_buf, norm_buf;
   DevBuf xf_t, xf_m, res, partials, active[2], active_count[2], lm;
   DevBuf frame_ptrs, mask_ptrs, zero_count, frame_list, tmp_in, tmp_mask, sum_tmp;
   int* pinned_count = nullptr;
@@ -404,7 +404,6 @@ extern "C" int ssg_registration_create(const ssg_image_registration_options* opt
   if (o.ecc.max_iterations < 1) return fail(SSG_E_INVALID_ARG, "ecc.max_iterations %d", o.ecc.max_iterations);
   if (o.motion_type < SSG_MOTION_TRANSLATION || o.motion_type > SSG_MOTION_AFFINE) return fail(SSG_E_UNSUPPORTED, "motion_type %d", o.motion_type);
   if (!(o.ecc.scale <= 0 || o.ecc.scale == 1 || fabs(o.ecc.scale - 0.5) < 1e-2)) return fail(SSG_E_UNSUPPORTED, "ecc.scale %g: only 1 and 0.5", o.ecc.scale);
-  if (o.ecc.normalization_scale > 0 && o.ecc.normalization_noise > 0) return fail(SSG_E_UNSUPPORTED, "ecc normalization");
   if (o.ecc.replace_planetary_disk_with_mask) return fail(SSG_E_UNSUPPORTED, "replace_planetary_disk_with_mask");
   if (o.enable_eccflow_registration && o.eccflow.downscale_method != SSG_ECCFLOW_DOWNSCALE_PYRAMID &&
       !(o.eccflow.scale_factor > 0 && o.eccflow.scale_factor < 1))
@@ -703,6 +702,67 @@ int run_corr(ssg_registration* r, int n, int pass, bool has_cmask) {
 }
 
 // prepares level-0 sources (optionally the 0.5 pyrDown of scaleImage) and the pyramids of n frames
+// ecc_normalize(src, mask, dst, lvl) (ecc2.cc:385-397) of n frames given by device pointers:
+// dst = src - pyrUp chain(pyrDown^lvl(src, BORDER_REPLICATE)) back through the same sizes
+// (ecc_downscale / ecc_upscale, ecc2.cc:224-275), then 0 where the mask is not 255.  masks:
+// device pointers (entries may be null) or an empty vector.  Chunks of 16 frames.
+int ecc_normalize_frames(const std::vector<const float*>& src, int spitch, const std::vector<const uint8_t*>& masks,
+                         int mpitch, int w, int h, int lvl, float* out, int opitch, long long ofs, hipStream_t s) {
+  const int n = (int)src.size();
+  if (lvl < 1) return fail(SSG_E_INVALID_ARG, "normalization level %d", lvl);
+  std::vector<int> lw(1, w), lh(1, h);
+  std::vector<size_t> loff(1, 0);
+  size_t tot = 0;
+  for (int l = 1; l <= lvl; ++l) {
+    lw.push_back((lw.back() + 1) / 2);
+    lh.push_back((lh.back() + 1) / 2);
+    if (lw[l] == lw[l - 1] && lh[l] == lh[l - 1])
+      return fail(SSG_E_UNSUPPORTED, "normalization level %d below a 1-pixel image (%dx%d)", lvl, w, h);
+    loff.push_back(tot);
+    tot += (size_t)lw[l] * lh[l];
+  }
+  constexpr int CH = 16;
+  const size_t n0 = (size_t)w * h;
+  DevBuf down, upa, upb, ptrs;
+  CHK(down.ensure(sizeof(float) * tot * CH));
+  CHK(upa.ensure(sizeof(float) * n0 * CH));
+  CHK(upb.ensure(sizeof(float) * n0 * CH));
+  CHK(ptrs.ensure(sizeof(void*) * 2 * CH));
+  auto L = [&](int l) { return down.as<float>() + loff[l] * CH; };
+  auto N = [&](int l) { return (long long)lw[l] * lh[l]; };
+  std::vector<const void*> hp(2 * CH);
+  for (int b = 0; b < n; b += CH) {
+    const int m = std::min(CH, n - b);
+    for (int i = 0; i < m; ++i)
+      HIPCHK(launch_pyrdown(src[b + i], 0, spitch, w, h, L(1) + i * N(1), 0, lw[1], lw[1], lh[1], 1, BORDER_REPLICATE,
+                            s));
+    for (int l = 2; l <= lvl; ++l)
+      HIPCHK(launch_pyrdown(L(l - 1), N(l - 1), lw[l - 1], lw[l - 1], lh[l - 1], L(l), N(l), lw[l], lw[l], lh[l], m,
+                            BORDER_REPLICATE, s));
+    const float* cur = L(lvl);
+    float* bufs[2] = {upa.as<float>(), upb.as<float>()};
+    for (int l = lvl - 1; l >= 0; --l) {
+      float* dst = bufs[l & 1];
+      HIPCHK(launch_pyrup(cur, N(l + 1), lw[l + 1], lh[l + 1], dst, N(l), lw[l], lh[l], m, s));
+      cur = dst;
+    }
+    for (int i = 0; i < m; ++i) {
+      hp[i] = src[b + i];
+      hp[CH + i] = masks.empty() ? nullptr : masks[b + i];
+    }
+    HIPCHK(hipMemcpyAsync(ptrs.p, hp.data(), sizeof(void*) * 2 * CH, hipMemcpyHostToDevice, s));
+    HIPCHK(launch_norm_sub(ptrs.as<const float*>(), spitch, cur, N(0),
+                           masks.empty() ? nullptr : ptrs.as<const uint8_t*>() + CH, mpitch, out + b * ofs, opitch, ofs,
+                           w, h, m, s));
+    HIPCHK(hipStreamSynchronize(s));  // the pointer table is reused
+  }
+  return SSG_OK;
+}
+
+bool use_normalize(const ssg_image_registration_options& o) {
+  return o.ecc.normalization_scale > 0 && o.ecc.normalization_noise > 0;
+}
+
 int prepare_batch(ssg_registration* r, int n, const ssg_image* frames, const ssg_image* masks, bool* has_cmask) {
   hipStream_t s = r->stream;
   CHK(ensure_workspace(r, n));
@@ -746,6 +806,34 @@ int prepare_batch(ssg_registration* r, int n, const ssg_image* frames, const ssg
       ptrs[f] = r->half_buf.as<float>() + (hp / 4) * r->eh * f;
     }
     src_pitch = (int)(hp / 4);
+  }
+  DevBuf tmpnm;
+  if (use_normalize(r->o)) {
+    // ecc_normalize of the (scaled) frames with their masks (c_frame_registration.cc:807-812)
+    std::vector<const uint8_t*> nm;
+    int nmp = 0;
+    if (masks) {
+      if (half) return fail(SSG_E_UNSUPPORTED, "current masks with ecc.scale 0.5");
+      const size_t mp = ((size_t)r->ref_w + 255) / 256 * 256;
+      CHK(tmpnm.ensure(mp * r->ref_h * n));
+      for (int f = 0; f < n; ++f) {
+        CHK(check_image(&masks[f], "mask"));
+        if (masks[f].type != SSG_8UC1 || masks[f].width != r->ref_w || masks[f].height != r->ref_h)
+          return fail(SSG_E_INVALID_ARG, "mask %d must be SSG_8UC1 of the frame size", f);
+        uint8_t* dst = tmpnm.as<uint8_t>() + mp * r->ref_h * f;
+        HIPCHK(hipMemcpy2DAsync(dst, mp, masks[f].data, (size_t)masks[f].step, r->ref_w, r->ref_h,
+                                masks[f].on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, s));
+        nm.push_back(dst);
+      }
+      nmp = (int)mp;
+    }
+    const size_t np = ((size_t)r->ew * 4 + 255) / 256 * 256;
+    CHK(r->norm_buf.ensure(np * r->eh * n));
+    const long long nfs = (long long)(np / 4) * r->eh;
+    CHK(ecc_normalize_frames(ptrs, src_pitch, nm, nmp, r->ew, r->eh, r->o.ecc.normalization_scale,
+                             r->norm_buf.as<float>(), (int)(np / 4), nfs, s));
+    for (int f = 0; f < n; ++f) ptrs[f] = r->norm_buf.as<float>() + nfs * f;
+    src_pitch = (int)(np / 4);
   }
   HIPCHK(hipMemcpyAsync(r->frame_ptrs.p, ptrs.data(), sizeof(void*) * n, hipMemcpyHostToDevice, s));
   r->prof.start(s);
@@ -927,6 +1015,23 @@ extern "C" int ssg_registration_setup_reference_frame(ssg_registration* r, const
                           BORDER_REFLECT101, s));
     src = halfb.as<const float>();
     src_pitch = (int)(hp / 4);
+  }
+  DevBuf normb, tmpnm;
+  if (use_normalize(r->o)) {
+    // ecc_normalize of the reference with its mask (c_frame_registration.cc:624-628)
+    std::vector<const uint8_t*> nm;
+    size_t nmp = 0;
+    if (mask) {
+      const void* mptr;
+      CHK(device_view(mask, tmpnm, s, &mptr, &nmp));
+      nm.push_back((const uint8_t*)mptr);
+    }
+    const size_t np = ((size_t)r->ew * 4 + 255) / 256 * 256;
+    CHK(normb.ensure(np * r->eh));
+    CHK(ecc_normalize_frames({src}, src_pitch, nm, (int)nmp, r->ew, r->eh, r->o.ecc.normalization_scale,
+                             normb.as<float>(), (int)(np / 4), 0, s));
+    src = normb.as<const float>();
+    src_pitch = (int)(np / 4);
   }
   HIPCHK(hipMemcpyAsync(r->frame_ptrs.p, &src, sizeof(void*), hipMemcpyHostToDevice, s));
   CHK(build_pyramids(r, r->frame_ptrs.as<const float*>(), src_pitch, 1, r->o.ecc.reference_smooth_sigma,
@@ -1657,6 +1762,52 @@ int focus_compute(ssg_accumulation* a) {
   return SSG_OK;
 }
 }  // namespace
+
+extern "C" int ssg_ecc_normalize(const ssg_image* src, const ssg_image* mask, ssg_image* dst, int32_t lvl) {
+  CHK(check_image(src, "src"));
+  CHK(check_image(dst, "dst"));
+  if (src->type != SSG_32FC1 || dst->type != SSG_32FC1 || dst->width != src->width || dst->height != src->height)
+    return fail(SSG_E_INVALID_ARG, "src and dst must be SSG_32FC1 of one size");
+  const int w = src->width, h = src->height;
+  if (mask) {
+    CHK(check_image(mask, "mask"));
+    if (mask->type != SSG_8UC1 || mask->width != w || mask->height != h)
+      return fail(SSG_E_INVALID_ARG, "mask must be SSG_8UC1 of the image size");
+  }
+  hipStream_t s = nullptr;
+  HIPCHK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+  struct StreamGuard {
+    hipStream_t s;
+    ~StreamGuard() {
+      (void)hipStreamSynchronize(s);
+      (void)hipStreamDestroy(s);
+    }
+  } guard{s};
+  DevBuf ts, tm, out;
+  const void* sp;
+  size_t spb;
+  CHK(device_view(src, ts, s, &sp, &spb));
+  std::vector<const uint8_t*> nm;
+  size_t mpb = 0;
+  if (mask) {
+    const void* mp;
+    CHK(device_view(mask, tm, s, &mp, &mpb));
+    nm.push_back((const uint8_t*)mp);
+  }
+  float* op = (float*)dst->data;
+  size_t opb = (size_t)dst->step;
+  if (!dst->on_device || (opb & 3)) {
+    opb = (size_t)w * 4;
+    CHK(out.ensure(opb * h));
+    op = out.as<float>();
+  }
+  CHK(ecc_normalize_frames({(const float*)sp}, (int)(spb / 4), nm, (int)mpb, w, h, lvl, op, (int)(opb / 4), 0, s));
+  if (op != dst->data)
+    HIPCHK(hipMemcpy2DAsync(dst->data, (size_t)dst->step, op, opb, (size_t)w * 4, h,
+                            dst->on_device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  return SSG_OK;
+}
 
 extern "C" int ssg_local_variance_map(const ssg_image* image, int32_t kradius, int32_t dscale, ssg_image* map,
                                       double* q) {

@@ -452,6 +452,27 @@ hipError_t launch_sharpness(const float* const* src_ptrs, int pitch, int w, int 
   return hipGetLastError();
 }
 
+// the last step of ecc_normalize (ecc2.cc:392-396): dst = src - up, then 0 where the mask is
+// not 255 (setTo(0, ~mask)); blockIdx.z = frame
+__global__ void __launch_bounds__(256) k_norm_sub(const float* const* __restrict__ src, int spitch,
+                                                  const float* __restrict__ up, long long ufs,
+                                                  const uint8_t* const* __restrict__ masks, int mpitch,
+                                                  float* __restrict__ dst, int dpitch, long long dfs, int w) {
+  const int x = blockIdx.x * 256 + threadIdx.x, y = blockIdx.y, f = blockIdx.z;
+  if (x >= w) return;
+  float v = src[f][(size_t)y * spitch + x] - up[f * ufs + (size_t)y * w + x];
+  if (masks && masks[f] && masks[f][(size_t)y * mpitch + x] != 255) v = 0.f;
+  dst[f * dfs + (size_t)y * dpitch + x] = v;
+}
+
+hipError_t launch_norm_sub(const float* const* src, int spitch, const float* up, long long ufs,
+                           const uint8_t* const* masks, int mpitch, float* dst, int dpitch, long long dfs, int w, int h,
+                           int nframes, hipStream_t s) {
+  hipLaunchKernelGGL(k_norm_sub, dim3((w + 255) / 256, h, nframes), dim3(256), 0, s, src, spitch, up, ufs, masks,
+                     mpitch, dst, dpitch, dfs, w);
+  return hipGetLastError();
+}
+
 __global__ void k_u16_to_f32(const uint16_t* __restrict__ src, int src_pitch, float* __restrict__ dst,
                              int dst_pitch, int w, float scale) {
   const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;

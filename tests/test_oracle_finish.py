@@ -157,3 +157,25 @@ def test_upscale_map_closed_forms(oracle32):
     np.testing.assert_array_equal(up[4:-4, 2:-2, 0], np.tile(np.arange(2, 2 * w - 2) / 2, (2 * h - 8, 1)))
     np.testing.assert_array_equal(up[2:-2, 4:-4, 1], np.tile((np.arange(2, 2 * h - 2) / 2)[:, None], (1, 2 * w - 8)))
     assert finish.upscale_size(finish.UPSCALE_X15, 33, 21) == (49, 31)
+
+
+def test_ecc_normalize_closed_forms(oracle32):
+    """ecc_normalize removes the local mean: a constant image gives exactly 0 (pyrDown and pyrUp
+    weights sum to 1 in powers of two), a linear ramp gives 0 away from the borders (both
+    filters reproduce linear functions there), and pixels off the mask are 0."""
+    from oracle import eccnorm
+
+    c = np.full((48, 64), 0.375, np.float32)
+    for lvl in (1, 2, 3):
+        assert not eccnorm.ecc_normalize(oracle32, c, lvl=lvl).any()
+    yy, xx = np.mgrid[0:48, 0:64].astype(np.float32)
+    ramp = (0.01 * xx + 0.02 * yy).astype(np.float32)
+    d = eccnorm.ecc_normalize(oracle32, ramp, lvl=2)
+    assert np.abs(d[12:-12, 12:-12]).max() < 1e-5
+    mask = np.full((48, 64), 255, np.uint8)
+    mask[10:20, 5:9] = 0
+    mask[30, 40] = 128
+    r = np.random.default_rng(2).random((48, 64)).astype(np.float32)
+    dm = eccnorm.ecc_normalize(oracle32, r, mask, 1)
+    assert not dm[mask != 255].any()
+    np.testing.assert_array_equal(dm[mask == 255], eccnorm.ecc_normalize(oracle32, r, None, 1)[mask == 255])
