@@ -520,7 +520,59 @@ __global__ void __launch_bounds__(ECC_THREADS) k_corr_pass(CorrPassArgs a) {
     const uint8_t* __restrict__ cmask = HAS_CMASK ? a.cmask + f * a.cmask_fstride : nullptr;
     const int x = x0 + lane;
     float n = 0, F1 = 0, G1 = 0, F2 = 0, G2 = 0, FG = 0;
-    for (int dy = 0; dy < ECC_RPT; ++dy) {
+    // Interior tiles (the warped corners of the clipped tile keep every tap inside the current
+    // image, no current mask): the mask test always passes and the sample needs no border
+    // handling, so four rows' loads go out together; the sums keep the row order.
+    bool fast = false;
+    if (!HAS_CMASK) {
+      const int cx[4] = {x0, min(x0 + CORR_TW - 1, a.w - 1), x0, min(x0 + CORR_TW - 1, a.w - 1)};
+      const int cy[4] = {y0, y0, min(y0 + CORR_TH - 1, a.h - 1), min(y0 + CORR_TH - 1, a.h - 1)};
+      float mnx = 1e30f, mxx = -1e30f, mny = 1e30f, mxy = -1e30f;
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        float sx, sy;
+        warp_xy<MOTION>(mc, cx[i], cy[i], sx, sy);
+        mnx = fminf(mnx, sx); mxx = fmaxf(mxx, sx); mny = fminf(mny, sy); mxy = fmaxf(mxy, sy);
+        if (sx != sx || sy != sy) mnx = -1e30f;
+      }
+      fast = mnx >= 0.25f && mny >= 0.25f && mxx <= (float)a.cw - 1.25f && mxy <= (float)a.ch - 1.25f;
+    }
+    if (fast) {
+      const int xc = min(x, a.w - 1);
+#pragma nounroll
+      for (int dy0 = 0; dy0 < ECC_RPT; dy0 += 4) {
+        float t[4][4], fv[4];
+        int q[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          const int yc = min(y0 + wv * ECC_RPT + dy0 + j, a.h - 1);
+          float sx, sy;
+          warp_xy<MOTION>(mc, xc, yc, sx, sy);
+          const int X = quant32(sx), Y = quant32(sy);
+          const float* r0 = cur + (size_t)(Y >> 5) * a.cur_pitch + (X >> 5);
+          t[j][0] = r0[0];
+          t[j][1] = r0[1];
+          t[j][2] = r0[a.cur_pitch];
+          t[j][3] = r0[a.cur_pitch + 1];
+          fv[j] = a.ref[(size_t)yc * a.ref_pitch + xc];
+          q[j] = ((Y & 31) << 5) | (X & 31);
+        }
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          const int y = y0 + wv * ECC_RPT + dy0 + j;
+          if (x >= a.w || y >= a.h) continue;
+          if (HAS_RMASK && a.rmask[(size_t)y * a.rmask_pitch + x] == 0) continue;
+          float w0, w1, w2, w3;
+          bilinear_w(q[j] & 31, q[j] >> 5, w0, w1, w2, w3);
+          const float gv = bilinear_sum(t[j][0], t[j][1], t[j][2], t[j][3], w0, w1, w2, w3);
+          const float fp = fv[j] - a.cf, gp = gv - a.cf;
+          n += 1.f;
+          F1 += fp; G1 += gp;
+          F2 = fmaf(fp, fp, F2); G2 = fmaf(gp, gp, G2); FG = fmaf(fp, gp, FG);
+        }
+      }
+    }
+    for (int dy = 0; dy < (fast ? 0 : ECC_RPT); ++dy) {
       const int y = y0 + wv * ECC_RPT + dy;
       if (x >= a.w || y >= a.h) continue;
       float sx, sy;
