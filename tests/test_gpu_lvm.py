@@ -88,3 +88,17 @@ def test_stack_with_local_variance_weight_maps(ssg, oracle32, oracle64):
     assert rel_l2(avg, ref64) <= STACK_REL_L2
     assert rel_l2(avg, A32) <= STACK_REL_L2
     assert rel_l2(wsum, W64) <= 1e-6
+
+
+@pytest.mark.parametrize("shape", [(161, 113), (320, 240), (1920, 1080)])
+def test_frame_sharpness_matches_oracle(ssg, oracle32, shape):
+    """The C1 per-frame weight, mean squared 3x3 laplacian (c_image_stacking_pipeline.cc:1466-1467):
+    per-pixel L in float in the oracle's order, summed in double (in another order), returned as
+    float."""
+    S = ssg
+    w, h = shape
+    r = np.random.default_rng(w)
+    frames = [r.random((h, w)).astype(np.float32) for _ in range(3)]
+    got = S.frame_sharpness([gpu(f) for f in frames])
+    want = np.array([oracle32.sharpness(f) for f in frames], np.float64)
+    np.testing.assert_allclose(got, want, rtol=2e-7, atol=0)
