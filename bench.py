@@ -83,7 +83,8 @@ def main():
 
     if a.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {a.gpus} is not supported "
-              f"(the path shards over frames, see DESIGN.md)", file=sys.stderr)
+              f"(the path shards over frames: each GPU registers and stacks its own share, and the "
+              f"accumulator planes are summed once at the end)", file=sys.stderr)
         sys.exit(2)
 
     import torch
