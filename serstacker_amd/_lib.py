@@ -15,6 +15,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libssg.so")
 
 SSG_OK = 0
+SSG_E_INVALID_ARG, SSG_E_HIP, SSG_E_NOT_READY, SSG_E_UNSUPPORTED, SSG_E_NO_MEMORY = -1, -2, -3, -4, -5
 SSG_8UC1, SSG_16UC1, SSG_32SC1, SSG_32FC1, SSG_32FC2, SSG_32SC3, SSG_32FC3 = 0, 2, 4, 5, 13, 20, 21
 SSG_64FC1, SSG_8UC3, SSG_16UC3, SSG_64FC3 = 6, 16, 18, 22
 UPSCALE_NONE, UPSCALE_X2, UPSCALE_X15, UPSCALE_X3 = 0, 1, 2, 3

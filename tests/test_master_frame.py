@@ -77,3 +77,20 @@ def test_stack_window_matches_oracle(S):
     for size, sfi, pos, m in itertools.product([2, 50, 101, 1000], [-5, 0, 10, 60], [0, 25, 50, 99, 500, 999],
                                                 [2, 3, 10, 49, 50, 51, 100, 2000]):
         assert S.master_stack_window(size, sfi, pos, m) == master.stack_window(size, sfi, pos, m), (size, sfi, pos, m)
+
+
+def test_host_entry_points_reject_bad_arguments(S):
+    """Argument checks of the host-only entry points (no GPU call is made)."""
+    import ctypes as C
+
+    lib = S._lib.load()
+    a, b = C.c_int32(), C.c_int32()
+    assert lib.ssg_master_scan_window(10, 5, None, C.byref(b)) == S._lib.SSG_E_INVALID_ARG
+    assert lib.ssg_master_stack_window(10, 0, 5, 4, C.byref(a), None) == S._lib.SSG_E_INVALID_ARG
+    assert lib.ssg_select_master_frame(3, None, 0, C.byref(a)) == S._lib.SSG_E_INVALID_ARG
+    assert lib.ssg_upscale_size(7, 10, 10, C.byref(a), C.byref(b)) == S._lib.SSG_E_INVALID_ARG
+    assert lib.ssg_upscale_size(-1, 10, 10, C.byref(a), C.byref(b)) == S._lib.SSG_E_INVALID_ARG
+    assert S.upscale_size(S._lib.UPSCALE_X15, 1920, 1080) == (2880, 1620)
+    assert S.upscale_size(S._lib.UPSCALE_X2, 33, 21) == (66, 42)
+    assert S.upscale_size(S._lib.UPSCALE_X3, 33, 21) == (99, 63)
+    assert S.upscale_size(S._lib.UPSCALE_NONE, 33, 21) == (33, 21)
