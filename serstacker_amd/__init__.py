@@ -39,21 +39,33 @@ def _is_torch(x) -> bool:
     return type(x).__module__.startswith("torch")
 
 
-def image_view(x) -> _lib.Image:
-    """ssg_image view of a numpy array or a torch GPU tensor."""
+def image_view(x, out: bool = False) -> _lib.Image:
+    """ssg_image view of a numpy array or a torch GPU tensor.
+
+    Arrays whose rows are contiguous (an ROI crop, a row slice) are viewed in place
+    with their row step.  Other host arrays are copied for inputs; the copy is kept
+    alive by the view (`view._owner`).  An output (`out=True`) must be viewable in
+    place, else ValueError: a copy would never reach the caller's array."""
     if x is None:
         return None
     if _is_torch(x):
         return _lib.device_image(x)
-    return _lib.host_image(np.ascontiguousarray(x))
+    a = np.asarray(x)
+    if not _lib.rows_contiguous(a):
+        if out:
+            raise ValueError("output array rows must be contiguous (last axis, and channels, dense)")
+        a = np.ascontiguousarray(a)
+    v = _lib.host_image(a)
+    v._owner = a
+    return v
 
 
-def image_views(frames):
+def image_views(frames, out: bool = False):
     """ctypes array of ssg_image views of a list of frames; a ctypes array of
     views passes through unchanged (build it once for repeated batch calls)."""
     if isinstance(frames, C.Array):
         return frames
-    return _lib.images([image_view(f) for f in frames])
+    return _lib.images([image_view(f, out) for f in frames])
 
 
 def c_image_registration_options(**ecc) -> _lib.ImageRegistrationOptions:
@@ -448,7 +460,7 @@ class c_eccflow:
             res = np.zeros((v.height, v.width, 2), np.float32)
             o = _lib.host_image(res)
         else:
-            res, o = out, image_view(out)
+            res, o = out, image_view(out, out=True)
         _lib.check(self._lib.ssg_eccflow_compute(self._h, C.byref(v), C.byref(m) if m else None,
                                                  C.byref(r) if r else None, C.byref(o)), "compute")
         return res
@@ -456,7 +468,7 @@ class c_eccflow:
     def compute_batch(self, frames, outs, rmaps=None, masks=None):
         """Batched compute over device frames into device maps outs[i] (H, W, 2)."""
         fv = image_views(frames)
-        ov = image_views(outs)
+        ov = image_views(outs, out=True)
         rv = image_views(rmaps) if rmaps is not None else None
         mv = image_views(masks) if masks is not None else None
         _lib.check(self._lib.ssg_eccflow_compute_batch(self._h, len(fv), fv, mv, rv, ov), "compute_batch")
@@ -510,7 +522,7 @@ class c_laplacian_pyramid_focus_stacking:
         """The fused image; into `out` (host array or device tensor of the frame shape) when given."""
         avg = out if out is not None else np.zeros((self.h, self.w) if self.cn == 1 else (self.h, self.w, self.cn),
                                                    np.float32)
-        av = image_view(avg)
+        av = image_view(avg, out=True)
         _lib.check(self._lib.ssg_accumulation_compute(self._h, C.byref(av), None, 1.0), "compute")
         return avg
 
@@ -548,7 +560,7 @@ def ecc_normalize(src, mask=None, lvl: int = 1, out=None):
     """ecc_normalize(src, mask, dst, lvl) (ecc2.cc:385-397) of an f32 image."""
     if out is None:
         out = np.empty(tuple(src.shape), np.float32)
-    sv, ov = image_view(src), image_view(out)
+    sv, ov = image_view(src), image_view(out, out=True)
     mv = image_view(mask) if mask is not None else None
     _lib.check(_lib.load().ssg_ecc_normalize(C.byref(sv), C.byref(mv) if mv is not None else None, C.byref(ov), lvl),
                "ssg_ecc_normalize")
@@ -719,13 +731,13 @@ class c_ser_reader:
         if out is None:
             shape, dt = self._shape_dtype()
             out = np.empty(shape, dt)
-        v = image_view(out)
+        v = image_view(out, out=True)
         _lib.check(self._lib.ssg_ser_read(self._h, C.byref(v)), "read")
         return out
 
     def read_batch(self, outs):
         """len(outs) successive frames into device tensors."""
-        views = image_views(outs)
+        views = image_views(outs, out=True)
         _lib.check(self._lib.ssg_ser_read_batch(self._h, len(views), views), "read_batch")
         return outs
 
@@ -742,7 +754,7 @@ def debayer(src, colorid: int, algorithm: int = -1, out=None):
         shape = ((h // 2, w // 2) if half else (h, w)) + ((3,) if cn == 3 else ())
         dt = src.cpu().numpy().dtype if _is_torch(src) else np.asarray(src).dtype
         out = np.empty(shape, dt)
-    sv, ov = image_view(src), image_view(out)
+    sv, ov = image_view(src), image_view(out, out=True)
     _lib.check(_lib.load().ssg_debayer(C.byref(sv), colorid, algorithm, C.byref(ov)), "debayer")
     return out
 
@@ -751,7 +763,7 @@ def read_input_frames(raws, colorid: int, bpp: int, algorithm: int = -1, dark=No
     """The raw-frame conversion of read_input_frame (c_image_stacking_pipeline_base.cc:124-279)
     for a list of frames; outs: f32 host arrays or device tensors of the debayered shape."""
     rv = image_views(raws)
-    ov = image_views(outs)
+    ov = image_views(outs, out=True)
     dv, fv = image_view(dark), image_view(flat)
     _lib.check(_lib.load().ssg_read_input_frames(len(rv), rv, colorid, bpp, algorithm,
                                                  C.byref(dv) if dv else None, C.byref(fv) if fv else None, ov),
@@ -777,7 +789,7 @@ def average_pyramid_inpaint(src, mask, max_levels: int = 100, out=None, out_mask
         out = np.empty(tuple(src.shape), np.float32)
     if out_mask is None:
         out_mask = np.empty((h, w), np.uint8)
-    sv, ov, dmv = image_view(src), image_view(out), image_view(out_mask)
+    sv, ov, dmv = image_view(src), image_view(out, out=True), image_view(out_mask, out=True)
     mv = image_view(mask) if mask is not None else None
     _lib.check(_lib.load().ssg_average_pyramid_inpaint(C.byref(sv), C.byref(mv) if mv is not None else None,
                                                        C.byref(ov), C.byref(dmv), max_levels),
@@ -799,7 +811,7 @@ def upscale_remaps(option: int, maps, out=None):
     dw, dh = upscale_size(option, w, h)
     if out is None:
         out = [np.empty((dh, dw, 2), np.float32) for _ in maps]
-    sv, dv = image_views(maps), image_views(out)
+    sv, dv = image_views(maps), image_views(out, out=True)
     _lib.check(_lib.load().ssg_upscale_remaps(option, len(maps), sv, dv), "ssg_upscale_remaps")
     return out
 
@@ -808,7 +820,7 @@ def unsharp_mask(src, sigma: float, alpha: float, out=None):
     """unsharp_mask(src, dst, sigma, alpha) (core/proc/unsharp_mask.cc:72-115) of an f32 image."""
     if out is None:
         out = np.empty(tuple(src.shape), np.float32) if not _is_torch(src) else src.clone()
-    sv, ov = image_view(src), image_view(out)
+    sv, ov = image_view(src), image_view(out, out=True)
     _lib.check(_lib.load().ssg_unsharp_mask(C.byref(sv), C.byref(ov), sigma, alpha), "unsharp_mask")
     return out
 

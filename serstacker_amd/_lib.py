@@ -302,10 +302,22 @@ _NP_TYPE = {(np.dtype(np.uint8), 1): SSG_8UC1, (np.dtype(np.uint16), 1): SSG_16U
 TYPE_NP = {v: k for k, v in _NP_TYPE.items()}
 
 
+def rows_contiguous(a: np.ndarray) -> bool:
+    """True when each row of a (H, W) or (H, W, C) array is one dense run of memory
+    (the pixels and their channels), rows at a positive step: an ssg_image view."""
+    if a.ndim not in (2, 3) or a.strides[0] <= 0:
+        return False
+    cn = 1 if a.ndim == 2 else a.shape[2]
+    if a.ndim == 3 and a.strides[2] != a.itemsize and cn > 1:
+        return False
+    return a.shape[1] <= 1 or a.strides[1] == cn * a.itemsize
+
+
 def host_image(a: np.ndarray) -> Image:
-    """View of a C-contiguous host array (H, W) or (H, W, C)."""
-    if not a.flags.c_contiguous:
-        raise ValueError("array must be C-contiguous")
+    """View of a host array (H, W) or (H, W, C) whose rows are contiguous; the step
+    is the array's row stride.  The caller keeps `a` alive while the view is used."""
+    if not rows_contiguous(a):
+        raise ValueError("array rows must be contiguous")
     cn = 1 if a.ndim == 2 else a.shape[2]
     t = _NP_TYPE[(a.dtype, cn)]
     return Image(a.ctypes.data, a.shape[1], a.shape[0], a.strides[0], t, 0)
@@ -332,7 +344,9 @@ def device_image(t) -> Image:
 
 
 def images(views) -> C.Array:
+    """ctypes array of views; it keeps the views (and the arrays they own) alive."""
     arr = (Image * len(views))()
     for i, v in enumerate(views):
         arr[i] = v
+    arr._keep = list(views)
     return arr

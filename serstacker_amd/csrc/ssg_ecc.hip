@@ -205,25 +205,61 @@ __device__ void ecc_solve(double* sm, FrameXf& X, FrameRes& R, const EccPassArgs
     }
   }
 
-  // ---- cv::invert failure rule: n <= 3 -> det == 0; n > 3 -> Cholesky pivot < FLT_EPSILON
+  // ---- cv::invert(DECOMP_CHOLESKY) failure rule: n <= 3 inverts through the determinant and
+  // fails only when it is 0 (a NaN Hessian goes on, as it does in the reference);
+  // n > 3 runs the Cholesky factorisation and fails on a pivot below FLT_EPSILON
   bool failed = false;
-  if (NP == 2) {
-    failed = (H[0] * H[3] - H[1] * H[2]) == 0.0;
-  } else if (NP == 3) {
-    const double d = H[0] * (H[4] * H[8] - H[5] * H[7]) - H[1] * (H[3] * H[8] - H[5] * H[6]) +
-                     H[2] * (H[3] * H[7] - H[4] * H[6]);
-    failed = d == 0.0;
-  }
-  for (int i = 0; i < NP && !failed; ++i) {
-    for (int j = 0; j < i; ++j) {
-      double s = H[i * NP + j];
-      for (int k = 0; k < j; ++k) s -= L[i * NP + k] * L[j * NP + k];
-      L[i * NP + j] = s / L[j * NP + j];
+  if constexpr (NP <= 3) {
+    double inv[9];
+    if (NP == 2) {
+      const double d = H[0] * H[3] - H[1] * H[2];
+      failed = d == 0.0;
+      const double r = 1.0 / d;
+      inv[0] = H[3] * r; inv[1] = -H[1] * r; inv[2] = -H[2] * r; inv[3] = H[0] * r;
+    } else {
+      const double d = H[0] * (H[4] * H[8] - H[5] * H[7]) - H[1] * (H[3] * H[8] - H[5] * H[6]) +
+                       H[2] * (H[3] * H[7] - H[4] * H[6]);
+      failed = d == 0.0;
+      const double r = 1.0 / d;
+      inv[0] = (H[4] * H[8] - H[5] * H[7]) * r;
+      inv[1] = (H[2] * H[7] - H[1] * H[8]) * r;
+      inv[2] = (H[1] * H[5] - H[2] * H[4]) * r;
+      inv[3] = (H[5] * H[6] - H[3] * H[8]) * r;
+      inv[4] = (H[0] * H[8] - H[2] * H[6]) * r;
+      inv[5] = (H[2] * H[3] - H[0] * H[5]) * r;
+      inv[6] = (H[3] * H[7] - H[4] * H[6]) * r;
+      inv[7] = (H[1] * H[6] - H[0] * H[7]) * r;
+      inv[8] = (H[0] * H[4] - H[1] * H[3]) * r;
     }
-    double s = H[i * NP + i];
-    for (int k = 0; k < i; ++k) s -= L[i * NP + k] * L[i * NP + k];
-    if (NP > 3 ? (s < (double)FLT_EPSILON) : !(s > 0.0)) failed = true;
-    else L[i * NP + i] = sqrt(s);
+    for (int i = 0; i < NP; ++i) {
+      double t = 0;
+      for (int j = 0; j < NP; ++j) t += inv[i * NP + j] * e[j];
+      z[i] = t;
+    }
+  } else {
+    for (int i = 0; i < NP && !failed; ++i) {
+      for (int j = 0; j < i; ++j) {
+        double s = H[i * NP + j];
+        for (int k = 0; k < j; ++k) s -= L[i * NP + k] * L[j * NP + k];
+        L[i * NP + j] = s / L[j * NP + j];
+      }
+      double s = H[i * NP + i];
+      for (int k = 0; k < i; ++k) s -= L[i * NP + k] * L[i * NP + k];
+      if (s < (double)FLT_EPSILON) failed = true;
+      else L[i * NP + i] = sqrt(s);
+    }
+    if (!failed) {
+      for (int i = 0; i < NP; ++i) {
+        double s = e[i];
+        for (int j = 0; j < i; ++j) s -= L[i * NP + j] * z[j];
+        z[i] = s / L[i * NP + i];
+      }
+      for (int i = NP - 1; i >= 0; --i) {
+        double s = z[i];
+        for (int j = i + 1; j < NP; ++j) s -= L[j * NP + i] * z[j];
+        z[i] = s / L[i * NP + i];
+      }
+    }
   }
   X.iters += 1;
   const int k = X.iters;
@@ -232,16 +268,6 @@ __device__ void ecc_solve(double* sm, FrameXf& X, FrameRes& R, const EccPassArgs
     R.level_failed[a.pass][a.level] = 1;
     X.flags &= ~XF_ACTIVE;
     return;
-  }
-  for (int i = 0; i < NP; ++i) {
-    double s = e[i];
-    for (int j = 0; j < i; ++j) s -= L[i * NP + j] * z[j];
-    z[i] = s / L[i * NP + i];
-  }
-  for (int i = NP - 1; i >= 0; --i) {
-    double s = z[i];
-    for (int j = i + 1; j < NP; ++j) s -= L[j * NP + i] * z[j];
-    z[i] = s / L[i * NP + i];
   }
   float dp[6] = {0, 0, 0, 0, 0, 0};
   for (int i = 0; i < NP; ++i) {

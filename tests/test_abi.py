@@ -128,3 +128,43 @@ def test_cpp_host_header_compiles():
         r = subprocess.run([gxx, "-std=c++17", "-Wall", "-Werror", "-fsyntax-only", f"-I{os.path.join(ROOT, 'include')}",
                             f], capture_output=True, text=True)
         assert r.returncode == 0, r.stderr
+
+
+def test_host_views_keep_their_memory():
+    """image_view of a row-strided array views it in place; of an array whose rows are
+    not contiguous it views a copy that the view itself keeps alive; an output array
+    that cannot be viewed in place is refused."""
+    import gc
+
+    import numpy as np
+
+    import serstacker_amd as S
+
+    a = np.arange(40 * 30, dtype=np.float32).reshape(30, 40)
+    crop = a[5:20, 3:33]
+    v = S.image_view(crop)
+    assert v.data == crop.ctypes.data and v.step == a.strides[0]
+    assert (v.width, v.height) == (30, 15)
+
+    t = a.T  # columns are rows now: not viewable in place
+    v = S.image_view(t)
+    assert v.data != t.ctypes.data and v.step == t.shape[1] * 4
+    gc.collect()
+    junk = [np.full(t.shape, -1.0, np.float32) for _ in range(8)]  # reuse freed blocks, if any
+    got = np.ctypeslib.as_array(C.cast(v.data, C.POINTER(C.c_float)), shape=(t.shape[0], t.shape[1]))
+    assert np.array_equal(got, t), "the copied input was freed while its view is alive"
+    del junk
+
+    views = S.image_views([a[:, ::2], a.T])
+    for i, ref in enumerate((a[:, ::2], a.T)):
+        got = np.ctypeslib.as_array(C.cast(views[i].data, C.POINTER(C.c_float)),
+                                    shape=(views[i].height, views[i].width))
+        assert np.array_equal(got, ref)
+
+    rgb = np.zeros((10, 12, 3), np.float32)
+    v = S.image_view(rgb[2:8, 1:9])
+    assert v.data == rgb[2:8, 1:9].ctypes.data and v.step == 12 * 3 * 4
+    with pytest.raises(ValueError):
+        S.image_view(a.T, out=True)
+    with pytest.raises(ValueError):
+        S.image_view(rgb[:, :, :2], out=True)
