@@ -346,6 +346,25 @@ int ssg_accumulation_get_counts(ssg_accumulation *a, ssg_image *counts);
 /* c_frame_accumulation::accumulated_frames (c_frame_accumulation.h:31). */
 int ssg_accumulation_accumulated_frames(ssg_accumulation *a, int32_t *n);
 
+/* c_frame_accumulation::reinitialize(src, accw) (c_frame_accumulation.h:25; c_weigthed_average
+ * :171-180): the accumulator restarts from a mean image `src` (f32, the accumulator's channels)
+ * with weights `accw` (SSG_32FC1), as one accumulated frame.  The sum-form planes hold src*accw
+ * exactly, so compute() returns src where accw > 0 (where accw == 0 it returns 0, the empty
+ * pixel of the sum form); per-pixel counts restart at 1 where accw > 0.  Bayer and focus
+ * accumulators return SSG_E_UNSUPPORTED, as the reference's return false (:486-490,1170-1173). */
+int ssg_accumulation_reinitialize(ssg_accumulation *a, const ssg_image *src, const ssg_image *accw);
+
+/* Accumulator checkpoint / resume (no reference counterpart: the reference keeps its state in
+ * cv::Mat members).  state_size gives the bytes of a state blob; state_save writes it into a host
+ * buffer; state_load restores it into an accumulator created with the same method, size,
+ * channels (and Bayer pattern / focus options), after which add() continues the stack and
+ * compute() returns what the saved accumulator would have: the sums are additive, so a stack
+ * split into sessions (or shards) resumes bit for bit.  The blob is a versioned header followed
+ * by the raw accumulator planes (host byte order). */
+int ssg_accumulation_state_size(ssg_accumulation *a, uint64_t *bytes);
+int ssg_accumulation_state_save(ssg_accumulation *a, void *buf, uint64_t bytes);
+int ssg_accumulation_state_load(ssg_accumulation *a, const void *buf, uint64_t bytes);
+
 /* c_frame_accumulation::clear (c_frame_accumulation.h:30). */
 int ssg_accumulation_clear(ssg_accumulation *a);
 

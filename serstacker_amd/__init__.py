@@ -210,7 +210,32 @@ class c_frame_registration:
         return dst, dm
 
 
-class c_weigthed_average:
+class _accumulator_state:
+    """Checkpoint / resume of an accumulator handle (ssg_accumulation_state_*)."""
+
+    def save_state(self) -> bytes:
+        n = C.c_uint64()
+        _lib.check(self._lib.ssg_accumulation_state_size(self._h, C.byref(n)), "state_size")
+        buf = C.create_string_buffer(n.value)
+        _lib.check(self._lib.ssg_accumulation_state_save(self._h, buf, n.value), "state_save")
+        return buf.raw
+
+    def load_state(self, state: bytes):
+        _lib.check(self._lib.ssg_accumulation_state_load(self._h, state, len(state)), "state_load")
+
+    def reinitialize(self, src, accw) -> bool:
+        """c_frame_accumulation::reinitialize(src, accw) (c_frame_accumulation.h:25): a weighted
+        average restarts from the mean `src` with weights `accw` (c_frame_accumulation.cc:171-180);
+        Bayer and focus accumulators return False, as the reference's do."""
+        sv, wv = image_view(src), image_view(accw)
+        rc = self._lib.ssg_accumulation_reinitialize(self._h, C.byref(sv), C.byref(wv))
+        if rc:
+            log.error("reinitialize: %s", self._lib.ssg_last_error().decode())
+            return False
+        return True
+
+
+class c_weigthed_average(_accumulator_state):
     """c_weigthed_average (core/average/c_frame_accumulation.h:40-70), sum form."""
 
     def __init__(self, width: int, height: int, channels: int = 1, method: int = _lib.ACC_WEIGHTED_AVERAGE):
@@ -285,7 +310,7 @@ class c_weigthed_average:
         _lib.check(self._lib.ssg_accumulation_clear(self._h), "clear")
 
 
-class c_bayer_average:
+class c_bayer_average(_accumulator_state):
     """c_bayer_average (core/average/c_frame_accumulation.h:222-262): raw Bayer frames
     accumulated per channel (BGR) through an optional remap."""
 
@@ -482,7 +507,7 @@ class c_eccflow:
         return p
 
 
-class c_laplacian_pyramid_focus_stacking:
+class c_laplacian_pyramid_focus_stacking(_accumulator_state):
     """c_laplacian_pyramid_focus_stacking (core/average/c_frame_accumulation.h:141-184);
     keyword options are the fields of ssg_focus_stacking_options."""
 

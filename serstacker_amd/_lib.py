@@ -247,6 +247,10 @@ _SIGS = [
     ("ssg_accumulation_set_remap", C.c_int, [C.c_void_p, _P(Image)]),
     ("ssg_accumulation_set_input_bpp", C.c_int, [C.c_void_p, C.c_int32]),
     ("ssg_frame_sharpness", C.c_int, [C.c_int, _P(Image), _P(C.c_float)]),
+    ("ssg_accumulation_reinitialize", C.c_int, [C.c_void_p, _P(Image), _P(Image)]),
+    ("ssg_accumulation_state_size", C.c_int, [C.c_void_p, _P(C.c_uint64)]),
+    ("ssg_accumulation_state_save", C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64]),
+    ("ssg_accumulation_state_load", C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64]),
     ("ssg_registration_set_profiling", C.c_int, [C.c_void_p, C.c_int]),
     ("ssg_registration_get_profile", C.c_int, [C.c_void_p, _P(Profile), C.c_int]),
     ("ssg_accumulation_set_profiling", C.c_int, [C.c_void_p, C.c_int]),

@@ -680,6 +680,35 @@ __global__ void k_acc_compute(const float* __restrict__ acc, int w, int h, int c
   if (mask) mask[(size_t)y * mask_pitch + x] = W > 0 ? 255 : 0;
 }
 
+// c_weigthed_average::reinitialize(src, accw) (c_frame_accumulation.cc:171-180) in the sum form:
+// the weighted sum becomes src * accw as an exact compensated pair (hi = the rounded product,
+// lo = its rounding residue), so compute() divides back to src exactly; the weight planes take
+// accw; the per-pixel counts restart at 1 where accw > 0.
+__global__ void k_acc_reinit(const float* __restrict__ src, int src_pitch, const float* __restrict__ accw,
+                             int accw_pitch, int w, int h, int cn, long long plane, float* __restrict__ acc,
+                             unsigned* __restrict__ cnt) {
+  const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
+  if (x >= w) return;
+  const size_t o = (size_t)y * w + x;
+  const float W = accw[(size_t)y * accw_pitch + x];
+  for (int c = 0; c < cn; ++c) {
+    const float v = src[(size_t)y * src_pitch + x * cn + c];
+    const float hi = v * W;
+    acc[(size_t)c * plane + o] = hi;
+    acc[(size_t)(cn + c) * plane + o] = fmaf(v, W, -hi);
+  }
+  acc[(size_t)(2 * cn) * plane + o] = W;
+  acc[(size_t)(2 * cn + 1) * plane + o] = 0.f;
+  cnt[o] = W > 0.f ? 1u : 0u;
+}
+
+hipError_t launch_acc_reinit(const float* src, int src_pitch, const float* accw, int accw_pitch, int w, int h, int cn,
+                             long long plane, float* acc, unsigned* cnt, hipStream_t s) {
+  dim3 grid((w + 255) / 256, h);
+  hipLaunchKernelGGL(k_acc_reinit, grid, dim3(256), 0, s, src, src_pitch, accw, accw_pitch, w, h, cn, plane, acc, cnt);
+  return hipGetLastError();
+}
+
 hipError_t launch_acc_compute(const float* acc, const float* accw, int w, int h, int cn, long long plane,
                               float* avg, int avg_pitch, uint8_t* mask, int mask_pitch, double dscale,
                               hipStream_t s) {

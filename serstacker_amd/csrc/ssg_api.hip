@@ -2345,6 +2345,138 @@ extern "C" int ssg_accumulation_accumulated_frames(ssg_accumulation* a, int32_t*
   return SSG_OK;
 }
 
+extern "C" int ssg_accumulation_reinitialize(ssg_accumulation* a, const ssg_image* src, const ssg_image* accw) {
+  if (!a) return fail(SSG_E_INVALID_ARG, "null handle");
+  // c_bayer_average and c_laplacian_pyramid_focus_stacking return false (c_frame_accumulation.cc:486-490,1170-1173)
+  if (a->bayer() || a->focus()) return fail(SSG_E_UNSUPPORTED, "reinitialize: weighted average accumulators only");
+  CHK(check_image(src, "src"));
+  CHK(check_image(accw, "accw"));
+  if (src->width != a->w || src->height != a->h || src->type != (a->cn == 3 ? SSG_32FC3 : SSG_32FC1))
+    return fail(SSG_E_INVALID_ARG, "reinitialize: src must be f32 with the accumulator's size and channels");
+  if (accw->width != a->w || accw->height != a->h || accw->type != SSG_32FC1)
+    return fail(SSG_E_INVALID_ARG, "reinitialize: accw must be SSG_32FC1 of the accumulator size");
+  hipStream_t s = a->stream;
+  DevBuf ts, tw;
+  const void *sp, *wp;
+  size_t spb, wpb;
+  CHK(device_view(src, ts, s, &sp, &spb));
+  CHK(device_view(accw, tw, s, &wp, &wpb));
+  if (spb % 4 || wpb % 4) return fail(SSG_E_INVALID_ARG, "reinitialize: row steps must be multiples of 4 bytes");
+  HIPCHK(launch_acc_reinit((const float*)sp, (int)(spb / 4), (const float*)wp, (int)(wpb / 4), a->w, a->h, a->cn,
+                           a->plane, a->acc.as<float>(), a->cnt.as<unsigned>(), s));
+  HIPCHK(hipStreamSynchronize(s));
+  a->accumulated = 1;
+  return SSG_OK;
+}
+
+namespace {
+// Accumulator state blob: a fixed header, then the device planes in a fixed order.
+struct AccStateHeader {
+  char magic[8];
+  int32_t version, method, width, height, channels, accumulated;
+  int32_t colorid, levels, policy, wchannels, reserved;
+  uint64_t payload;
+};
+constexpr char kAccMagic[8] = {'S', 'S', 'G', 'A', 'C', 'C', 'S', 'T'};
+constexpr int kAccStateVersion = 1;
+
+struct Seg {
+  void* p;
+  size_t bytes;
+};
+
+// the device buffers that hold the state: weighted average (2 cn + 2) f32 planes + u32 counts;
+// Bayer 6 f32 planes + 3 u32 count planes; focus stacking the fused pyramid and, for the
+// weighted policies, its weight pyramid (the layout fixed by the options at the first add)
+std::vector<Seg> acc_segments(ssg_accumulation* a) {
+  std::vector<Seg> v;
+  if (a->focus()) {
+    if (a->L == 0) return v;
+    v.push_back({a->facc.p, sizeof(float) * a->cn * a->P});
+    if (a->fo.fusing_policy == SSG_FOCUS_LPG_WEIGHTS) v.push_back({a->fw.p, sizeof(float) * a->P});
+    else if (a->fo.fusing_policy == SSG_FOCUS_WEIGHTED_AVERAGE) v.push_back({a->fw.p, sizeof(float) * a->wc * a->P});
+    return v;
+  }
+  v.push_back({a->acc.p, a->acc_bytes()});
+  v.push_back({a->cnt.p, a->cnt_bytes()});
+  return v;
+}
+
+AccStateHeader acc_header(ssg_accumulation* a) {
+  AccStateHeader h{};
+  memcpy(h.magic, kAccMagic, 8);
+  h.version = kAccStateVersion;
+  h.method = a->method;
+  h.width = a->w;
+  h.height = a->h;
+  h.channels = a->cn;
+  h.accumulated = a->accumulated;
+  h.colorid = a->bayer() ? a->colorid : 0;
+  h.levels = a->focus() ? a->L : 0;
+  h.policy = a->focus() ? a->fo.fusing_policy : 0;
+  h.wchannels = a->focus() ? a->wc : 0;
+  for (const Seg& g : acc_segments(a)) h.payload += g.bytes;
+  return h;
+}
+}  // namespace
+
+extern "C" int ssg_accumulation_state_size(ssg_accumulation* a, uint64_t* bytes) {
+  if (!a || !bytes) return fail(SSG_E_INVALID_ARG, "null argument");
+  *bytes = sizeof(AccStateHeader) + acc_header(a).payload;
+  return SSG_OK;
+}
+
+extern "C" int ssg_accumulation_state_save(ssg_accumulation* a, void* buf, uint64_t bytes) {
+  if (!a || !buf) return fail(SSG_E_INVALID_ARG, "null argument");
+  const AccStateHeader h = acc_header(a);
+  if (bytes < sizeof(h) + h.payload)
+    return fail(SSG_E_INVALID_ARG, "state_save: %llu bytes given, %llu needed", (unsigned long long)bytes,
+                (unsigned long long)(sizeof(h) + h.payload));
+  memcpy(buf, &h, sizeof(h));
+  char* dst = (char*)buf + sizeof(h);
+  for (const Seg& g : acc_segments(a)) {
+    HIPCHK(hipMemcpyAsync(dst, g.p, g.bytes, hipMemcpyDeviceToHost, a->stream));
+    dst += g.bytes;
+  }
+  HIPCHK(hipStreamSynchronize(a->stream));
+  return SSG_OK;
+}
+
+extern "C" int ssg_accumulation_state_load(ssg_accumulation* a, const void* buf, uint64_t bytes) {
+  if (!a || !buf) return fail(SSG_E_INVALID_ARG, "null argument");
+  AccStateHeader h;
+  if (bytes < sizeof(h)) return fail(SSG_E_INVALID_ARG, "state_load: %llu bytes is no accumulator state",
+                                     (unsigned long long)bytes);
+  memcpy(&h, buf, sizeof(h));
+  if (memcmp(h.magic, kAccMagic, 8) || h.version != kAccStateVersion)
+    return fail(SSG_E_INVALID_ARG, "state_load: not an accumulator state of version %d", kAccStateVersion);
+  if (h.method != a->method || h.width != a->w || h.height != a->h || h.channels != a->cn)
+    return fail(SSG_E_INVALID_ARG, "state_load: state of method %d %dx%dx%d, accumulator of method %d %dx%dx%d",
+                h.method, h.width, h.height, h.channels, a->method, a->w, a->h, a->cn);
+  if (a->bayer() && h.colorid != a->colorid)
+    return fail(SSG_E_INVALID_ARG, "state_load: bayer pattern %d, accumulator %d", h.colorid, a->colorid);
+  if (a->focus()) {
+    if (h.policy != a->fo.fusing_policy) return fail(SSG_E_INVALID_ARG, "state_load: another fusing policy");
+    if (h.accumulated > 0 && a->L == 0) CHK(focus_prepare(a));
+    if (h.accumulated > 0 && (h.levels != a->L || h.wchannels != a->wc))
+      return fail(SSG_E_INVALID_ARG, "state_load: %d pyramid levels, the options give %d", h.levels, a->L);
+  }
+  if (bytes < sizeof(h) + h.payload) return fail(SSG_E_INVALID_ARG, "state_load: truncated state");
+  const std::vector<Seg> segs = h.accumulated > 0 || !a->focus() ? acc_segments(a) : std::vector<Seg>();
+  uint64_t tot = 0;
+  for (const Seg& g : segs) tot += g.bytes;
+  if (tot != h.payload) return fail(SSG_E_INVALID_ARG, "state_load: payload of %llu bytes, %llu expected",
+                                    (unsigned long long)h.payload, (unsigned long long)tot);
+  const char* src = (const char*)buf + sizeof(h);
+  for (const Seg& g : segs) {
+    HIPCHK(hipMemcpyAsync(g.p, src, g.bytes, hipMemcpyHostToDevice, a->stream));
+    src += g.bytes;
+  }
+  HIPCHK(hipStreamSynchronize(a->stream));
+  a->accumulated = h.accumulated;
+  return SSG_OK;
+}
+
 extern "C" int ssg_registration_set_profiling(ssg_registration* r, int enable) {
   if (!r) return fail(SSG_E_INVALID_ARG, "null handle");
   r->prof.on = enable != 0;

@@ -528,6 +528,7 @@ hipError_t launch_level_begin(const LevelBeginArgs& a, hipStream_t s) {
 template <int MOTION, bool HAS_CMASK, bool HAS_RMASK>
 __global__ void __launch_bounds__(ECC_THREADS) k_corr_pass(CorrPassArgs a) {
   __shared__ double dred[ECC_WAVES][8];
+  __shared__ float fred[ECC_WAVES][4];
   const int tid = threadIdx.x, lane = tid & (WAVE - 1), wv = tid / WAVE;
   const int items = a.nlist * a.ntiles;
   const int G = gridDim.x, b = blockIdx.x;
@@ -546,6 +547,8 @@ __global__ void __launch_bounds__(ECC_THREADS) k_corr_pass(CorrPassArgs a) {
     const uint8_t* __restrict__ cmask = HAS_CMASK ? a.cmask + f * a.cmask_fstride : nullptr;
     const int x = x0 + lane;
     float n = 0, F1 = 0, G1 = 0, F2 = 0, G2 = 0, FG = 0;
+    // extremes of f' and g' over the mask: a constant image is recognised exactly
+    float fmn = INFINITY, fmx = -INFINITY, gmn = INFINITY, gmx = -INFINITY;
     // Interior tiles (the warped corners of the clipped tile keep every tap inside the current
     // image, no current mask): the mask test always passes and the sample needs no border
     // handling, so four rows' loads go out together; the sums keep the row order.
@@ -595,6 +598,7 @@ __global__ void __launch_bounds__(ECC_THREADS) k_corr_pass(CorrPassArgs a) {
           n += 1.f;
           F1 += fp; G1 += gp;
           F2 = fmaf(fp, fp, F2); G2 = fmaf(gp, gp, G2); FG = fmaf(fp, gp, FG);
+          fmn = fminf(fmn, fp); fmx = fmaxf(fmx, fp); gmn = fminf(gmn, gp); gmx = fmaxf(gmx, gp);
         }
       }
     }
@@ -613,15 +617,31 @@ __global__ void __launch_bounds__(ECC_THREADS) k_corr_pass(CorrPassArgs a) {
       n += 1.f;
       F1 += fp; G1 += gp;
       F2 = fmaf(fp, fp, F2); G2 = fmaf(gp, gp, G2); FG = fmaf(fp, gp, FG);
+      fmn = fminf(fmn, fp); fmx = fmaxf(fmx, fp); gmn = fminf(gmn, gp); gmx = fmaxf(gmx, gp);
     }
     double v[6] = {n, F1, G1, F2, G2, FG};
 #pragma unroll
     for (int i = 0; i < 6; ++i) v[i] = wave_sum(v[i]);
-    if (lane == 0)
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+      fmn = fminf(fmn, __shfl_xor(fmn, o));
+      fmx = fmaxf(fmx, __shfl_xor(fmx, o));
+      gmn = fminf(gmn, __shfl_xor(gmn, o));
+      gmx = fmaxf(gmx, __shfl_xor(gmx, o));
+    }
+    if (lane == 0) {
       for (int i = 0; i < 6; ++i) dred[wv][i] = v[i];
+      fred[wv][0] = fmn; fred[wv][1] = fmx; fred[wv][2] = gmn; fred[wv][3] = gmx;
+    }
     __syncthreads();
     double* part = reinterpret_cast<double*>(a.partials) + ((size_t)f * a.ntiles + tile) * NMOM_CORR;
-    if (tid < NMOM_CORR) part[tid] = ((dred[0][tid] + dred[1][tid]) + dred[2][tid]) + dred[3][tid];
+    if (tid < 6) part[tid] = ((dred[0][tid] + dred[1][tid]) + dred[2][tid]) + dred[3][tid];
+    if (tid >= 6 && tid < NMOM_CORR) {
+      const int q = tid - 6;
+      float e = fred[0][q];
+      for (int w = 1; w < ECC_WAVES; ++w) e = (q & 1) ? fmaxf(e, fred[w][q]) : fminf(e, fred[w][q]);
+      part[tid] = (double)e;
+    }
     __syncthreads();  // dred is rewritten by the next item
   }
 }
@@ -629,25 +649,49 @@ __global__ void __launch_bounds__(ECC_THREADS) k_corr_pass(CorrPassArgs a) {
 // One block per listed frame: tile sums in tile order, rho, and the gate.
 __global__ void __launch_bounds__(256) k_corr_finalize(CorrPassArgs a) {
   __shared__ double red[ECC_WAVES][8];
+  __shared__ double ered[ECC_WAVES][4];
   const int f = a.frames[blockIdx.x];
   if (a.res[f].rejected) return;
   const int tid = threadIdx.x, lane = tid & (WAVE - 1), wv = tid / WAVE;
   const double* base = reinterpret_cast<const double*>(a.partials) + (size_t)f * a.ntiles * NMOM_CORR;
   double S[6] = {0, 0, 0, 0, 0, 0};
-  for (int t = tid; t < a.ntiles; t += blockDim.x)
+  double E[4] = {INFINITY, -INFINITY, INFINITY, -INFINITY};  // min f', max f', min g', max g'
+  for (int t = tid; t < a.ntiles; t += blockDim.x) {
     for (int i = 0; i < 6; ++i) S[i] += base[(size_t)t * NMOM_CORR + i];
+    for (int i = 0; i < 4; ++i) {
+      const double e = base[(size_t)t * NMOM_CORR + 6 + i];
+      E[i] = (i & 1) ? fmax(E[i], e) : fmin(E[i], e);
+    }
+  }
   for (int i = 0; i < 6; ++i) S[i] = wave_sum(S[i]);
-  if (lane == 0)
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1)
+    for (int i = 0; i < 4; ++i) {
+      const double e = __shfl_xor(E[i], o);
+      E[i] = (i & 1) ? fmax(E[i], e) : fmin(E[i], e);
+    }
+  if (lane == 0) {
     for (int i = 0; i < 6; ++i) red[wv][i] = S[i];
+    for (int i = 0; i < 4; ++i) ered[wv][i] = E[i];
+  }
   __syncthreads();
   if (tid == 0) {
     for (int i = 0; i < 6; ++i) S[i] = ((red[0][i] + red[1][i]) + red[2][i]) + red[3][i];
+    for (int w = 1; w < ECC_WAVES; ++w)
+      for (int i = 0; i < 4; ++i) ered[0][i] = (i & 1) ? fmax(ered[0][i], ered[w][i]) : fmin(ered[0][i], ered[w][i]);
+    // the image or the reference constant over the mask: the reference's centred covariance
+    // is exactly 0 (its float mean is that constant) and its rho 0 / (s1 s2)
+    const bool flat = ered[0][0] == ered[0][1] || ered[0][2] == ered[0][3];
     const double inv = S[0] > 0 ? 1.0 / S[0] : 0.0;
     const double mf = S[1] * inv, mg = S[2] * inv;
     const double vf = S[3] * inv - mf * mf, vg = S[4] * inv - mg * mg;
     const double sf = sqrt(vf > 0 ? vf : 0.0), sg = sqrt(vg > 0 ? vg : 0.0);
     const double cov = S[5] * inv - mf * mg;
-    const double rho = cov / (sf * sg);
+    // a constant image over the mask (or an empty mask): the raw moments here can leave a
+    // rounding residue in cov and the variance, so the reference's value is set explicitly --
+    // 0 / 0 = NaN, its result whenever its double sums of the constant are exact (0 and
+    // short-mantissa values; other constants make its own variance a rounding residue)
+    const double rho = (flat || S[0] == 0) ? __builtin_nan("") : cov / (sf * sg);
     FrameRes& R = a.res[f];
     R.rho[a.pass] = rho;
     // `rho < thr` as in the reference: a NaN rho passes the gate

@@ -66,7 +66,7 @@ struct FrameRes {
 // moment counts of the fused ECC pass
 constexpr int NMOM_FULL = 41;   // affine moments (see ssg_ecc.hip)
 constexpr int NMOM_TRANS = 14;  // translation moments
-constexpr int NMOM_CORR = 6;    // correlation moments
+constexpr int NMOM_CORR = 10;   // correlation moments (6 sums, the extremes of f' and g')
 constexpr int NMOM_LM_FULL = 25;  // c_ecclm pass: 18 Hessian + 6 J^T rhs moments + |rhs|^2
 constexpr int NMOM_LM_TRANS = 6;  // gx^2, gx gy, gy^2, gx rhs, gy rhs, |rhs|^2
 constexpr int NMOM_IC = 8;        // inverse-compositional pass: 6 J^T rhs moments, |rhs|^2, good pixels
@@ -317,6 +317,8 @@ hipError_t launch_accum_merge(const float* part, const unsigned* part_cnt, int n
 hipError_t launch_acc_add(const float* src, int src_pitch, int w, int h, int cn, const uint8_t* mask, int mask_pitch,
                           const float* wmap, int wmap_pitch, float* acc, float* accw, unsigned* cnt,
                           long long plane, hipStream_t s);
+hipError_t launch_acc_reinit(const float* src, int src_pitch, const float* accw, int accw_pitch, int w, int h, int cn,
+                             long long plane, float* acc, unsigned* cnt, hipStream_t s);
 hipError_t launch_acc_compute(const float* acc, const float* accw, int w, int h, int cn, long long plane,
                               float* avg, int avg_pitch, uint8_t* mask, int mask_pitch, double dscale,
                               hipStream_t s);
