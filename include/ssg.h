@@ -374,6 +374,20 @@ int ssg_accumulation_clear(ssg_accumulation *a);
  * SSG_32FC1 device images of one size; weights: n floats (host). */
 int ssg_frame_sharpness(int n, const ssg_image *frames, float *weights);
 
+/* Allocation-free form of ssg_frame_sharpness, same arguments and the same results bit for bit.
+ * No counterpart in the reference: it is for callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).  It calls
+ * no hipMalloc/hipFree and creates no stream: the pointer table, the per-block partial sums
+ * and the weights live in a static device array of SSG_SHARPNESS_NOALLOC_SCRATCH_BYTES per GPU,
+ * held until the process ends, and the work runs on the null stream of the current device.
+ * Limits: n <= SSG_SHARPNESS_NOALLOC_MAX_FRAMES and 24576 + 8 * n * ceil(height / 16) bytes
+ * <= SSG_SHARPNESS_NOALLOC_SCRATCH_BYTES, else SSG_E_INVALID_ARG before any GPU work.  Calls
+ * from threads of one process are serialised; calls that overlap on one GPU from other
+ * processes, or GPU work that writes the scratch array concurrently, race. */
+#define SSG_SHARPNESS_NOALLOC_SCRATCH_BYTES (2 * 1024 * 1024)
+#define SSG_SHARPNESS_NOALLOC_MAX_FRAMES 2048
+int ssg_frame_sharpness_noalloc(int n, const ssg_image *frames, float *weights);
+
 /* ---- FFT phase correlation (no counterpart in the reference, which has none on this path;
  * its nearest code is core/proc/fft.cc:17-160).  Optional 2x2 binning (the luminance of raw
  * Bayer frames), mean removal, separable Hann apodisation, zero padding to powers of two, normalised

@@ -251,6 +251,7 @@ _SIGS = [
     ("ssg_accumulation_state_size", C.c_int, [C.c_void_p, _P(C.c_uint64)]),
     ("ssg_accumulation_state_save", C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64]),
     ("ssg_accumulation_state_load", C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64]),
+    ("ssg_frame_sharpness_noalloc", C.c_int, [C.c_int, _P(Image), _P(C.c_float)]),
     ("ssg_registration_set_profiling", C.c_int, [C.c_void_p, C.c_int]),
     ("ssg_registration_get_profile", C.c_int, [C.c_void_p, _P(Profile), C.c_int]),
     ("ssg_accumulation_set_profiling", C.c_int, [C.c_void_p, C.c_int]),

@@ -2707,6 +2707,75 @@ extern "C" int ssg_frame_sharpness(int n, const ssg_image* frames, float* weight
   return SSG_OK;
 }
 
+// ---- allocation-free variant of ssg_frame_sharpness --------------------------------------------
+// The scratch memory lives in a file-scope device array (one copy per GPU, held until the
+// process ends).  Host code never passes `scratch` itself to a kernel or a copy: its device
+// address comes from hipGetSymbolAddress, once per device, and is kept in g_scratch_ptr.
+// Layout: [0, 16 KiB) the frame pointer table (2048 entries), [16 KiB, 24 KiB) the weights
+// (2048 floats), [24 KiB, end) the per-block partial sums (doubles).
+static __device__ __align__(256) unsigned char scratch[SSG_SHARPNESS_NOALLOC_SCRATCH_BYTES];
+
+namespace {
+constexpr size_t kTabOff = 0, kOutOff = 16384, kPartOff = 24576;
+static_assert(SSG_SHARPNESS_NOALLOC_MAX_FRAMES * sizeof(void*) <= kOutOff - kTabOff, "pointer table");
+static_assert(SSG_SHARPNESS_NOALLOC_MAX_FRAMES * sizeof(float) <= kPartOff - kOutOff, "weights");
+
+std::mutex g_scratch_mu;
+constexpr int kMaxDevices = 64;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+// the device address of the current device's scratch array (caller holds g_scratch_mu)
+int scratch_address(unsigned char** out) {
+  int dev = -1;
+  HIPCHK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) return fail(SSG_E_UNSUPPORTED, "device %d: at most %d devices", dev, kMaxDevices);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    HIPCHK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (!p) return fail(SSG_E_HIP, "hipGetSymbolAddress(scratch) returned a null address");
+    g_scratch_ptr[dev] = p;
+  }
+  *out = (unsigned char*)g_scratch_ptr[dev];
+  return SSG_OK;
+}
+}  // namespace
+
+extern "C" int ssg_frame_sharpness_noalloc(int n, const ssg_image* frames, float* weights) {
+  if (n <= 0) return SSG_OK;
+  if (!frames || !weights) return fail(SSG_E_INVALID_ARG, "null frames or weights");
+  for (int f = 0; f < n; ++f) {
+    CHK(check_image(&frames[f], "frame"));
+    if (!frames[f].on_device || frames[f].type != SSG_32FC1 || frames[f].width != frames[0].width ||
+        frames[f].height != frames[0].height || frames[f].step != frames[0].step || frames[f].step % 4)
+      return fail(SSG_E_INVALID_ARG, "frame %d: SSG_32FC1 device images of one size and step", f);
+  }
+  const int w = frames[0].width, h = frames[0].height;
+  const int nb = (h + 15) / 16;
+  const size_t need = kPartOff + sizeof(double) * (size_t)n * nb;
+  if (n > SSG_SHARPNESS_NOALLOC_MAX_FRAMES || need > SSG_SHARPNESS_NOALLOC_SCRATCH_BYTES)
+    return fail(SSG_E_INVALID_ARG,
+                "ssg_frame_sharpness_noalloc: %d frames of height %d need %zu bytes of scratch; the maximum is %d "
+                "frames and %d bytes (24576 + 8 * n * ceil(height / 16))",
+                n, h, need, SSG_SHARPNESS_NOALLOC_MAX_FRAMES, SSG_SHARPNESS_NOALLOC_SCRATCH_BYTES);
+  const float* tab[SSG_SHARPNESS_NOALLOC_MAX_FRAMES];
+  for (int f = 0; f < n; ++f) tab[f] = (const float*)frames[f].data;
+  // one call at a time per process: every call on a GPU shares its scratch array
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  unsigned char* base = nullptr;
+  CHK(scratch_address(&base));
+  hipStream_t s = nullptr;  // the null stream: no stream is created
+  const float* const* dtab = (const float* const*)(base + kTabOff);
+  float* dout = (float*)(base + kOutOff);
+  double* dpart = (double*)(base + kPartOff);
+  // every slot that is read is written first (table, partials, weights): nothing to clear
+  HIPCHK(hipMemcpyAsync((void*)dtab, tab, sizeof(void*) * n, hipMemcpyHostToDevice, s));
+  HIPCHK(launch_sharpness(dtab, (int)(frames[0].step / 4), w, h, n, dpart, s));
+  HIPCHK(launch_sharpness_finish(dpart, nb, n, (double)w * h, dout, s));
+  HIPCHK(hipMemcpyAsync(weights, dout, sizeof(float) * n, hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  return SSG_OK;
+}
+
 // =====================================================================================================
 // dense ECC flow: c_eccflow (core/proc/image_registration/ecc2.cc:2103-2865)
 // =====================================================================================================

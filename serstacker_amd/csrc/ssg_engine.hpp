@@ -287,6 +287,7 @@ hipError_t launch_count_zero_u8(const uint8_t* src, long long fstride, int pitch
 hipError_t launch_sum_f32(const float* src, int pitch, int w, int h, double* out, hipStream_t s);
 hipError_t launch_sharpness(const float* const* src_ptrs, int pitch, int w, int h, int nframes, double* partial,
                             hipStream_t s);
+hipError_t launch_sharpness_finish(const double* partial, int nb, int nframes, double area, float* out, hipStream_t s);
 // dst = src - up, 0 where masks[f] (if any) is not 255; src / masks: device arrays of per-frame pointers
 hipError_t launch_norm_sub(const float* const* src, int spitch, const float* up, long long ufs,
                            const uint8_t* const* masks, int mpitch, float* dst, int dpitch, long long dfs, int w, int h,

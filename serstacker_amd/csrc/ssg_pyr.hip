@@ -452,6 +452,23 @@ hipError_t launch_sharpness(const float* const* src_ptrs, int pitch, int w, int 
   return hipGetLastError();
 }
 
+// The frames' weights from the partials of k_sharpness: per frame the partials summed in
+// block order, then divided by the pixel count (the same double operations in the same order
+// as the host reduction of ssg_frame_sharpness).  One thread per frame.
+__global__ void __launch_bounds__(64) k_sharpness_finish(const double* __restrict__ partial, int nb, int nframes,
+                                                         double area, float* __restrict__ out) {
+  const int f = blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= nframes) return;
+  double t = 0;
+  for (int b = 0; b < nb; ++b) t += partial[(size_t)f * nb + b];
+  out[f] = (float)(t / area);
+}
+
+hipError_t launch_sharpness_finish(const double* partial, int nb, int nframes, double area, float* out, hipStream_t s) {
+  hipLaunchKernelGGL(k_sharpness_finish, dim3((nframes + 63) / 64), dim3(64), 0, s, partial, nb, nframes, area, out);
+  return hipGetLastError();
+}
+
 // the last step of ecc_normalize (ecc2.cc:392-396): dst = src - up, then 0 where the mask is
 // not 255 (setTo(0, ~mask)); blockIdx.z = frame
 __global__ void __launch_bounds__(256) k_norm_sub(const float* const* __restrict__ src, int spitch,
