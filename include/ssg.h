@@ -302,6 +302,19 @@ int ssg_accumulation_add(ssg_accumulation *a, const ssg_image *src, const ssg_im
  * launch; the results equal those of the single adds. */
 int ssg_accumulation_add_frames(ssg_accumulation *a, int n, const ssg_image *frames);
 
+/* Focus stacking of several independent sequences in one batch (no reference counterpart: one
+ * c_laplacian_pyramid_focus_stacking per sequence, c_frame_accumulation.cc:491-683, each fed
+ * by its own pipeline).  accs[q] are focus accumulators with the SSG_FOCUS_LPG_WEIGHTS policy,
+ * of any sizes and 1 or 3 channels; frames[q][0..nframes[q]) are f32 device frames of that
+ * accumulator's size (one row step per sequence); shifts[q] (optional, may be NULL) holds per
+ * frame the translation of its registration (tx, ty): the frame is remapped from (x + tx,
+ * y + ty) with INTER_LINEAR and BORDER_REFLECT101 before it is added.  Every step of a chunk of
+ * frames is one launch over all sequences; the sums equal those of one add() per aligned frame,
+ * bit for bit.  At most 32 sequences per call; the accumulators' profiling counters are those of
+ * accs[0] (SSG_PROF_FOCUS_*). */
+int ssg_focus_stack_batch(int32_t nseq, ssg_accumulation *const *accs, const int32_t *nframes,
+                          const ssg_image *const *frames, const float *const *shifts);
+
 /* Fused remap + base_remap mask + weighted add of n registered frames (the
  * per-frame sequence custom_remap -> weights*mask/255 -> add of
  * c_image_stacking_pipeline.cc:1644-1776 for every frame at once).
@@ -458,7 +471,8 @@ int ssg_eccflow_compute_batch(ssg_eccflow *f, int n, const ssg_image *inputs, co
  * the per-pixel minimum traffic of each kernel. ------------------------------------------ */
 enum { SSG_PROF_ECC_TRANSLATION = 0, SSG_PROF_ECC_MAIN = 1, SSG_PROF_CORRELATION = 2, SSG_PROF_PYRAMID = 3,
        SSG_PROF_ACCUMULATE = 4, SSG_PROF_ECC_SOLVE = 5, SSG_PROF_FFT = 6, SSG_PROF_ECCFLOW = 7,
-       SSG_PROF_COUNT = 8 };
+       SSG_PROF_FOCUS_PREP = 8, SSG_PROF_FOCUS_BLUR = 9, SSG_PROF_FOCUS_DOWN = 10, SSG_PROF_FOCUS_FUSE = 11,
+       SSG_PROF_COUNT = 12 };
 typedef struct ssg_profile {
   int64_t launches[SSG_PROF_COUNT];
   double ms[SSG_PROF_COUNT];     /* summed kernel durations */

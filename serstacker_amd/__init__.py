@@ -32,7 +32,7 @@ __all__ = ["c_image_registration_options", "c_frame_registration", "c_weigthed_a
            "inpaint_linear", "average_pyramid_inpaint", "unsharp_mask", "local_variance_map", "upscale_size",
            "upscale_remaps",
            "master_frame_quality", "master_scan_window", "select_master_frame", "master_stack_window",
-           "ecc_normalize"]
+           "ecc_normalize", "focus_stack_batch"]
 
 
 def _is_torch(x) -> bool:
@@ -555,6 +555,25 @@ class c_laplacian_pyramid_focus_stacking(_accumulator_state):
         n = C.c_int32()
         _lib.check(self._lib.ssg_accumulation_accumulated_frames(self._h, C.byref(n)), "accumulated_frames")
         return n.value
+
+
+def focus_stack_batch(stacks, frames, shifts=None):
+    """ssg_focus_stack_batch: several focus sequences at once.  stacks: list of
+    c_laplacian_pyramid_focus_stacking (lpg-weight policy); frames[q]: list of device tensors
+    of stack q's shape; shifts[q] (optional): (n, 2) translations (tx, ty) of the frames."""
+    n = len(stacks)
+    lib = _lib.load()
+    handles = (C.c_void_p * n)(*[st._h.value for st in stacks])
+    counts = (C.c_int32 * n)(*[len(f) for f in frames])
+    views = [image_views(f) for f in frames]
+    vptr = (C.POINTER(_lib.Image) * n)(*[C.cast(v, C.POINTER(_lib.Image)) for v in views])
+    sh_arrays, sptr = [], None
+    if shifts is not None:
+        for sh in shifts:
+            sh_arrays.append(None if sh is None else np.ascontiguousarray(sh, np.float32).reshape(-1))
+        sptr = (C.POINTER(C.c_float) * n)(*[
+            a.ctypes.data_as(C.POINTER(C.c_float)) if a is not None else C.POINTER(C.c_float)() for a in sh_arrays])
+    _lib.check(lib.ssg_focus_stack_batch(n, handles, counts, vptr, sptr), "ssg_focus_stack_batch")
 
 
 def lpg(src, k: float = 6.0, p: float = 2.0):

@@ -10,7 +10,7 @@
 // reference's incremental mean (c_frame_accumulation.cc:115-121); both give
 // A = sum(w I) / sum(w).  The pairs are kept with TwoSum so 2000-frame sums
 // lose no precision, and compute() divides in FP64.
-#include "ssg_engine.hpp"
+#include "ssg_stencil.hpp"
 
 namespace ssg {
 
@@ -32,47 +32,7 @@ __device__ __forceinline__ void two_sum(float a, float b, float& s, float& e) {
   e = (a - (s - bb)) + (b - bb);
 }
 
-// ---- per-channel samplers --------------------------------------------------------------------
-template <int CN>
-__device__ __forceinline__ void lin_fast(const float* __restrict__ img, int pitch, int X, int Y, float* out) {
-  const int ix = X >> 5, iy = Y >> 5;
-  float w0, w1, w2, w3;
-  bilinear_w(X & 31, Y & 31, w0, w1, w2, w3);
-  const float* r0 = img + (size_t)iy * pitch + ix * CN;
-  const float* r1 = r0 + pitch;
-#pragma unroll
-  for (int c = 0; c < CN; ++c) out[c] = bilinear_sum(r0[c], r0[CN + c], r1[c], r1[CN + c], w0, w1, w2, w3);
-}
-
-template <int CN>
-__device__ __forceinline__ bool lin_general(const float* __restrict__ img, int pitch, int w, int h, int X, int Y,
-                                            int border, float cval, float* out) {
-  const int ix = sat_short(X >> 5), iy = sat_short(Y >> 5);
-  if ((unsigned)ix < (unsigned)(w - 1) && (unsigned)iy < (unsigned)(h - 1)) {
-    lin_fast<CN>(img, pitch, X, Y, out);
-    return true;
-  }
-  if (border == BORDER_CONSTANT && (ix >= w || ix + 1 < 0 || iy >= h || iy + 1 < 0)) {
-#pragma unroll
-    for (int c = 0; c < CN; ++c) out[c] = cval;
-    return true;
-  }
-  if (border == BORDER_TRANSPARENT) return false;
-  float w0, w1, w2, w3;
-  bilinear_w(X & 31, Y & 31, w0, w1, w2, w3);
-  const int x0 = border_interp(ix, w, border), x1 = border_interp(ix + 1, w, border);
-  const int y0 = border_interp(iy, h, border), y1 = border_interp(iy + 1, h, border);
-#pragma unroll
-  for (int c = 0; c < CN; ++c) {
-    const float s00 = (x0 >= 0 && y0 >= 0) ? img[(size_t)y0 * pitch + x0 * CN + c] : cval;
-    const float s01 = (x1 >= 0 && y0 >= 0) ? img[(size_t)y0 * pitch + x1 * CN + c] : cval;
-    const float s10 = (x0 >= 0 && y1 >= 0) ? img[(size_t)y1 * pitch + x0 * CN + c] : cval;
-    const float s11 = (x1 >= 0 && y1 >= 0) ? img[(size_t)y1 * pitch + x1 * CN + c] : cval;
-    out[c] = bilinear_sum(s00, s01, s10, s11, w0, w1, w2, w3);
-  }
-  return true;
-}
-
+// ---- per-channel samplers (bilinear: lin_fast / lin_general in ssg_stencil.hpp) ----------------
 // remapBicubic: interior rows as fma chains added row by row; border pixels
 // as sum = cval + sum over in-image taps of (S - cval) * w (imgwarp.cpp).
 template <int CN>

@@ -1257,6 +1257,7 @@ struct ssg_accumulation {
   int L = 0, wc = 1;
   DevBuf facc, fw, fp, fup, ft0, ft1, ft2, ft3, fgray, fm, fW, fptrs, fstd;
   DevBuf fbpyr, fbup, fbW;  // batch planes of add_frames, kept for the next batch
+  DevBuf fms, fmtab;        // ssg_focus_stack_batch: chunk scratch and job tables (first handle)
   std::vector<const float*> hptrs;
   int slot = 0;
   SepKernel G{}, Wk{};
@@ -1959,6 +1960,188 @@ extern "C" int ssg_accumulation_add_frames(ssg_accumulation* a, int n, const ssg
   if (!frames) return fail(SSG_E_INVALID_ARG, "null frames");
   if (a->focus()) return focus_add_frames(a, n, frames);
   for (int f = 0; f < n; ++f) CHK(ssg_accumulation_add(a, &frames[f], nullptr));
+  return SSG_OK;
+}
+
+// Several focus sequences at once (see ssg_focus_multi.hip).  The frames of every sequence go
+// through in chunks of m frames; each step of a chunk is one launch over all sequences.
+extern "C" int ssg_focus_stack_batch(int32_t nseq, ssg_accumulation* const* accs, const int32_t* nframes,
+                                     const ssg_image* const* frames, const float* const* shifts) {
+  if (nseq <= 0) return SSG_OK;
+  if (!accs || !nframes || !frames) return fail(SSG_E_INVALID_ARG, "null argument");
+  if (nseq > FM_MAXJOBS / 2) return fail(SSG_E_UNSUPPORTED, "at most %d sequences per call", FM_MAXJOBS / 2);
+  int maxn = 0;
+  for (int q = 0; q < nseq; ++q) {
+    ssg_accumulation* a = accs[q];
+    if (!a || !a->focus()) return fail(SSG_E_INVALID_ARG, "sequence %d: not a focus stacking accumulator", q);
+    for (int r = 0; r < q; ++r)
+      if (accs[r] == a) return fail(SSG_E_INVALID_ARG, "sequence %d repeats the accumulator of sequence %d", q, r);
+    if (a->fo.fusing_policy != SSG_FOCUS_LPG_WEIGHTS)
+      return fail(SSG_E_UNSUPPORTED, "sequence %d: the batched form takes the lpg-weight fusing policy", q);
+    if (nframes[q] < 0) return fail(SSG_E_INVALID_ARG, "sequence %d: %d frames", q, nframes[q]);
+    if (nframes[q] > 0 && !frames[q]) return fail(SSG_E_INVALID_ARG, "sequence %d: null frames", q);
+    for (int k = 0; k < nframes[q]; ++k) {
+      const ssg_image& im = frames[q][k];
+      CHK(check_image(&im, "frame"));
+      if (!im.on_device || im.type != (a->cn == 3 ? SSG_32FC3 : SSG_32FC1) || im.width != a->w || im.height != a->h ||
+          im.step != frames[q][0].step || im.step % 4)
+        return fail(SSG_E_INVALID_ARG,
+                    "sequence %d frame %d: f32 device images with the accumulator's size and channels, one step", q, k);
+    }
+    maxn = std::max(maxn, (int)nframes[q]);
+  }
+  if (maxn == 0) return SSG_OK;
+  for (int q = 0; q < nseq; ++q) {
+    ssg_accumulation* a = accs[q];
+    if (a->accumulated == 0) CHK(focus_prepare(a));
+    if (a->L > FM_MAXL) return fail(SSG_E_UNSUPPORTED, "sequence %d: %d pyramid levels (max %d)", q, a->L, FM_MAXL);
+    if (a->Wk.nx != a->Wk.ny || !(a->Wk.nx & 1) || a->Wk.nx > SEP_MAX_TAPS)
+      return fail(SSG_E_UNSUPPORTED, "sequence %d: smoothing kernel of %d taps", q, a->Wk.nx);
+    HIPCHK(hipStreamSynchronize(a->stream));
+  }
+  ssg_accumulation* a0 = accs[0];
+  hipStream_t s = a0->stream;
+  // chunk size: all sequences' chunk planes within a fixed scratch budget
+  std::vector<size_t> per(nseq);
+  size_t per_all = 0;
+  for (int q = 0; q < nseq; ++q) {
+    const ssg_accumulation* a = accs[q];
+    const size_t n0 = (size_t)a->w * a->h;
+    per[q] = sizeof(float) * ((size_t)a->cn * a->P + (size_t)a->P + 2 * n0);
+    if (nframes[q] > 0) per_all += per[q];
+  }
+  constexpr size_t kBudget = (size_t)8 << 30;
+  const int m = (int)std::max<size_t>(1, std::min<size_t>({(size_t)32, (size_t)maxn, kBudget / std::max<size_t>(per_all, 1)}));
+  std::vector<size_t> soff(nseq + 1, 0);
+  for (int q = 0; q < nseq; ++q) soff[q + 1] = soff[q] + (nframes[q] > 0 ? ((per[q] * m + 255) / 256) * 256 : 0);
+  CHK(a0->fms.ensure(soff[nseq]));
+  // device tables for every chunk: FmSeq[nchunks][nseq], the frame pointers, the shifts
+  const int nchunks = (maxn + m - 1) / m;
+  int nptr = 0;
+  for (int q = 0; q < nseq; ++q) nptr += nframes[q];
+  const size_t tab_bytes = sizeof(FmSeq) * (size_t)nchunks * nseq;
+  const size_t ptr_off = (tab_bytes + 255) / 256 * 256;
+  const size_t sh_off = ptr_off + ((sizeof(void*) * (size_t)nptr + 255) / 256) * 256;
+  const size_t total = sh_off + sizeof(float) * 2 * (size_t)nptr;
+  CHK(a0->fmtab.ensure(total));
+  std::vector<char> host(total, 0);
+  FmSeq* tab = (FmSeq*)host.data();
+  const float** hptr = (const float**)(host.data() + ptr_off);
+  float* hsh = (float*)(host.data() + sh_off);
+  char* dbase = a0->fmtab.as<char>();
+  std::vector<int> pbase(nseq, 0);
+  for (int q = 0, o = 0; q < nseq; ++q) {
+    pbase[q] = o;
+    for (int k = 0; k < nframes[q]; ++k) {
+      hptr[o + k] = (const float*)frames[q][k].data;
+      hsh[2 * (o + k)] = shifts && shifts[q] ? shifts[q][2 * k] : 0.f;
+      hsh[2 * (o + k) + 1] = shifts && shifts[q] ? shifts[q][2 * k + 1] : 0.f;
+    }
+    o += nframes[q];
+  }
+  const double l2 = (100.f / 4.f) * (100.f / 4.f), g2 = (100.f / 36.f) * (100.f / 36.f);
+  for (int c = 0; c < nchunks; ++c)
+    for (int q = 0; q < nseq; ++q) {
+      const ssg_accumulation* a = accs[q];
+      FmSeq& t = tab[(size_t)c * nseq + q];
+      t.w = a->w;
+      t.h = a->h;
+      t.cn = a->cn;
+      t.L = a->L;
+      t.m = std::max(0, std::min(m, nframes[q] - c * m));
+      for (int i = 0; i < a->L; ++i) {
+        t.lw[i] = a->lw[i];
+        t.lh[i] = a->lh[i];
+        t.loff[i] = a->loff[i];
+      }
+      const size_t n0 = (size_t)a->w * a->h;
+      float* base = (float*)(a0->fms.as<char>() + soff[q]);
+      t.G = base;
+      t.W = base + (size_t)a->cn * a->P * m;
+      t.gray = t.W + (size_t)a->P * m;
+      t.M = t.gray + n0 * m;
+      t.acc = a->facc.as<float>();
+      t.wsum = a->fw.as<float>();
+      t.frames = (const float* const*)(dbase + ptr_off) + pbase[q] + (size_t)c * m;
+      t.shift = shifts && shifts[q] ? (const float*)(dbase + sh_off) + 2 * ((size_t)pbase[q] + (size_t)c * m) : nullptr;
+      t.frame_pitch = nframes[q] > 0 ? (int)(frames[q][0].step / 4) : 0;
+      const double kk = a->fo.lpg_k;
+      t.alpha = (float)(kk / (kk + 1)) * (float)l2;
+      t.beta = (float)(1. / (kk + 1)) * (float)g2;
+      t.eps = 1e-9f;
+      t.p = (float)a->fo.lpg_p;
+      t.K = a->Wk;
+    }
+  HIPCHK(hipMemcpyAsync(dbase, host.data(), total, hipMemcpyHostToDevice, s));
+  HIPCHK(hipStreamSynchronize(s));
+  int maxL = 0;
+  for (int q = 0; q < nseq; ++q) maxL = std::max(maxL, accs[q]->L);
+  auto& prof = a0->prof;
+  for (int c = 0; c < nchunks; ++c) {
+    const FmSeq* hs = tab + (size_t)c * nseq;
+    const FmSeq* ds = (const FmSeq*)dbase + (size_t)c * nseq;
+    auto run = [&](int kind, auto&& add_jobs, int prof_kind) -> int {
+      FmJobs J{};
+      long long end = 0;
+      add_jobs([&](int q, int k, int level) {
+        const long long b = fm_job_blocks(hs[q], k, level);
+        if (b <= 0) return;
+        end += b;
+        J.j[J.n++] = FmJob{q, level, k, (int)end};
+      });
+      if (J.n == 0) return SSG_OK;
+      if (end > INT32_MAX) return fail(SSG_E_UNSUPPORTED, "step of %lld blocks", end);
+      prof.start(s);
+      HIPCHK(launch_fm_step(kind, J, ds, s));
+      prof.stop(s, prof_kind);
+      return SSG_OK;
+    };
+    auto each = [&](int kind) {
+      return [&, kind](auto&& add) {
+        for (int q = 0; q < nseq; ++q)
+          if (hs[q].m > 0) add(q, kind, 0);
+      };
+    };
+    CHK(run(FM_PREP, each(FM_PREP), SSG_PROF_FOCUS_PREP));
+    CHK(run(FM_LPG, each(FM_LPG), SSG_PROF_FOCUS_PREP));
+    CHK(run(FM_BLUR, each(FM_BLUR), SSG_PROF_FOCUS_BLUR));
+    for (int i = 0; i + 1 < maxL; ++i) {
+      CHK(run(FM_DOWN_G, [&](auto&& add) {
+        for (int q = 0; q < nseq; ++q)
+          if (hs[q].m > 0 && i + 1 < hs[q].L) {
+            add(q, FM_DOWN_G, i);
+            add(q, FM_DOWN_W, i);
+          }
+      }, SSG_PROF_FOCUS_DOWN));
+      CHK(run(FM_FUSE, [&](auto&& add) {
+        for (int q = 0; q < nseq; ++q)
+          if (hs[q].m > 0 && i + 1 < hs[q].L) add(q, FM_FUSE, i);
+      }, SSG_PROF_FOCUS_FUSE));
+    }
+    CHK(run(FM_TOP, [&](auto&& add) {
+      for (int q = 0; q < nseq; ++q)
+        if (hs[q].m > 0) add(q, FM_TOP, hs[q].L - 1);
+    }, SSG_PROF_FOCUS_FUSE));
+  }
+  HIPCHK(hipStreamSynchronize(s));
+  if (prof.on) {
+    // algorithmic bytes per frame (cn channels, N0 level-0 pixels, P pyramid pixels): prep reads
+    // the frame and writes the planes and the gray image (8 cn + 4) N0; lpg 8 N0; the smoothing
+    // 8 N0; each pyrDown reads its level and writes the next, (cn + 1)(n_i + n_i+1); the fuse reads
+    // each level's planes and weights once, (cn + 1) P, plus the next level through L2
+    for (int q = 0; q < nseq; ++q) {
+      const ssg_accumulation* a = accs[q];
+      const double n0 = (double)a->w * a->h, nf = nframes[q];
+      double down = 0;
+      for (int i = 0; i + 1 < a->L; ++i) down += (double)(a->cn + 1) * 4.0 * ((double)a->lw[i] * a->lh[i] + (double)a->lw[i + 1] * a->lh[i + 1]);
+      prof.add_bytes(SSG_PROF_FOCUS_PREP, nf * ((8.0 * a->cn + 4.0) * n0 + 8.0 * n0));
+      prof.add_bytes(SSG_PROF_FOCUS_BLUR, nf * 8.0 * n0);
+      prof.add_bytes(SSG_PROF_FOCUS_DOWN, nf * down);
+      prof.add_bytes(SSG_PROF_FOCUS_FUSE, nf * (a->cn + 1.0) * 4.0 * (double)a->P);
+    }
+    prof.collect();
+  }
+  for (int q = 0; q < nseq; ++q) accs[q]->accumulated += nframes[q];
   return SSG_OK;
 }
 

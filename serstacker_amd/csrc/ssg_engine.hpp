@@ -473,4 +473,38 @@ struct SynthFrame {
 
 void set_cubic_table(const float tab[32][4], hipStream_t s);
 
+// ---- multi-sequence focus stacking (ssg_focus_multi.hip) -------------------------------------
+// One launch per step serves every sequence of a batch: the blocks of a launch are split over
+// jobs (a sequence, a level, a kind) by their prefix block counts.
+constexpr int FM_MAXL = 16;
+struct FmSeq {
+  int w, h, cn, L, m;               // level-0 size, channels, levels, frames in this chunk
+  int lw[FM_MAXL], lh[FM_MAXL];
+  long long loff[FM_MAXL];          // single-plane element offset of each level
+  float* G;     // chunk pyramids: level i at G + m*cn*loff[i], planes [frame][channel] of lw*lh
+  float* W;     // chunk weight pyramids: level i at W + m*loff[i], planes [frame]
+  float* gray;  // [frame][pixel] channel average of the aligned level-0 frame
+  float* M;     // [frame][pixel] lpg maps
+  float* acc;   // fused sums: level i at acc + cn*loff[i], planes [channel]
+  float* wsum;  // weight sums: level i at wsum + loff[i]
+  const float* const* frames;  // m device frames, interleaved cn channels
+  const float* shift;          // 2 floats per frame (source offset x, y) or null
+  int frame_pitch;             // floats per frame row
+  float alpha, beta, eps, p;   // lpg coefficients and power
+  SepKernel K;                 // the weight smoothing (symmetric gaussian)
+};
+enum { FM_PREP = 0, FM_LPG = 1, FM_BLUR = 2, FM_DOWN_G = 3, FM_DOWN_W = 4, FM_FUSE = 5, FM_TOP = 6 };
+struct FmJob {
+  int seq, level, kind, end;  // end: the launch's block index one past this job's last block
+};
+constexpr int FM_MAXJOBS = 64;
+struct FmJobs {
+  int n;
+  FmJob j[FM_MAXJOBS];
+};
+// blocks of one job (kind, level) of sequence q
+long long fm_job_blocks(const FmSeq& q, int kind, int level);
+// step `kind` (FM_PREP, FM_LPG, FM_BLUR, FM_DOWN_*, FM_FUSE/FM_TOP) of every job in J; seqs: device table
+hipError_t launch_fm_step(int kind, const FmJobs& J, const FmSeq* seqs, hipStream_t s);
+
 }  // namespace ssg

@@ -7,7 +7,7 @@
 // formula keeps the reference's order of operations with no fused multiply-add
 // (the library builds with -ffp-contract=off), so the results reproduce the
 // FP32 oracle bit for bit.
-#include "ssg_engine.hpp"
+#include "ssg_stencil.hpp"
 
 namespace ssg {
 
@@ -40,39 +40,19 @@ __global__ void __launch_bounds__(256) k_gray(const float* planes, size_t n, int
     g[i] = planes[i];
     return;
   }
-  const float s = (planes[i] + planes[n + i]) + planes[2 * n + i];
-  g[i] = (float)((double)s * (1.0 / 3.0));
+  g[i] = gray3(planes[i], planes[n + i], planes[2 * n + i]);
 }
 
-// compute_lpg_5x5_parallel (lpg.cc:63-127) at every pixel of the clamped grid
-// (its edge rows/columns copy the nearest computed pixel), then pow(m, p)
+// compute_lpg_5x5_parallel (lpg.cc:63-127) at every pixel of the clamped grid, then pow(m, p)
 __global__ void __launch_bounds__(256) k_lpg(const float* g, int w, int h, float alpha, float beta, float eps,
                                              float p, float* m) {
   const size_t i = gidx();
   if (i >= (size_t)w * h) return;
   const int y = (int)(i / w), x = (int)(i - (size_t)y * w);
-  const int cy = min(max(y, 2), h - 3), cx = min(max(x, 2), w - 3);
-  auto R = [&](int dy, int dx) { return g[(size_t)(cy + dy) * w + cx + dx]; };
-  const float two = 2.f, four = 4.f;
-  auto col = [&](int dx) { return (((R(-2, dx) + two * R(-1, dx)) + four * R(0, dx)) + two * R(1, dx)) + R(2, dx); };
-  auto row = [&](int dy) { return (((R(dy, -2) + two * R(dy, -1)) + four * R(dy, 0)) + two * R(dy, 1)) + R(dy, 2); };
-  auto c3 = [&](int dx) { return (R(-1, dx) + two * R(0, dx)) + R(1, dx); };
-  auto r3 = [&](int dy) { return (R(dy, -1) + two * R(dy, 0)) + R(dy, 1); };
-  const float gx = (col(2) - col(-2)) + two * (c3(1) - c3(-1));
-  const float gy = (row(2) - row(-2)) + two * (r3(1) - r3(-1));
-  const float grad = gx * gx + gy * gy;
-  const float t = ((16.f * R(0, 0) - two * (((R(-1, 0) + R(1, 0)) + R(0, -1)) + R(0, 1))) -
-                   1.f * (((R(-1, -1) + R(-1, 1)) + R(1, -1)) + R(1, 1))) -
-                  1.f * (((R(-2, 0) + R(2, 0)) + R(0, -2)) + R(0, 2));
-  float v = (alpha * (t * t) + beta * grad) + eps;
-  if (p == 2.f) v = v * v;
-  else if (p != 0.f && p != 1.f) v = powf(v, p);
-  m[i] = v;
+  m[i] = lpg_at(g, w, w, h, x, y, alpha, beta, eps, p);
 }
 
-// cv::pyrUp (float) of nplanes planes: the oracle's canonical form -- horizontally
-// upsampled rows (left REFLECT101, right replicated), then the vertical pass; an odd
-// extra column repeats the last one, an odd extra row repeats row 2*sh-2
+// cv::pyrUp (float) of nplanes planes (pyrup_at, the oracle's canonical form)
 __global__ void __launch_bounds__(256) k_pyrup(const float* src, long long src_fstride, int sw, int sh, float* dst,
                                                long long dst_fstride, int dw, int dh) {
   const size_t i = gidx();
@@ -80,24 +60,7 @@ __global__ void __launch_bounds__(256) k_pyrup(const float* src, long long src_f
   const int yo = (int)(i / dw), xo = (int)(i - (size_t)yo * dw);
   const float* S = src + blockIdx.y * src_fstride;
   float* D = dst + blockIdx.y * dst_fstride;
-  const int ye = (dh > 2 * sh && yo == 2 * sh) ? 2 * sh - 2 : yo;  // the extra odd row copies row 2 sh - 2
-  const int y = ye >> 1;
-  auto srow = [&](int r) {  // source rows -1..sh: REFLECT101 above, replicated below
-    if (r < 0) r = sh > 1 ? 1 : 0;
-    if (r >= sh) r = sh - 1;
-    return S + (size_t)r * sw;
-  };
-  const float scale = 1.f / 64.f;
-  float v;
-  if (ye & 1) {
-    // a row 2y+1 that exists only as min(2y+1, dh-1) when dh is odd and smaller is
-    // written by the even branch below (the oracle writes t1 first, then t0)
-    v = (hup(srow(y), sw, xo) + hup(srow(y + 1), sw, xo)) * 4.f * scale;
-  } else {
-    const float r0 = hup(srow(y - 1), sw, xo), r1 = hup(srow(y), sw, xo), r2 = hup(srow(y + 1), sw, xo);
-    v = fmaf(r1, 6.f, r0 + r2) * scale;
-  }
-  D[(size_t)yo * dw + xo] = v;
+  D[(size_t)yo * dw + xo] = pyrup_at(S, sw, sw, sh, dh, xo, yo);
 }
 
 // elementwise helpers over n elements

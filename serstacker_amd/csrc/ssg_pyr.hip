@@ -6,7 +6,7 @@
 // column-filtered output with coalesced stores.
 #include <type_traits>
 
-#include "ssg_engine.hpp"
+#include "ssg_stencil.hpp"
 
 namespace ssg {
 
@@ -16,44 +16,6 @@ namespace ssg {
 // filter.simd.hpp, with fused multiply-adds).
 // ---------------------------------------------------------------------------
 constexpr int SF_TW = 64, SF_TH = 16, SF_MAXR = (SEP_MAX_TAPS - 1) / 2;
-
-__device__ __forceinline__ float row_filter(const float* c, const SepKernel& K) {
-  // c points at the centre tap
-  const int n = K.nx, a = n / 2;
-  const float* k = K.kx;
-  if (K.tx == 1 && n <= 5) {
-    float s = c[0] * k[a];
-    if (n >= 3) s = fmaf(c[-1] + c[1], k[a + 1], s);
-    if (n >= 5) s = fmaf(c[-2] + c[2], k[a + 2], s);
-    return s;
-  }
-  if (K.tx == 2 && n >= 3 && n <= 5) {
-    float s = (c[1] - c[-1]) * k[a + 1];
-    if (n >= 5) s = fmaf(c[2] - c[-2], k[a + 2], s);
-    return s;
-  }
-  float s = 0.f;
-  for (int i = 0; i < n; ++i) s = fmaf(c[i - a], k[i], s);
-  return s;
-}
-
-__device__ __forceinline__ float col_filter(const float* c, int stride, const SepKernel& K) {
-  const int n = K.ny, a = n / 2;
-  const float* k = K.ky;
-  if (K.ty == 1) {
-    float s = fmaf(c[0], k[a], K.delta);
-    for (int i = 1; i <= a; ++i) s = fmaf(c[-i * stride] + c[i * stride], k[a + i], s);
-    return s;
-  }
-  if (K.ty == 2) {
-    float s = K.delta;
-    for (int i = 1; i <= a; ++i) s = fmaf(c[i * stride] - c[-i * stride], k[a + i], s);
-    return s;
-  }
-  float s = K.delta;
-  for (int i = 0; i < n; ++i) s = fmaf(c[(i - a) * stride], k[i], s);
-  return s;
-}
 
 __global__ void __launch_bounds__(256) k_sepfilter(const float* const* __restrict__ src_ptrs, int src_pitch,
                                                    float* __restrict__ dst, long long dst_fstride, int dst_pitch,
