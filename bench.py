@@ -70,10 +70,13 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="the CPU sample stops after this much time")
     ap.add_argument("--oracle-frames", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--workload", choices=["affine", "turbulence", "sweep"], default="affine",
+    ap.add_argument("--workload", choices=["affine", "turbulence", "focus", "sweep"], default="affine",
                     help="affine: batched affine ECC + weighted stack of 1080p frames (the headline metric); "
                          "turbulence: phase correlation + eccflow + bicubic remap + average of 1024x1024 frames; "
+                         "focus: 8 mixed-size RGB focus sequences, translation ECC + one batched focus stack; "
                          "sweep: each remaining kernel family at its configuration's frame size")
+    ap.add_argument("--focus-scale", type=float, default=1.0,
+                    help="focus workload: multiply every sequence's frame count by this (1 = 40..200 frames)")
     ap.add_argument("--ecc-method", choices=list(ECC_METHODS), default="fa",
                     help="ECC_ALIGN_METHOD of the affine workload (the headline is forward additive)")
     ap.add_argument("--copy-calib-mib", type=int, default=0,
@@ -101,6 +104,8 @@ def main():
         return bench_turbulence(a, S, _lib, torch)
     if a.workload == "sweep":
         return bench_sweep(a, S, _lib, torch)
+    if a.workload == "focus":
+        return bench_focus(a, S, _lib, torch)
     n, W, H = a.frames, a.width, a.height
 
     # ---- inputs resident in HBM (not timed)
@@ -478,6 +483,218 @@ def bench_turbulence(a, S, _lib, torch):
         "flow_epe_vs_fp64": epe64,
         "kernels": {k: {kk: (round(vv, 3) if isinstance(vv, float) else vv) for kk, vv in v.items()}
                     for k, v in kern.items()},
+    }
+    print(json.dumps(out))
+
+
+FOCUS_SIZES = [(1024, 768), (2048, 1536), (2560, 2048), (3072, 2048)]
+
+
+def focus_sequences(torch, dev, seed=5, scale=1.0):
+    """The focus workload's inputs, built on the GPU (untimed): 8 sequences with sizes drawn from
+    FOCUS_SIZES and 40..200 RGB f32 frames each.  The scene is a sum of 6 octaves of value noise
+    per channel (lattice values bicubically upsampled, amplitude halving per octave); frame k of a
+    sequence is the scene blurred per pixel with sigma = |z(x, y) - f_k| * 1.2 px, z a smooth depth
+    map in [0, 8] and f_k a focus sweep over [0, 8] (the blur interpolated between a stack of
+    gaussian blurs at sigma 0, 0.6, 1.2, 2.4, 4.8, 9.6), then translated by U(-6, 6) px (bilinear,
+    border replicated).  Returns [(w, h, frames (n, h, w, 3) BGR interleaved, true shifts (n, 2))]."""
+    import torch.nn.functional as F
+
+    g = torch.Generator(device="cpu").manual_seed(seed)
+    out = []
+    for q in range(8):
+        w, h = FOCUS_SIZES[int(torch.randint(0, 4, (1,), generator=g))]
+        n = max(2, int(round(int(torch.randint(40, 201, (1,), generator=g)) * scale)))
+
+        def noise(octaves, base, ch):
+            acc = torch.zeros((ch, h, w), device=dev)
+            amp = 1.0
+            for o in range(octaves):
+                gw, gh = max(2, w // (base >> o) + 2), max(2, h // (base >> o) + 2)
+                lat = torch.rand((1, ch, gh, gw), generator=g).to(dev)
+                acc += amp * F.interpolate(lat, size=(h, w), mode="bicubic", align_corners=False)[0]
+                amp *= 0.5
+            return acc / (2.0 - 2.0 ** (1 - octaves))
+
+        scene = noise(6, 256, 3).clamp(0, 1)
+        z = noise(2, 512, 1)[0]
+        z = (z - z.min()) / (z.max() - z.min()).clamp_min(1e-6) * 8.0
+        sig = [0.0, 0.6, 1.2, 2.4, 4.8, 9.6]
+        stack = [scene]
+        for sg in sig[1:]:
+            r = int(3 * sg + 0.5)
+            t = torch.arange(-r, r + 1, device=dev, dtype=torch.float32)
+            k = torch.exp(-t * t / (2 * sg * sg))
+            k = k / k.sum()
+            x = F.pad(scene[None], (r, r, 0, 0), mode="replicate")
+            x = F.conv2d(x, k.view(1, 1, 1, -1).repeat(3, 1, 1, 1), groups=3)
+            x = F.pad(x, (0, 0, r, r), mode="replicate")
+            x = F.conv2d(x, k.view(1, 1, -1, 1).repeat(3, 1, 1, 1), groups=3)
+            stack.append(x[0])
+        stack = torch.stack(stack)  # (6, 3, h, w)
+        sigt = torch.tensor(sig, device=dev)
+        frames = torch.empty((n, h, w, 3), device=dev)
+        shifts = (torch.rand((n, 2), generator=g) * 12.0 - 6.0).to(dev)
+        yy, xx = torch.meshgrid(torch.arange(h, device=dev, dtype=torch.float32),
+                                torch.arange(w, device=dev, dtype=torch.float32), indexing="ij")
+        for k in range(n):
+            fk = 8.0 * k / max(n - 1, 1)
+            s_ = (z - fk).abs() * 1.2
+            j = torch.searchsorted(sigt, s_.clamp(max=9.6).contiguous(), right=True).clamp(1, 5)
+            lo, hi = sigt[j - 1], sigt[j]
+            wgt = ((s_.clamp(max=9.6) - lo) / (hi - lo))[None]
+            a = torch.gather(stack, 0, (j - 1)[None, None].expand(1, 3, h, w))[0]
+            b = torch.gather(stack, 0, j[None, None].expand(1, 3, h, w))[0]
+            img = a * (1 - wgt) + b * wgt
+            gx = (xx - shifts[k, 0]) / (w - 1) * 2 - 1
+            gy = (yy - shifts[k, 1]) / (h - 1) * 2 - 1
+            img = F.grid_sample(img[None], torch.stack([gx, gy], -1)[None], mode="bilinear", padding_mode="border",
+                                align_corners=True)[0]
+            frames[k] = img.permute(1, 2, 0)
+        out.append((w, h, frames, shifts.cpu().numpy()))
+        del stack
+    torch.cuda.synchronize()
+    return out
+
+
+def bench_focus(a, S, _lib, torch):
+    """Focus stacking in mixed-size batches: 8 sequences (sizes from 1024x768 .. 3072x2048, 40..200
+    RGB f32 frames each).  One step = translation registration of every sequence (forward-additive
+    ECC on the gray channel, 3 levels, against the sequence's first frame) + ONE batched focus stack
+    of all sequences (lpg k=6 p=2 weights, gaussian sigma 5, 6-level laplacian fusion) + compute of
+    the 8 stacks.  Frames resident in HBM before the timed region."""
+    dev = torch.device("cuda:0")
+    lib = _lib.load()
+    t0 = time.time()
+    seqs = focus_sequences(torch, dev, scale=a.focus_scale)
+    nf = sum(x[2].shape[0] for x in seqs)
+    px = sum(x[0] * x[1] * x[2].shape[0] for x in seqs)
+    print(f"[focus] synthesized {nf} frames in {time.time() - t0:.1f}s: " +
+          ", ".join(f"{w}x{h}x{f.shape[0]}" for w, h, f, _ in seqs), file=sys.stderr, flush=True)
+    regs, views, stacks = [], [], []
+    for w, h, f, _ in seqs:
+        o = S.c_image_registration_options(motion_type=S.MOTION_TRANSLATION, scale=1.0, eps=0.02, ecch_max_level=3,
+                                           max_iterations=50)
+        r = S.c_frame_registration(o)
+        assert r.setup_reference_frame(f[0])
+        regs.append(r)
+        views.append(_lib.images([S.image_view(f[k]) for k in range(f.shape[0])]))
+
+    def step():
+        shifts = []
+        for r, v in zip(regs, views):
+            st = r.register_batch(v)
+            shifts.append(np.array([[s.params[0], s.params[1]] for s in st], np.float32))
+        stacks.clear()
+        for w, h, f, _ in seqs:
+            stacks.append(S.c_laplacian_pyramid_focus_stacking(w, h, 3, fusing_policy=2, lpg_k=6.0, lpg_p=2.0,
+                                                               lpg_sigma=5.0, max_levels=6))
+        S.focus_stack_batch(stacks, views, shifts)
+        outs = [st.compute(torch.empty((h, w, 3), device=dev)) for st, (w, h, _, _) in zip(stacks, seqs)]
+        return shifts, outs
+
+    for _ in range(a.warmup):
+        step()
+    torch.cuda.synchronize()
+    # profile the batched focus stack of one step (the library's HIP events on its stream)
+    prof_on = []
+    t_reg = 0.0
+    t_stack = 0.0
+    t0 = time.perf_counter()
+    for it in range(a.steps):
+        ts = time.perf_counter()
+        shifts = []
+        for r, v in zip(regs, views):
+            st = r.register_batch(v)
+            shifts.append(np.array([[s.params[0], s.params[1]] for s in st], np.float32))
+        torch.cuda.synchronize()
+        t_reg += time.perf_counter() - ts
+        ts = time.perf_counter()
+        stacks = [S.c_laplacian_pyramid_focus_stacking(w, h, 3, fusing_policy=2, lpg_k=6.0, lpg_p=2.0, lpg_sigma=5.0,
+                                                       max_levels=6) for w, h, _, _ in seqs]
+        if it == a.steps - 1:
+            lib.ssg_accumulation_set_profiling(stacks[0]._h, 1)
+            prof_on = stacks[0]
+        S.focus_stack_batch(stacks, views, shifts)
+        outs = [st.compute(torch.empty((h, w, 3), device=dev)) for st, (w, h, _, _) in zip(stacks, seqs)]
+        torch.cuda.synchronize()
+        t_stack += time.perf_counter() - ts
+    elapsed = time.perf_counter() - t0
+    prof = _lib.Profile()
+    lib.ssg_accumulation_get_profile(prof_on._h, C.byref(prof), 1)
+    fps = nf * a.steps / elapsed
+    ms_step = elapsed * 1e3 / a.steps
+    kinds = {"focus_prep": _lib.PROF_FOCUS_PREP, "focus_blur": _lib.PROF_FOCUS_BLUR,
+             "focus_down": _lib.PROF_FOCUS_DOWN, "focus_fuse": _lib.PROF_FOCUS_FUSE}
+    kernels = {}
+    for name, k in kinds.items():
+        kk = {"launches": int(prof.launches[k]), "ms": prof.ms[k], "bytes": prof.bytes[k]}
+        kk["avg_us"] = 1e3 * kk["ms"] / max(kk["launches"], 1)
+        kk["gbs"] = kk["bytes"] / max(kk["ms"], 1e-9) / 1e6
+        kk["frac"] = kk["gbs"] / HBM_PEAK_GBS
+        kernels[name] = kk
+    dom_name = max(kernels, key=lambda k: kernels[k]["ms"])
+    dom = kernels[dom_name]
+    # frame k is the scene moved by d_k, the reference (frame 0) by d_0: the registration finds d_k - d_0
+    err = max(float(np.abs(sh - (x[3] - x[3][0])).max()) for sh, x in zip(shifts, seqs))
+    print(f"[focus] {fps:.1f} frames/s ({ms_step:.1f} ms/step: registration {1e3 * t_reg / a.steps:.1f} ms, "
+          f"stack {1e3 * t_stack / a.steps:.1f} ms); max |shift - true shift| {err:.3f} px", file=sys.stderr,
+          flush=True)
+    print("[focus] kernels: " + ", ".join(f"{k}: {v['ms']:.1f} ms {v['gbs']:.0f} GB/s x{v['launches']}"
+                                          for k, v in kernels.items()), file=sys.stderr, flush=True)
+
+    cpu = None
+    if not a.no_cpu:
+        # the FP32 oracle (checker; a bounded sample): translation ECC + remap + lpg-weight focus add of frames
+        # of the first 1024x768 sequence (or the smallest), until about cpu_seconds of CPU work
+        from oracle import focus as ofocus
+        from oracle.oracle import Oracle, reg_options
+
+        threads = min(16, os.cpu_count() or 1)
+        o32 = Oracle(32)
+        o32.set_threads(threads)
+        w, h, f, _ = min(seqs, key=lambda x: x[0] * x[1])
+        host = f.cpu().numpy()
+        gray = lambda im: ((im[..., 0] * np.float32(0.114) + im[..., 1] * np.float32(0.587)) +
+                           im[..., 2] * np.float32(0.299)).astype(np.float32)
+        oreg = o32.registration(reg_options(motion_type=0, scale=1.0, eps=0.02, ecch_max_level=3, max_iterations=50))
+        oreg.setup_reference_frame(gray(host[0]))
+        fs = ofocus.FocusStack(o32, ofocus.LPG_WEIGHTS, lpg_k=6.0, lpg_p=2.0, lpg_sigma=5.0, max_levels=6)
+        done, cpu_s = 0, 0.0
+        for k in range(host.shape[0]):
+            ts = time.perf_counter()
+            st = oreg.register_frame(gray(host[k]))
+            t = o32.xform(0, [st.params[0], st.params[1]])
+            rmap = o32.create_map(t, w, h)
+            fs.add([o32.remap(np.ascontiguousarray(host[k][..., c]), rmap, interp=1, border=4) for c in range(3)])
+            cpu_s += time.perf_counter() - ts
+            done += 1
+            if cpu_s >= a.cpu_seconds:
+                break
+        cpu = {"value": round(done / cpu_s, 4), "unit": "frames/s", "cores": threads, "kind": "port",
+               "sample": f"{done} frames of a {w}x{h} RGB sequence ({cpu_s:.1f} s): translation ECC, remap and the "
+                         f"lpg-weight focus add, FP32 oracle, {threads} threads (the frame size mix of the workload "
+                         f"averages {px / nf / (w * h):.2f}x this size)"}
+        print(f"[focus] cpu oracle: {cpu['value']} frames/s of {w}x{h}", file=sys.stderr, flush=True)
+
+    out = {
+        "metric": "focus-stacked frames/s (8 mixed-size RGB sequences, one batched launch per step), % HBM roofline",
+        "value": round(fps, 2), "unit": "frames/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
+        "ms_per_step": round(ms_step, 3), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "focus stacking of 8 mixed-size RGB sequences: translation ECC per sequence + one "
+                               "batched lpg(k=6,p=2)/sigma-5/6-level focus stack",
+                   "sequences": [f"{w}x{h}x{f.shape[0]}" for w, h, f, _ in seqs], "frames": nf,
+                   "megapixels": round(px / 1e6, 1)},
+        "roofline": {"bound": "hbm", "kernel": dom_name, "achieved": round(dom["gbs"], 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(dom["frac"], 4), "traffic": None,
+                     "avg_launch_us": round(dom["avg_us"], 2), "launches": dom["launches"]},
+        "cpu_baseline": cpu,
+        "registration_ms_per_step": round(1e3 * t_reg / a.steps, 2),
+        "stack_ms_per_step": round(1e3 * t_stack / a.steps, 2),
+        "max_shift_error_px": round(err, 4),
+        "kernels": {k: {kk: (round(vv, 3) if isinstance(vv, float) else vv) for kk, vv in v.items()}
+                    for k, v in kernels.items()},
     }
     print(json.dumps(out))
 

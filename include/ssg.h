@@ -119,6 +119,13 @@ typedef struct ssg_eccflow_registration_options {
   int32_t downscale_method; /* ECCFlowDownscaleMethod, ecc2.h:507-511 */
 } ssg_eccflow_registration_options;
 
+/* color_channel_type (core/proc/extract_channel.h:13-50): the channel ECC registers on.  Colour
+ * (SSG_32FC3, BGR) frames take GRAY (cv::cvtColor BGR2GRAY, the reference default), BLUE / GREEN
+ * / RED or a channel index 0..2; gray frames are used as they are. */
+enum { SSG_COLOR_CHANNEL_0 = 0, SSG_COLOR_CHANNEL_1 = 1, SSG_COLOR_CHANNEL_2 = 2, SSG_COLOR_CHANNEL_DONT_CHANGE = 100,
+       SSG_COLOR_CHANNEL_RED = 101, SSG_COLOR_CHANNEL_GREEN = 102, SSG_COLOR_CHANNEL_BLUE = 103,
+       SSG_COLOR_CHANNEL_GRAY = 104 };
+
 /* c_image_registration_options, c_frame_registration.h:119-136 */
 typedef struct ssg_image_registration_options {
   int32_t motion_type;

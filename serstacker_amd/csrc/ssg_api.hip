@@ -325,7 +325,7 @@ struct ssg_registration {
   double rma[MAX_LEVELS] = {};
   // batch workspace
   int cap = 0;
-  DevBuf cur_pyr, cur_mask_pyr, half_buf, norm_buf;
+  DevBuf cur_pyr, cur_mask_pyr, half_buf, norm_buf, gray_buf;
   DevBuf xf_t, xf_m, res, partials, active[2], active_count[2], lm;
   DevBuf frame_ptrs, mask_ptrs, zero_count, frame_list, tmp_in, tmp_mask, sum_tmp;
   int* pinned_count = nullptr;
@@ -342,7 +342,7 @@ extern "C" void ssg_default_image_registration_options(ssg_image_registration_op
   if (!o) return;
   memset(o, 0, sizeof(*o));
   o->motion_type = SSG_MOTION_AFFINE;
-  o->ecc_registration_channel = 0;  // color_channel_gray
+  o->ecc_registration_channel = SSG_COLOR_CHANNEL_GRAY;
   o->interpolation = SSG_INTER_LINEAR;
   o->border_mode = SSG_BORDER_REFLECT101;
   o->ecc.scale = 0.5;
@@ -883,12 +883,76 @@ int prepare_batch(ssg_registration* r, int n, const ssg_image* frames, const ssg
   return SSG_OK;
 }
 
+// create_ecc_image's channel extraction (c_frame_registration.cc:1009-1022, extract_channel with
+// ecc_registration_channel): colour (BGR f32) frames become device gray images in `buf`, `views`
+// holds their views and *out points at them; gray frames pass through unchanged.
+int ecc_channel_views(ssg_registration* r, int n, const ssg_image* frames, DevBuf& buf, std::vector<ssg_image>& views,
+                      const ssg_image** out) {
+  *out = frames;
+  if (n <= 0 || frames[0].type != SSG_32FC3) return SSG_OK;
+  int ch;
+  switch (r->o.ecc_registration_channel) {
+    case SSG_COLOR_CHANNEL_GRAY: ch = -1; break;
+    case SSG_COLOR_CHANNEL_BLUE: case SSG_COLOR_CHANNEL_0: ch = 0; break;
+    case SSG_COLOR_CHANNEL_GREEN: case SSG_COLOR_CHANNEL_1: ch = 1; break;
+    case SSG_COLOR_CHANNEL_RED: case SSG_COLOR_CHANNEL_2: ch = 2; break;
+    default:
+      return fail(SSG_E_UNSUPPORTED, "ecc_registration_channel %d on colour frames (gray, blue, green, red, 0..2)",
+                  r->o.ecc_registration_channel);
+  }
+  const int w = frames[0].width, h = frames[0].height;
+  for (int f = 0; f < n; ++f) {
+    CHK(check_image(&frames[f], "frame"));
+    if (frames[f].type != SSG_32FC3 || frames[f].width != w || frames[f].height != h)
+      return fail(SSG_E_INVALID_ARG, "frame %d: colour frames of one size", f);
+  }
+  hipStream_t s = r->stream;
+  const size_t gp = ((size_t)w * 4 + 255) / 256 * 256, cp = ((size_t)w * 12 + 255) / 256 * 256;
+  bool direct = true;
+  for (int f = 0; f < n; ++f)
+    if (!frames[f].on_device || frames[f].step != frames[0].step || frames[f].step % 4) direct = false;
+  CHK(buf.ensure(gp * h * n + (direct ? 0 : cp * h * n) + sizeof(void*) * n));
+  char* gbase = buf.as<char>();
+  char* cbase = gbase + gp * h * n;
+  std::vector<const float*> src(n);
+  int spitch;
+  if (direct) {
+    for (int f = 0; f < n; ++f) src[f] = (const float*)frames[f].data;
+    spitch = (int)(frames[0].step / 4);
+  } else {
+    for (int f = 0; f < n; ++f) {
+      char* d = cbase + cp * h * f;
+      HIPCHK(hipMemcpy2DAsync(d, cp, frames[f].data, (size_t)frames[f].step, (size_t)w * 12, h,
+                              frames[f].on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, s));
+      src[f] = (const float*)d;
+    }
+    spitch = (int)(cp / 4);
+  }
+  const float** dsrc = (const float**)(gbase + gp * h * n + (direct ? 0 : cp * h * n));
+  HIPCHK(hipMemcpyAsync((void*)dsrc, src.data(), sizeof(void*) * n, hipMemcpyHostToDevice, s));
+  HIPCHK(launch_ecc_channel(dsrc, spitch, w, h, n, ch, (float*)gbase, (long long)(gp / 4) * h, (int)(gp / 4), s));
+  HIPCHK(hipStreamSynchronize(s));
+  views.assign(n, ssg_image{});
+  for (int f = 0; f < n; ++f) {
+    views[f].data = gbase + gp * h * f;
+    views[f].width = w;
+    views[f].height = h;
+    views[f].step = (int64_t)gp;
+    views[f].type = SSG_32FC1;
+    views[f].on_device = 1;
+  }
+  *out = views.data();
+  return SSG_OK;
+}
+
 int register_n(ssg_registration* r, int n, const ssg_image* frames, const ssg_image* masks, ssg_registration_status* st,
                bool plain) {
   if (!r->have_ref) return fail(SSG_E_NOT_READY, "setup_reference_frame() was not called");
   if (n <= 0) return SSG_OK;
   if (!frames || !st) return fail(SSG_E_INVALID_ARG, "null frames or status");
   hipStream_t s = r->stream;
+  std::vector<ssg_image> gv;
+  CHK(ecc_channel_views(r, n, frames, r->gray_buf, gv, &frames));
   bool has_cmask = false;
   CHK(prepare_batch(r, n, frames, masks, &has_cmask));
   HIPCHK(hipMemsetAsync(r->res.p, 0, sizeof(FrameRes) * n, s));
@@ -956,7 +1020,10 @@ int register_n(ssg_registration* r, int n, const ssg_image* frames, const ssg_im
 extern "C" int ssg_registration_setup_reference_frame(ssg_registration* r, const ssg_image* image, const ssg_image* mask) {
   if (!r) return fail(SSG_E_INVALID_ARG, "null handle");
   CHK(check_image(image, "reference image"));
-  if (image->type != SSG_32FC1) return fail(SSG_E_UNSUPPORTED, "reference image: SSG_32FC1 (gray) only");
+  DevBuf gref;
+  std::vector<ssg_image> gview;
+  CHK(ecc_channel_views(r, 1, image, gref, gview, &image));
+  if (image->type != SSG_32FC1) return fail(SSG_E_UNSUPPORTED, "reference image: SSG_32FC1 (gray) or SSG_32FC3 (BGR)");
   if (mask) {
     CHK(check_image(mask, "reference mask"));
     if (mask->type != SSG_8UC1 || mask->width != image->width || mask->height != image->height)
@@ -1100,7 +1167,17 @@ extern "C" int ssg_registration_register_frame(ssg_registration* r, const ssg_im
   r->have_status = false;
   CHK(register_n(r, 1, image, mask, status, true));
   r->last = *status;
-  // create_remap(reference size) then the eccflow refine (c_frame_registration.cc:883-917)
+  // create_remap(reference size) then the eccflow refine (c_frame_registration.cc:883-917) on the
+  // registration channel (register_n left it in gray_buf for colour frames)
+  ssg_image gv{};
+  if (image && image->type == SSG_32FC3) {
+    gv = *image;
+    gv.data = r->gray_buf.p;
+    gv.step = (int64_t)(((size_t)image->width * 4 + 255) / 256 * 256);
+    gv.type = SSG_32FC1;
+    gv.on_device = 1;
+    image = &gv;
+  }
   if (r->flow && status->ok) CHK(refine_current_remap(r, image, mask));
   r->have_status = true;
   return SSG_OK;
@@ -2010,8 +2087,11 @@ extern "C" int ssg_focus_stack_batch(int32_t nseq, ssg_accumulation* const* accs
     per[q] = sizeof(float) * ((size_t)a->cn * a->P + (size_t)a->P + 2 * n0);
     if (nframes[q] > 0) per_all += per[q];
   }
-  constexpr size_t kBudget = (size_t)8 << 30;
-  const int m = (int)std::max<size_t>(1, std::min<size_t>({(size_t)32, (size_t)maxn, kBudget / std::max<size_t>(per_all, 1)}));
+  // at most 24 GiB and a third of the free device memory: m frames per sequence and chunk (<= 32)
+  size_t free_b = 0, total_b = 0;
+  HIPCHK(hipMemGetInfo(&free_b, &total_b));
+  const size_t budget = std::min<size_t>((size_t)24 << 30, free_b / 3);
+  const int m = (int)std::max<size_t>(1, std::min<size_t>({(size_t)32, (size_t)maxn, budget / std::max<size_t>(per_all, 1)}));
   std::vector<size_t> soff(nseq + 1, 0);
   for (int q = 0; q < nseq; ++q) soff[q + 1] = soff[q] + (nframes[q] > 0 ? ((per[q] * m + 255) / 256) * 256 : 0);
   CHK(a0->fms.ensure(soff[nseq]));
@@ -3659,6 +3739,9 @@ extern "C" int ssg_registration_register_batch_remaps(ssg_registration* r, int n
         remaps[f].height != r->ref_h || remaps[f].step != remaps[0].step || remaps[f].step % 8)
       return fail(SSG_E_INVALID_ARG, "remap %d: device SSG_32FC2 maps of the reference size and one step", f);
   }
+  DevBuf gb;
+  std::vector<ssg_image> gv;
+  CHK(ecc_channel_views(r, n, frames, gb, gv, &frames));
   if (r->flow) {
     for (int f = 0; f < n; ++f) {
       if (!frames[f].on_device || frames[f].step != frames[0].step || frames[f].step % 4)

@@ -287,6 +287,8 @@ hipError_t launch_count_zero_u8(const uint8_t* src, long long fstride, int pitch
 hipError_t launch_sum_f32(const float* src, int pitch, int w, int h, double* out, hipStream_t s);
 hipError_t launch_sharpness(const float* const* src_ptrs, int pitch, int w, int h, int nframes, double* partial,
                             hipStream_t s);
+hipError_t launch_ecc_channel(const float* const* src, int pitch, int w, int h, int n, int channel, float* dst,
+                              long long dst_fs, int dst_pitch, hipStream_t s);
 hipError_t launch_sharpness_finish(const double* partial, int nb, int nframes, double area, float* out, hipStream_t s);
 // dst = src - up, 0 where masks[f] (if any) is not 255; src / masks: device arrays of per-frame pointers
 hipError_t launch_norm_sub(const float* const* src, int spitch, const float* up, long long ufs,

@@ -414,6 +414,25 @@ hipError_t launch_sharpness(const float* const* src_ptrs, int pitch, int w, int 
   return hipGetLastError();
 }
 
+// create_ecc_image's channel extraction of BGR f32 frames (extract_channel,
+// core/proc/extract_channel.cc:693-697): channel < 0 is cv::cvtColor(COLOR_BGR2GRAY) as
+// (b * 0.114f + g * 0.587f) + r * 0.299f, else cv::extractChannel(channel); blockIdx.z = frame
+__global__ void __launch_bounds__(256) k_ecc_channel(const float* const* __restrict__ src, int pitch, int w, int channel,
+                                                     float* __restrict__ dst, long long dst_fs, int dst_pitch) {
+  const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y, f = blockIdx.z;
+  if (x >= w) return;
+  const float* p = src[f] + (size_t)y * pitch + 3 * (size_t)x;
+  dst[f * dst_fs + (size_t)y * dst_pitch + x] =
+      channel < 0 ? (p[0] * 0.114f + p[1] * 0.587f) + p[2] * 0.299f : p[channel];
+}
+
+hipError_t launch_ecc_channel(const float* const* src, int pitch, int w, int h, int n, int channel, float* dst,
+                              long long dst_fs, int dst_pitch, hipStream_t s) {
+  hipLaunchKernelGGL(k_ecc_channel, dim3((w + 255) / 256, h, n), dim3(256), 0, s, src, pitch, w, channel, dst, dst_fs,
+                     dst_pitch);
+  return hipGetLastError();
+}
+
 // The frames' weights from the partials of k_sharpness: per frame the partials summed in
 // block order, then divided by the pixel count (the same double operations in the same order
 // as the host reduction of ssg_frame_sharpness).  One thread per frame.

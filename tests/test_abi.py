@@ -57,6 +57,7 @@ def test_abi_version_and_defaults(lib):
     assert o.ecc.ecc_method == _lib.ECC_ALIGN_LM and o.ecc.ecch_estimate_translation_first == 1
     assert (o.eccflow.support_scale, o.eccflow.max_iterations, o.eccflow.scale_factor) == (4, 3, 0.75)
     assert o.enable_feature_registration == 1 and o.enable_ecc_registration == 0
+    assert o.ecc_registration_channel == 104  # color_channel_gray (core/proc/extract_channel.h:31)
 
 
 def test_struct_layout_matches_header(tmp_path):

@@ -317,3 +317,41 @@ def test_singular_and_nan_frames_follow_the_oracle(ssg, oracle32, oracle64, moti
     d32 = synth_np.max_displacement(o32.register_frame(cases[3]).matrix, o.matrix, w, h)
     assert st[3].ok == o.ok == 1
     assert synth_np.max_displacement(st[3].matrix, o.matrix, w, h) <= warp_bar(d32)
+
+
+# ------------------------------------------------------------------------------------------------
+@pytest.mark.parametrize("channel,pick", [(104, None), (103, 0), (101, 2), (1, 1)])
+def test_colour_frames_register_on_their_channel(ssg, channel, pick):
+    """BGR f32 frames register on ecc_registration_channel (create_ecc_image's extract_channel,
+    core/proc/extract_channel.cc:693-697): gray = cv::cvtColor(COLOR_BGR2GRAY) as
+    (b * 0.114 + g * 0.587) + r * 0.299 in float, or one channel; the result equals registering
+    those gray images, bit for bit."""
+    S = ssg
+    ref, frames, _ = _register_colour_setup()
+    kw = dict(motion_type=3, scale=1.0, eps=0.02, ecch_max_level=2, max_iterations=30, ecc_registration_channel=channel)
+
+    def gray(im):
+        if pick is not None:
+            return np.ascontiguousarray(im[..., pick])
+        return ((im[..., 0] * np.float32(0.114) + im[..., 1] * np.float32(0.587)) +
+                im[..., 2] * np.float32(0.299)).astype(np.float32)
+
+    col = S.c_frame_registration(S.c_image_registration_options(**kw))
+    assert col.setup_reference_frame(gpu_tensor(ref))
+    st_c = col.register_batch([gpu_tensor(f) for f in frames])
+    one = S.c_frame_registration(S.c_image_registration_options(**kw))
+    assert one.setup_reference_frame(gray(ref))
+    st_g = one.register_batch([gpu_tensor(gray(f)) for f in frames])
+    for a, b in zip(st_c, st_g):
+        assert (a.ok, a.code, a.iterations) == (b.ok, b.code, b.iterations)
+        assert list(a.params) == list(b.params)
+    # the per-frame form with a host colour frame agrees too
+    assert col.register_frame(frames[0])
+    assert list(col.status().params) == list(st_g[0].params)
+
+
+def _register_colour_setup():
+    ref, frames, _ = synth_np.frames(3, 200, 150, radius=55, seed=12, shift=3.0, rot_deg=0.5, noise=0.002)
+    tint = np.array([0.8, 1.0, 1.15], np.float32)
+    return ((ref[..., None] * tint).astype(np.float32),
+            [(f[..., None] * tint).astype(np.float32) for f in frames], None)
