@@ -571,7 +571,7 @@ def bench_focus(a, S, _lib, torch):
     px = sum(x[0] * x[1] * x[2].shape[0] for x in seqs)
     print(f"[focus] synthesized {nf} frames in {time.time() - t0:.1f}s: " +
           ", ".join(f"{w}x{h}x{f.shape[0]}" for w, h, f, _ in seqs), file=sys.stderr, flush=True)
-    regs, views, stacks = [], [], []
+    regs, views = [], []
     for w, h, f, _ in seqs:
         o = S.c_image_registration_options(motion_type=S.MOTION_TRANSLATION, scale=1.0, eps=0.02, ecch_max_level=3,
                                            max_iterations=50)
@@ -579,47 +579,49 @@ def bench_focus(a, S, _lib, torch):
         assert r.setup_reference_frame(f[0])
         regs.append(r)
         views.append(_lib.images([S.image_view(f[k]) for k in range(f.shape[0])]))
+    stacks = [S.c_laplacian_pyramid_focus_stacking(w, h, 3, fusing_policy=2, lpg_k=6.0, lpg_p=2.0, lpg_sigma=5.0,
+                                                   max_levels=6) for w, h, _, _ in seqs]
+    outs = [torch.empty((h, w, 3), device=dev) for w, h, _, _ in seqs]
+    # the sequences register concurrently: one host thread per sequence, each handle on its own HIP
+    # stream (the library calls release the interpreter lock)
+    from concurrent.futures import ThreadPoolExecutor
+    pool = ThreadPoolExecutor(max_workers=len(seqs))
+
+    def register(q):
+        st = regs[q].register_batch(views[q])
+        return np.array([[x.params[0], x.params[1]] for x in st], np.float32)
+
+    times = {"reg": 0.0, "stack": 0.0}
 
     def step():
-        shifts = []
-        for r, v in zip(regs, views):
-            st = r.register_batch(v)
-            shifts.append(np.array([[s.params[0], s.params[1]] for s in st], np.float32))
-        stacks.clear()
-        for w, h, f, _ in seqs:
-            stacks.append(S.c_laplacian_pyramid_focus_stacking(w, h, 3, fusing_policy=2, lpg_k=6.0, lpg_p=2.0,
-                                                               lpg_sigma=5.0, max_levels=6))
+        ts = time.perf_counter()
+        shifts = list(pool.map(register, range(len(seqs))))
+        t1 = time.perf_counter()
+        for st in stacks:
+            st.clear()
         S.focus_stack_batch(stacks, views, shifts)
-        outs = [st.compute(torch.empty((h, w, 3), device=dev)) for st, (w, h, _, _) in zip(stacks, seqs)]
-        return shifts, outs
+        for st, o in zip(stacks, outs):
+            st.compute(o)
+        torch.cuda.synchronize()
+        times["reg"] += t1 - ts
+        times["stack"] += time.perf_counter() - t1
+        return shifts
 
     for _ in range(a.warmup):
         step()
     torch.cuda.synchronize()
-    # profile the batched focus stack of one step (the library's HIP events on its stream)
-    prof_on = []
-    t_reg = 0.0
-    t_stack = 0.0
+    times["reg"] = times["stack"] = 0.0
     t0 = time.perf_counter()
     for it in range(a.steps):
-        ts = time.perf_counter()
-        shifts = []
-        for r, v in zip(regs, views):
-            st = r.register_batch(v)
-            shifts.append(np.array([[s.params[0], s.params[1]] for s in st], np.float32))
-        torch.cuda.synchronize()
-        t_reg += time.perf_counter() - ts
-        ts = time.perf_counter()
-        stacks = [S.c_laplacian_pyramid_focus_stacking(w, h, 3, fusing_policy=2, lpg_k=6.0, lpg_p=2.0, lpg_sigma=5.0,
-                                                       max_levels=6) for w, h, _, _ in seqs]
         if it == a.steps - 1:
             lib.ssg_accumulation_set_profiling(stacks[0]._h, 1)
-            prof_on = stacks[0]
-        S.focus_stack_batch(stacks, views, shifts)
-        outs = [st.compute(torch.empty((h, w, 3), device=dev)) for st, (w, h, _, _) in zip(stacks, seqs)]
-        torch.cuda.synchronize()
-        t_stack += time.perf_counter() - ts
+            prof_reset = _lib.Profile()
+            lib.ssg_accumulation_get_profile(stacks[0]._h, C.byref(prof_reset), 1)
+        shifts = step()
     elapsed = time.perf_counter() - t0
+    pool.shutdown()
+    t_reg, t_stack = times["reg"], times["stack"]
+    prof_on = stacks[0]
     prof = _lib.Profile()
     lib.ssg_accumulation_get_profile(prof_on._h, C.byref(prof), 1)
     fps = nf * a.steps / elapsed

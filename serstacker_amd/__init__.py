@@ -223,6 +223,10 @@ class _accumulator_state:
     def load_state(self, state: bytes):
         _lib.check(self._lib.ssg_accumulation_state_load(self._h, state, len(state)), "state_load")
 
+    def clear(self):
+        """c_frame_accumulation::clear (c_frame_accumulation.h:30)."""
+        _lib.check(self._lib.ssg_accumulation_clear(self._h), "clear")
+
     def reinitialize(self, src, accw) -> bool:
         """c_frame_accumulation::reinitialize(src, accw) (c_frame_accumulation.h:25): a weighted
         average restarts from the mean `src` with weights `accw` (c_frame_accumulation.cc:171-180);

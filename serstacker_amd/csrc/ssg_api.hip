@@ -1576,10 +1576,12 @@ int focus_prepare(ssg_accumulation* a) {
   CHK(a->facc.ensure(sizeof(float) * cn * a->P));
   CHK(a->fp.ensure(sizeof(float) * cn * a->P));
   CHK(a->fup.ensure(sizeof(float) * cn * n0));
-  CHK(a->ft0.ensure(sizeof(float) * cn * n0));
-  CHK(a->ft1.ensure(sizeof(float) * cn * n0));
-  CHK(a->ft2.ensure(sizeof(float) * cn * n0));
-  CHK(a->ft3.ensure(sizeof(float) * cn * n0));
+  if (o.fusing_policy != SSG_FOCUS_LPG_WEIGHTS) {  // energy scratch of the reference's own policies
+    CHK(a->ft0.ensure(sizeof(float) * cn * n0));
+    CHK(a->ft1.ensure(sizeof(float) * cn * n0));
+    CHK(a->ft2.ensure(sizeof(float) * cn * n0));
+    CHK(a->ft3.ensure(sizeof(float) * cn * n0));
+  }
   CHK(a->fstd.ensure(sizeof(double) * 2 * cn));
   CHK(a->fptrs.ensure(sizeof(float*) * 256));
   a->hptrs.assign(256, nullptr);
@@ -2156,6 +2158,11 @@ extern "C" int ssg_focus_stack_batch(int32_t nseq, ssg_accumulation* const* accs
   HIPCHK(hipStreamSynchronize(s));
   int maxL = 0;
   for (int q = 0; q < nseq; ++q) maxL = std::max(maxL, accs[q]->L);
+  // one smoothing kernel for every sequence: the blur runs its register-blocked form
+  int blur_r = accs[0]->Wk.nx / 2;
+  for (int q = 1; q < nseq; ++q)
+    if (memcmp(&accs[q]->Wk, &accs[0]->Wk, sizeof(SepKernel)) != 0) blur_r = 0;
+  if (accs[0]->Wk.tx != 1 || accs[0]->Wk.ty != 1) blur_r = 0;
   auto& prof = a0->prof;
   for (int c = 0; c < nchunks; ++c) {
     const FmSeq* hs = tab + (size_t)c * nseq;
@@ -2172,7 +2179,7 @@ extern "C" int ssg_focus_stack_batch(int32_t nseq, ssg_accumulation* const* accs
       if (J.n == 0) return SSG_OK;
       if (end > INT32_MAX) return fail(SSG_E_UNSUPPORTED, "step of %lld blocks", end);
       prof.start(s);
-      HIPCHK(launch_fm_step(kind, J, ds, s));
+      HIPCHK(launch_fm_step(kind, J, ds, blur_r, s));
       prof.stop(s, prof_kind);
       return SSG_OK;
     };

@@ -506,7 +506,8 @@ struct FmJobs {
 };
 // blocks of one job (kind, level) of sequence q
 long long fm_job_blocks(const FmSeq& q, int kind, int level);
-// step `kind` (FM_PREP, FM_LPG, FM_BLUR, FM_DOWN_*, FM_FUSE/FM_TOP) of every job in J; seqs: device table
-hipError_t launch_fm_step(int kind, const FmJobs& J, const FmSeq* seqs, hipStream_t s);
+// step `kind` (FM_PREP, FM_LPG, FM_BLUR, FM_DOWN_*, FM_FUSE/FM_TOP) of every job in J; seqs: device table;
+// blur_radius: the smoothing radius shared by every job of an FM_BLUR launch (0: they differ)
+hipError_t launch_fm_step(int kind, const FmJobs& J, const FmSeq* seqs, int blur_radius, hipStream_t s);
 
 }  // namespace ssg
