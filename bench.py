@@ -11,23 +11,28 @@ pass, rho gates at min_rho 0.8.  Stacking: fused remap (bilinear,
 REFLECT101) + base_remap mask + weighted average with the per-frame
 sharpness weight, FP64 final normalisation.
 
-One step = weights + batched registration + fused warp/accumulate + compute
-of all frames, with the frames resident in HBM before the timed region.
+One step = ssg_stack_batch (weights of the unaligned frames, batched registration and the fused
+warp/accumulate, every per-frame decision on the device, replayed as one captured hipGraph) +
+compute of the stack, with the frames resident in HBM before the timed region.
 
 Also reported:
   * roofline of the dominant kernel (the kernel family with the largest
     summed time, the translation-first ECC iteration pass on this workload):
     algorithmic bytes / summed kernel time from HIP events on the library's
-    stream, against the 8 TB/s HBM peak; `traffic` is the measured HBM bytes
+    stream, over the same steps run eagerly right after the timed graph
+    replays (a replayed graph times as one unit); `traffic` is the measured HBM bytes
     per launch of that kernel (rocprofv3 FETCH_SIZE + WRITE_SIZE, scaled by
     the same counters on a copy of known size), read from
     profiles/pmc_traffic.json when it names the same kernel;
   * plain per-frame GPU form (one frame per launch sequence, host sync per ECC
     iteration), frames/s on a sample of frames;
   * CPU baseline: the FP32 oracle (plain C + OpenMP restatement of the
-    reference algorithm) on a bounded sample (frames until about 12 s of CPU
-    work), frames/s;
-  * warp error vs the FP64 oracle on a few frames (px).
+    reference algorithm) built on the host with -O3 -march=native -ffast-math (the
+    reference's own flags), all the process's cores, on the first 100 frames (bounded
+    to about 40 s); the same without -ffast-math on the parity frames;
+  * parity of the timed batch on its first 16 frames: warps vs the FP64 oracle (and the
+    FP32 oracle's own distance), the stack of those frames vs the FP32 oracle with
+    bit-exact per-pixel counts.
 """
 from __future__ import annotations
 
@@ -66,10 +71,13 @@ def main():
     ap.add_argument("--width", type=int, default=1920)
     ap.add_argument("--height", type=int, default=1080)
     ap.add_argument("--plain-frames", type=int, default=24)
-    ap.add_argument("--cpu-frames", type=int, default=64, help="upper bound of the CPU sample")
-    ap.add_argument("--cpu-seconds", type=float, default=12.0, help="the CPU sample stops after this much time")
-    ap.add_argument("--oracle-frames", type=int, default=2)
+    ap.add_argument("--cpu-frames", type=int, default=100, help="upper bound of the CPU sample")
+    ap.add_argument("--cpu-seconds", type=float, default=40.0, help="the CPU sample stops after this much time")
+    ap.add_argument("--oracle-frames", type=int, default=16,
+                    help="frames checked against the FP64 oracle (warps) and the FP32 oracle (stack, counts)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-graph", action="store_true",
+                    help="run the batched steps eagerly instead of replaying the captured hipGraph")
     ap.add_argument("--workload", choices=["affine", "turbulence", "focus", "sweep"], default="affine",
                     help="affine: batched affine ECC + weighted stack of 1080p frames (the headline metric); "
                          "turbulence: phase correlation + eccflow + bicubic remap + average of 1024x1024 frames; "
@@ -126,38 +134,43 @@ def main():
     acc = S.c_weigthed_average(W, H, 1)
     views = _lib.images([S.image_view(f) for f in frames])
 
-    def step():
-        wts = S.frame_sharpness(views)
-        st = reg.register_batch(views)
+    def step(graph=not a.no_graph):
+        # weights + registration + fused warp/accumulate, every per-frame decision on the device
+        # (ssg_stack_batch; with a graph: one replay of the captured launch sequence), then compute
         acc.clear()
-        acc.add_registered_batch(views, st, S.MOTION_AFFINE, frame_weights=wts)
+        st = reg.stack_batch(acc, views, use_graph=graph)
         avg, mask = acc.compute()
         return st, avg
 
     for _ in range(a.warmup):
         step()
     torch.cuda.synchronize()
-    lib.ssg_registration_set_profiling(reg._h, 1)
-    lib.ssg_accumulation_set_profiling(acc._h, 1)
-    prof_r, prof_a = _lib.Profile(), _lib.Profile()
-    lib.ssg_registration_get_profile(reg._h, C.byref(prof_r), 1)
-    lib.ssg_accumulation_get_profile(acc._h, C.byref(prof_a), 1)
-    torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(a.steps):
         st, avg = step()
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
-    lib.ssg_registration_get_profile(reg._h, C.byref(prof_r), 1)
-    lib.ssg_accumulation_get_profile(acc._h, C.byref(prof_a), 1)
-    lib.ssg_registration_set_profiling(reg._h, 0)
-    lib.ssg_accumulation_set_profiling(acc._h, 0)
     ms_step = elapsed * 1e3 / a.steps
     fps = n * a.steps / elapsed
+    # the same steps run eagerly with every kernel family bracketed by HIP events on the library's
+    # stream (a replayed graph times as one unit): the per-kernel roofline figures
+    lib.ssg_registration_set_profiling(reg._h, 1)
+    prof_r, prof_a = _lib.Profile(), _lib.Profile()
+    lib.ssg_registration_get_profile(reg._h, C.byref(prof_r), 1)
+    torch.cuda.synchronize()
+    te = time.perf_counter()
+    for _ in range(a.steps):
+        st, avg = step(graph=False)
+    torch.cuda.synchronize()
+    eager_ms = (time.perf_counter() - te) * 1e3 / a.steps
+    lib.ssg_registration_get_profile(reg._h, C.byref(prof_r), 1)
+    lib.ssg_registration_set_profiling(reg._h, 0)
+    prof_a = prof_r  # the stack batch accounts its accumulate launches on the registration stream
     nok = int(sum(s.ok for s in st))
     its = np.array([s.iterations for s in st])
     its_t = np.array([s.iterations_translation for s in st])
-    print(f"[bench] batched: {fps:.1f} frames/s ({ms_step:.1f} ms/step), {nok}/{n} registered, "
+    print(f"[bench] batched ({'eager' if a.no_graph else 'hipGraph replay'}): {fps:.1f} frames/s ({ms_step:.1f} "
+          f"ms/step; profiled eager steps {eager_ms:.1f} ms), {nok}/{n} registered, "
           f"main iterations mean {its.mean():.1f} min {its.min()} max {its.max()}, translation-first "
           f"mean {its_t.mean():.1f} max {its_t.max()}", file=sys.stderr, flush=True)
 
@@ -211,57 +224,122 @@ def main():
         plain_fps = npl / (time.perf_counter() - t0)
         print(f"[bench] plain per-frame GPU: {plain_fps:.1f} frames/s on {npl} frames", file=sys.stderr, flush=True)
 
-    # ---- CPU baseline and FP64 warp error (oracle; checker only)
-    cpu = None
-    warp_err = None
+    # ---- CPU baseline and parity of the timed batch (the oracle is the checker here, never the thing measured
+    # on the GPU side)
+    cpu, cpu_plain, warp, parity = None, None, None, None
     if not a.no_cpu and a.cpu_frames > 0:
-        from oracle.oracle import Oracle, reg_options
+        import tempfile
 
-        threads = min(16, os.cpu_count() or 1)
+        from oracle.oracle import Oracle, build_native, reg_options
+
+        threads = len(os.sched_getaffinity(0))
+        cpu_model = "unknown"
+        try:
+            with open("/proc/cpuinfo") as fh:
+                cpu_model = next(ln.split(":", 1)[1].strip() for ln in fh if ln.startswith("model name"))
+        except (OSError, StopIteration):
+            pass
+        libs = build_native(os.path.join(tempfile.gettempdir(), "ssg_oracle_native"))
         kw = dict(motion_type=3, scale=1.0, eps=0.02, ecch_max_level=4, max_iterations=50, min_rho=0.8,
                   ecc_method=ECC_METHODS[a.ecc_method])
-        o32 = Oracle(32)
-        o32.set_threads(threads)
         ref_h = ref.cpu().numpy()
-        oreg = o32.registration(reg_options(**kw))
-        oreg.setup_reference_frame(ref_h)
-        A = np.zeros((H, W), np.float32)
-        Wt = np.zeros((H, W), np.float32)
-        done = 0
-        cpu_s = 0.0
-        for i in range(min(a.cpu_frames, n)):
-            f = frames[i].cpu().numpy()
-            t0 = time.perf_counter()
-            wgt = o32.sharpness(f)
-            ost = oreg.register_frame(f)
-            t = o32.xform(3, list(ost.params))
-            rmap = o32.create_map(t, W, H)
-            d, m = o32.base_remap(f, rmap, interp=1, border=4)
-            wimg = np.full((H, W), np.float32(wgt))
-            wr, _ = o32.base_remap(wimg, rmap, interp=1, border=0, want_mask=False)
-            o32.weighted_average_update(d, A, Wt, weights=(wr * (m / np.float32(255))).astype(np.float32))
-            cpu_s += time.perf_counter() - t0
-            done += 1
-            if cpu_s >= a.cpu_seconds:
-                break
-        cpu = {"value": round(done / cpu_s, 4), "unit": "frames/s", "cores": threads, "kind": "port",
-               "sample": f"{done} of the {n} frames ({W}x{H}, {cpu_s:.1f} s): sharpness, registration (translation-first + "
-                         f"affine ECC, 4 levels), base_remap and weighted-average update, FP32 oracle, "
-                         f"{threads} OpenMP threads"}
-        print(f"[bench] cpu oracle: {cpu['value']} frames/s ({threads} threads)", file=sys.stderr, flush=True)
-        if a.oracle_frames > 0:
+
+        def run_cpu(library, max_frames, max_s):
+            """register + remap + weighted-average update of frames 0.. until max_frames or max_s of CPU time"""
+            o32 = Oracle(32, library=library)
+            o32.set_threads(threads)
+            oreg = o32.registration(reg_options(**kw))
+            oreg.setup_reference_frame(ref_h)
+            A = np.zeros((H, W), np.float32)
+            Wt = np.zeros((H, W), np.float32)
+            done, cpu_s, stats = 0, 0.0, []
+            for i in range(min(max_frames, n)):
+                f = frames[i].cpu().numpy()
+                t0 = time.perf_counter()
+                wgt = o32.sharpness(f)
+                ost = oreg.register_frame(f)
+                t = o32.xform(3, list(ost.params))
+                rmap = o32.create_map(t, W, H)
+                d, m = o32.base_remap(f, rmap, interp=1, border=4)
+                wimg = np.full((H, W), np.float32(wgt))
+                wr, _ = o32.base_remap(wimg, rmap, interp=1, border=0, want_mask=False)
+                o32.weighted_average_update(d, A, Wt, weights=(wr * (m / np.float32(255))).astype(np.float32))
+                cpu_s += time.perf_counter() - t0
+                done += 1
+                stats.append(np.array(ost.matrix[:]))
+                if cpu_s >= max_s:
+                    break
+            return done, cpu_s, stats
+
+        k = min(a.oracle_frames, n)
+        if k > 0:
+            # the reference-faithful FP32 build (contraction off; also timed, on these frames) and the FP64 oracle
+            done32, s32, m32 = run_cpu(libs["native"], k, 1e9)
+            cpu_plain = {"value": round(done32 / s32, 4), "unit": "frames/s", "cores": threads,
+                         "sample": f"{done32} frames, -O3 -march=native without -ffast-math"}
             o64 = Oracle(64)
             o64.set_threads(threads)
             r64 = o64.registration(reg_options(**kw))
             r64.setup_reference_frame(ref_h)
-            errs = []
-            for i in range(min(a.oracle_frames, n)):
+            c = np.array([[0, 0, 1], [W - 1, 0, 1], [0, H - 1, 1], [W - 1, H - 1, 1]], np.float64).T
+            disp = lambda m1, m2: float(np.sqrt((((np.asarray(m1) - np.asarray(m2)).reshape(2, 3) @ c) ** 2).sum(0)).max())
+            eg, eo = [], []
+            for i in range(k):
                 o = r64.register_frame(frames[i].cpu().numpy())
-                d = (np.array(st[i].matrix[:]) - np.array(o.matrix[:])).reshape(2, 3)
-                c = np.array([[0, 0, 1], [W - 1, 0, 1], [0, H - 1, 1], [W - 1, H - 1, 1]], np.float64).T
-                errs.append(float(np.sqrt(((d @ c) ** 2).sum(0)).max()))
-            warp_err = max(errs)
-            print(f"[bench] warp error vs FP64 oracle: {warp_err:.5f} px", file=sys.stderr, flush=True)
+                eg.append(disp(st[i].matrix[:], o.matrix[:]))
+                eo.append(disp(m32[i], o.matrix[:]))
+            warp = {"frames": k, "gpu_vs_fp64_max_px": round(max(eg), 5), "gpu_vs_fp64_median_px": round(float(np.median(eg)), 5),
+                    "within_0.01px": int(sum(e <= 0.01 for e in eg)),
+                    "fp32_oracle_vs_fp64_max_px": round(max(eo), 5),
+                    "fp32_oracle_vs_fp64_median_px": round(float(np.median(eo)), 5)}
+            print(f"[bench] warps vs FP64 over {k} frames: GPU max {max(eg):.5f} median {np.median(eg):.5f} px; "
+                  f"FP32 oracle max {max(eo):.5f} median {np.median(eo):.5f} px", file=sys.stderr, flush=True)
+            # the stack of those frames with the batch's own parameters and weights: counts bit-exact and the
+            # average against the FP32 incremental mean and the FP64 sums
+            wts = S.frame_sharpness(frames[:k])
+            sub = (_lib.RegistrationStatus * k)(*st[:k])
+            acck = S.c_weigthed_average(W, H, 1)
+            acck.add_registered_batch(frames[:k], sub, S.MOTION_AFFINE, frame_weights=wts)
+            gavg, _ = acck.compute()
+            ora = Oracle(32)
+            ora.set_threads(threads)
+            A32 = np.zeros((H, W), np.float32)
+            W32 = np.zeros((H, W), np.float32)
+            Ssum = np.zeros((H, W))
+            Wsum = np.zeros((H, W))
+            cnt = np.zeros((H, W), np.int64)
+            for i in range(k):
+                if not st[i].ok:
+                    continue
+                f = frames[i].cpu().numpy()
+                rmap = ora.create_map(ora.xform(3, list(st[i].params)), W, H)
+                d, m = ora.base_remap(f, rmap, interp=1, border=4)
+                wr, _ = ora.base_remap(np.full((H, W), np.float32(wts[i])), rmap, interp=1, border=0, want_mask=False)
+                ora.weighted_average_update(d, A32, W32, weights=(wr * (m / np.float32(255))).astype(np.float32))
+                mb = m > 0
+                Ssum += np.where(mb, np.float64(np.float32(wts[i])) * d, 0)
+                Wsum += np.where(mb, np.float64(np.float32(wts[i])), 0)
+                cnt += mb
+            rel = lambda x, y: float(np.linalg.norm(np.float64(x) - np.float64(y)) / max(np.linalg.norm(np.float64(y)), 1e-300))
+            f64avg = np.where(Wsum > 0, Ssum / np.maximum(Wsum, 1e-300), 0)
+            parity = {"frames": k, "counts_bitexact": bool(np.array_equal(acck.counts(), cnt)),
+                      "accumulated_frames_match": acck.accumulated_frames() == int(sum(s.ok for s in st[:k])),
+                      "stack_rel_l2_vs_fp32_oracle": rel(gavg, A32), "stack_rel_l2_vs_fp64_sums": rel(gavg, f64avg)}
+            print(f"[bench] stack of {k} frames: counts bit-exact {parity['counts_bitexact']}, rel-L2 vs FP32 oracle "
+                  f"{parity['stack_rel_l2_vs_fp32_oracle']:.2e}, vs FP64 {parity['stack_rel_l2_vs_fp64_sums']:.2e}",
+                  file=sys.stderr, flush=True)
+
+        # last: loading a -ffast-math library sets flush-to-zero for the process, so the checks above
+        # run before it
+        done, cpu_s, _ = run_cpu(libs["native_fastmath"], a.cpu_frames, a.cpu_seconds)
+        per_frame = cpu_s / done
+        cpu = {"value": round(done / cpu_s, 4), "unit": "frames/s", "cores": threads, "kind": "port",
+               "sample": f"{done} of the {n} frames ({W}x{H}, {cpu_s:.1f} s; the whole batch would take "
+                         f"{per_frame * n:.0f} s): sharpness, registration (translation-first + affine ECC, 4 levels), "
+                         f"base_remap and weighted-average update; the FP32 oracle built -O3 -march=native "
+                         f"-ffast-math (the reference's flags) on this host, {threads} OpenMP threads, {cpu_model}"}
+        print(f"[bench] cpu oracle (native, fast-math): {cpu['value']} frames/s, {threads} threads, {cpu_model}",
+              file=sys.stderr, flush=True)
 
     out = {
         "metric": "aligned+stacked frames/s at 1080p (batched vs plain GPU vs CPU), % HBM roofline",
@@ -284,12 +362,16 @@ def main():
                      "unit": "GB/s", "frac": round(dom["frac"], 4), "traffic": traffic,
                      "avg_launch_us": round(dom["avg_us"], 2), "launches": dom["launches"]},
         "cpu_baseline": cpu,
+        "graph": not a.no_graph,
+        "eager_profiled_ms_per_step": round(eager_ms, 3),
         "plain_gpu": {"value": round(plain_fps, 2), "unit": "frames/s", "frames": npl} if plain_fps else None,
         "speedup_vs_plain": round(fps / plain_fps, 2) if plain_fps else None,
         "speedup_vs_cpu": round(fps / cpu["value"], 1) if cpu else None,
         "registered": nok,
         "main_iterations": {"mean": round(float(its.mean()), 2), "min": int(its.min()), "max": int(its.max())},
-        "warp_err_vs_fp64_px": warp_err,
+        "cpu_baseline_no_fastmath": cpu_plain,
+        "warp_vs_fp64": warp,
+        "parity": parity,
         "kernels": {k: {kk: (round(vv, 3) if isinstance(vv, float) else vv) for kk, vv in v.items()}
                     for k, v in kernels.items()},
     }

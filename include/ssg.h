@@ -340,6 +340,21 @@ int ssg_accumulation_add_registered_batch(ssg_accumulation *a, int n, const ssg_
                                           const float *frame_weights, const ssg_image *weight_maps,
                                           int32_t interpolation, int32_t border_mode, double border_value);
 
+/* The per-frame pipeline of a batch with every per-frame decision on the device (no reference
+ * counterpart: c_image_stacking_pipeline.cc:1466-1790 runs it frame by frame): the weight of each
+ * unaligned frame (the ssg_frame_sharpness measure), the registration of register_batch, and the
+ * fused warp + mask + weighted accumulate (add_registered_batch with those weights) of the frames
+ * that pass the rho gates -- with no host round trip between them: each pyramid level queues its
+ * whole pass budget and the per-frame convergence mask ends every frame's iterations on the
+ * device.  use_graph != 0 captures that fixed launch sequence into a hipGraph on the first call
+ * and replays it while the handles, frame count and frame buffers stay the same (one graph
+ * launch per batch).  frames: SSG_32FC1 device frames of the reference size and one step;
+ * `a`: a 1-channel weighted-average accumulator of the reference size.  Statuses as
+ * register_batch.  Not with masks, normalization_scale > 0 or the eccflow refine. */
+int ssg_stack_batch(ssg_registration *r, ssg_accumulation *a, int32_t n, const ssg_image *frames,
+                    int32_t interpolation, int32_t border_mode, double border_value, int32_t use_graph,
+                    ssg_registration_status *status);
+
 /* The same fused remap -> mask -> weight -> accumulate with a dense remap field per frame
  * instead of a transform: the frames warped by c_frame_registration::remap with the
  * eccflow-refined current remap (c_frame_registration.cc:1405-1417), then accumulated
@@ -479,7 +494,7 @@ int ssg_eccflow_compute_batch(ssg_eccflow *f, int n, const ssg_image *inputs, co
 enum { SSG_PROF_ECC_TRANSLATION = 0, SSG_PROF_ECC_MAIN = 1, SSG_PROF_CORRELATION = 2, SSG_PROF_PYRAMID = 3,
        SSG_PROF_ACCUMULATE = 4, SSG_PROF_ECC_SOLVE = 5, SSG_PROF_FFT = 6, SSG_PROF_ECCFLOW = 7,
        SSG_PROF_FOCUS_PREP = 8, SSG_PROF_FOCUS_BLUR = 9, SSG_PROF_FOCUS_DOWN = 10, SSG_PROF_FOCUS_FUSE = 11,
-       SSG_PROF_COUNT = 12 };
+       SSG_PROF_GRAPH = 12, SSG_PROF_COUNT = 13 };
 typedef struct ssg_profile {
   int64_t launches[SSG_PROF_COUNT];
   double ms[SSG_PROF_COUNT];     /* summed kernel durations */

@@ -106,13 +106,24 @@ def _load(bits: int):
     return lib
 
 
+def build_native(out_dir: str) -> dict:
+    """The host-tuned FP32 builds (-march=native, with and without -ffast-math) into out_dir, a
+    directory of the machine that runs them; returns {variant: library path}."""
+    subprocess.run(["make", "-s", "-C", HERE, "native", f"NATIVE_DIR={out_dir}"], check=True)
+    return {"native": os.path.join(out_dir, "liboracle_f32_native.so"),
+            "native_fastmath": os.path.join(out_dir, "liboracle_f32_native_fastmath.so")}
+
+
 class Oracle:
-    def __init__(self, bits: int = 64):
+    def __init__(self, bits: int = 64, library: str | None = None):
+        """bits: 32 (reference-faithful FP32) or 64; library: an FP32 build of the same source to
+        load instead (a host-tuned variant from build_native, for timing)."""
         if bits not in (32, 64):
             raise ValueError("bits must be 32 or 64")
-        if bits not in _libs:
-            _libs[bits] = _load(bits)
-        self.lib = _libs[bits]
+        key = library or bits
+        if key not in _libs:
+            _libs[key] = C.CDLL(library, mode=C.RTLD_LOCAL) if library else _load(bits)
+        self.lib = _libs[key]
         self.bits = bits
         self.dt = np.float64 if bits == 64 else np.float32
         rp = C.POINTER(C.c_double if bits == 64 else C.c_float)

@@ -141,6 +141,20 @@ class c_frame_registration:
             raise SsgError(self.last_error)
         return st
 
+    def stack_batch(self, acc, frames, interpolation=_lib.INTER_LINEAR, border_mode=_lib.BORDER_REFLECT101,
+                    border_value=0.0, use_graph=True):
+        """ssg_stack_batch: weights, registration and the fused warp + weighted accumulate of a
+        batch of device frames into `acc` (a c_weigthed_average), every per-frame decision on the
+        device; use_graph replays a captured hipGraph of the launch sequence.  Returns the statuses."""
+        views = image_views(frames)
+        st = (_lib.RegistrationStatus * len(views))()
+        rc = self._lib.ssg_stack_batch(self._h, acc._h, len(views), views, interpolation, border_mode, border_value,
+                                       int(bool(use_graph)), st)
+        if rc:
+            self._fail(rc, "stack_batch")
+            raise SsgError(self.last_error)
+        return st
+
     def register_batch_remaps(self, frames, remaps, masks=None):
         """Batched registration that also writes each frame's remap (device tensors
         (H, W, 2)), refined by the dense ECC flow when eccflow registration is enabled."""

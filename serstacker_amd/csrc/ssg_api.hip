@@ -450,7 +450,12 @@ extern "C" int ssg_registration_create(const ssg_image_registration_options* opt
   return SSG_OK;
 }
 
+namespace {
+void drop_stack_state(const ssg_registration* r);  // ssg_stack_api.inc
+}
+
 extern "C" void ssg_registration_destroy(ssg_registration* r) {
+  if (r) drop_stack_state(r);
   if (!r) return;
   if (r->stream) (void)hipStreamSynchronize(r->stream);
   if (r->flow) ssg_eccflow_destroy(r->flow);
@@ -562,7 +567,7 @@ int ensure_workspace(ssg_registration* r, int n) {
 // One ECC pass (translation-first or main) over all levels for n frames.
 // plain: the host waits for each iteration and stops when no frame is active,
 // the per-frame form of the reference loop.
-int run_ecc(ssg_registration* r, int n, int pass, bool has_cmask, bool plain) {
+int run_ecc(ssg_registration* r, int n, int pass, bool has_cmask, bool plain, bool poll = true) {
   hipStream_t s = r->stream;
   const int motion = pass == 0 ? SSG_MOTION_TRANSLATION : r->o.motion_type;
   const int method = r->o.ecc.ecc_method;
@@ -656,7 +661,9 @@ int run_ecc(ssg_registration* r, int n, int pass, bool has_cmask, bool plain) {
       if (method == ECC_FA) HIPCHK(launch_ecc_solve(motion, a, s));
       else HIPCHK(launch_lm_solve(motion, a, s));
       r->prof.stop(s, SSG_PROF_ECC_SOLVE);
-      const bool check = plain || ((it & 3) == 3);
+      // device-driven batches (poll == false) never wait on the host: passes over an empty active
+      // list return at once, so the level's full pass budget is queued without a round trip
+      const bool check = poll && (plain || ((it & 3) == 3));
       if (check && it + 1 < max_passes) {
         HIPCHK(hipMemcpyAsync(r->pinned_count, a.active_count_out, sizeof(int), hipMemcpyDeviceToHost, s));
         HIPCHK(hipStreamSynchronize(s));
@@ -945,6 +952,10 @@ int ecc_channel_views(ssg_registration* r, int n, const ssg_image* frames, DevBu
   return SSG_OK;
 }
 
+void fill_statuses(ssg_registration* r, int n, const FrameXf* xf, const FrameRes* res, ssg_registration_status* st);
+
+void account_ecc_bytes(ssg_registration* r, int n, const FrameRes* res, bool has_cmask);
+
 int register_n(ssg_registration* r, int n, const ssg_image* frames, const ssg_image* masks, ssg_registration_status* st,
                bool plain) {
   if (!r->have_ref) return fail(SSG_E_NOT_READY, "setup_reference_frame() was not called");
@@ -955,7 +966,7 @@ int register_n(ssg_registration* r, int n, const ssg_image* frames, const ssg_im
   CHK(ecc_channel_views(r, n, frames, r->gray_buf, gv, &frames));
   bool has_cmask = false;
   CHK(prepare_batch(r, n, frames, masks, &has_cmask));
-  HIPCHK(hipMemsetAsync(r->res.p, 0, sizeof(FrameRes) * n, s));
+  HIPCHK(launch_zero(r->res.p, sizeof(FrameRes) * n, s));
   const bool tfirst = r->o.motion_type != SSG_MOTION_TRANSLATION && r->o.ecc.ecch_estimate_translation_first &&
                       r->o.ecc.ecch_max_level != 0;
   if (tfirst) {
@@ -970,9 +981,20 @@ int register_n(ssg_registration* r, int n, const ssg_image* frames, const ssg_im
   HIPCHK(hipMemcpyAsync(res.data(), r->res.p, sizeof(FrameRes) * n, hipMemcpyDeviceToHost, s));
   HIPCHK(hipStreamSynchronize(s));
   if (r->prof.on) {
-    // algorithmic bytes of the ECC passes: every iteration of a frame at level l
-    // reads the warped current level and the reference level once (4 + 4 B/px,
-    // +1 B/px per mask); correlation passes read both level-0 images
+    account_ecc_bytes(r, n, res.data(), has_cmask);
+    r->prof.collect();
+  }
+  fill_statuses(r, n, xf.data(), res.data(), st);
+  return SSG_OK;
+}
+
+// algorithmic bytes of the ECC passes of a batch, from its per-frame records
+void account_ecc_bytes(ssg_registration* r, int n, const FrameRes* res, bool has_cmask) {
+  const bool tfirst = r->o.motion_type != SSG_MOTION_TRANSLATION && r->o.ecc.ecch_estimate_translation_first &&
+                      r->o.ecc.ecch_max_level != 0;
+  {
+    // every iteration of a frame at level l reads the warped current level and the reference
+    // level once (4 + 4 B/px, +1 B/px per mask); correlation passes read both level-0 images
     const int maxit = r->o.ecc.max_iterations;
     for (int f = 0; f < n; ++f) {
       for (int p = 0; p < 2; ++p) {
@@ -987,8 +1009,13 @@ int register_n(ssg_registration* r, int n, const ssg_image* frames, const ssg_im
       const int checks = (tfirst ? 1 : 0) + (res[f].rejected == SSG_REG_RHO_TRANSLATION ? 0 : 1);
       r->prof.add_bytes(SSG_PROF_CORRELATION, checks * 8.0 * r->lw[0] * r->lh[0]);
     }
-    r->prof.collect();
   }
+}
+
+// ssg_registration_status of n frames from the device records of a batch
+void fill_statuses(ssg_registration* r, int n, const FrameXf* xf, const FrameRes* res, ssg_registration_status* st) {
+  const bool tfirst = r->o.motion_type != SSG_MOTION_TRANSLATION && r->o.ecc.ecch_estimate_translation_first &&
+                      r->o.ecc.ecch_max_level != 0;
   const int np = motion_nparams(r->o.motion_type);
   for (int f = 0; f < n; ++f) {
     ssg_registration_status& o = st[f];
@@ -1007,12 +1034,18 @@ int register_n(ssg_registration* r, int n, const ssg_image* frames, const ssg_im
     }
     o.nparams = np;
     float p[6];
-    for (int i = 0; i < 6; ++i) p[i] = xf[f].p[i];
-    if (use_half(r->o)) scale_params(r->o.motion_type, p, 1.0 / r->o.ecc.scale);
+    if (R.rejected == SSG_REG_RHO_TRANSLATION) {
+      // the translation gate returns before the main transform takes the translation estimate
+      // (c_frame_registration.cc:842-849): the transform keeps its default (identity) parameters
+      const float ident[4][6] = {{0, 0}, {0, 0, 0}, {0, 0, 0, 1}, {1, 0, 0, 0, 1, 0}};
+      for (int i = 0; i < 6; ++i) p[i] = ident[r->o.motion_type][i];
+    } else {
+      for (int i = 0; i < 6; ++i) p[i] = xf[f].p[i];
+      if (use_half(r->o)) scale_params(r->o.motion_type, p, 1.0 / r->o.ecc.scale);
+    }
     for (int i = 0; i < 6; ++i) o.params[i] = i < np ? p[i] : 0.f;
     params_matrix(r->o.motion_type, p, o.matrix);
   }
-  return SSG_OK;
 }
 
 }  // namespace
@@ -1032,6 +1065,7 @@ extern "C" int ssg_registration_setup_reference_frame(ssg_registration* r, const
   hipStream_t s = r->stream;
   r->have_ref = false;
   r->cap = 0;
+  drop_stack_state(r);  // a captured batch graph holds this reference's levels and constants
   r->ref_w = image->width;
   r->ref_h = image->height;
   const bool half = use_half(r->o);
@@ -3808,4 +3842,5 @@ extern "C" int ssg_registration_remap(ssg_registration* r, const ssg_image* src,
 }
 
 #include "ssg_input_api.inc"
+#include "ssg_stack_api.inc"
 #include "ssg_finish_api.inc"

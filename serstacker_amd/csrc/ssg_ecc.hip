@@ -511,6 +511,30 @@ __global__ void __launch_bounds__(1024) k_level_begin(LevelBeginArgs a, int use_
   compact_active(a.pass == 0 ? a.xf_t : a.xf_m, a.nframes, a.active_out, a.active_count_out);
 }
 
+// The accumulation inputs of a registered batch, formed on the device: the main pass's parameters
+// scaled back by 1/ecc.scale when the ECC images were halved, turned into map coefficients with
+// the centre at the origin (the host's map_coef of the reported parameters), and the frames that
+// are added -- not rejected by a rho gate, with a positive weight (c_frame_accumulation.cc:112).
+__global__ void __launch_bounds__(256) k_xf_to_acc(const FrameXf* __restrict__ xf, const FrameRes* __restrict__ res,
+                                                   const float* __restrict__ w, int n, int motion, double inv_scale,
+                                                   MapCoef* __restrict__ maps, int* __restrict__ valid) {
+  const int f = blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= n) return;
+  FrameXf X = xf[f];
+  if (inv_scale != 1.0) scale_xf(motion, X, inv_scale);
+  X.C[0] = X.C[1] = 0.f;
+  update_map(motion, X);
+  maps[f] = X.mc;
+  valid[f] = res[f].rejected == 0 && w[f] > 0.f;
+}
+
+hipError_t launch_xf_to_acc(const FrameXf* xf, const FrameRes* res, const float* w, int n, int motion,
+                            double inv_scale, MapCoef* maps, int* valid, hipStream_t s) {
+  hipLaunchKernelGGL(k_xf_to_acc, dim3((n + 255) / 256), dim3(256), 0, s, xf, res, w, n, motion, inv_scale, maps,
+                     valid);
+  return hipGetLastError();
+}
+
 hipError_t launch_level_begin(const LevelBeginArgs& a, hipStream_t s) {
   const int use_t = a.pass == 1 && a.xf_t != nullptr;
   LevelBeginArgs b = a;

@@ -289,6 +289,9 @@ hipError_t launch_sharpness(const float* const* src_ptrs, int pitch, int w, int 
                             hipStream_t s);
 hipError_t launch_ecc_channel(const float* const* src, int pitch, int w, int h, int n, int channel, float* dst,
                               long long dst_fs, int dst_pitch, hipStream_t s);
+hipError_t launch_zero(void* p, size_t bytes, hipStream_t s);
+hipError_t launch_xf_to_acc(const FrameXf* xf, const FrameRes* res, const float* w, int n, int motion,
+                            double inv_scale, MapCoef* maps, int* valid, hipStream_t s);
 hipError_t launch_sharpness_finish(const double* partial, int nb, int nframes, double area, float* out, hipStream_t s);
 // dst = src - up, 0 where masks[f] (if any) is not 255; src / masks: device arrays of per-frame pointers
 hipError_t launch_norm_sub(const float* const* src, int spitch, const float* up, long long ufs,

@@ -433,6 +433,19 @@ hipError_t launch_ecc_channel(const float* const* src, int pitch, int w, int h, 
   return hipGetLastError();
 }
 
+// zero-fill of device memory by a kernel (a memset node is not wanted in captured graphs)
+__global__ void __launch_bounds__(256) k_zero_u32(uint32_t* __restrict__ p, size_t n) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = 0u;
+}
+
+hipError_t launch_zero(void* p, size_t bytes, hipStream_t s) {
+  if (bytes % 4) return hipErrorInvalidValue;
+  const size_t n = bytes / 4;
+  const unsigned g = (unsigned)std::min<size_t>(4096, (n + 255) / 256);
+  if (n) hipLaunchKernelGGL(k_zero_u32, dim3(g ? g : 1), dim3(256), 0, s, (uint32_t*)p, n);
+  return hipGetLastError();
+}
+
 // The frames' weights from the partials of k_sharpness: per frame the partials summed in
 // block order, then divided by the pixel count (the same double operations in the same order
 // as the host reduction of ssg_frame_sharpness).  One thread per frame.
