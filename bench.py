@@ -76,6 +76,7 @@ def main():
     ap.add_argument("--oracle-frames", type=int, default=16,
                     help="frames checked against the FP64 oracle (warps) and the FP32 oracle (stack, counts)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--retune", action="store_true", help="tune launch shapes again instead of using the cache")
     ap.add_argument("--no-graph", action="store_true",
                     help="run the batched steps eagerly instead of replaying the captured hipGraph")
     ap.add_argument("--workload", choices=["affine", "turbulence", "focus", "sweep"], default="affine",
@@ -134,11 +135,41 @@ def main():
     acc = S.c_weigthed_average(W, H, 1)
     views = _lib.images([S.image_view(f) for f in frames])
 
+    # launch shapes and frames per launch: the project's autotuner (cached per device and size class;
+    # the shipped choices for MI355X come from serstacker_amd/tuning.json), untimed
+    from serstacker_amd.autotune import Autotuner, size_class
+
+    batches = {}
+
+    def chunk_views(k):
+        if k not in batches:
+            batches[k] = [_lib.images([S.image_view(f) for f in frames[i:i + k]]) for i in range(0, n, k)]
+        return batches[k]
+
+    def run_batches(k, graph=True):
+        acc.clear()
+        sts = []
+        for v in chunk_views(k):
+            sts.extend(reg.stack_batch(acc, v, use_graph=graph))
+        return sts
+
+    tune_space = {"frames_per_launch": [n, max(1, n // 2), max(1, n // 4)], "acc_rows": [4, 2],
+                  "acc_blocks": [4096, 2048, 8192], "ecc_grid": [8, 6, 4, 12]}
+    t_tune = time.perf_counter()
+    tuner = Autotuner()
+    choice, tune_times, tune_cached = tuner.tune(
+        "stack_batch_affine_fa" if a.ecc_method == "fa" else f"stack_batch_affine_{a.ecc_method}",
+        size_class(W, H, n), tune_space,
+        lambda c: (run_batches(c["frames_per_launch"]), torch.cuda.synchronize()), retune=a.retune)
+    t_tune = time.perf_counter() - t_tune
+    kpl = choice["frames_per_launch"]
+    print(f"[bench] tuning ({'cached' if tune_cached else f'{t_tune:.1f} s'}): {choice}", file=sys.stderr, flush=True)
+
     def step(graph=not a.no_graph):
         # weights + registration + fused warp/accumulate, every per-frame decision on the device
-        # (ssg_stack_batch; with a graph: one replay of the captured launch sequence), then compute
-        acc.clear()
-        st = reg.stack_batch(acc, views, use_graph=graph)
+        # (ssg_stack_batch; with a graph: one replay of the captured launch sequence per launch of
+        # `kpl` frames), then compute
+        st = run_batches(kpl, graph)
         avg, mask = acc.compute()
         return st, avg
 
@@ -363,6 +394,8 @@ def main():
                      "avg_launch_us": round(dom["avg_us"], 2), "launches": dom["launches"]},
         "cpu_baseline": cpu,
         "graph": not a.no_graph,
+        "tuning": {"choice": choice, "cached": tune_cached, "seconds": round(t_tune, 2),
+                   "size_class": size_class(W, H, n)},
         "eager_profiled_ms_per_step": round(eager_ms, 3),
         "plain_gpu": {"value": round(plain_fps, 2), "unit": "frames/s", "frames": npl} if plain_fps else None,
         "speedup_vs_plain": round(fps / plain_fps, 2) if plain_fps else None,
@@ -689,6 +722,23 @@ def bench_focus(a, S, _lib, torch):
         times["stack"] += time.perf_counter() - t1
         return shifts
 
+    # frames per sequence and chunk of the batched focus stack: the project's autotuner (cached)
+    from serstacker_amd.autotune import Autotuner, size_class
+
+    shifts0 = list(pool.map(register, range(len(seqs))))
+
+    def stack_only(_choice):
+        for st in stacks:
+            st.clear()
+        S.focus_stack_batch(stacks, views, shifts0)
+        torch.cuda.synchronize()
+
+    t_tune = time.perf_counter()
+    choice, _, tune_cached = Autotuner().tune("focus_stack_batch", size_class(int(px / nf), 1, nf),
+                                              {"focus_chunk": [32, 16, 8]}, stack_only, retune=a.retune)
+    t_tune = time.perf_counter() - t_tune
+    print(f"[focus] tuning ({'cached' if tune_cached else f'{t_tune:.1f} s'}): {choice}", file=sys.stderr, flush=True)
+
     for _ in range(a.warmup):
         step()
     torch.cuda.synchronize()
@@ -777,6 +827,7 @@ def bench_focus(a, S, _lib, torch):
         "registration_ms_per_step": round(1e3 * t_reg / a.steps, 2),
         "stack_ms_per_step": round(1e3 * t_stack / a.steps, 2),
         "max_shift_error_px": round(err, 4),
+        "tuning": {"choice": choice, "cached": tune_cached, "seconds": round(t_tune, 2)},
         "kernels": {k: {kk: (round(vv, 3) if isinstance(vv, float) else vv) for kk, vv in v.items()}
                     for k, v in kernels.items()},
     }

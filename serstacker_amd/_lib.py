@@ -131,6 +131,7 @@ PROF_ECC_TRANSLATION, PROF_ECC_MAIN, PROF_CORRELATION, PROF_PYRAMID, PROF_ACCUMU
 PROF_ECC_SOLVE, PROF_FFT, PROF_ECCFLOW = 5, 6, 7
 PROF_FOCUS_PREP, PROF_FOCUS_BLUR, PROF_FOCUS_DOWN, PROF_FOCUS_FUSE, PROF_GRAPH = 8, 9, 10, 11, 12
 PROF_COUNT = 13
+TUNE_ACC_ROWS, TUNE_ACC_BLOCKS, TUNE_ECC_GRID, TUNE_FOCUS_CHUNK = 0, 1, 2, 3
 ECCFLOW_DOWNSCALE_RECURSIVE_RESIZE, ECCFLOW_DOWNSCALE_FULL_RESIZE, ECCFLOW_DOWNSCALE_PYRAMID = 0, 1, 2
 
 
@@ -250,6 +251,8 @@ _SIGS = [
     ("ssg_accumulation_set_remap", C.c_int, [C.c_void_p, _P(Image)]),
     ("ssg_accumulation_set_input_bpp", C.c_int, [C.c_void_p, C.c_int32]),
     ("ssg_frame_sharpness", C.c_int, [C.c_int, _P(Image), _P(C.c_float)]),
+    ("ssg_tune_set", C.c_int, [C.c_int32, C.c_int32]),
+    ("ssg_tune_get", C.c_int, [C.c_int32, _P(C.c_int32)]),
     ("ssg_stack_batch", C.c_int, [C.c_void_p, C.c_void_p, C.c_int32, _P(Image), C.c_int32, C.c_int32, C.c_double, C.c_int32, _P(RegistrationStatus)]),
     ("ssg_focus_stack_batch", C.c_int, [C.c_int32, _P(C.c_void_p), _P(C.c_int32), _P(_P(Image)), _P(_P(C.c_float))]),
     ("ssg_accumulation_reinitialize", C.c_int, [C.c_void_p, _P(Image), _P(Image)]),

@@ -167,32 +167,33 @@ __device__ __forceinline__ int mask_value(const uint8_t* m, int pitch, int w, in
 // (remap(255) >= 255, erode 5x5 with BORDER_CONSTANT 255), the frame sample
 // (cv::remap with the option interpolation and border) and the weighted
 // update, all from registers; one write of the chunk partials at the end.
-constexpr int AC_TW = 64, AC_RPT = 4, AC_WAVES = 4, AC_TH = AC_RPT * AC_WAVES;
+constexpr int AC_TW = 64, AC_WAVES = 4;
 constexpr int AC_GROUP = 16;  // frames summed plainly before folding into the compensated pair
 
 // WMAP: per-frame weight images (frames' own grid, f32) remapped with the
 // frame's interpolation and BORDER_CONSTANT 0; else a per-frame scalar weight,
 // remapped as a constant image.  Either is then gated by the mask (weights *
 // mask / 255, c_image_stacking_pipeline.cc:1704-1712, the mask being 0 or 255).
-template <int MOTION, int INTERP, int CN, bool WMAP>
+template <int MOTION, int INTERP, int CN, bool WMAP, int RPT>
 __global__ void __launch_bounds__(256) k_accumulate(AccumArgs a) {
+  constexpr int TH = RPT * AC_WAVES;  // tile rows: RPT rows per thread, AC_WAVES waves
   // MOTION_MAP: the source positions come from a dense per-frame field, staged in LDS for the
   // tile and its 2-px erosion halo (the whole tile takes the bordered path)
   constexpr bool DENSE = MOTION == MOTION_MAP;
-  __shared__ uint8_t mk[AC_TH + 4][AC_TW + 4];
-  __shared__ uint8_t rmk[AC_TH + 4][AC_TW];
-  __shared__ float2 smap[DENSE ? (AC_TH + 4) * (AC_TW + 4) : 1];
+  __shared__ uint8_t mk[TH + 4][AC_TW + 4];
+  __shared__ uint8_t rmk[TH + 4][AC_TW];
+  __shared__ float2 smap[DENSE ? (TH + 4) * (AC_TW + 4) : 1];
   __shared__ float sred[DENSE ? AC_WAVES * 4 : 1];
   const int tid = threadIdx.x, lane = tid & (WAVE - 1), wv = tid / WAVE;
-  const int x0 = blockIdx.x * AC_TW, y0 = blockIdx.y * AC_TH;
+  const int x0 = blockIdx.x * AC_TW, y0 = blockIdx.y * TH;
   const int chunk = blockIdx.z;
   const int kb = chunk * a.per_chunk, kend = min(kb + a.per_chunk, a.nframes);
   const int x = x0 + lane;
-  float shi[AC_RPT][CN], slo[AC_RPT][CN], sg[AC_RPT][CN];
-  float whi[AC_RPT], wlo[AC_RPT], wg[AC_RPT];
-  unsigned cnt[AC_RPT];
+  float shi[RPT][CN], slo[RPT][CN], sg[RPT][CN];
+  float whi[RPT], wlo[RPT], wg[RPT];
+  unsigned cnt[RPT];
 #pragma unroll
-  for (int r = 0; r < AC_RPT; ++r) {
+  for (int r = 0; r < RPT; ++r) {
 #pragma unroll
     for (int c = 0; c < CN; ++c) shi[r][c] = slo[r][c] = sg[r][c] = 0.f;
     whi[r] = wlo[r] = wg[r] = 0.f;
@@ -214,7 +215,7 @@ __global__ void __launch_bounds__(256) k_accumulate(AccumArgs a) {
     const float* __restrict__ img = a.frames[k];
     // uniform test: tile + erosion halo inside the sampling fast region
     const int ex0 = max(x0 - 2, 0), ey0 = max(y0 - 2, 0);
-    const int ex1 = min(x0 + AC_TW + 1, a.dw - 1), ey1 = min(y0 + AC_TH + 1, a.dh - 1);
+    const int ex1 = min(x0 + AC_TW + 1, a.dw - 1), ey1 = min(y0 + TH + 1, a.dh - 1);
     float mnx = 1e30f, mxx = -1e30f, mny = 1e30f, mxy = -1e30f;
     if constexpr (!DENSE) {
       const int cx[4] = {ex0, ex1, ex0, ex1}, cy[4] = {ey0, ey0, ey1, ey1};
@@ -228,7 +229,7 @@ __global__ void __launch_bounds__(256) k_accumulate(AccumArgs a) {
     }
     if constexpr (DENSE) {
       // stage the field of the tile + halo and reduce its extent over the block
-      for (int i = tid; i < (AC_TH + 4) * (AC_TW + 4); i += 256) {
+      for (int i = tid; i < (TH + 4) * (AC_TW + 4); i += 256) {
         const int ly = i / (AC_TW + 4), lx = i - ly * (AC_TW + 4);
         const int px = x0 - 2 + lx, py = y0 - 2 + ly;
         if ((unsigned)px < (unsigned)a.dw && (unsigned)py < (unsigned)a.dh) {
@@ -294,10 +295,10 @@ __global__ void __launch_bounds__(256) k_accumulate(AccumArgs a) {
         const bool pair = k2 < kend;
         const int xc = min(x, a.dw - 1);
         const uint32_t pb = (uint32_t)a.src_pitch * 4u;
-        int X1[AC_RPT], Y1[AC_RPT], X2[AC_RPT], Y2[AC_RPT];
+        int X1[RPT], Y1[RPT], X2[RPT], Y2[RPT];
 #pragma unroll
-        for (int r = 0; r < AC_RPT; ++r) {
-          const int yc = min(y0 + wv * AC_RPT + r, a.dh - 1);
+        for (int r = 0; r < RPT; ++r) {
+          const int yc = min(y0 + wv * RPT + r, a.dh - 1);
           float sx, sy;
           if constexpr (DENSE) {
             const float2 q = smap[(yc - y0 + 2) * (AC_TW + 4) + (xc - x0) + 2];
@@ -316,9 +317,9 @@ __global__ void __launch_bounds__(256) k_accumulate(AccumArgs a) {
         }
         const rsrc_t r1 = make_rsrc(img, (uint32_t)a.sh * pb);
         const rsrc_t r2 = make_rsrc(pair ? a.frames[k2] : img, (uint32_t)a.sh * pb);
-        float t1[AC_RPT][4][CN], t2[AC_RPT][4][CN];
+        float t1[RPT][4][CN], t2[RPT][4][CN];
 #pragma unroll
-        for (int r = 0; r < AC_RPT; ++r) {
+        for (int r = 0; r < RPT; ++r) {
           const uint32_t o = (uint32_t)(Y1[r] >> 5) * pb + (uint32_t)(X1[r] >> 5) * (4u * CN);
 #pragma unroll
           for (int c = 0; c < CN; ++c) {
@@ -330,7 +331,7 @@ __global__ void __launch_bounds__(256) k_accumulate(AccumArgs a) {
         }
         if (pair) {
 #pragma unroll
-          for (int r = 0; r < AC_RPT; ++r) {
+          for (int r = 0; r < RPT; ++r) {
             const uint32_t o = (uint32_t)(Y2[r] >> 5) * pb + (uint32_t)(X2[r] >> 5) * (4u * CN);
 #pragma unroll
             for (int c = 0; c < CN; ++c) {
@@ -344,8 +345,8 @@ __global__ void __launch_bounds__(256) k_accumulate(AccumArgs a) {
         const float wk2 = pair ? a.weights[k2] : 0.f;
         const float* __restrict__ wimg = WMAP ? a.wmaps[k] : nullptr;
 #pragma unroll
-        for (int r = 0; r < AC_RPT; ++r) {
-          const int y = y0 + wv * AC_RPT + r;
+        for (int r = 0; r < RPT; ++r) {
+          const int y = y0 + wv * RPT + r;
           if (x >= a.dw || y >= a.dh) continue;
           float w0, w1, w2, w3;
           bilinear_w(X1[r] & 31, Y1[r] & 31, w0, w1, w2, w3);
@@ -381,7 +382,7 @@ __global__ void __launch_bounds__(256) k_accumulate(AccumArgs a) {
         if (ingroup == AC_GROUP) {
           ingroup = 0;
 #pragma unroll
-          for (int r = 0; r < AC_RPT; ++r) {
+          for (int r = 0; r < RPT; ++r) {
 #pragma unroll
             for (int c = 0; c < CN; ++c) {
               float s_, e_;
@@ -402,7 +403,7 @@ __global__ void __launch_bounds__(256) k_accumulate(AccumArgs a) {
     }
     if (!interior) {
       // warped mask for the tile + 2-px halo; outside the destination counts as 255
-      for (int i = tid; i < (AC_TH + 4) * (AC_TW + 4); i += 256) {
+      for (int i = tid; i < (TH + 4) * (AC_TW + 4); i += 256) {
         const int ly = i / (AC_TW + 4), lx = i - ly * (AC_TW + 4);
         const int px = x0 - 2 + lx, py = y0 - 2 + ly;
         uint8_t v = 1;
@@ -420,15 +421,15 @@ __global__ void __launch_bounds__(256) k_accumulate(AccumArgs a) {
         mk[ly][lx] = v;
       }
       __syncthreads();
-      for (int i = tid; i < (AC_TH + 4) * AC_TW; i += 256) {
+      for (int i = tid; i < (TH + 4) * AC_TW; i += 256) {
         const int ly = i / AC_TW, lx = i - ly * AC_TW;
         rmk[ly][lx] = mk[ly][lx] & mk[ly][lx + 1] & mk[ly][lx + 2] & mk[ly][lx + 3] & mk[ly][lx + 4];
       }
       __syncthreads();
     }
 #pragma unroll
-    for (int r = 0; r < AC_RPT; ++r) {
-      const int ly = wv * AC_RPT + r, y = y0 + ly;
+    for (int r = 0; r < RPT; ++r) {
+      const int ly = wv * RPT + r, y = y0 + ly;
       if (x >= a.dw || y >= a.dh) continue;
       bool m = true;
       if (!interior) m = rmk[ly][lane] & rmk[ly + 1][lane] & rmk[ly + 2][lane] & rmk[ly + 3][lane] & rmk[ly + 4][lane];
@@ -479,7 +480,7 @@ __global__ void __launch_bounds__(256) k_accumulate(AccumArgs a) {
     if (++ingroup == AC_GROUP) {
       ingroup = 0;
 #pragma unroll
-      for (int r = 0; r < AC_RPT; ++r) {
+      for (int r = 0; r < RPT; ++r) {
 #pragma unroll
         for (int c = 0; c < CN; ++c) {
           float s, e;
@@ -497,8 +498,8 @@ __global__ void __launch_bounds__(256) k_accumulate(AccumArgs a) {
     }
   }
 #pragma unroll
-  for (int r = 0; r < AC_RPT; ++r) {
-    const int y = y0 + wv * AC_RPT + r;
+  for (int r = 0; r < RPT; ++r) {
+    const int y = y0 + wv * RPT + r;
     if (x >= a.dw || y >= a.dh) continue;
     const size_t o = (size_t)y * a.dw + x;
     float* P = a.part + (size_t)chunk * (2 * CN + 2) * a.plane;
@@ -519,12 +520,19 @@ __global__ void __launch_bounds__(256) k_accumulate(AccumArgs a) {
 
 hipError_t launch_accumulate(int motion, const AccumArgs& a, hipStream_t s) {
   const int nch = a.nchunks;
-  dim3 grid((a.dw + AC_TW - 1) / AC_TW, (a.dh + AC_TH - 1) / AC_TH, nch);
+  // tile height: a.rows_per_thread rows per thread (2 or 4) times AC_WAVES waves
+  const int rpt = a.rows_per_thread == 2 ? 2 : 4;
+  dim3 grid((a.dw + AC_TW - 1) / AC_TW, (a.dh + rpt * AC_WAVES - 1) / (rpt * AC_WAVES), nch);
   const bool cubic = a.interp == INTER_CUBIC;
-#define SSG_ACC(M, I, C)                                                                        \
-  do {                                                                                          \
-    if (a.wmaps) hipLaunchKernelGGL((k_accumulate<M, I, C, true>), grid, dim3(256), 0, s, a);   \
-    else hipLaunchKernelGGL((k_accumulate<M, I, C, false>), grid, dim3(256), 0, s, a);          \
+#define SSG_ACC(M, I, C)                                                                                  \
+  do {                                                                                                    \
+    if (rpt == 2) {                                                                                       \
+      if (a.wmaps) hipLaunchKernelGGL((k_accumulate<M, I, C, true, 2>), grid, dim3(256), 0, s, a);        \
+      else hipLaunchKernelGGL((k_accumulate<M, I, C, false, 2>), grid, dim3(256), 0, s, a);               \
+    } else {                                                                                              \
+      if (a.wmaps) hipLaunchKernelGGL((k_accumulate<M, I, C, true, 4>), grid, dim3(256), 0, s, a);        \
+      else hipLaunchKernelGGL((k_accumulate<M, I, C, false, 4>), grid, dim3(256), 0, s, a);               \
+    }                                                                                                     \
   } while (0)
 #define SSG_ACC_IC(M)                                   \
   do {                                                  \

@@ -9,6 +9,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <atomic>
 #include <mutex>
 #include <string>
 #include <cstring>
@@ -371,6 +372,40 @@ extern "C" void ssg_default_image_registration_options(ssg_image_registration_op
   o->enable_feature_registration = 1;
   o->enable_ecc_registration = 0;
   o->enable_eccflow_registration = 0;
+}
+
+// ---- tuning knobs -----------------------------------------------------------------------------
+namespace {
+std::atomic<int> g_tune[SSG_TUNE_COUNT] = {{4}, {4096}, {8}, {32}};
+const int g_tune_lo[SSG_TUNE_COUNT] = {2, 256, 1, 1}, g_tune_hi[SSG_TUNE_COUNT] = {4, 65536, 16, 32};
+}  // namespace
+
+namespace ssg {
+int tune_value(int knob) { return knob >= 0 && knob < SSG_TUNE_COUNT ? g_tune[knob].load() : 0; }
+
+int accumulate_chunks(int n, int w, int h) {
+  const int th = 4 * tune_value(SSG_TUNE_ACC_ROWS);
+  const long long tiles = (long long)((w + 63) / 64) * ((h + th - 1) / th);
+  const long long target = tune_value(SSG_TUNE_ACC_BLOCKS);
+  int nch = (int)std::max(1LL, std::min<long long>(16, (target + tiles - 1) / tiles));
+  nch = std::min(nch, std::max(n, 1));
+  const int per = (n + nch - 1) / nch;
+  return (n + per - 1) / per;
+}
+}  // namespace ssg
+
+extern "C" int ssg_tune_set(int32_t knob, int32_t value) {
+  if (knob < 0 || knob >= SSG_TUNE_COUNT) return fail(SSG_E_INVALID_ARG, "tuning knob %d", knob);
+  if (value < g_tune_lo[knob] || value > g_tune_hi[knob] || (knob == SSG_TUNE_ACC_ROWS && value != 2 && value != 4))
+    return fail(SSG_E_INVALID_ARG, "tuning knob %d: value %d out of range", knob, value);
+  g_tune[knob].store(value);
+  return SSG_OK;
+}
+
+extern "C" int ssg_tune_get(int32_t knob, int32_t* value) {
+  if (knob < 0 || knob >= SSG_TUNE_COUNT || !value) return fail(SSG_E_INVALID_ARG, "tuning knob %d", knob);
+  *value = g_tune[knob].load();
+  return SSG_OK;
 }
 
 extern "C" int ssg_abi_version(void) { return SSG_ABI_VERSION; }
@@ -2127,7 +2162,8 @@ extern "C" int ssg_focus_stack_batch(int32_t nseq, ssg_accumulation* const* accs
   size_t free_b = 0, total_b = 0;
   HIPCHK(hipMemGetInfo(&free_b, &total_b));
   const size_t budget = std::min<size_t>((size_t)24 << 30, free_b / 3);
-  const int m = (int)std::max<size_t>(1, std::min<size_t>({(size_t)32, (size_t)maxn, budget / std::max<size_t>(per_all, 1)}));
+  const int m = (int)std::max<size_t>(1, std::min<size_t>({(size_t)tune_value(SSG_TUNE_FOCUS_CHUNK), (size_t)maxn,
+                                                             budget / std::max<size_t>(per_all, 1)}));
   std::vector<size_t> soff(nseq + 1, 0);
   for (int q = 0; q < nseq; ++q) soff[q + 1] = soff[q] + (nframes[q] > 0 ? ((per[q] * m + 255) / 256) * 256 : 0);
   CHK(a0->fms.ensure(soff[nseq]));
@@ -2427,13 +2463,9 @@ int accumulate_fused(ssg_accumulation* a, int n, const ssg_image* frames, int mo
   g.cn = a->cn;
   g.interp = interpolation == SSG_INTER_CUBIC ? INTER_CUBIC : INTER_LINEAR;
   g.wmaps = wmaps;
-  // chunks: about 4096 blocks over the tiles, each chunk a contiguous frame range (measured:
-  // more chunks of fewer frames each run slower)
-  const long long tiles = (long long)((a->w + 63) / 64) * ((a->h + 15) / 16);
-  int nch = (int)std::max(1LL, std::min<long long>(16, (4096 + tiles - 1) / tiles));
-  nch = std::min(nch, n);
+  // chunks: about SSG_TUNE_ACC_BLOCKS blocks over the tiles, each chunk a contiguous frame range
+  const int nch = accumulate_chunks(n, a->w, a->h);
   const int per = (n + nch - 1) / nch;
-  nch = (n + per - 1) / per;
   CHK(a->part.ensure(sizeof(float) * a->plane * (2 * a->cn + 2) * nch));
   CHK(a->part_cnt.ensure(sizeof(unsigned) * a->plane * nch));
   g.frames = a->frame_ptrs.as<const float*>();
@@ -2456,6 +2488,7 @@ int accumulate_fused(ssg_accumulation* a, int n, const ssg_image* frames, int mo
   g.part_cnt = a->part_cnt.as<unsigned>();
   g.plane = a->plane;
   g.nchunks = nch;
+  g.rows_per_thread = tune_value(SSG_TUNE_ACC_ROWS);
   a->prof.start(s);
   HIPCHK(launch_accumulate(motion, g, s));
   a->prof.stop(s, SSG_PROF_ACCUMULATE);

@@ -28,6 +28,7 @@
 
 #include <algorithm>
 
+#include "../../include/ssg.h"
 #include "ssg_engine.hpp"
 
 namespace ssg {
@@ -366,7 +367,7 @@ static int device_cus() {
 
 // upper bound of the persistent grids (8 blocks per CU); each pass kernel
 // launches at most the blocks that are resident at once at its own VGPR/LDS use
-int ecc_persistent_grid() { return device_cus() * 8; }
+int ecc_persistent_grid() { return device_cus() * tune_value(SSG_TUNE_ECC_GRID); }
 
 template <int MOTION, bool F, bool CM, bool RM>
 static int pass_resident_grid() {

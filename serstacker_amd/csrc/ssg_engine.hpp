@@ -238,7 +238,13 @@ struct AccumArgs {
   unsigned* part_cnt;
   long long plane;             // elements per plane (dw*dh)
   int nchunks;
+  int rows_per_thread;         // tile height / 4 waves: 2 or 4 (the tuning knob SSG_TUNE_ACC_ROWS)
 };
+
+// the tuning knobs (ssg_tune_set): the value in force for `knob`
+int tune_value(int knob);
+// chunks of the fused accumulate for n frames on a w x h accumulator (frames per chunk follow)
+int accumulate_chunks(int n, int w, int h);
 
 // c_bayer_average state: planes [c][h*w] for c = B, G, R (the reference's BGR order)
 struct BayerState {

@@ -169,3 +169,34 @@ def test_host_views_keep_their_memory():
         S.image_view(a.T, out=True)
     with pytest.raises(ValueError):
         S.image_view(rgb[:, :, :2], out=True)
+
+
+def test_tuning_knobs_and_autotuner_cache(tmp_path):
+    """The tuning knobs keep their ranges; the autotuner's coordinate descent keeps the fastest
+    candidate of each choice, caches it per device / workload / size class and reuses it (a stub
+    workload whose time depends on the choice; no GPU work)."""
+    from serstacker_amd import _lib, autotune
+
+    L = _lib.load()
+    v = C.c_int32()
+    assert L.ssg_tune_get(_lib.TUNE_ACC_ROWS, C.byref(v)) == 0 and v.value == 4
+    assert L.ssg_tune_set(_lib.TUNE_ACC_ROWS, 3) == -1
+    assert L.ssg_tune_set(_lib.TUNE_ECC_GRID, 99) == -1
+    assert L.ssg_tune_set(7, 1) == -1
+    assert autotune.size_class(1920, 1080, 2000) == "2048kpxx2048"
+    calls = []
+
+    def run(choice):
+        calls.append(dict(choice))
+        import time as _t
+        _t.sleep(0.001 * (abs(choice["acc_blocks"] - 2048) / 1024 + (choice["frames_per_launch"] != 500)))
+
+    space = {"frames_per_launch": [1000, 500], "acc_blocks": [4096, 2048]}
+    t = autotune.Autotuner(str(tmp_path / "tune.json"))
+    choice, times, cached = t.tune("stub", "k", space, run, repeats=1)
+    assert not cached and choice == {"frames_per_launch": 500, "acc_blocks": 2048}
+    assert autotune.get_knob("acc_blocks") == 2048
+    n = len(calls)
+    again, _, cached = autotune.Autotuner(str(tmp_path / "tune.json")).tune("stub", "k", space, run)
+    assert cached and again == choice and len(calls) == n
+    autotune.set_knob("acc_blocks", 4096)
