@@ -142,6 +142,19 @@ class c_frame_registration {
                      border_mode < 0 ? opts_.border_value[0] : border_value) == SSG_OK;
   }
 
+  // weights + registration + fused warp/accumulate of a device batch with every per-frame decision
+  // on the device, replayed as a captured hipGraph when use_graph (ssg_stack_batch)
+  template <class Acc>
+  bool stack_batch(Acc &acc, const std::vector<image> &frames, std::vector<registration_status> &out,
+                   bool use_graph = true, int interpolation = SSG_INTER_LINEAR,
+                   int border_mode = SSG_BORDER_REFLECT101, double border_value = 0) {
+    if (!h_) return fail_state();
+    out.resize(frames.size());
+    std::vector<ssg_image> f(frames.begin(), frames.end());
+    return ssg_stack_batch(h_, acc.handle(), (int)f.size(), f.data(), interpolation, border_mode, border_value,
+                           use_graph ? 1 : 0, out.data()) == SSG_OK;
+  }
+
   ssg_registration *handle() const { return h_; }
 
  private:
@@ -167,9 +180,25 @@ class c_frame_accumulation {
   virtual bool add(const image &src, const image &mask = image()) = 0;
   virtual bool compute(image &avg, image *mask = nullptr, double dscale = 1.0) const = 0;
   virtual bool get_acc_counters(image &accw) const = 0;
+  // c_frame_accumulation::reinitialize (c_frame_accumulation.h:25)
+  virtual bool reinitialize(const image &src, const image &accw) = 0;
   virtual void clear() = 0;
   virtual std::pair<int, int> accumulator_size() const = 0;
   virtual int accumulated_frames() const = 0;
+  // checkpoint / resume of the accumulator (ssg_accumulation_state_save / _load)
+  virtual bool save_state(std::vector<uint8_t> &out) const = 0;
+  virtual bool load_state(const std::vector<uint8_t> &in) = 0;
+
+ protected:
+  static bool save_state_of(ssg_accumulation *a, std::vector<uint8_t> &out) {
+    uint64_t n = 0;
+    if (!a || ssg_accumulation_state_size(a, &n) != SSG_OK) return false;
+    out.resize(n);
+    return ssg_accumulation_state_save(a, out.data(), n) == SSG_OK;
+  }
+  static bool load_state_of(ssg_accumulation *a, const std::vector<uint8_t> &in) {
+    return a && ssg_accumulation_state_load(a, in.data(), in.size()) == SSG_OK;
+  }
 };
 
 // c_weigthed_average (c_frame_accumulation.h:39-63, c_frame_accumulation.cc:20-260)
@@ -236,6 +265,13 @@ class c_weigthed_average : public c_frame_accumulation {
     if (a_) ssg_accumulation_accumulated_frames(a_, &n);
     return n;
   }
+  // c_weigthed_average::reinitialize (c_frame_accumulation.cc:171-180)
+  bool reinitialize(const image &src, const image &accw) override {
+    return a_ && ssg_accumulation_reinitialize(a_, &src, &accw) == SSG_OK;
+  }
+  bool save_state(std::vector<uint8_t> &out) const override { return save_state_of(a_, out); }
+  bool load_state(const std::vector<uint8_t> &in) override { return load_state_of(a_, in); }
+  ssg_accumulation *handle() const { return a_; }
 
  private:
   int w_, h_, cn_;
@@ -277,6 +313,12 @@ class accumulation_handle : public c_frame_accumulation {
     if (a_) ssg_accumulation_accumulated_frames(a_, &n);
     return n;
   }
+  // Bayer and focus accumulators refuse it, as the reference's return false
+  bool reinitialize(const image &src, const image &accw) override {
+    return a_ && ssg_accumulation_reinitialize(a_, &src, &accw) == SSG_OK;
+  }
+  bool save_state(std::vector<uint8_t> &out) const override { return save_state_of(a_, out); }
+  bool load_state(const std::vector<uint8_t> &in) override { return load_state_of(a_, in); }
   ssg_accumulation *handle() const { return a_; }
 
  protected:

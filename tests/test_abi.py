@@ -131,6 +131,41 @@ def test_cpp_host_header_compiles():
         assert r.returncode == 0, r.stderr
 
 
+def test_integration_shim_compiles():
+    """The reference-side shim printed in INTEGRATION.md compiles against include/ and the
+    restated reference interfaces under tests/cpp/refshim (the registration class's virtual
+    operations and protected state, the accumulator base with its pure virtuals, the option
+    structs); every override is marked override, so a signature that drifts from the
+    reference's fails here."""
+    import shutil
+    import subprocess
+    import tempfile
+
+    gxx = shutil.which("g++")
+    if gxx is None:
+        pytest.skip("g++ not available")
+    text = open(os.path.join(ROOT, "INTEGRATION.md")).read()
+    m = re.search(r"<!-- integration-shim -->\s*```cpp\n(.*?)\n```", text, re.S)
+    assert m, "shim block missing"
+    code = m.group(1)
+    for cls in ("c_frame_registration_mi355x", "c_weigthed_average_mi355x", "c_bayer_average_mi355x",
+                "c_laplacian_pyramid_focus_stacking_mi355x"):
+        assert f"class {cls}" in code
+    # instantiate every class, so a pure virtual left unimplemented is an error
+    main = ("\nint main() {\n  c_image_registration_options o;\n  c_frame_registration_mi355x r(o);\n"
+            "  c_weigthed_average_mi355x a;\n  c_bayer_average_mi355x b;\n"
+            "  c_laplacian_pyramid_focus_stacking_mi355x f{c_laplacian_pyramid_focus_stacking::options()};\n"
+            "  (void)r; (void)a; (void)b; (void)f;\n  return 0;\n}\n")
+    with tempfile.TemporaryDirectory() as d:
+        f = os.path.join(d, "shim.cpp")
+        with open(f, "w") as fh:
+            fh.write(code.replace("#pragma once\n", "") + main)
+        r = subprocess.run([gxx, "-std=c++17", "-Wall", "-Wextra", "-Werror", "-Wsuggest-override", "-fsyntax-only",
+                            f"-I{os.path.join(ROOT, 'include')}", f"-I{os.path.join(ROOT, 'tests', 'cpp', 'refshim')}",
+                            f], capture_output=True, text=True)
+        assert r.returncode == 0, r.stderr
+
+
 def test_host_views_keep_their_memory():
     """image_view of a row-strided array views it in place; of an array whose rows are
     not contiguous it views a copy that the view itself keeps alive; an output array
