@@ -153,14 +153,26 @@ def main():
             sts.extend(reg.stack_batch(acc, v, use_graph=graph))
         return sts
 
-    tune_space = {"frames_per_launch": [n, max(1, n // 2), max(1, n // 4)], "acc_rows": [4, 2],
-                  "acc_blocks": [4096, 2048, 8192], "ecc_grid": [8, 6, 4, 12]}
+    # two stages: the accumulate's tile shape and chunking on the fused warp/accumulate alone (one
+    # registration of the batch gives its statuses and weights), then the ECC grid and the frames
+    # per launch on the whole step
     t_tune = time.perf_counter()
     tuner = Autotuner()
+    wkey = "stack_batch_affine_fa" if a.ecc_method == "fa" else f"stack_batch_affine_{a.ecc_method}"
+    acc_choice = tuner.lookup(wkey + "_accumulate", size_class(W, H, n))
+    if acc_choice is None or a.retune:
+        st_t = reg.register_batch(views)
+        w_t = S.frame_sharpness(views)
+        acc_choice, _, _ = tuner.tune(
+            wkey + "_accumulate", size_class(W, H, n), {"acc_rows": [4, 2], "acc_blocks": [4096, 2048, 8192, 16384]},
+            lambda c: (acc.clear(), acc.add_registered_batch(views, st_t, S.MOTION_AFFINE, frame_weights=w_t),
+                       torch.cuda.synchronize()), repeats=3, retune=a.retune)
+    Autotuner.apply(acc_choice)
+    tune_space = {"frames_per_launch": [n, max(1, n // 2), max(1, n // 4)], "ecc_grid": [8, 6, 4, 12]}
     choice, tune_times, tune_cached = tuner.tune(
-        "stack_batch_affine_fa" if a.ecc_method == "fa" else f"stack_batch_affine_{a.ecc_method}",
-        size_class(W, H, n), tune_space,
+        wkey, size_class(W, H, n), tune_space,
         lambda c: (run_batches(c["frames_per_launch"]), torch.cuda.synchronize()), retune=a.retune)
+    choice = {**acc_choice, **choice}
     t_tune = time.perf_counter() - t_tune
     kpl = choice["frames_per_launch"]
     print(f"[bench] tuning ({'cached' if tune_cached else f'{t_tune:.1f} s'}): {choice}", file=sys.stderr, flush=True)

@@ -167,7 +167,8 @@ typedef struct ssg_registration_status {
 /* ---- library ------------------------------------------------------------------------------- */
 int ssg_abi_version(void);
 const char *ssg_last_error(void);
-/* Number of HIP devices and the name of the current one; fails if no GPU. */
+/* Number of HIP devices and the name of the current one, "<name> (<gcnArchName>, <n> CUs)";
+ * fails if no GPU. */
 int ssg_device_info(int *num_devices, char *name, int name_len);
 
 /* ---- registration: c_frame_registration (c_frame_registration.h:210-354) ------------------- */

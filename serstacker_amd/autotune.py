@@ -11,10 +11,11 @@ A workload is described by a name and a size class; its tunable choices are
 
 `Autotuner.tune` times the candidates on the device by coordinate descent (each choice in turn,
 the others held at their best so far; the median of `repeats` timed runs after one untimed run,
-which also captures any graph), keeps the fastest, caches it in a JSON file keyed by device name,
-workload and size class, and applies it.  `Autotuner.lookup` returns a cached choice; the
-shipped `tuning.json` beside this module holds the choices measured on MI355X, and a user cache
-($SSG_TUNE_CACHE, default ~/.cache/serstacker_amd/tune.json) takes precedence over it.
+which also captures any graph), keeps the fastest, caches it in a JSON file keyed by device
+(architecture and CU count), workload and size class, and applies it.  `Autotuner.lookup`
+returns a cached choice; the shipped `tuning.json` beside this module holds the choices measured
+on MI355X, and a user cache ($SSG_TUNE_CACHE, default ~/.cache/serstacker_amd/tune.json) takes
+precedence over it.
 """
 from __future__ import annotations
 
@@ -55,6 +56,14 @@ def device_name() -> str:
     return buf.value.decode(errors="replace") or "unknown"
 
 
+def device_key(name: str | None = None) -> str:
+    """The cache key of a device: its architecture and CU count (the text in the parentheses of
+    ssg_device_info's name), which the marketing name in front of it may not be stable about."""
+    name = device_name() if name is None else name
+    lo, hi = name.rfind("("), name.rfind(")")
+    return name[lo + 1:hi] if 0 <= lo < hi else name
+
+
 def get_knob(name: str) -> int:
     v = C.c_int32()
     _lib.check(_lib.load().ssg_tune_get(KNOBS[name], C.byref(v)), "ssg_tune_get")
@@ -69,7 +78,7 @@ class Autotuner:
     def __init__(self, cache_path: str | None = None):
         self.path = cache_path or os.environ.get("SSG_TUNE_CACHE") or os.path.join(
             os.path.expanduser("~"), ".cache", "serstacker_amd", "tune.json")
-        self.device = device_name()
+        self.device = device_key()
 
     @staticmethod
     def _read(path):

@@ -170,6 +170,24 @@ __device__ __forceinline__ int mask_value(const uint8_t* m, int pitch, int w, in
 constexpr int AC_TW = 64, AC_WAVES = 4;
 constexpr int AC_GROUP = 16;  // frames summed plainly before folding into the compensated pair
 
+// Whether every source position of the destination rectangle [ex0, ex1] x [ey0, ey1] under the
+// parametric map lies in [lo, hx] x [lo, hy]: the positions are monotone in x and in y, so the
+// corners bound them; a NaN coefficient fails the test.
+template <int MOTION>
+__device__ __forceinline__ bool tile_inside(const MapCoef& mc, int ex0, int ey0, int ex1, int ey1, float lo, float hx,
+                                            float hy) {
+  float mnx = 1e30f, mxx = -1e30f, mny = 1e30f, mxy = -1e30f;
+  const int cx[4] = {ex0, ex1, ex0, ex1}, cy[4] = {ey0, ey0, ey1, ey1};
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    float sx, sy;
+    warp_xy<MOTION>(mc, cx[i], cy[i], sx, sy);
+    mnx = fminf(mnx, sx); mxx = fmaxf(mxx, sx); mny = fminf(mny, sy); mxy = fmaxf(mxy, sy);
+    if (sx != sx || sy != sy) mnx = -1e30f;
+  }
+  return mnx >= lo && mny >= lo && mxx <= hx && mxy <= hy;
+}
+
 // WMAP: per-frame weight images (frames' own grid, f32) remapped with the
 // frame's interpolation and BORDER_CONSTANT 0; else a per-frame scalar weight,
 // remapped as a constant image.  Either is then gated by the mask (weights *
@@ -180,15 +198,31 @@ __global__ void __launch_bounds__(256) k_accumulate(AccumArgs a) {
   // MOTION_MAP: the source positions come from a dense per-frame field, staged in LDS for the
   // tile and its 2-px erosion halo (the whole tile takes the bordered path)
   constexpr bool DENSE = MOTION == MOTION_MAP;
+  // parametric motion, bilinear sampling and scalar weights: the path of the stack batch
+  constexpr bool FAST = !DENSE && !WMAP && INTERP == INTER_LINEAR;
+  // the bordered path's warped mask and its erosion
   __shared__ uint8_t mk[TH + 4][AC_TW + 4];
   __shared__ uint8_t rmk[TH + 4][AC_TW];
   __shared__ float2 smap[DENSE ? (TH + 4) * (AC_TW + 4) : 1];
   __shared__ float sred[DENSE ? AC_WAVES * 4 : 1];
   const int tid = threadIdx.x, lane = tid & (WAVE - 1), wv = tid / WAVE;
-  const int x0 = blockIdx.x * AC_TW, y0 = blockIdx.y * TH;
-  const int chunk = blockIdx.z;
+  // XCD-aware tile order: blocks are dispatched round-robin over the 8 XCDs, so dispatch slot b
+  // runs on XCD b % 8; each XCD takes a contiguous range of row-major tiles, whose neighbours
+  // (left/right and above/below) then fetch the source lines they share through that XCD's L2
+  int tx, ty, chunk;
+  {
+    const int G = gridDim.x * gridDim.y * gridDim.z;
+    const int b = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
+    const int q = G >> 3, rm = G & 7, j = b & 7;
+    const int t = j * q + min(j, rm) + (b >> 3);
+    tx = t % gridDim.x;
+    const int rest = t / gridDim.x;
+    ty = rest % gridDim.y;
+    chunk = rest / gridDim.y;
+  }
+  const int x0 = tx * AC_TW, y0 = ty * TH;
   const int kb = chunk * a.per_chunk, kend = min(kb + a.per_chunk, a.nframes);
-  const int x = x0 + lane;
+  const int x = x0 + lane, xc = min(x, a.dw - 1);  // lanes past the right edge sample column dw - 1
   float shi[RPT][CN], slo[RPT][CN], sg[RPT][CN];
   float whi[RPT], wlo[RPT], wg[RPT];
   unsigned cnt[RPT];
@@ -199,12 +233,65 @@ __global__ void __launch_bounds__(256) k_accumulate(AccumArgs a) {
     whi[r] = wlo[r] = wg[r] = 0.f;
     cnt[r] = 0;
   }
+  int ingroup = 0;
+  // every AC_GROUP frames the plain group sums fold into the compensated pairs
+  auto fold = [&]() {
+    ingroup = 0;
+#pragma unroll
+    for (int r = 0; r < RPT; ++r) {
+#pragma unroll
+      for (int c = 0; c < CN; ++c) {
+        float s_, e_;
+        two_sum(shi[r][c], sg[r][c], s_, e_);
+        shi[r][c] = s_;
+        slo[r][c] += e_;
+        sg[r][c] = 0.f;
+      }
+      float s_, e_;
+      two_sum(whi[r], wg[r], s_, e_);
+      whi[r] = s_;
+      wlo[r] += e_;
+      wg[r] = 0.f;
+    }
+  };
   const float lo_m = INTERP == INTER_CUBIC ? 1.25f : 0.25f;
   const float hi_x = (float)a.sw - (INTERP == INTER_CUBIC ? 3.25f : 1.25f);
   const float hi_y = (float)a.sh - (INTERP == INTER_CUBIC ? 3.25f : 1.25f);
-  int ingroup = 0;
-  for (int k = kb; k < kend; ++k) {
-    if (!a.valid[k]) continue;
+  // INTER_LINEAR tap weights of the 32 x 32 sub-pixel positions (cv::remap's table)
+  __shared__ float4 wtab[FAST ? 1024 : 1];
+  if constexpr (FAST) {
+    for (int i = tid; i < 1024; i += 256) {
+      float w0, w1, w2, w3;
+      bilinear_w(i & 31, i >> 5, w0, w1, w2, w3);
+      wtab[i] = make_float4(w0, w1, w2, w3);
+    }
+    __syncthreads();
+  }
+  // tile + erosion halo, clipped to the destination
+  const int ex0 = max(x0 - 2, 0), ey0 = max(y0 - 2, 0);
+  const int ex1 = min(x0 + AC_TW + 1, a.dw - 1), ey1 = min(y0 + TH + 1, a.dh - 1);
+  const uint32_t pb = (uint32_t)a.src_pitch * 4u;
+  // source byte offset of tap (ix, iy) from the ubfe fields of quant32_bits (ix + 2^17, iy + 2^17)
+  const uint32_t obase = 0u - 0x20000u * (4u * CN + pb);
+  unsigned nint = 0;  // frames added through the interior path: they count for every pixel
+  for (int g0 = kb; g0 < kend; g0 += WAVE) {
+    // lane l classifies frame g0 + l: valid, and (parametric motions) whether the tile and its
+    // erosion halo sample inside the fast region, so one ballot covers 64 frames
+    unsigned long long vmask, imask;
+    {
+      const int kl = g0 + lane;
+      bool v = false, in = false;
+      if (kl < kend && a.valid[kl]) {
+        v = true;
+        if constexpr (!DENSE) in = tile_inside<MOTION>(a.maps[kl], ex0, ey0, ex1, ey1, lo_m, hi_x, hi_y);
+      }
+      vmask = __ballot(v);
+      imask = __ballot(in);
+    }
+  while (vmask) {
+    const int kj = __builtin_ctzll(vmask);
+    vmask &= vmask - 1;
+    const int k = g0 + kj;
     MapCoef mc;
     if constexpr (!DENSE) {
 #pragma unroll
@@ -213,22 +300,10 @@ __global__ void __launch_bounds__(256) k_accumulate(AccumArgs a) {
     const float2* __restrict__ rm = DENSE ? a.rmaps[k] : nullptr;
     const float wk = a.weights[k];
     const float* __restrict__ img = a.frames[k];
-    // uniform test: tile + erosion halo inside the sampling fast region
-    const int ex0 = max(x0 - 2, 0), ey0 = max(y0 - 2, 0);
-    const int ex1 = min(x0 + AC_TW + 1, a.dw - 1), ey1 = min(y0 + TH + 1, a.dh - 1);
-    float mnx = 1e30f, mxx = -1e30f, mny = 1e30f, mxy = -1e30f;
-    if constexpr (!DENSE) {
-      const int cx[4] = {ex0, ex1, ex0, ex1}, cy[4] = {ey0, ey0, ey1, ey1};
-#pragma unroll
-      for (int i = 0; i < 4; ++i) {
-        float sx, sy;
-        warp_xy<MOTION>(mc, cx[i], cy[i], sx, sy);
-        mnx = fminf(mnx, sx); mxx = fmaxf(mxx, sx); mny = fminf(mny, sy); mxy = fmaxf(mxy, sy);
-        if (sx != sx || sy != sy) mnx = -1e30f;  // NaN positions take the bordered path
-      }
-    }
+    bool interior = (imask >> kj) & 1;
     if constexpr (DENSE) {
       // stage the field of the tile + halo and reduce its extent over the block
+      float mnx = 1e30f, mxx = -1e30f, mny = 1e30f, mxy = -1e30f;
       for (int i = tid; i < (TH + 4) * (AC_TW + 4); i += 256) {
         const int ly = i / (AC_TW + 4), lx = i - ly * (AC_TW + 4);
         const int px = x0 - 2 + lx, py = y0 - 2 + ly;
@@ -260,42 +335,105 @@ __global__ void __launch_bounds__(256) k_accumulate(AccumArgs a) {
         mny = fminf(mny, sred[q * 4 + 2]);
         mxy = fmaxf(mxy, sred[q * 4 + 3]);
       }
+      interior = mnx >= lo_m && mny >= lo_m && mxx <= hi_x && mxy <= hi_y;
     }
-    const bool interior = mnx >= lo_m && mny >= lo_m && mxx <= hi_x && mxy <= hi_y;
-    // Interior tiles with bilinear sampling take a phased path: every lane's source positions
-    // first, then all loads (raw buffer loads; lanes outside the destination sample a clamped
-    // pixel, whose position lies inside the tile's hull), then the weighted updates.  Two
-    // interior frames with scalar weights go together, so both frames' samples are in flight at
-    // once; per pixel the updates stay in frame order and the compensated fold keeps its
-    // 16-frame boundaries, so the sums are those of the one-frame path.
-    if constexpr (INTERP == INTER_LINEAR) {
+    // Interior tiles, parametric motion, bilinear sampling, scalar weights: every lane's source
+    // positions first, then all loads (raw buffer loads; lanes outside the destination sample a
+    // clamped pixel, whose position lies inside the tile's hull), then the weighted updates.  Two
+    // interior frames go together, so both frames' samples are in flight at once; per pixel the
+    // updates stay in frame order and the compensated fold keeps its 16-frame boundaries, so the
+    // sums are those of the one-frame path.  The positions quantise with quant32_bits, the tap
+    // weights come from the table.
+    if constexpr (FAST) {
       if (interior) {
-        int k2 = kend;
+        int k2 = -1;
+        if (ingroup + 2 <= AC_GROUP && vmask) {
+          const int j2 = __builtin_ctzll(vmask);
+          if ((imask >> j2) & 1) {
+            k2 = g0 + j2;
+            vmask &= vmask - 1;
+          }
+        }
+        const bool pair = k2 >= 0;
         MapCoef mc2;
-        if constexpr (!DENSE && !WMAP) {
-          if (ingroup + 2 <= AC_GROUP) {
-            int kk = k + 1;
-            while (kk < kend && !a.valid[kk]) ++kk;
-            if (kk < kend) {
 #pragma unroll
-              for (int i = 0; i < 8; ++i) mc2.c[i] = a.maps[kk].c[i];
-              float n0 = 1e30f, n1 = -1e30f, n2 = 1e30f, n3 = -1e30f;
-              const int cx[4] = {ex0, ex1, ex0, ex1}, cy[4] = {ey0, ey0, ey1, ey1};
+        for (int i = 0; i < 8; ++i) mc2.c[i] = a.maps[pair ? k2 : k].c[i];
+        uint32_t o1[RPT], q1[RPT], o2[RPT], q2[RPT];
 #pragma unroll
-              for (int i = 0; i < 4; ++i) {
-                float sx, sy;
-                warp_xy<MOTION>(mc2, cx[i], cy[i], sx, sy);
-                n0 = fminf(n0, sx); n1 = fmaxf(n1, sx); n2 = fminf(n2, sy); n3 = fmaxf(n3, sy);
-                if (sx != sx || sy != sy) n0 = -1e30f;
-              }
-              if (n0 >= lo_m && n2 >= lo_m && n1 <= hi_x && n3 <= hi_y) k2 = kk;
+        for (int r = 0; r < RPT; ++r) {
+          const int yc = min(y0 + wv * RPT + r, a.dh - 1);
+          float sx, sy;
+          warp_xy<MOTION>(mc, xc, yc, sx, sy);
+          uint32_t tx = quant32_bits(sx), ty = quant32_bits(sy);
+          o1[r] = __umul24(__builtin_amdgcn_ubfe(ty, 5, 18), pb) + (__builtin_amdgcn_ubfe(tx, 5, 18) * (4u * CN) + obase);
+          q1[r] = ((ty & 31u) << 5) | (tx & 31u);
+          if (pair) {
+            warp_xy<MOTION>(mc2, xc, yc, sx, sy);
+            tx = quant32_bits(sx);
+            ty = quant32_bits(sy);
+            o2[r] = __umul24(__builtin_amdgcn_ubfe(ty, 5, 18), pb) + (__builtin_amdgcn_ubfe(tx, 5, 18) * (4u * CN) + obase);
+            q2[r] = ((ty & 31u) << 5) | (tx & 31u);
+          }
+        }
+        const rsrc_t r1 = make_rsrc(img, (uint32_t)a.sh * pb);
+        const rsrc_t r2 = make_rsrc(pair ? a.frames[k2] : img, (uint32_t)a.sh * pb);
+        float t1[RPT][4][CN], t2[RPT][4][CN];
+#pragma unroll
+        for (int r = 0; r < RPT; ++r) {
+#pragma unroll
+          for (int c = 0; c < CN; ++c) {
+            t1[r][0][c] = bload_f32(r1, o1[r] + 4u * c, 0u);
+            t1[r][1][c] = bload_f32(r1, o1[r] + 4u * (CN + c), 0u);
+            t1[r][2][c] = bload_f32(r1, o1[r] + 4u * c, pb);
+            t1[r][3][c] = bload_f32(r1, o1[r] + 4u * (CN + c), pb);
+          }
+        }
+        if (pair) {
+#pragma unroll
+          for (int r = 0; r < RPT; ++r) {
+#pragma unroll
+            for (int c = 0; c < CN; ++c) {
+              t2[r][0][c] = bload_f32(r2, o2[r] + 4u * c, 0u);
+              t2[r][1][c] = bload_f32(r2, o2[r] + 4u * (CN + c), 0u);
+              t2[r][2][c] = bload_f32(r2, o2[r] + 4u * c, pb);
+              t2[r][3][c] = bload_f32(r2, o2[r] + 4u * (CN + c), pb);
             }
           }
         }
-        const bool pair = k2 < kend;
-        const int xc = min(x, a.dw - 1);
-        const uint32_t pb = (uint32_t)a.src_pitch * 4u;
-        int X1[RPT], Y1[RPT], X2[RPT], Y2[RPT];
+        const float wk2 = pair ? a.weights[k2] : 0.f;
+        // lanes and rows outside the destination accumulate their clamped pixel too (never
+        // written), which keeps the updates free of branches
+#pragma unroll
+        for (int r = 0; r < RPT; ++r) {
+          float4 w = wtab[q1[r]];
+          // the scalar weight remapped as a constant image (const_sample inside the image)
+          const float wp = bilinear_sum(wk, wk, wk, wk, w.x, w.y, w.z, w.w);
+#pragma unroll
+          for (int c = 0; c < CN; ++c)
+            sg[r][c] = fmaf(wp, bilinear_sum(t1[r][0][c], t1[r][1][c], t1[r][2][c], t1[r][3][c], w.x, w.y, w.z, w.w),
+                            sg[r][c]);
+          wg[r] += wp;
+          if (pair) {
+            w = wtab[q2[r]];
+            const float wp2 = bilinear_sum(wk2, wk2, wk2, wk2, w.x, w.y, w.z, w.w);
+#pragma unroll
+            for (int c = 0; c < CN; ++c)
+              sg[r][c] = fmaf(wp2, bilinear_sum(t2[r][0][c], t2[r][1][c], t2[r][2][c], t2[r][3][c], w.x, w.y, w.z, w.w),
+                              sg[r][c]);
+            wg[r] += wp2;
+          }
+        }
+        const int done = pair ? 2 : 1;
+        nint += done;
+        ingroup += done;
+        if (ingroup == AC_GROUP) fold();
+        continue;
+      }
+    }
+    // the interior path of the dense fields and the weight images: one frame at a time
+    if constexpr (INTERP == INTER_LINEAR && !FAST) {
+      if (interior) {
+        int X1[RPT], Y1[RPT];
 #pragma unroll
         for (int r = 0; r < RPT; ++r) {
           const int yc = min(y0 + wv * RPT + r, a.dh - 1);
@@ -309,15 +447,9 @@ __global__ void __launch_bounds__(256) k_accumulate(AccumArgs a) {
           }
           X1[r] = quant32(sx);
           Y1[r] = quant32(sy);
-          if (pair) {
-            warp_xy<MOTION>(mc2, xc, yc, sx, sy);
-            X2[r] = quant32(sx);
-            Y2[r] = quant32(sy);
-          }
         }
         const rsrc_t r1 = make_rsrc(img, (uint32_t)a.sh * pb);
-        const rsrc_t r2 = make_rsrc(pair ? a.frames[k2] : img, (uint32_t)a.sh * pb);
-        float t1[RPT][4][CN], t2[RPT][4][CN];
+        float t1[RPT][4][CN];
 #pragma unroll
         for (int r = 0; r < RPT; ++r) {
           const uint32_t o = (uint32_t)(Y1[r] >> 5) * pb + (uint32_t)(X1[r] >> 5) * (4u * CN);
@@ -329,20 +461,6 @@ __global__ void __launch_bounds__(256) k_accumulate(AccumArgs a) {
             t1[r][3][c] = bload_f32(r1, o + pb + 4u * (CN + c), 0u);
           }
         }
-        if (pair) {
-#pragma unroll
-          for (int r = 0; r < RPT; ++r) {
-            const uint32_t o = (uint32_t)(Y2[r] >> 5) * pb + (uint32_t)(X2[r] >> 5) * (4u * CN);
-#pragma unroll
-            for (int c = 0; c < CN; ++c) {
-              t2[r][0][c] = bload_f32(r2, o + 4u * c, 0u);
-              t2[r][1][c] = bload_f32(r2, o + 4u * (CN + c), 0u);
-              t2[r][2][c] = bload_f32(r2, o + pb + 4u * c, 0u);
-              t2[r][3][c] = bload_f32(r2, o + pb + 4u * (CN + c), 0u);
-            }
-          }
-        }
-        const float wk2 = pair ? a.weights[k2] : 0.f;
         const float* __restrict__ wimg = WMAP ? a.wmaps[k] : nullptr;
 #pragma unroll
         for (int r = 0; r < RPT; ++r) {
@@ -364,40 +482,9 @@ __global__ void __launch_bounds__(256) k_accumulate(AccumArgs a) {
                             sg[r][c]);
           wg[r] += wp;
           cnt[r] += 1;
-          if (pair) {
-            bilinear_w(X2[r] & 31, Y2[r] & 31, w0, w1, w2, w3);
-            const float wp2 = const_sample<INTERP>(wk2, a.sw, a.sh, X2[r], Y2[r]);
-#pragma unroll
-            for (int c = 0; c < CN; ++c)
-              sg[r][c] = fmaf(wp2, bilinear_sum(t2[r][0][c], t2[r][1][c], t2[r][2][c], t2[r][3][c], w0, w1, w2, w3),
-                              sg[r][c]);
-            wg[r] += wp2;
-            cnt[r] += 1;
-          }
         }
         if (DENSE) __syncthreads();  // the next frame rewrites the staged field
-        const int done = pair ? 2 : 1;
-        if (pair) k = k2;
-        ingroup += done;
-        if (ingroup == AC_GROUP) {
-          ingroup = 0;
-#pragma unroll
-          for (int r = 0; r < RPT; ++r) {
-#pragma unroll
-            for (int c = 0; c < CN; ++c) {
-              float s_, e_;
-              two_sum(shi[r][c], sg[r][c], s_, e_);
-              shi[r][c] = s_;
-              slo[r][c] += e_;
-              sg[r][c] = 0.f;
-            }
-            float s_, e_;
-            two_sum(whi[r], wg[r], s_, e_);
-            whi[r] = s_;
-            wlo[r] += e_;
-            wg[r] = 0.f;
-          }
-        }
+        if (++ingroup == AC_GROUP) fold();
         continue;
       }
     }
@@ -440,7 +527,7 @@ __global__ void __launch_bounds__(256) k_accumulate(AccumArgs a) {
         sx = mv.x;
         sy = mv.y;
       } else {
-        warp_xy<MOTION>(mc, x, y, sx, sy);
+        warp_xy<MOTION>(mc, xc, min(y, a.dh - 1), sx, sy);  // x < dw, y < dh here
       }
       const int X = quant32(sx), Y = quant32(sy);
       float v[CN];
@@ -477,25 +564,8 @@ __global__ void __launch_bounds__(256) k_accumulate(AccumArgs a) {
     }
     // the next frame rewrites the LDS tiles (masks, staged field)
     if (DENSE || !interior) __syncthreads();
-    if (++ingroup == AC_GROUP) {
-      ingroup = 0;
-#pragma unroll
-      for (int r = 0; r < RPT; ++r) {
-#pragma unroll
-        for (int c = 0; c < CN; ++c) {
-          float s, e;
-          two_sum(shi[r][c], sg[r][c], s, e);
-          shi[r][c] = s;
-          slo[r][c] += e;
-          sg[r][c] = 0.f;
-        }
-        float s, e;
-        two_sum(whi[r], wg[r], s, e);
-        whi[r] = s;
-        wlo[r] += e;
-        wg[r] = 0.f;
-      }
-    }
+    if (++ingroup == AC_GROUP) fold();
+  }
   }
 #pragma unroll
   for (int r = 0; r < RPT; ++r) {
@@ -514,12 +584,14 @@ __global__ void __launch_bounds__(256) k_accumulate(AccumArgs a) {
     two_sum(whi[r], wg[r], s, e);
     P[(size_t)(2 * CN) * a.plane + o] = s;
     P[(size_t)(2 * CN + 1) * a.plane + o] = wlo[r] + e;
-    a.part_cnt[(size_t)chunk * a.plane + o] = cnt[r];
+    a.part_cnt[(size_t)chunk * a.plane + o] = cnt[r] + nint;
   }
 }
 
 hipError_t launch_accumulate(int motion, const AccumArgs& a, hipStream_t s) {
   const int nch = a.nchunks;
+  // the interior path forms tap offsets with 24-bit multiplies and positions with quant32_bits
+  if ((long long)a.src_pitch * 4 >= (1LL << 24) || a.sw >= (1 << 17) || a.sh >= (1 << 17)) return hipErrorInvalidValue;
   // tile height: a.rows_per_thread rows per thread (2 or 4) times AC_WAVES waves
   const int rpt = a.rows_per_thread == 2 ? 2 : 4;
   dim3 grid((a.dw + AC_TW - 1) / AC_TW, (a.dh + rpt * AC_WAVES - 1) / (rpt * AC_WAVES), nch);

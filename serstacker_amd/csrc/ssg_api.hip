@@ -421,7 +421,7 @@ extern "C" int ssg_device_info(int* num_devices, char* name, int name_len) {
     HIPCHK(hipGetDevice(&dev));
     hipDeviceProp_t prop;
     HIPCHK(hipGetDeviceProperties(&prop, dev));
-    snprintf(name, name_len, "%s (%s)", prop.name, prop.gcnArchName);
+    snprintf(name, name_len, "%s (%s, %d CUs)", prop.name, prop.gcnArchName, prop.multiProcessorCount);
   }
   return SSG_OK;
 }

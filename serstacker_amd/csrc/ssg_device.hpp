@@ -62,6 +62,12 @@ __device__ __forceinline__ int quant32(float v) {
   return __float2int_rn(s);
 }
 
+// quant32 for |v| < 2^17 as the bits of one fma: v*32 is exact, and adding 1.5*2^23 rounds it to
+// the nearest integer, ties to even, into the low mantissa bits, so the result's bits are
+// 0x4B400000 + quant32(v): bits 0..4 are quant32(v) & 31, and ubfe(bits, 5, 18) is
+// (quant32(v) >> 5) + 2^17
+__device__ __forceinline__ uint32_t quant32_bits(float v) { return __float_as_uint(fmaf(v, 32.f, 12582912.f)); }
+
 __device__ __forceinline__ int sat_short(int v) { return min(max(v, -32768), 32767); }
 
 // INTER_NEAREST position of a float map coordinate: saturate_cast<short>(float),

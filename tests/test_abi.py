@@ -219,6 +219,11 @@ def test_tuning_knobs_and_autotuner_cache(tmp_path):
     assert L.ssg_tune_set(_lib.TUNE_ECC_GRID, 99) == -1
     assert L.ssg_tune_set(7, 1) == -1
     assert autotune.size_class(1920, 1080, 2000) == "2048kpxx2048"
+    assert autotune.device_key("AMD Instinct MI355X (gfx950:sramecc+:xnack-, 256 CUs)") == \
+        "gfx950:sramecc+:xnack-, 256 CUs"
+    import json
+    shipped = json.load(open(autotune.SHIPPED))
+    assert shipped and all(k.split("|")[0].startswith("gfx950") for k in shipped)
     calls = []
 
     def run(choice):
