@@ -641,7 +641,7 @@ int ssg_master_stack_window(int32_t size, int32_t start_frame_index, int32_t mas
  * or SSG_32FC3, host or device; mask: SSG_8UC1 of the image size (not modified). */
 int ssg_inpaint_linear(ssg_image *image, const ssg_image *mask);
 
-/* ---- stacking at a larger size: upscale of the registration remaps ---------------------------- */
+/* ---- stacking at a larger size: upscale of the frames before align, of the remaps after ------- */
 
 /* frame_upscale_option (c_image_stacking_pipeline.h:43-49). */
 enum { SSG_UPSCALE_NONE = 0, SSG_UPSCALE_X2 = 1, SSG_UPSCALE_X15 = 2, SSG_UPSCALE_X3 = 3 };
@@ -656,6 +656,19 @@ int ssg_upscale_size(int32_t option, int32_t w, int32_t h, int32_t *dw, int32_t 
  * go in one launch; host maps are staged.  The upscaled remaps feed
  * ssg_accumulation_add_remapped_batch with an accumulator of the upscaled size. */
 int ssg_upscale_remaps(int32_t option, int32_t n, const ssg_image *src, ssg_image *dst);
+/* upscale_image(scale, src, srcmask, dst, dstmask) (c_image_stacking_pipeline.cc:1949-2000) of n
+ * frames, the frame_upscale_before_align steps: the reference frame and its mask before
+ * setup_reference_frame (:914-918), each frame with its mask and its weight map before
+ * register_frame (:1474-1492).  x1.5 cv::resize INTER_LINEAR, x2 cv::pyrUp, x3 INTER_LINEAR_EXACT
+ * (float data: INTER_LINEAR); the masks take OpenCV's 8-bit arithmetic of the same resize and are
+ * then compare(dstmask, 255, CMP_GE) (0 / 255); none copies both.  src: SSG_32FC1 or SSG_32FC3
+ * of one size and type, host or device, 4-byte aligned steps; dst: the same type at
+ * ssg_upscale_size.  srcmask: NULL (no masks) or n SSG_8UC1 images of the source size, an entry
+ * with data NULL meaning that frame has no mask; dstmask: n SSG_8UC1 of the upscaled size (the
+ * entries of frames without a mask are not written).  Device images with one step per side go
+ * in one launch per 1024; host images are staged. */
+int ssg_upscale_images(int32_t option, int32_t n, const ssg_image *src, const ssg_image *srcmask, ssg_image *dst,
+                       ssg_image *dstmask);
 
 /* average_pyramid_inpaint(src, mask, dst, dstmask, max_levels)
  * (core/proc/inpaint/average_pyramid_inpaint.cc:96-126), the pipeline's fill of the accumulated

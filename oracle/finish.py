@@ -1,6 +1,8 @@
 """CPU oracle of the stack finish (test infrastructure only): linear-interpolation
-inpainting (core/proc/inpaint/linear_interpolation_inpaint.cc) and unsharp_mask
-(core/proc/unsharp_mask.cc:15-115).
+inpainting (core/proc/inpaint/linear_interpolation_inpaint.cc), unsharp_mask
+(core/proc/unsharp_mask.cc:15-115), average_pyramid_inpaint, and the frame and remap
+upscales of stacking at a larger size (c_image_stacking_pipeline.cc:1869-2000) with
+OpenCV's resize / pyrUp on float and 8-bit data restated from its generic C++ paths.
 
 Only tests/, ``__graft_entry__.smoke()`` and bench.py's CPU leg use this module.
 The inpainting follows the reference's run loops literally (pure Python over
@@ -231,6 +233,107 @@ UPSCALE_NONE, UPSCALE_X2, UPSCALE_X15, UPSCALE_X3 = 0, 1, 2, 3
 def upscale_size(option, w, h):
     return {UPSCALE_X2: (2 * w, 2 * h), UPSCALE_X15: (w * 3 // 2, h * 3 // 2), UPSCALE_X3: (3 * w, 3 * h)}.get(
         option, (w, h))
+
+
+def _fix11(w):
+    """resize.cpp's 8-bit weights: saturate_cast<short>(w * INTER_RESIZE_COEF_SCALE), 11 bits,
+    rounded half to even."""
+    return np.rint(w.astype(np.float32) * np.float32(2048)).astype(np.int64)
+
+
+def resize_linear_u8(src, dw, dh):
+    """cv::resize INTER_LINEAR of an 8-bit plane (resize.cpp's generic fixed-point path): 11-bit
+    weights, the horizontal pass in int (S[x0] * a0 + S[x1] * a1), the vertical pass of the
+    8-bit VResizeLinear: ((b0 * (R0 >> 4)) >> 16) + ((b1 * (R1 >> 4)) >> 16) + 2) >> 2."""
+    from oracle import lvm
+
+    h, w = src.shape
+    x0, x1, a0, a1 = lvm.resize_linear_coeffs(w, dw)
+    y0, y1, b0, b1 = lvm.resize_linear_vcoeffs(h, dh)
+    s = src.astype(np.int64)
+    rows = s[:, x0] * _fix11(a0) + s[:, x1] * _fix11(a1)
+    v = (((_fix11(b0)[:, None] * (rows[y0] >> 4)) >> 16) + ((_fix11(b1)[:, None] * (rows[y1] >> 4)) >> 16) + 2) >> 2
+    return v.astype(np.uint8)
+
+
+def _exact_coeffs(ssize, dsize):
+    """interpolationLinear::getCoeffs (resize.cpp, bit-exact resize): fval = scale * (d + 0.5)
+    - 0.5 in double; inside the source the weights are 8-bit fixed point, c1 = round-half-even
+    (256 (fval - floor(fval))), c0 = 256 - c1; before the source or from its last sample on, the
+    edge sample with (256, 0)."""
+    scale = 1.0 / (dsize / ssize)
+    x0 = np.zeros(dsize, np.int64)
+    c1 = np.zeros(dsize, np.int64)
+    for d in range(dsize):
+        f = scale * (d + 0.5) - 0.5
+        i = math.floor(f)
+        if i >= 0 and ssize > 1:
+            if i < ssize - 1:
+                x0[d], c1[d] = i, int(np.rint((f - i) * 256.0))
+            else:
+                x0[d] = ssize - 1
+    return x0, np.minimum(x0 + 1, ssize - 1), 256 - c1, c1
+
+
+def resize_linear_exact_u8(src, dw, dh):
+    """cv::resize INTER_LINEAR_EXACT of an 8-bit plane: the horizontal pass exact in 8-bit fixed
+    point, the vertical one exact in 16-bit, rounded half up: (sum + 2^15) >> 16."""
+    h, w = src.shape
+    x0, x1, c0, c1 = _exact_coeffs(w, dw)
+    y0, y1, d0, d1 = _exact_coeffs(h, dh)
+    s = src.astype(np.int64)
+    rows = s[:, x0] * c0 + s[:, x1] * c1
+    v = rows[y0] * d0[:, None] + rows[y1] * d1[:, None]
+    return ((v + 32768) >> 16).astype(np.uint8)
+
+
+def pyr_up_u8(src):
+    """cv::pyrUp of an 8-bit plane to (2w, 2h): [1 6 1] / [4 4] per axis summed in int, left/top
+    REFLECT101, right/bottom edge replicated, (sum + 32) >> 6."""
+    h, w = src.shape
+    s = src.astype(np.int64)
+    if w == 1:
+        rows = np.repeat(s * 8, 2, axis=1)
+    else:
+        left = np.concatenate([s[:, 1:2], s[:, :-1]], axis=1)
+        right = np.concatenate([s[:, 1:], s[:, -1:]], axis=1)
+        rows = np.empty((h, 2 * w), np.int64)
+        rows[:, 0::2] = left + 6 * s + right
+        rows[:, 1::2] = 4 * (s + right)
+    up = rows[1 if h > 1 else 0][None, :]
+    down = rows[-1][None, :]
+    prev = np.concatenate([up, rows[:-1]], axis=0)
+    nxt = np.concatenate([rows[1:], down], axis=0)
+    out = np.empty((2 * h, 2 * w), np.int64)
+    out[0::2] = prev + 6 * rows + nxt
+    out[1::2] = 4 * (rows + nxt)
+    return ((out + 32) >> 6).astype(np.uint8)
+
+
+def upscale_image(ora, src, mask, option):
+    """upscale_image (c_image_stacking_pipeline.cc:1949-2000) of an f32 image (h, w) or (h, w, cn)
+    and its optional 8-bit mask: x1.5 cv::resize INTER_LINEAR, x2 cv::pyrUp, x3 INTER_LINEAR_EXACT
+    (on float data the same as INTER_LINEAR); the upscaled mask is then compare(>= 255) (0 / 255).
+    The none option copies both."""
+    from oracle import lvm
+
+    src = np.asarray(src, np.float32)
+    if option == UPSCALE_NONE:
+        return src.copy(), (None if mask is None else mask.copy())
+    h, w = src.shape[:2]
+    dw, dh = upscale_size(option, w, h)
+    planes = [src] if src.ndim == 2 else [np.ascontiguousarray(src[..., c]) for c in range(src.shape[2])]
+    if option == UPSCALE_X2:
+        up = [ora.pyr_up(p) for p in planes]
+    else:
+        up = [lvm.resize_linear(p, dw, dh) for p in planes]
+    dst = up[0] if src.ndim == 2 else np.stack(up, -1)
+    dmask = None
+    if mask is not None:
+        m = {UPSCALE_X15: lambda a: resize_linear_u8(a, dw, dh), UPSCALE_X2: pyr_up_u8,
+             UPSCALE_X3: lambda a: resize_linear_exact_u8(a, dw, dh)}[option](np.asarray(mask, np.uint8))
+        dmask = np.where(m >= 255, 255, 0).astype(np.uint8)
+    return dst.astype(np.float32), dmask
 
 
 def upscale_map(ora, m, option):

@@ -11,7 +11,9 @@ oracle's pyrDown and numpy float32 elementwise arithmetic:
      (the reference sums in float with an atomic compare-exchange per TBB range, so its
      low bits depend on the scheduling),
   5. map = G^3 * float(depthScale^3) + float(0.05 Q), depthScale = 20 for float images,
-  6. cv::resize INTER_LINEAR back to the input size (half-pixel centres, clamped taps).
+  6. cv::resize INTER_LINEAR back to the input size (half-pixel centres; columns outside
+     the source take the edge column with weights (1, 0), rows outside it are clipped to
+     the edge row with both weights kept, as resize.cpp's generic path does).
 
 8-bit and 16-bit single-channel images stay integer through the pyramid (cv::pyrDown
 on integer depths sums the 5x5 taps in int and rounds with (sum + 128) >> 8) and use
@@ -69,7 +71,9 @@ def morph_gradient(m, r):
 
 
 def resize_linear_coeffs(ssize, dsize):
-    """cv::resize INTER_LINEAR source taps and float weights along one axis."""
+    """cv::resize INTER_LINEAR source columns and float weights (imgproc resize.cpp, generic
+    path): fx = (float)((dx + 0.5) * scale - 0.5), sx = floor(fx), fx -= sx; a column left of
+    the source (sx < 0) or at/after its last one takes that edge column with fx = 0."""
     scale = 1.0 / (dsize / ssize)  # resize.cpp: scale_x = 1 / inv_scale_x
     sx = np.zeros(dsize, np.int64)
     fx = np.zeros(dsize, F32)
@@ -85,10 +89,26 @@ def resize_linear_coeffs(ssize, dsize):
     return sx, np.minimum(sx + 1, ssize - 1), (F32(1) - fx), fx
 
 
+def resize_linear_vcoeffs(ssize, dsize):
+    """The rows of cv::resize INTER_LINEAR: fy and sy as for the columns, but fy is never
+    reset; the two rows sy, sy + 1 are clipped to [0, ssize - 1] and keep the weights
+    (1 - fy, fy) (resizeGeneric_Invoker's clip(sy, 0, ssize.height))."""
+    scale = 1.0 / (dsize / ssize)
+    sy = np.zeros(dsize, np.int64)
+    fy = np.zeros(dsize, F32)
+    for d in range(dsize):
+        f = np.float32((d + 0.5) * scale - 0.5)
+        s = int(np.floor(f))
+        sy[d], fy[d] = s, np.float32(f - np.float32(s))
+    return np.clip(sy, 0, ssize - 1), np.clip(sy + 1, 0, ssize - 1), (F32(1) - fy), fy
+
+
 def resize_linear(m, dw, dh):
+    """cv::resize INTER_LINEAR of a float32 plane: the horizontal pass S[x0] * a0 + S[x1] * a1,
+    then the vertical R[y0] * b0 + R[y1] * b1, each in float32."""
     h, w = m.shape
     x0, x1, a0, a1 = resize_linear_coeffs(w, dw)
-    y0, y1, b0, b1 = resize_linear_coeffs(h, dh)
+    y0, y1, b0, b1 = resize_linear_vcoeffs(h, dh)
     rows = m[:, x0] * a0 + m[:, x1] * a1  # horizontal pass, float32
     return (rows[y0] * b0[:, None] + rows[y1] * b1[:, None]).astype(F32)
 

@@ -30,7 +30,7 @@ log = logging.getLogger("serstacker_amd")
 __all__ = ["c_image_registration_options", "c_frame_registration", "c_weigthed_average", "image_view",
            "c_eccflow", "synth_params", "synth_render", "SsgError", "c_ser_reader", "debayer", "read_input_frames",
            "inpaint_linear", "average_pyramid_inpaint", "unsharp_mask", "local_variance_map", "upscale_size",
-           "upscale_remaps",
+           "upscale_remaps", "upscale_images",
            "master_frame_quality", "master_scan_window", "select_master_frame", "master_stack_window",
            "ecc_normalize", "focus_stack_batch"]
 
@@ -876,6 +876,27 @@ def upscale_remaps(option: int, maps, out=None):
     sv, dv = image_views(maps), image_views(out, out=True)
     _lib.check(_lib.load().ssg_upscale_remaps(option, len(maps), sv, dv), "ssg_upscale_remaps")
     return out
+
+
+def upscale_images(option: int, frames, masks=None, out=None, out_masks=None):
+    """upscale_image (c_image_stacking_pipeline.cc:1949-2000) of f32 frames ((H, W) or (H, W, 3),
+    host arrays or device tensors) and their optional u8 masks (a list; None entries for frames
+    without one).  Returns (frames, masks) at the upscaled size: host arrays when `out` /
+    `out_masks` are omitted, masks None where a frame had none."""
+    h, w = frames[0].shape[:2]
+    dw, dh = upscale_size(option, w, h)
+    if out is None:
+        out = [np.empty((dh, dw) + tuple(frames[0].shape[2:]), np.float32) for _ in frames]
+    sv, dv = image_views(frames), image_views(out, out=True)
+    mv = dmv = None
+    if masks is not None:
+        if out_masks is None:
+            out_masks = [None if m is None else np.empty((dh, dw), np.uint8) for m in masks]
+        empty = _lib.Image()  # data NULL: this frame has no mask
+        mv = _lib.images([empty if m is None else image_view(m) for m in masks])
+        dmv = _lib.images([empty if m is None else image_view(om, out=True) for m, om in zip(masks, out_masks)])
+    _lib.check(_lib.load().ssg_upscale_images(option, len(frames), sv, mv, dv, dmv), "ssg_upscale_images")
+    return out, (out_masks if masks is not None else None)
 
 
 def unsharp_mask(src, sigma: float, alpha: float, out=None):

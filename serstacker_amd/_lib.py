@@ -239,6 +239,7 @@ _SIGS = [
     ("ssg_average_pyramid_inpaint", C.c_int, [_P(Image), _P(Image), _P(Image), _P(Image), C.c_int32]),
     ("ssg_upscale_size", C.c_int, [C.c_int32, C.c_int32, C.c_int32, _P(C.c_int32), _P(C.c_int32)]),
     ("ssg_upscale_remaps", C.c_int, [C.c_int32, C.c_int32, _P(Image), _P(Image)]),
+    ("ssg_upscale_images", C.c_int, [C.c_int32, C.c_int32, _P(Image), _P(Image), _P(Image), _P(Image)]),
     ("ssg_read_input_frames", C.c_int, [C.c_int32, _P(Image), C.c_int32, C.c_int32, C.c_int32, _P(Image), _P(Image),
                                         _P(Image)]),
     ("ssg_local_variance_map", C.c_int, [_P(Image), C.c_int32, C.c_int32, _P(Image), _P(C.c_double)]),

@@ -94,6 +94,18 @@ __device__ __forceinline__ void lin_tap(int d, double scale, int ssize, int& s0,
   a1 = f;
 }
 
+// The rows of the same resize: fy is never reset; the rows sy and sy + 1 are clipped to the
+// source and keep the weights (1 - fy, fy) (resize.cpp resizeGeneric_Invoker, clip(sy, 0, h))
+__device__ __forceinline__ void lin_tap_v(int d, double scale, int ssize, int& s0, int& s1, float& b0, float& b1) {
+  float f = (float)(((double)d + 0.5) * scale - 0.5);
+  const int s = (int)floorf(f);
+  f -= (float)s;
+  s0 = min(max(s, 0), ssize - 1);
+  s1 = min(max(s + 1, 0), ssize - 1);
+  b0 = 1.f - f;
+  b1 = f;
+}
+
 // cv::pyrUp (float), horizontal pass: the upsampled value at output column xo of source row S
 // (ST floats between pixels): [1 6 1] at even columns, [4 4] at odd ones, left REFLECT101, right
 // replicated; an odd extra column repeats the last one

@@ -210,6 +210,12 @@ void upscale_size(int option, int w, int h, int* dw, int* dh);
 // n interleaved 2-channel f32 fields (pitches in floats) upscaled by the option (not NONE)
 hipError_t launch_upscale_maps(int option, int n, const float* const* src, int spitch, int sw, int sh,
                                float* const* dst, int dpitch, hipStream_t s);
+// n interleaved cn-channel f32 images (cn 1..3, pitches in floats) upscaled by the option (not NONE)
+hipError_t launch_upscale_f32(int option, int cn, int n, const float* const* src, int spitch, int sw, int sh,
+                              float* const* dst, int dpitch, hipStream_t s);
+// n 8-bit masks (pitches in bytes) upscaled by the option (not NONE), then compared with 255
+hipError_t launch_upscale_masks(int option, int n, const uint8_t* const* src, int spitch, int sw, int sh,
+                                uint8_t* const* dst, int dpitch, hipStream_t s);
 hipError_t launch_apr_init(const float* src, int spitch, const uint8_t* mask, int mpitch, int w, int h, int cn,
                            float* img, float* msk, unsigned* nz, hipStream_t s);
 hipError_t launch_apr_down(const float* img, const float* msk, int w, int h, int cn, float* dimg, float* dmsk, int dw,

@@ -79,7 +79,7 @@ __global__ void __launch_bounds__(256) k_resize_linear(const float* src, int sw,
   int x0, x1, y0, y1;
   float a0, a1, b0, b1;
   lin_tap(x, scale_x, sw, x0, x1, a0, a1);
-  lin_tap(y, scale_y, sh, y0, y1, b0, b1);
+  lin_tap_v(y, scale_y, sh, y0, y1, b0, b1);
   const float* r0 = src + (size_t)y0 * sw;
   const float* r1 = src + (size_t)y1 * sw;
   const float h0 = r0[x0] * a0 + r0[x1] * a1, h1 = r1[x0] * a0 + r1[x1] * a1;

@@ -1,4 +1,5 @@
-"""Small seeded planetary-disk frames in numpy (test inputs).
+"""Small seeded planetary-disk frames in numpy (test inputs), and the warp measures the
+registration tests compare with.
 
 A limb-darkened disk (I = 1 - 0.6 (1 - mu)) textured by gaussian-filtered
 uniform noise at 10%, warped by a known affine map, optionally blurred and
@@ -105,3 +106,23 @@ def blob_scene(w: int, h: int, seed: int, n: int = 60, smin: float = 3.0, smax: 
 
     render.at = at
     return render
+
+
+def warp_bar(d32, tol=0.01):
+    """The warp bar of a registration run to convergence: `tol` px, or twice the FP32 spread
+    where that is larger."""
+    return max(tol, 2.0 * d32)
+
+
+def fp32_spread(oreg32, o64, f, mask, w, h, seed, copies=8):
+    """Farthest end point of the FP32 oracle from the FP64 one, over the frame and `copies`
+    copies with each pixel scaled by (1 +- 2^-23): how far the reference algorithm at its own
+    precision is determined on this frame."""
+    rng = np.random.default_rng(seed)
+    d = 0.0
+    for k in range(copies + 1):
+        fk = f if k == 0 else (f * (1.0 + rng.choice([-1.0, 1.0], f.shape) * 2.0**-23)).astype(np.float32)
+        o32 = oreg32.register_frame(fk, mask)
+        if o32.ok:
+            d = max(d, max_displacement(o32.matrix, o64.matrix, w, h))
+    return d

@@ -55,22 +55,8 @@ def _frames(n=4, w=320, h=240, seed=11, scale=1.0):
     return ref, frames, truth
 
 
-def warp_bar(d32):
-    """0.01 px, or twice the FP32 spread where that is larger."""
-    return max(WARP_TOL_PX, 2.0 * d32)
-
-
-def fp32_spread(oreg32, o64, f, mask, w, h, seed, copies=8):
-    """Farthest end point of the FP32 oracle from the FP64 one, over the frame and `copies`
-    copies with each pixel scaled by (1 +- 2^-23)."""
-    rng = np.random.default_rng(seed)
-    d = 0.0
-    for k in range(copies + 1):
-        fk = f if k == 0 else (f * (1.0 + rng.choice([-1.0, 1.0], f.shape) * 2.0**-23)).astype(np.float32)
-        o32 = oreg32.register_frame(fk, mask)
-        if o32.ok:
-            d = max(d, synth_np.max_displacement(o32.matrix, o64.matrix, w, h))
-    return d
+warp_bar = synth_np.warp_bar
+fp32_spread = synth_np.fp32_spread
 
 
 def _compare(S, oracle64, kw, ref, frames, masks=None, ref_mask=None, same_counts=False, oracle32=None):

@@ -1,5 +1,6 @@
 """Pins of the stack-finish oracle (oracle/finish.py): linear-interpolation inpainting
-(linear_interpolation_inpaint.cc) and unsharp_mask (unsharp_mask.cc) by closed forms."""
+(linear_interpolation_inpaint.cc), unsharp_mask (unsharp_mask.cc) and the frame / mask / remap
+upscales of stacking at a larger size by closed forms and by the pinned FP64 pyrUp."""
 import math
 
 import numpy as np
@@ -157,6 +158,86 @@ def test_upscale_map_closed_forms(oracle32):
     np.testing.assert_array_equal(up[4:-4, 2:-2, 0], np.tile(np.arange(2, 2 * w - 2) / 2, (2 * h - 8, 1)))
     np.testing.assert_array_equal(up[2:-2, 4:-4, 1], np.tile((np.arange(2, 2 * h - 2) / 2)[:, None], (1, 2 * w - 8)))
     assert finish.upscale_size(finish.UPSCALE_X15, 33, 21) == (49, 31)
+
+
+@pytest.mark.parametrize("shape", [(8, 12), (7, 9), (1, 5), (5, 1)])
+def test_upscale_u8_constants(shape):
+    """Every constant 8-bit image stays that constant through the three 8-bit upscales (the
+    rounding terms of the fixed-point passes: + 2 >> 2 after the >> 4 / >> 16 products, + 2^15
+    >> 16, + 32 >> 6)."""
+    h, w = shape
+    for c in range(256):
+        m = np.full(shape, c, np.uint8)
+        assert np.all(F.resize_linear_u8(m, w * 3 // 2 or 1, h * 3 // 2 or 1) == c)
+        assert np.all(F.resize_linear_exact_u8(m, 3 * w, 3 * h) == c)
+        assert np.all(F.pyr_up_u8(m) == c)
+
+
+def test_upscale_u8_linear_tracks_the_float_resize():
+    """The fixed-point INTER_LINEAR of random 8-bit data is within 1 of the float32 resize of the
+    same data (same taps, weights exact to 2^-11): a swapped weight or a wrong tap is off by far
+    more."""
+    from oracle import lvm
+
+    r = np.random.default_rng(4)
+    for (h, w), (dh, dw) in (((20, 30), (30, 45)), ((13, 17), (19, 25)), ((6, 4), (9, 6))):
+        m = r.integers(0, 256, (h, w), dtype=np.uint8)
+        got = F.resize_linear_u8(m, dw, dh).astype(np.float64)
+        ref = lvm.resize_linear(m.astype(np.float32), dw, dh).astype(np.float64)
+        assert np.abs(got - ref).max() <= 1.0
+
+
+def test_upscale_u8_exact_x3_impulse():
+    """INTER_LINEAR_EXACT x3 of a single 255 sample: the 8-bit weights of 1/3 and 2/3 are 85 and
+    171, so the row through it reads ... 0, 85, 170, 255, 170, 85, 0 ... rounded half up
+    (255 * 171 / 256 = 170.3, 255 * 85 / 256 = 84.7), centred on output 3j + 1."""
+    m = np.zeros((5, 7), np.uint8)
+    m[2, 3] = 255
+    out = F.resize_linear_exact_u8(m, 21, 15)
+    np.testing.assert_array_equal(out[7, 7:14], [0, 85, 170, 255, 170, 85, 0])
+    np.testing.assert_array_equal(out[4:11, 10], [0, 85, 170, 255, 170, 85, 0])
+    # left / top edges replicate the first sample, the right / bottom ones the last
+    e = np.zeros((3, 4), np.uint8)
+    e[:, 0] = 200
+    e[:, 3] = 90
+    out = F.resize_linear_exact_u8(e, 12, 9)
+    assert np.all(out[:, :2] == 200) and np.all(out[:, -2:] == 90)
+
+
+def test_upscale_u8_pyr_up_is_the_rounded_float_pyr_up(oracle64):
+    """The 8-bit pyrUp is the FP64 pyrUp (pinned by brute force) of the same samples rounded half
+    up: the integer sums are exact in both."""
+    r = np.random.default_rng(6)
+    for shape in ((9, 13), (2, 2), (1, 6), (6, 1), (16, 20)):
+        m = r.integers(0, 256, shape, dtype=np.uint8)
+        want = np.floor(oracle64.pyr_up(m.astype(np.float64)) + 0.5)
+        np.testing.assert_array_equal(F.pyr_up_u8(m), want)
+
+
+@pytest.mark.parametrize("option", [F.UPSCALE_X15, F.UPSCALE_X2])
+def test_upscale_image_binary_mask_support(oracle64, option):
+    """A 0 / 255 mask upscaled and compared with 255 keeps exactly the output pixels whose
+    taps (nonzero weights) are all valid: the upscale of the indicator of the holes is 0 there
+    (weights of x1.5 on even sizes are >= 1/6, so no tap is negligible)."""
+    from oracle import lvm
+
+    r = np.random.default_rng(8)
+    h, w = 16, 22
+    mask = np.where(r.random((h, w)) < 0.08, 0, 255).astype(np.uint8)
+    img = r.random((h, w, 3)).astype(np.float32)
+    dst, dm = F.upscale_image(oracle64, img, mask, option)
+    dw, dh = F.upscale_size(option, w, h)
+    assert dst.shape == (dh, dw, 3) and dm.shape == (dh, dw)
+    holes = (mask == 0).astype(np.float32)
+    spread = oracle64.pyr_up(holes.astype(np.float64)) if option == F.UPSCALE_X2 else \
+        lvm.resize_linear(holes, dw, dh)
+    np.testing.assert_array_equal(dm == 255, spread == 0)
+    assert set(np.unique(dm)) <= {0, 255}
+    # no mask: the image alone; none: copies
+    d2, m2 = F.upscale_image(oracle64, img, None, option)
+    assert m2 is None and np.array_equal(d2, dst)
+    d3, m3 = F.upscale_image(oracle64, img, mask, F.UPSCALE_NONE)
+    assert np.array_equal(d3, img) and np.array_equal(m3, mask)
 
 
 def test_ecc_normalize_closed_forms(oracle32):
