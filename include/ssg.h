@@ -597,10 +597,12 @@ int ssg_ser_read_batch(ssg_ser_reader *r, int32_t n, ssg_image *images);
 
 /* debayer() (core/io/debayer.cc:1197-1355) on the GPU; src and dst device or host images.
  * src: one channel, SSG_8UC1, SSG_16UC1 or SSG_32FC1, even size for the Bayer methods.  dst has
- * the source depth: 3 channels for NN2 / MATRIX / SP (SP at half size), 1 channel for DISABLE
- * (a copy), PLANE_k and AVGBP (half size).  NN, EA and VNG on 8/16-bit sources call OpenCV's
- * demosaicing in the reference and return SSG_E_UNSUPPORTED here; on f32 sources (and VNG on
- * 16-bit) they fall back to NN2 as the reference does.  DEFAULT is NN2. */
+ * the source depth: 3 channels for NN / EA / NN2 / MATRIX / SP (SP at half size), 1 channel for
+ * DISABLE (a copy), PLANE_k and AVGBP (half size).  NN and EA on 8/16-bit sources are OpenCV's
+ * cv::demosaicing COLOR_Bayer*2BGR (bilinear) and COLOR_Bayer*2BGR_EA (edge-aware), borders copied
+ * from their inner neighbours; VNG on 8-bit sources (OpenCV's VNG) returns SSG_E_UNSUPPORTED; on
+ * f32 sources NN / EA / VNG (and VNG on 16-bit) fall back to NN2 as the reference does.  DEFAULT
+ * is NN2. */
 int ssg_debayer(const ssg_image *src, int32_t colorid, int32_t algorithm, ssg_image *dst);
 
 /* The conversion of c_image_stacking_pipeline_base::read_input_frame

@@ -129,6 +129,58 @@ def matrix(src: np.ndarray, colorid: int) -> np.ndarray:
     return out
 
 
+def _opencv_demosaic(src: np.ndarray, colorid: int, edge_aware: bool) -> np.ndarray:
+    """cv::demosaicing of an 8/16-bit Bayer image to BGR (the reference's DEBAYER_NN and
+    DEBAYER_EA, debayer.cc:1272-1355): OpenCV's Bayer2RGB_ (COLOR_Bayer*2BGR, bilinear) or
+    Bayer2RGB_EdgeAware_T (COLOR_Bayer*2BGR_EA), restated from OpenCV's imgproc demosaicing.cpp
+    (third-party, absent here).  Inside (rows and columns 1 .. n-2) each pixel keeps its own
+    colour; at a green site the colour of its left/right neighbours is (L + R + 1) >> 1 and that of
+    its up/down neighbours (U + D + 1) >> 1; at a red or blue site the opposite colour is
+    (UL + UR + DL + DR + 2) >> 2 and green is (U + D + L + R + 2) >> 2 (bilinear) or, edge-aware,
+    (U + D + 1) >> 1 when |L - R| > |D - U| and (L + R + 1) >> 1 otherwise.  Columns 0 and w-1
+    copy columns 1 and w-2, then rows 0 and h-1 copy rows 1 and h-2; an image of 2 rows or
+    columns or fewer comes out 0."""
+    h, w = src.shape
+    out = np.zeros((h, w, 3), src.dtype)
+    if h <= 2 or w <= 2:
+        return out
+    s = src.astype(np.int64)
+    c = s[1:-1, 1:-1]
+    U, D_, L, R = s[:-2, 1:-1], s[2:, 1:-1], s[1:-1, :-2], s[1:-1, 2:]
+    diag = (s[:-2, :-2] + s[:-2, 2:] + s[2:, :-2] + s[2:, 2:] + 2) >> 2
+    hor, ver = (L + R + 1) >> 1, (U + D_ + 1) >> 1
+    if edge_aware:
+        g = np.where(np.abs(L - R) > np.abs(D_ - U), ver, hor)
+    else:
+        g = (U + D_ + L + R + 2) >> 2
+    cm = color_map(colorid, h, w)
+    site = cm[1:-1, 1:-1]
+    hcol = cm[1:-1, :-2]  # colour of the left neighbour
+    vcol = cm[:-2, 1:-1]  # colour of the upper neighbour
+    inner = np.zeros((h - 2, w - 2, 3), np.int64)
+    for k in range(3):
+        green_site = np.where(hcol == k, hor, np.where(vcol == k, ver, 0))
+        other_site = np.where(k == 1, g, diag)
+        inner[..., k] = np.where(site == k, c, np.where(site == 1, green_site, other_site))
+    full = np.empty((h, w, 3), np.int64)
+    full[1:-1, 1:-1] = inner
+    full[1:-1, 0] = inner[:, 0]
+    full[1:-1, -1] = inner[:, -1]
+    full[0] = full[1]
+    full[-1] = full[-2]
+    return full.astype(src.dtype)
+
+
+def opencv_bilinear(src: np.ndarray, colorid: int) -> np.ndarray:
+    """DEBAYER_NN on 8/16-bit data: cv::demosaicing(COLOR_Bayer*2BGR)."""
+    return _opencv_demosaic(src, colorid, False)
+
+
+def opencv_ea(src: np.ndarray, colorid: int) -> np.ndarray:
+    """DEBAYER_EA on 8/16-bit data: cv::demosaicing(COLOR_Bayer*2BGR_EA)."""
+    return _opencv_demosaic(src, colorid, True)
+
+
 def debayer(src: np.ndarray, colorid: int, algo: int = DEFAULT) -> np.ndarray:
     """debayer() (debayer.cc:1197-1355) for one-channel sources."""
     if algo == DEFAULT:
@@ -140,8 +192,12 @@ def debayer(src: np.ndarray, colorid: int, algo: int = DEFAULT) -> np.ndarray:
     integral = src.dtype in (np.uint8, np.uint16)
     if algo == DISABLE:
         return src.copy()
-    if algo in (NN, EA) and integral or (algo == VNG and src.dtype == np.uint8):
-        raise NotImplementedError("OpenCV demosaicing")
+    if algo == NN and integral:
+        return opencv_bilinear(src, colorid)
+    if algo == EA and integral:
+        return opencv_ea(src, colorid)
+    if algo == VNG and src.dtype == np.uint8:
+        raise NotImplementedError("OpenCV VNG demosaicing of 8-bit data")
     if algo in (NN, EA, VNG, NN2):
         return nn2(src, colorid)
     if algo == SP:

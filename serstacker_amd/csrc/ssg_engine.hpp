@@ -165,7 +165,9 @@ struct LevelBeginArgs {
 };
 
 // ---- input stage (raw frames -> pipeline f32 images, debayer) ----------------------------------
-enum : int { INPUT_COPY = 0, INPUT_NN2 = 1, INPUT_MATRIX = 2, INPUT_SP = 3, INPUT_AVGBP = 4, INPUT_PLANE0 = 5 };
+// INPUT_PLANE0 + k (k = 0..3): Bayer plane k; INPUT_CV_*: OpenCV's bilinear / edge-aware demosaicing
+enum : int { INPUT_COPY = 0, INPUT_NN2 = 1, INPUT_MATRIX = 2, INPUT_SP = 3, INPUT_AVGBP = 4, INPUT_PLANE0 = 5,
+             INPUT_CV_BILINEAR = 9, INPUT_CV_EA = 10 };
 
 struct InputArgs {
   const void* src;         // frame f at src + f * src_fstride (bytes), rows of src_pitch bytes

@@ -3,7 +3,8 @@
 // darkbayer subtraction, flatbayer division and debayer of
 // c_image_stacking_pipeline_base::read_input_frame
 // (c_image_stacking_pipeline_base.cc:124-279), and the debayer algorithms of
-// core/io/debayer.cc on their own (NN2, SP, MATRIX, PLANE_k, AVGBP).
+// core/io/debayer.cc on their own (NN2, SP, MATRIX, PLANE_k, AVGBP, and OpenCV's bilinear and
+// edge-aware demosaicing behind DEBAYER_NN / DEBAYER_EA on 8/16-bit data).
 //
 // One thread per output pixel; a launch covers a batch of frames (grid.z).
 // The reference's arithmetic is kept exactly: integer sources are averaged in
@@ -148,6 +149,38 @@ __global__ void __launch_bounds__(256) k_input(InputArgs a) {
       const int X = 2 * x, Y = 2 * y;
       o[x] = out_value<TO, W>(a, avg4<W>(S::at(a, f, X, Y), S::at(a, f, X + 1, Y), S::at(a, f, X, Y + 1),
                                          S::at(a, f, X + 1, Y + 1)));
+      return;
+    }
+    case INPUT_CV_BILINEAR:
+    case INPUT_CV_EA: {
+      // cv::demosaicing COLOR_Bayer*2BGR / COLOR_Bayer*2BGR_EA (debayer.cc:1272-1355; OpenCV's
+      // Bayer2RGB_ and Bayer2RGB_EdgeAware_T): the interior from the 3x3 neighbourhood in int;
+      // columns 0 and w-1 copy columns 1 and w-2, then rows 0 and h-1 copy rows 1 and h-2, so a
+      // border pixel is its clamped interior pixel; images of 2 rows or columns or fewer are 0
+      if (w <= 2 || h <= 2) {
+        for (int k = 0; k < 3; ++k) o[3 * x + k] = out_value<TO, W>(a, (W)0);
+        return;
+      }
+      const int xi = min(max(x, 1), w - 2), yi = min(max(y, 1), h - 2);
+      const int c = site_color(a.pattern, xi, yi);
+      const W L = S::at(a, f, xi - 1, yi), R = S::at(a, f, xi + 1, yi), U = S::at(a, f, xi, yi - 1),
+              D = S::at(a, f, xi, yi + 1);
+      W bgr[3];
+      bgr[c] = S::at(a, f, xi, yi);
+      if (c == 1) {
+        bgr[site_color(a.pattern, xi - 1, yi)] = avg2<W>(L, R);
+        bgr[site_color(a.pattern, xi, yi - 1)] = avg2<W>(U, D);
+      } else {
+        if (a.method == INPUT_CV_EA) {
+          const W dh = L > R ? L - R : R - L, dv = D > U ? D - U : U - D;
+          bgr[1] = dh > dv ? avg2<W>(U, D) : avg2<W>(L, R);
+        } else {
+          bgr[1] = avg4<W>(U, L, R, D);
+        }
+        bgr[2 - c] = avg4<W>(S::at(a, f, xi - 1, yi - 1), S::at(a, f, xi + 1, yi - 1), S::at(a, f, xi - 1, yi + 1),
+                             S::at(a, f, xi + 1, yi + 1));
+      }
+      for (int k = 0; k < 3; ++k) o[3 * x + k] = out_value<TO, W>(a, bgr[k]);
       return;
     }
     default: {

@@ -1,6 +1,6 @@
 """GPU input stage against the oracle (oracle/debayer.py, oracle/ser.py), through the C ABI:
-debayer() of core/io/debayer.cc (NN2, SP, MATRIX, PLANE_k, AVGBP, DISABLE; the fallbacks of
-NN/EA/VNG), the raw-frame conversion of read_input_frame
+debayer() of core/io/debayer.cc (NN2, SP, MATRIX, PLANE_k, AVGBP, DISABLE; OpenCV's bilinear
+and edge-aware demosaicing behind NN / EA on 8/16-bit data; the fallbacks of NN/EA/VNG), the raw-frame conversion of read_input_frame
 (c_image_stacking_pipeline_base.cc:124-279) and the SER reader into device memory.
 Integer and float arithmetic follow the reference exactly: compared bit for bit."""
 import numpy as np
@@ -66,13 +66,46 @@ def test_debayer_bitexact(S, dtype, algo, colorid):
     np.testing.assert_array_equal(S.debayer(src, colorid, algo), want)
 
 
+@pytest.mark.parametrize("dtype", [np.uint8, np.uint16])
+@pytest.mark.parametrize("algo", [D.NN, D.EA])
+@pytest.mark.parametrize("colorid", PATTERNS)
+@pytest.mark.parametrize("shape", [(34, 46), (4, 6), (2, 8), (8, 2)])
+def test_debayer_opencv_bilinear_and_edge_aware(S, dtype, algo, colorid, shape):
+    """DEBAYER_NN and DEBAYER_EA on 8/16-bit data are cv::demosaicing's bilinear and edge-aware
+    modes (copied borders; 0 for 2-row / 2-column images): bit-exact, device and host."""
+    src = _src(dtype, shape, colorid * 11 + algo + shape[0])
+    want = D.debayer(src, colorid, algo)
+    out = empty_gpu(want.shape, dtype)
+    S.debayer(to_gpu(src), colorid, algo, out=out)
+    np.testing.assert_array_equal(from_gpu(out, dtype), want)
+    np.testing.assert_array_equal(S.debayer(src, colorid, algo), want)
+
+
+@pytest.mark.parametrize("algo", [D.NN, D.EA])
+def test_read_input_frames_opencv_demosaic(S, algo):
+    """12-bit raw frames debayered by OpenCV's modes in 16 bits, then converted to f32 (no
+    calibration); with a dark frame the data is float and both modes fall back to nn2."""
+    import torch
+
+    n, h, w, bpp = 3, 30, 44, 12
+    rng = np.random.default_rng(algo)
+    raw = rng.integers(0, 4096, (n, h, w)).astype(np.uint16)
+    dark = (rng.random((h, w)) * 0.02).astype(np.float32)
+    for dk in (None, dark):
+        want = [D.read_input_frame(raw[i], 9, bpp, algo, dk) for i in range(n)]
+        slab = to_gpu(raw)
+        outs = torch.empty((n,) + want[0].shape, dtype=torch.float32, device="cuda")
+        S.read_input_frames([slab[i] for i in range(n)], 9, bpp, algo, dark=dk, outs=[outs[i] for i in range(n)])
+        for i in range(n):
+            np.testing.assert_array_equal(outs[i].cpu().numpy(), want[i])
+
+
 def test_debayer_opencv_methods(S):
-    """NN and EA on 8/16-bit data, and VNG on 8-bit data, are OpenCV's demosaicing in the
-    reference: rejected; on f32 data (and VNG on 16-bit) they fall back to NN2."""
+    """VNG on 8-bit data is OpenCV's VNG demosaicing in the reference: rejected; NN / EA / VNG on
+    f32 data and VNG on 16-bit data fall back to NN2, as in debayer() (debayer.cc:1243-1262)."""
     u8, u16, f32 = _src(np.uint8, (8, 8), 1), _src(np.uint16, (8, 8), 2), _src(np.float32, (8, 8), 3)
-    for src, algo in ((u8, D.NN), (u16, D.NN), (u8, D.EA), (u16, D.EA), (u8, D.VNG)):
-        with pytest.raises(RuntimeError):
-            S.debayer(src, 8, algo)
+    with pytest.raises(RuntimeError):
+        S.debayer(u8, 8, D.VNG)
     np.testing.assert_array_equal(S.debayer(u16, 8, D.VNG), D.nn2(u16, 8))
     for algo in (D.NN, D.EA, D.VNG):
         np.testing.assert_array_equal(S.debayer(f32, 8, algo), D.nn2(f32, 8))

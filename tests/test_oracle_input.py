@@ -100,7 +100,56 @@ def test_read_input_frame_conversion_rules():
     # mono frames are only converted
     np.testing.assert_array_equal(D.read_input_frame(raw, 0, 12), raw.astype(np.float32) / np.float32(4096))
     with pytest.raises(NotImplementedError):
-        D.debayer(raw, 8, D.NN)
+        D.debayer(raw.astype(np.uint8), 8, D.VNG)
+
+
+@pytest.mark.parametrize("colorid", [8, 9, 10, 11])
+@pytest.mark.parametrize("dtype", [np.uint8, np.uint16])
+def test_opencv_bilinear_is_nn2_inside_with_copied_borders(colorid, dtype):
+    """OpenCV's bilinear demosaic computes every interior pixel with the same integer averages
+    as the reference's own nn2 (pinned on its own): the two agree on rows and columns 1 .. n-2;
+    the border columns and rows are copies of their inner neighbours."""
+    rng = np.random.default_rng(colorid)
+    raw = rng.integers(0, np.iinfo(dtype).max + 1, (12, 18)).astype(dtype)
+    got = D.debayer(raw, colorid, D.NN)
+    assert got.dtype == dtype and got.shape == (12, 18, 3)
+    np.testing.assert_array_equal(got[1:-1, 1:-1], D.nn2(raw, colorid)[1:-1, 1:-1])
+    np.testing.assert_array_equal(got[1:-1, 0], got[1:-1, 1])
+    np.testing.assert_array_equal(got[1:-1, -1], got[1:-1, -2])
+    np.testing.assert_array_equal(got[0], got[1])
+    np.testing.assert_array_equal(got[-1], got[-2])
+    # each pixel keeps its own sample in its own channel
+    cm = D.color_map(colorid, 12, 18)
+    inner = got[1:-1, 1:-1]
+    np.testing.assert_array_equal(np.take_along_axis(inner, cm[1:-1, 1:-1, None], 2)[..., 0], raw[1:-1, 1:-1])
+
+
+@pytest.mark.parametrize("colorid", [8, 11])
+def test_opencv_ea_follows_the_flatter_direction(colorid):
+    """Edge-aware green at red / blue sites: on horizontal stripes (rows constant, so |L - R| = 0)
+    it is the left/right average, i.e. the site's own row value; on vertical stripes (columns
+    constant) it is the up/down average, the site's own column value.  Bilinear mixes in the
+    other direction instead.  The other channels are the bilinear ones."""
+    h, w = 10, 14
+    rows = np.tile((np.arange(h) * 37 % 251)[:, None], (1, w)).astype(np.uint8)
+    cols = np.tile((np.arange(w) * 53 % 251)[None, :], (h, 1)).astype(np.uint8)
+    cm = D.color_map(colorid, h, w)[1:-1, 1:-1]
+    for img, own in ((rows, rows), (cols, cols)):
+        ea = D.debayer(img, colorid, D.EA)
+        nn = D.debayer(img, colorid, D.NN)
+        g = ea[1:-1, 1:-1, 1]
+        np.testing.assert_array_equal(g[cm != 1], own[1:-1, 1:-1][cm != 1])
+        assert not np.array_equal(nn[1:-1, 1:-1, 1][cm != 1], own[1:-1, 1:-1][cm != 1])
+        for k in (0, 2):
+            np.testing.assert_array_equal(ea[..., k], nn[..., k])
+
+
+def test_opencv_demosaic_constants_and_small_images():
+    for algo in (D.NN, D.EA):
+        for c in (0, 1, 200, 255):
+            np.testing.assert_array_equal(D.debayer(np.full((8, 10), c, np.uint8), 8, algo), c)
+        np.testing.assert_array_equal(D.debayer(np.full((2, 10), 9, np.uint16), 9, algo), 0)
+        np.testing.assert_array_equal(D.debayer(np.full((8, 2), 9, np.uint16), 9, algo), 0)
 
 
 def test_ser_header_layout_follows_the_format():
