@@ -149,6 +149,16 @@ __device__ __forceinline__ void bilinear_w(int fx, int fy, float& w0, float& w1,
   w3 = ty * tx;
 }
 
+// the weights of the 32 x 32 sub-pixel positions, entry fy * 32 + fx (cv::remap's bilinear table),
+// filled by the nt threads of a block into LDS; the caller synchronises before reading it
+__device__ __forceinline__ void fill_bilinear_table(float4* t, int tid, int nt) {
+  for (int i = tid; i < 1024; i += nt) {
+    float w0, w1, w2, w3;
+    bilinear_w(i & 31, i >> 5, w0, w1, w2, w3);
+    t[i] = make_float4(w0, w1, w2, w3);
+  }
+}
+
 __device__ __forceinline__ float bilinear_sum(float s00, float s01, float s10, float s11, float w0, float w1,
                                               float w2, float w3) {
   float v = s00 * w0;

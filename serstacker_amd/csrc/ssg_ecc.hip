@@ -554,7 +554,10 @@ template <int MOTION, bool HAS_CMASK, bool HAS_RMASK>
 __global__ void __launch_bounds__(ECC_THREADS) k_corr_pass(CorrPassArgs a) {
   __shared__ double dred[ECC_WAVES][8];
   __shared__ float fred[ECC_WAVES][4];
+  __shared__ float4 wtab[1024];  // bilinear weights of the 32 x 32 sub-pixel positions
   const int tid = threadIdx.x, lane = tid & (WAVE - 1), wv = tid / WAVE;
+  fill_bilinear_table(wtab, tid, ECC_THREADS);
+  __syncthreads();
   const int items = a.nlist * a.ntiles;
   const int G = gridDim.x, b = blockIdx.x;
   const int vb = (G % 8 == 0) ? (b % 8) * (G / 8) + b / 8 : b;
@@ -592,7 +595,12 @@ __global__ void __launch_bounds__(ECC_THREADS) k_corr_pass(CorrPassArgs a) {
       fast = mnx >= 0.25f && mny >= 0.25f && mxx <= (float)a.cw - 1.25f && mxy <= (float)a.ch - 1.25f;
     }
     if (fast) {
+      // every position lies in [0.25, size - 1.25]: quantised without the clamp, the taps read with
+      // buffer loads at 32-bit offsets, the weights from the table
       const int xc = min(x, a.w - 1);
+      const uint32_t cpb = (uint32_t)a.cur_pitch * 4u, rpb = (uint32_t)a.ref_pitch * 4u;
+      const rsrc_t rc = make_rsrc(cur, (uint32_t)a.ch * cpb);
+      const rsrc_t rr = make_rsrc(a.ref, (uint32_t)a.h * rpb);
 #pragma nounroll
       for (int dy0 = 0; dy0 < ECC_RPT; dy0 += 4) {
         float t[4][4], fv[4];
@@ -602,13 +610,13 @@ __global__ void __launch_bounds__(ECC_THREADS) k_corr_pass(CorrPassArgs a) {
           const int yc = min(y0 + wv * ECC_RPT + dy0 + j, a.h - 1);
           float sx, sy;
           warp_xy<MOTION>(mc, xc, yc, sx, sy);
-          const int X = quant32(sx), Y = quant32(sy);
-          const float* r0 = cur + (size_t)(Y >> 5) * a.cur_pitch + (X >> 5);
-          t[j][0] = r0[0];
-          t[j][1] = r0[1];
-          t[j][2] = r0[a.cur_pitch];
-          t[j][3] = r0[a.cur_pitch + 1];
-          fv[j] = a.ref[(size_t)yc * a.ref_pitch + xc];
+          const int X = quant32_inrange(sx), Y = quant32_inrange(sy);
+          const uint32_t o = (uint32_t)(Y >> 5) * cpb + (uint32_t)(X >> 5) * 4u;
+          t[j][0] = bload_f32(rc, o, 0u);
+          t[j][1] = bload_f32(rc, o + 4u, 0u);
+          t[j][2] = bload_f32(rc, o, cpb);
+          t[j][3] = bload_f32(rc, o + 4u, cpb);
+          fv[j] = bload_f32(rr, (uint32_t)xc * 4u, (uint32_t)yc * rpb);
           q[j] = ((Y & 31) << 5) | (X & 31);
         }
 #pragma unroll
@@ -616,9 +624,8 @@ __global__ void __launch_bounds__(ECC_THREADS) k_corr_pass(CorrPassArgs a) {
           const int y = y0 + wv * ECC_RPT + dy0 + j;
           if (x >= a.w || y >= a.h) continue;
           if (HAS_RMASK && a.rmask[(size_t)y * a.rmask_pitch + x] == 0) continue;
-          float w0, w1, w2, w3;
-          bilinear_w(q[j] & 31, q[j] >> 5, w0, w1, w2, w3);
-          const float gv = bilinear_sum(t[j][0], t[j][1], t[j][2], t[j][3], w0, w1, w2, w3);
+          const float4 w = wtab[q[j]];
+          const float gv = bilinear_sum(t[j][0], t[j][1], t[j][2], t[j][3], w.x, w.y, w.z, w.w);
           const float fp = fv[j] - a.cf, gp = gv - a.cf;
           n += 1.f;
           F1 += fp; G1 += gp;
