@@ -1107,6 +1107,9 @@ extern "C" int ssg_registration_setup_reference_frame(ssg_registration* r, const
   r->ew = half ? (r->ref_w + 1) / 2 : r->ref_w;
   r->eh = half ? (r->ref_h + 1) / 2 : r->ref_h;
   if (half && mask) return fail(SSG_E_UNSUPPORTED, "reference mask with ecc.scale 0.5");
+  // the ECC kernels address a level's samples with 24-bit products of 18-bit tap indices
+  if (r->ref_w >= (1 << 17) || r->ref_h >= (1 << 17))
+    return fail(SSG_E_UNSUPPORTED, "reference frame %dx%d: sides of 131072 pixels or more", r->ref_w, r->ref_h);
   // level count (ecc2.cc:1009-1026)
   const int min_size = std::max(4, r->o.ecc.ecch_minimum_image_size);
   int tw = r->ew, th = r->eh, lv = 1;

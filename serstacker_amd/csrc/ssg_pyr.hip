@@ -378,8 +378,11 @@ hipError_t launch_sum_f32(const float* src, int pitch, int w, int h, double* out
 
 // Per-frame sharpness weight: mean of the squared 3x3 laplacian
 // [0 1 0; 1 -4 1; 0 1 0], BORDER_REPLICATE.  Canonical per pixel
-// L = fma(-4, c, (up + down) + (left + right)).  One block per (row strip,
-// frame) writes a double partial; the host sums the partials in order.
+// L = fma(-4, c, (up + down) + (left + right)), L^2 exact in double.  One block per (row strip,
+// frame) writes a double partial, summed in strip order afterwards.  Frames whose rows are
+// 16-byte aligned runs of a multiple of 4 pixels: each thread owns 4 adjacent columns, walks the
+// strip with the rows above, at and below in registers (one 16-byte load per row) and keeps one
+// double sum per column; other frames: one column per thread.
 __global__ void __launch_bounds__(256) k_sharpness(const float* const* __restrict__ src_ptrs, int pitch, int w, int h,
                                                    int rows_per_block, double* __restrict__ partial) {
   __shared__ double red[4];
@@ -387,17 +390,40 @@ __global__ void __launch_bounds__(256) k_sharpness(const float* const* __restric
   const float* __restrict__ S = src_ptrs[f];
   const int y0 = blockIdx.x * rows_per_block, y1 = min(h, y0 + rows_per_block);
   double acc = 0;
-  for (int y = y0; y < y1; ++y) {
-    const float* up = S + (size_t)max(y - 1, 0) * pitch;
-    const float* c = S + (size_t)y * pitch;
-    const float* dn = S + (size_t)min(y + 1, h - 1) * pitch;
-    float part = 0.f;
-    for (int x = threadIdx.x; x < w; x += blockDim.x) {
-      const float l = c[max(x - 1, 0)], r = c[min(x + 1, w - 1)];
-      const float L = fmaf(-4.f, c[x], (up[x] + dn[x]) + (l + r));
-      acc += (double)L * (double)L;
+  if ((((uintptr_t)S & 15u) == 0) && (pitch & 3) == 0 && (w & 3) == 0) {
+    double a4[4] = {0, 0, 0, 0};
+    for (int x = 4 * (int)threadIdx.x; x < w; x += 4 * (int)blockDim.x) {
+      const int xl = max(x - 1, 0), xr = min(x + 4, w - 1);
+      float4 up = *(const float4*)(S + (size_t)max(y0 - 1, 0) * pitch + x);
+      float4 c = *(const float4*)(S + (size_t)y0 * pitch + x);
+      for (int y = y0; y < y1; ++y) {
+        const float* cr = S + (size_t)y * pitch;
+        const float4 dn = *(const float4*)(S + (size_t)min(y + 1, h - 1) * pitch + x);
+        const float l = cr[xl], r = cr[xr];
+        const float L0 = fmaf(-4.f, c.x, (up.x + dn.x) + (l + c.y));
+        const float L1 = fmaf(-4.f, c.y, (up.y + dn.y) + (c.x + c.z));
+        const float L2 = fmaf(-4.f, c.z, (up.z + dn.z) + (c.y + c.w));
+        const float L3 = fmaf(-4.f, c.w, (up.w + dn.w) + (c.z + r));
+        a4[0] = fma((double)L0, (double)L0, a4[0]);
+        a4[1] = fma((double)L1, (double)L1, a4[1]);
+        a4[2] = fma((double)L2, (double)L2, a4[2]);
+        a4[3] = fma((double)L3, (double)L3, a4[3]);
+        up = c;
+        c = dn;
+      }
     }
-    (void)part;
+    acc = (a4[0] + a4[1]) + (a4[2] + a4[3]);
+  } else {
+    for (int y = y0; y < y1; ++y) {
+      const float* up = S + (size_t)max(y - 1, 0) * pitch;
+      const float* c = S + (size_t)y * pitch;
+      const float* dn = S + (size_t)min(y + 1, h - 1) * pitch;
+      for (int x = threadIdx.x; x < w; x += blockDim.x) {
+        const float l = c[max(x - 1, 0)], r = c[min(x + 1, w - 1)];
+        const float L = fmaf(-4.f, c[x], (up[x] + dn[x]) + (l + r));
+        acc = fma((double)L, (double)L, acc);
+      }
+    }
   }
   acc = wave_sum(acc);
   if ((threadIdx.x & (WAVE - 1)) == 0) red[threadIdx.x / WAVE] = acc;
