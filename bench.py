@@ -919,6 +919,12 @@ def bench_sweep(a, S, _lib, torch):
           lambda: _lib.check(lib.ssg_read_input_frames(ni, rin, _lib.COLORID_BAYER_RGGB, 12, _lib.DEBAYER_NN2, None,
                                                        None, rout), "read_input_frames"),
           ni * (2 + 12) * Wi * Hi, ni, "frames")
+    # the same conversion with OpenCV's bilinear (DEBAYER_NN) and edge-aware (DEBAYER_EA) demosaicing
+    for algo, aname in ((_lib.DEBAYER_NN, "nn"), (_lib.DEBAYER_EA, "ea")):
+        timed(f"input_opencv_{aname}_{ni}x{Wi}x{Hi}_u16_to_f32bgr",
+              lambda algo=algo: _lib.check(lib.ssg_read_input_frames(ni, rin, _lib.COLORID_BAYER_RGGB, 12, algo, None,
+                                                                     None, rout), "read_input_frames"),
+              ni * (2 + 12) * Wi * Hi, ni, "frames")
     del bgr
     import struct
     import tempfile
@@ -972,6 +978,19 @@ def bench_sweep(a, S, _lib, torch):
           lambda: _lib.check(lib.ssg_upscale_remaps(_lib.UPSCALE_X15, nu, uin, uout), "upscale_remaps"),
           nu * 8 * (W * H + dwu * dhu), nu, "maps")
     del src_maps, dst_maps
+    # upscale before align: 64 mono f32 frames of 1920x1080 with their 0 / 255 masks to x1.5
+    ufr = torch.rand((nu, H, W), generator=g).to(dev)
+    umk = torch.where(torch.rand((nu, H, W), generator=g) < 0.01, 0, 255).to(torch.uint8).to(dev)
+    ufo = torch.empty((nu, dhu, dwu), dtype=torch.float32, device=dev)
+    umo = torch.empty((nu, dhu, dwu), dtype=torch.uint8, device=dev)
+    fin = _lib.images([S.image_view(ufr[i]) for i in range(nu)])
+    fmi = _lib.images([S.image_view(umk[i]) for i in range(nu)])
+    fout = _lib.images([S.image_view(ufo[i]) for i in range(nu)])
+    fmo = _lib.images([S.image_view(umo[i]) for i in range(nu)])
+    timed(f"upscale_images_x1.5_{nu}x{W}x{H}_with_masks",
+          lambda: _lib.check(lib.ssg_upscale_images(_lib.UPSCALE_X15, nu, fin, fmi, fout, fmo), "upscale_images"),
+          nu * 5 * (W * H + dwu * dhu), nu, "frames")
+    del ufr, umk, ufo, umo
     # finish of a x1.5 stack (2880x1620 RGB) with 5% scattered holes and a band of missing rows
     img = torch.rand((dhu, dwu, 3), generator=g).to(dev)
     hole = torch.rand((dhu, dwu), generator=g) < 0.05

@@ -1,8 +1,9 @@
 // ssg_upscale.hip -- the upscales of stacking at a larger size (c_image_stacking_pipeline.cc:
 // 1869-2000): upscale_remap of registration remaps (interleaved 2-channel f32 fields) and
 // upscale_image of frames (1- or 3-channel f32, weight maps included) and their 8-bit masks.
-// A batch of same-size planes per launch (blockIdx.z = plane), one thread per output pixel,
-// 64 x 4 output pixels per block so each wave writes one row segment.
+// A batch of same-size planes per launch (blockIdx.z = plane), one thread per output pixel of a
+// float plane (64 x 4 per block, so each wave writes one row segment) and per four adjacent
+// outputs of a mask.
 //   x1.5: cv::resize INTER_LINEAR;  x3: INTER_LINEAR_EXACT (on float data INTER_LINEAR);
 //   x2:   cv::pyrUp.
 // Float data follows resize.cpp's generic float path: the horizontal pass S[x0] a0 + S[x1] a1,
@@ -109,48 +110,73 @@ __device__ __forceinline__ int hup_u8(const uint8_t* R, int sw, int xo) {
   return R[x - 1] + R[x] * 6 + R[x + 1];
 }
 
-// MODE 0: x1.5 INTER_LINEAR (fixed point), 1: x3 INTER_LINEAR_EXACT, 2: x2 pyrUp; then >= 255
+// MODE 0: x1.5 INTER_LINEAR (fixed point), 1: x3 INTER_LINEAR_EXACT, 2: x2 pyrUp; then >= 255.
+// Each thread writes MX_PT adjacent outputs of one row (the row's taps formed once), as one 4-byte
+// store where the row allows it.
+constexpr int MX_PT = 4;
+
 template <int MODE>
 __global__ void __launch_bounds__(256) k_up_mask(const uint8_t* const* src, int spitch, int sw, int sh,
                                                  uint8_t* const* dst, int dpitch, int dw, int dh, double scale_x,
                                                  double scale_y) {
-  const int x = blockIdx.x * UP_BX + threadIdx.x, y = blockIdx.y * UP_BY + threadIdx.y;
-  if (x >= dw || y >= dh) return;
+  const int xb = (blockIdx.x * UP_BX + threadIdx.x) * MX_PT, y = blockIdx.y * UP_BY + threadIdx.y;
+  if (xb >= dw || y >= dh) return;
   const uint8_t* S = src[blockIdx.z];
-  int v;
+  uint8_t* D = dst[blockIdx.z] + (size_t)y * dpitch;
+  uint32_t packed = 0;
+  int yt0, yt1, wy0 = 0, wy1 = 0;  // the row's taps and weights
   if constexpr (MODE == 0) {
-    int x0, x1, y0, y1;
-    float a0, a1, b0, b1;
-    lin_tap(x, scale_x, sw, x0, x1, a0, a1);
-    lin_tap_v(y, scale_y, sh, y0, y1, b0, b1);
+    float b0, b1;
+    lin_tap_v(y, scale_y, sh, yt0, yt1, b0, b1);
     // saturate_cast<short>(w * INTER_RESIZE_COEF_SCALE): round half to even
-    const int ia0 = __float2int_rn(a0 * 2048.f), ia1 = __float2int_rn(a1 * 2048.f);
-    const int ib0 = __float2int_rn(b0 * 2048.f), ib1 = __float2int_rn(b1 * 2048.f);
-    const uint8_t* r0 = S + (size_t)y0 * spitch;
-    const uint8_t* r1 = S + (size_t)y1 * spitch;
-    const int h0 = r0[x0] * ia0 + r0[x1] * ia1, h1 = r1[x0] * ia0 + r1[x1] * ia1;
-    v = (((ib0 * (h0 >> 4)) >> 16) + ((ib1 * (h1 >> 4)) >> 16) + 2) >> 2;
+    wy0 = __float2int_rn(b0 * 2048.f);
+    wy1 = __float2int_rn(b1 * 2048.f);
   } else if constexpr (MODE == 1) {
-    int x0, x1, c0, c1, y0, y1, d0, d1;
-    exact_tap(x, scale_x, sw, x0, x1, c0, c1);
-    exact_tap(y, scale_y, sh, y0, y1, d0, d1);
-    const uint8_t* r0 = S + (size_t)y0 * spitch;
-    const uint8_t* r1 = S + (size_t)y1 * spitch;
-    const int h0 = r0[x0] * c0 + r0[x1] * c1, h1 = r1[x0] * c0 + r1[x1] * c1;
-    v = (h0 * d0 + h1 * d1 + 32768) >> 16;
+    exact_tap(y, scale_y, sh, yt0, yt1, wy0, wy1);
   } else {
-    auto srow = [&](int r) {
-      if (r < 0) r = sh > 1 ? 1 : 0;
-      if (r >= sh) r = sh - 1;
-      return S + (size_t)r * spitch;
-    };
-    const int yy = y >> 1;
-    if (y & 1)
-      v = ((hup_u8(srow(yy), sw, x) + hup_u8(srow(yy + 1), sw, x)) * 4 + 32) >> 6;
-    else
-      v = (hup_u8(srow(yy - 1), sw, x) + hup_u8(srow(yy), sw, x) * 6 + hup_u8(srow(yy + 1), sw, x) + 32) >> 6;
+    yt0 = yt1 = 0;
   }
-  dst[blockIdx.z][(size_t)y * dpitch + x] = v >= 255 ? 255 : 0;
+  const int n = min(MX_PT, dw - xb);
+#pragma unroll
+  for (int k = 0; k < MX_PT; ++k) {
+    if (k >= n) break;
+    const int x = xb + k;
+    int v;
+    if constexpr (MODE == 0) {
+      int x0, x1;
+      float a0, a1;
+      lin_tap(x, scale_x, sw, x0, x1, a0, a1);
+      const int ia0 = __float2int_rn(a0 * 2048.f), ia1 = __float2int_rn(a1 * 2048.f);
+      const uint8_t* r0 = S + (size_t)yt0 * spitch;
+      const uint8_t* r1 = S + (size_t)yt1 * spitch;
+      const int h0 = r0[x0] * ia0 + r0[x1] * ia1, h1 = r1[x0] * ia0 + r1[x1] * ia1;
+      v = (((wy0 * (h0 >> 4)) >> 16) + ((wy1 * (h1 >> 4)) >> 16) + 2) >> 2;
+    } else if constexpr (MODE == 1) {
+      int x0, x1, c0, c1;
+      exact_tap(x, scale_x, sw, x0, x1, c0, c1);
+      const uint8_t* r0 = S + (size_t)yt0 * spitch;
+      const uint8_t* r1 = S + (size_t)yt1 * spitch;
+      const int h0 = r0[x0] * c0 + r0[x1] * c1, h1 = r1[x0] * c0 + r1[x1] * c1;
+      v = (h0 * wy0 + h1 * wy1 + 32768) >> 16;
+    } else {
+      auto srow = [&](int r) {
+        if (r < 0) r = sh > 1 ? 1 : 0;
+        if (r >= sh) r = sh - 1;
+        return S + (size_t)r * spitch;
+      };
+      const int yy = y >> 1;
+      if (y & 1)
+        v = ((hup_u8(srow(yy), sw, x) + hup_u8(srow(yy + 1), sw, x)) * 4 + 32) >> 6;
+      else
+        v = (hup_u8(srow(yy - 1), sw, x) + hup_u8(srow(yy), sw, x) * 6 + hup_u8(srow(yy + 1), sw, x) + 32) >> 6;
+    }
+    packed |= (v >= 255 ? 255u : 0u) << (8 * k);
+  }
+  if (n == MX_PT && (((uintptr_t)(D + xb)) & 3u) == 0) {
+    *(uint32_t*)(D + xb) = packed;
+  } else {
+    for (int k = 0; k < n; ++k) D[xb + k] = (uint8_t)(packed >> (8 * k));
+  }
 }
 
 // resize(): inv_scale = dsize / ssize, scale = 1 / inv_scale, both in double
@@ -201,7 +227,8 @@ hipError_t launch_upscale_masks(int option, int n, const uint8_t* const* src, in
                                 uint8_t* const* dst, int dpitch, hipStream_t s) {
   int dw, dh;
   upscale_size(option, sw, sh, &dw, &dh);
-  const dim3 g((unsigned)((dw + UP_BX - 1) / UP_BX), (unsigned)((dh + UP_BY - 1) / UP_BY), (unsigned)n),
+  const dim3 g((unsigned)((dw + UP_BX * MX_PT - 1) / (UP_BX * MX_PT)), (unsigned)((dh + UP_BY - 1) / UP_BY),
+               (unsigned)n),
       b(UP_BX, UP_BY);
   const double sx = resize_scale(dw, sw), sy = resize_scale(dh, sh);
   switch (option) {
