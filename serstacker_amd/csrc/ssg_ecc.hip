@@ -577,13 +577,15 @@ __global__ void __launch_bounds__(ECC_THREADS) k_corr_pass(CorrPassArgs a) {
     float n = 0, F1 = 0, G1 = 0, F2 = 0, G2 = 0, FG = 0;
     // extremes of f' and g' over the mask: a constant image is recognised exactly
     float fmn = INFINITY, fmx = -INFINITY, gmn = INFINITY, gmx = -INFINITY;
-    // Interior tiles (the warped corners of the clipped tile keep every tap inside the current
-    // image, no current mask): the mask test always passes and the sample needs no border
-    // handling, so four rows' loads go out together; the sums keep the row order.
+    // Interior strips (the warped corners of the wave's clipped 16-row strip keep every tap
+    // inside the current image, no current mask): the mask test always passes and the sample
+    // needs no border handling, so four rows' loads go out together; the sums keep the row order.
     bool fast = false;
-    if (!HAS_CMASK) {
+    const int ys0 = y0 + wv * ECC_RPT;
+    if (!HAS_CMASK && ys0 < a.h) {
+      const int ys1 = min(ys0 + ECC_RPT - 1, a.h - 1);
       const int cx[4] = {x0, min(x0 + CORR_TW - 1, a.w - 1), x0, min(x0 + CORR_TW - 1, a.w - 1)};
-      const int cy[4] = {y0, y0, min(y0 + CORR_TH - 1, a.h - 1), min(y0 + CORR_TH - 1, a.h - 1)};
+      const int cy[4] = {ys0, ys0, ys1, ys1};
       float mnx = 1e30f, mxx = -1e30f, mny = 1e30f, mxy = -1e30f;
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
