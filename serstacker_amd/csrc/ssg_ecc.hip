@@ -603,6 +603,18 @@ __global__ void __launch_bounds__(ECC_THREADS) k_corr_pass(CorrPassArgs a) {
       const uint32_t cpb = (uint32_t)a.cur_pitch * 4u, rpb = (uint32_t)a.ref_pitch * 4u;
       const rsrc_t rc = make_rsrc(cur, (uint32_t)a.ch * cpb);
       const rsrc_t rr = make_rsrc(a.ref, (uint32_t)a.h * rpb);
+      // a translation: the x position of a column is the same in every row and the y position of
+      // a row the same in every column -- the column's tap offset and fraction once per lane, the
+      // row's (formed by lane j for row j of the strip) read with readlane; the same samples
+      int txo = 0, tfx = 0, tro = 0, tqy = 0;
+      if constexpr (MOTION == MOTION_TRANSLATION) {
+        const int X = quant32_inrange((float)xc + mc.c[0]);
+        txo = (X >> 5) * 4;
+        tfx = X & 31;
+        const int Y = quant32_inrange((float)min(y0 + wv * ECC_RPT + (lane & (ECC_RPT - 1)), a.h - 1) + mc.c[1]);
+        tro = (Y >> 5) * (int)cpb;
+        tqy = (Y & 31) << 5;
+      }
 #pragma nounroll
       for (int dy0 = 0; dy0 < ECC_RPT; dy0 += 4) {
         float t[4][4], fv[4];
@@ -610,16 +622,25 @@ __global__ void __launch_bounds__(ECC_THREADS) k_corr_pass(CorrPassArgs a) {
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
           const int yc = min(y0 + wv * ECC_RPT + dy0 + j, a.h - 1);
-          float sx, sy;
-          warp_xy<MOTION>(mc, xc, yc, sx, sy);
-          const int X = quant32_inrange(sx), Y = quant32_inrange(sy);
-          const uint32_t o = (uint32_t)(Y >> 5) * cpb + (uint32_t)(X >> 5) * 4u;
-          t[j][0] = bload_f32(rc, o, 0u);
-          t[j][1] = bload_f32(rc, o + 4u, 0u);
-          t[j][2] = bload_f32(rc, o, cpb);
-          t[j][3] = bload_f32(rc, o + 4u, cpb);
+          if constexpr (MOTION == MOTION_TRANSLATION) {
+            const uint32_t r = (uint32_t)__builtin_amdgcn_readlane(tro, dy0 + j);
+            t[j][0] = bload_f32(rc, (uint32_t)txo, r);
+            t[j][1] = bload_f32(rc, (uint32_t)txo + 4u, r);
+            t[j][2] = bload_f32(rc, (uint32_t)txo, r + cpb);
+            t[j][3] = bload_f32(rc, (uint32_t)txo + 4u, r + cpb);
+            q[j] = __builtin_amdgcn_readlane(tqy, dy0 + j) | tfx;
+          } else {
+            float sx, sy;
+            warp_xy<MOTION>(mc, xc, yc, sx, sy);
+            const int X = quant32_inrange(sx), Y = quant32_inrange(sy);
+            const uint32_t o = (uint32_t)(Y >> 5) * cpb + (uint32_t)(X >> 5) * 4u;
+            t[j][0] = bload_f32(rc, o, 0u);
+            t[j][1] = bload_f32(rc, o + 4u, 0u);
+            t[j][2] = bload_f32(rc, o, cpb);
+            t[j][3] = bload_f32(rc, o + 4u, cpb);
+            q[j] = ((Y & 31) << 5) | (X & 31);
+          }
           fv[j] = bload_f32(rr, (uint32_t)xc * 4u, (uint32_t)yc * rpb);
-          q[j] = ((Y & 31) << 5) | (X & 31);
         }
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
