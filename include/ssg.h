@@ -512,17 +512,20 @@ int ssg_eccflow_get_profile(ssg_eccflow *f, ssg_profile *out, int reset);
 
 /* ---- tuning knobs (no reference counterpart) -------------------------------------------------
  * Launch-shape choices of the kernels, process-wide, set by the project's autotuner
- * (serstacker_amd.autotune) per size class, or by hand.  ACC_ROWS, ECC_GRID and FOCUS_CHUNK change
- * speed only (every value adds the same values in the same order); ACC_BLOCKS regroups a batch's
- * frames into chunks whose compensated partial sums are then merged, so the stack agrees to the
- * precision of those compensated sums rather than bit for bit (counts stay exact).
+ * (serstacker_amd.autotune) per size class, or by hand.  ACC_ROWS, ECC_GRID, FOCUS_CHUNK and
+ * PYR_STRIP change speed only (every value adds the same values in the same order); ACC_BLOCKS
+ * regroups a batch's frames into chunks whose compensated partial sums are then merged, so the
+ * stack agrees to the precision of those compensated sums rather than bit for bit (counts stay
+ * exact).
  *   SSG_TUNE_ACC_ROWS     rows per thread of the fused warp+accumulate tile: 2 (64x8) or 4 (64x16)
  *   SSG_TUNE_ACC_BLOCKS   blocks the fused accumulate aims at per launch (256..65536): the frames
  *                         per chunk follow (more chunks = more partial planes to merge)
  *   SSG_TUNE_ECC_GRID     blocks per CU of the persistent ECC passes (1..16; capped by occupancy)
- *   SSG_TUNE_FOCUS_CHUNK  frames per sequence and chunk of ssg_focus_stack_batch (1..32) */
+ *   SSG_TUNE_FOCUS_CHUNK  frames per sequence and chunk of ssg_focus_stack_batch (1..32)
+ *   SSG_TUNE_PYR_STRIP    1 (default): the registration pyramid by column strips, its smoothed
+ *                         level 0 and level 1 in one pass; 0: tiled smoothing, then tiled pyrDown */
 enum { SSG_TUNE_ACC_ROWS = 0, SSG_TUNE_ACC_BLOCKS = 1, SSG_TUNE_ECC_GRID = 2, SSG_TUNE_FOCUS_CHUNK = 3,
-       SSG_TUNE_COUNT = 4 };
+       SSG_TUNE_PYR_STRIP = 4, SSG_TUNE_COUNT = 5 };
 int ssg_tune_set(int32_t knob, int32_t value);
 int ssg_tune_get(int32_t knob, int32_t *value);
 

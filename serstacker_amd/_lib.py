@@ -131,7 +131,7 @@ PROF_ECC_TRANSLATION, PROF_ECC_MAIN, PROF_CORRELATION, PROF_PYRAMID, PROF_ACCUMU
 PROF_ECC_SOLVE, PROF_FFT, PROF_ECCFLOW = 5, 6, 7
 PROF_FOCUS_PREP, PROF_FOCUS_BLUR, PROF_FOCUS_DOWN, PROF_FOCUS_FUSE, PROF_GRAPH = 8, 9, 10, 11, 12
 PROF_COUNT = 13
-TUNE_ACC_ROWS, TUNE_ACC_BLOCKS, TUNE_ECC_GRID, TUNE_FOCUS_CHUNK = 0, 1, 2, 3
+TUNE_ACC_ROWS, TUNE_ACC_BLOCKS, TUNE_ECC_GRID, TUNE_FOCUS_CHUNK, TUNE_PYR_STRIP = 0, 1, 2, 3, 4
 ECCFLOW_DOWNSCALE_RECURSIVE_RESIZE, ECCFLOW_DOWNSCALE_FULL_RESIZE, ECCFLOW_DOWNSCALE_PYRAMID = 0, 1, 2
 
 

@@ -28,7 +28,7 @@ import time
 from . import _lib
 
 KNOBS = {"acc_rows": _lib.TUNE_ACC_ROWS, "acc_blocks": _lib.TUNE_ACC_BLOCKS, "ecc_grid": _lib.TUNE_ECC_GRID,
-         "focus_chunk": _lib.TUNE_FOCUS_CHUNK}
+         "focus_chunk": _lib.TUNE_FOCUS_CHUNK, "pyr_strip": _lib.TUNE_PYR_STRIP}
 SHIPPED = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuning.json")
 
 

@@ -376,8 +376,8 @@ extern "C" void ssg_default_image_registration_options(ssg_image_registration_op
 
 // ---- tuning knobs -----------------------------------------------------------------------------
 namespace {
-std::atomic<int> g_tune[SSG_TUNE_COUNT] = {{4}, {4096}, {8}, {32}};
-const int g_tune_lo[SSG_TUNE_COUNT] = {2, 256, 1, 1}, g_tune_hi[SSG_TUNE_COUNT] = {4, 65536, 16, 32};
+std::atomic<int> g_tune[SSG_TUNE_COUNT] = {{4}, {4096}, {8}, {32}, {1}};
+const int g_tune_lo[SSG_TUNE_COUNT] = {2, 256, 1, 1, 0}, g_tune_hi[SSG_TUNE_COUNT] = {4, 65536, 16, 32, 1};
 }  // namespace
 
 namespace ssg {
@@ -508,16 +508,36 @@ bool use_half(const ssg_image_registration_options& o) { return o.ecc.scale > 0 
 int build_pyramids(ssg_registration* r, const float* const* src_ptrs, int src_pitch, int n, double sigma,
                    float* pyr) {
   hipStream_t s = r->stream;
+  // strip kernels (SSG_TUNE_PYR_STRIP 1): level 0 smoothed together with level 1, the other levels
+  // by pyrDown strips; else the tiled smoothing and pyrDown (the same values)
+  const bool strip = tune_value(SSG_TUNE_PYR_STRIP) != 0;
+  int l1 = 1;  // first level formed by pyrDown alone
   if (sigma > 0) {
     const SepKernel k = gaussian_sep(sigma);
-    HIPCHK(launch_sepfilter(src_ptrs, src_pitch, pyr + r->loff[0], r->pyr_elems, r->lpitch[0], r->lw[0], r->lh[0], n, k,
-                            BORDER_REPLICATE, s));
+    if (strip && r->nlevels > 1 &&
+        smooth_pyrdown_supported(k, src_pitch, pyr + r->loff[0], r->pyr_elems, r->lpitch[0], r->lw[0], r->lh[0],
+                                 r->loff[1] - r->loff[0], r->lpitch[1], r->lw[1], r->lh[1], n)) {
+      HIPCHK(launch_smooth_pyrdown(src_ptrs, src_pitch, pyr + r->loff[0], r->pyr_elems, r->lpitch[0], r->lw[0],
+                                   r->lh[0], r->loff[1] - r->loff[0], r->lpitch[1], r->lw[1], r->lh[1], n, k, s));
+      l1 = 2;
+    } else {
+      HIPCHK(launch_sepfilter(src_ptrs, src_pitch, pyr + r->loff[0], r->pyr_elems, r->lpitch[0], r->lw[0], r->lh[0],
+                              n, k, BORDER_REPLICATE, s));
+    }
   } else {
     HIPCHK(launch_copy_frames(src_ptrs, src_pitch, pyr + r->loff[0], r->pyr_elems, r->lpitch[0], r->lw[0], r->lh[0], n, s));
   }
-  for (int l = 1; l < r->nlevels; ++l)
-    HIPCHK(launch_pyrdown(pyr + r->loff[l - 1], r->pyr_elems, r->lpitch[l - 1], r->lw[l - 1], r->lh[l - 1],
-                          pyr + r->loff[l], r->pyr_elems, r->lpitch[l], r->lw[l], r->lh[l], n, BORDER_REFLECT101, s));
+  for (int l = l1; l < r->nlevels; ++l) {
+    const float* sp = pyr + r->loff[l - 1];
+    float* dp = pyr + r->loff[l];
+    if (strip && strip_pyrdown_supported(sp, r->pyr_elems, r->lpitch[l - 1], r->lw[l - 1], r->lh[l - 1], dp,
+                                         r->pyr_elems, r->lpitch[l], r->lw[l], r->lh[l], n))
+      HIPCHK(launch_strip_pyrdown(sp, r->pyr_elems, r->lpitch[l - 1], r->lw[l - 1], r->lh[l - 1], dp, r->pyr_elems,
+                                  r->lpitch[l], r->lw[l], r->lh[l], n, s));
+    else
+      HIPCHK(launch_pyrdown(sp, r->pyr_elems, r->lpitch[l - 1], r->lw[l - 1], r->lh[l - 1], dp, r->pyr_elems,
+                            r->lpitch[l], r->lw[l], r->lh[l], n, BORDER_REFLECT101, s));
+  }
   return SSG_OK;
 }
 

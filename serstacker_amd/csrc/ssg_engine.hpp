@@ -289,6 +289,19 @@ hipError_t launch_sepfilter(const float* const* src_ptrs, int src_pitch, float* 
 hipError_t launch_pyrdown(const float* src, long long src_fstride, int src_pitch, int sw, int sh, float* dst,
                           long long dst_fstride, int dst_pitch, int dw, int dh, int nframes, int border,
                           hipStream_t s);
+// level 0 smoothed (a symmetric gaussian of 3..9 taps, BORDER_REPLICATE) and level 1 (pyrDown,
+// BORDER_REFLECT101) of a pyramid in one pass; level 1 starts off1 floats after level 0 in each
+// frame's block of fstride floats
+bool smooth_pyrdown_supported(const SepKernel& k, int src_pitch, const float* pyr, long long fstride, int pitch0,
+                              int w, int h, long long off1, int pitch1, int dw, int dh, int nframes);
+hipError_t launch_smooth_pyrdown(const float* const* src_ptrs, int src_pitch, float* pyr, long long fstride,
+                                 int pitch0, int w, int h, long long off1, int pitch1, int dw, int dh, int nframes,
+                                 const SepKernel& k, hipStream_t s);
+// pyrDown (BORDER_REFLECT101) by column strips, one wave each: the values of launch_pyrdown
+bool strip_pyrdown_supported(const float* src, long long src_fstride, int src_pitch, int w, int h, const float* dst,
+                             long long dst_fstride, int dst_pitch, int dw, int dh, int nframes);
+hipError_t launch_strip_pyrdown(const float* src, long long src_fstride, int src_pitch, int w, int h, float* dst,
+                                long long dst_fstride, int dst_pitch, int dw, int dh, int nframes, hipStream_t s);
 hipError_t launch_copy_frames(const float* const* src_ptrs, int src_pitch, float* dst, long long dst_fstride,
                               int dst_pitch, int w, int h, int nframes, hipStream_t s);
 hipError_t launch_mask_prepare(const uint8_t* const* src_ptrs, int src_pitch, uint8_t* dst, long long dst_fstride,

@@ -1,9 +1,11 @@
 // ssg_pyr.hip -- image pyramid kernels for gfx950: separable smoothing
 // (cv::sepFilter2D), pyrDown, mask preparation, small reductions.
 //
-// All kernels are LDS-tiled: a block loads its tile plus halo once with
-// coalesced reads, filters rows into a second LDS buffer and writes the
-// column-filtered output with coalesced stores.
+// The tiled kernels load a tile plus halo into LDS once with coalesced reads,
+// filter rows into a second LDS buffer and write the column-filtered output
+// with coalesced stores. The registration pyramid is built by column strips
+// instead (k_strip_pyrdown): one wave per strip, neighbours by DPP, windows in
+// registers, level 0 smoothed and level 1 formed in the same pass.
 #include <type_traits>
 
 #include "ssg_stencil.hpp"
@@ -265,6 +267,283 @@ hipError_t launch_pyrdown(const float* src, long long src_fstride, int src_pitch
   dim3 grid((dw + PD_TW - 1) / PD_TW, (dh + PD_TH - 1) / PD_TH, nframes);
   hipLaunchKernelGGL(k_pyrdown, grid, dim3(256), 0, s, src, src_fstride, src_pitch, sw, sh, dst, dst_fstride,
                      dst_pitch, dw, dh, border);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// pyrDown of a level with no LDS (R == 0), or the registration pyramid's
+// smoothed level 0 (a symmetric gaussian of radius R with BORDER_REPLICATE)
+// and level 1 in one pass (R > 0), so that level 0 is stored once and never
+// read back. Each wave owns a strip of 240 source columns (4 per lane, 8 more
+// either side as halo) and slides down 64 source rows. A lane takes its
+// neighbours' columns with DPP lane shifts for the row filters; the smoothing
+// column window and pyrDown's five rows (BORDER_REFLECT101) stay in
+// registers; source rows are loaded SW_D rows ahead. Same expressions in the
+// same order as k_sepfilter_t and k_pyrdown, so the same values.
+// ---------------------------------------------------------------------------
+constexpr int SW_COLS = 240;  // source columns a strip stores (lanes 2..61)
+constexpr int SW_RB = 64;     // source rows a wave covers
+constexpr int SW_D = 8;       // source rows in flight per wave
+
+__device__ __forceinline__ float from_lane_below(float v) {  // lane i receives lane i-1 (wave_shr:1)
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x138, 0xF, 0xF, true));
+}
+__device__ __forceinline__ float from_lane_above(float v) {  // lane i receives lane i+1 (wave_shl:1)
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x130, 0xF, 0xF, true));
+}
+
+struct StripArgs {
+  const float* const* src_ptrs;  // R > 0: frame f's source
+  const float* src;              // R == 0: frame f at src + f * src_fstride
+  long long src_fstride;
+  int src_pitch, w, h;
+  float* d0;                     // R > 0: the smoothed level, frame f at d0 + f * d0_fstride
+  long long d0_fstride;
+  int pitch0;
+  float* d1;                     // pyrDown output, frame f at d1 + f * d1_fstride
+  long long d1_fstride;
+  int pitch1, dw, dh;
+  int nstrips, nchunks, nwaves, per_xcd;
+};
+
+template <int R>
+__global__ void __launch_bounds__(256) k_strip_pyrdown(StripArgs a, SepKernel K) {
+  static_assert(R >= 0 && R <= 4, "a lane's neighbours hold 4 columns each");
+  constexpr bool SMOOTH = R > 0;
+  const int lane = threadIdx.x & 63;
+  // consecutive waves (neighbouring strips and chunks, which share halo rows and columns) on one XCD
+  const int lb = (blockIdx.x & 7) * a.per_xcd + (blockIdx.x >> 3);
+  const int gw = __builtin_amdgcn_readfirstlane(lb * 4 + (threadIdx.x >> 6));
+  if (gw >= a.nwaves) return;
+  const int w = a.w, h = a.h, dw = a.dw, dh = a.dh;
+  const int sx = gw % a.nstrips, t = gw / a.nstrips, cy = t % a.nchunks, f = t / a.nchunks;
+  const float* __restrict__ S = SMOOTH ? a.src_ptrs[f] : a.src + f * a.src_fstride;
+  float* __restrict__ D0 = SMOOTH ? a.d0 + f * a.d0_fstride : nullptr;
+  float* __restrict__ D1 = a.d1 + f * a.d1_fstride;
+  const int xb = sx * SW_COLS - 8, c = xb + 4 * lane;
+  const bool vec = ((uintptr_t)S & 15) == 0 && c >= 0 && c + 3 < w;
+  const int cx0 = min(max(c, 0), w - 1), cx1 = min(max(c + 1, 0), w - 1), cx2 = min(max(c + 2, 0), w - 1),
+            cx3 = min(max(c + 3, 0), w - 1);
+  // rows and columns outside the frame read the nearest edge sample (the smoothing's
+  // BORDER_REPLICATE; pyrDown alone reads none that it uses)
+  auto load_row = [&](int r) -> float4 {
+    const float* row = S + (size_t)min(max(r, 0), h - 1) * a.src_pitch;
+    if (vec) return *reinterpret_cast<const float4*>(row + c);
+    return make_float4(row[cx0], row[cx1], row[cx2], row[cx3]);
+  };
+  auto row_filt = [&](float4 q) -> float4 {
+    constexpr int RR = SMOOTH ? R : 1;
+    const float lo[4] = {from_lane_below(q.x), from_lane_below(q.y), from_lane_below(q.z), from_lane_below(q.w)};
+    const float hi[4] = {from_lane_above(q.x), from_lane_above(q.y), from_lane_above(q.z), from_lane_above(q.w)};
+    float e[4 + 2 * RR];
+#pragma unroll
+    for (int i = 0; i < RR; ++i) {
+      e[i] = lo[4 - RR + i];
+      e[RR + 4 + i] = hi[i];
+    }
+    e[RR] = q.x;
+    e[RR + 1] = q.y;
+    e[RR + 2] = q.z;
+    e[RR + 3] = q.w;
+    float4 o;
+    o.x = row_filter_r<RR>(&e[RR], K);
+    o.y = row_filter_r<RR>(&e[RR + 1], K);
+    o.z = row_filter_r<RR>(&e[RR + 2], K);
+    o.w = row_filter_r<RR>(&e[RR + 3], K);
+    return o;
+  };
+  const bool xedge = xb < 0 || xb + 256 > w;  // pyrDown reads reflected columns (BORDER_REFLECT101)
+  // pyrDown's row pass at this lane's two even columns c and c + 2
+  auto pyr_row = [&](float4 s) -> float2 {
+    float e[7] = {from_lane_below(s.z), from_lane_below(s.w), s.x, s.y, s.z, s.w, from_lane_above(s.x)};
+    if (xedge) {
+      float m[7];
+#pragma unroll
+      for (int i = 0; i < 7; ++i) {
+        const int j = border_interp(c - 2 + i, w, BORDER_REFLECT101) - (c - 2);
+        float v = e[i];
+#pragma unroll
+        for (int k = 0; k < 7; ++k) v = (j == k) ? e[k] : v;
+        m[i] = v;
+      }
+#pragma unroll
+      for (int i = 0; i < 7; ++i) e[i] = m[i];
+    }
+    return make_float2(fmaf(e[1] + e[3], 4.f, fmaf(e[2], 6.f, e[0] + e[4])),
+                       fmaf(e[3] + e[5], 4.f, fmaf(e[4], 6.f, e[2] + e[6])));
+  };
+  const bool st_lane = lane >= 2 && lane < 62;
+  const int X = c >> 1;
+  auto store1 = [&](int Y, float2 r0, float2 r1, float2 r2, float2 r3, float2 r4) {
+    if (st_lane && X < dw) {
+      float2 v;
+      v.x = fmaf(r1.x + r3.x, 4.f, fmaf(r2.x, 6.f, r0.x + r4.x)) * (1.f / 256.f);
+      v.y = fmaf(r1.y + r3.y, 4.f, fmaf(r2.y, 6.f, r0.y + r4.y)) * (1.f / 256.f);
+      float* d = D1 + (size_t)Y * a.pitch1 + X;
+      if (X + 1 < dw)
+        *reinterpret_cast<float2*>(d) = v;
+      else
+        d[0] = v.x;
+    }
+  };
+  const int ys = cy * SW_RB, y_end0 = min(ys + SW_RB, h);
+  const int Ya = ys >> 1, Yb = min((ys + SW_RB) >> 1, dh);
+  // source rows formed: s_first .. s_stop-1, those the stores and the level-1 rows read
+  const int s_first = ys == 0 ? 0 : ys - 2, s_stop = min(max(SMOOTH ? y_end0 : 0, 2 * Yb + 1), h);
+  constexpr int NW = SMOOTH ? 2 * R + 1 : 1;
+  float4 wq[NW], pq[SW_D];
+  if constexpr (SMOOTH) {
+    float4 init[2 * R];
+#pragma unroll
+    for (int i = 0; i < 2 * R; ++i) init[i] = load_row(s_first - R + i);
+#pragma unroll
+    for (int d = 0; d < SW_D; ++d) pq[d] = load_row(s_first + R + d);
+#pragma unroll
+    for (int i = 0; i < 2 * R; ++i) wq[i + 1] = row_filt(init[i]);
+  } else {
+#pragma unroll
+    for (int d = 0; d < SW_D; ++d) pq[d] = load_row(s_first + d);
+  }
+  float2 hist[5];
+#pragma unroll
+  for (int i = 0; i < 5; ++i) hist[i] = make_float2(0.f, 0.f);
+#pragma nounroll
+  for (int sr = s_first; sr < s_stop; ++sr) {
+    const float4 raw = pq[0];
+#pragma unroll
+    for (int d = 0; d < SW_D - 1; ++d) pq[d] = pq[d + 1];
+    pq[SW_D - 1] = load_row(sr + R + SW_D);
+    float4 s;
+    if constexpr (SMOOTH) {
+#pragma unroll
+      for (int i = 0; i < 2 * R; ++i) wq[i] = wq[i + 1];
+      wq[2 * R] = row_filt(raw);
+      float a0[NW], a1[NW], a2[NW], a3[NW];
+#pragma unroll
+      for (int i = 0; i < NW; ++i) {
+        a0[i] = wq[i].x;
+        a1[i] = wq[i].y;
+        a2[i] = wq[i].z;
+        a3[i] = wq[i].w;
+      }
+      s = make_float4(col_filter_r<R>(a0, K), col_filter_r<R>(a1, K), col_filter_r<R>(a2, K), col_filter_r<R>(a3, K));
+      if (sr >= ys && sr < y_end0 && st_lane && c < w) {
+        float* d = D0 + (size_t)sr * a.pitch0 + c;
+        if (c + 3 < w) {
+          *reinterpret_cast<float4*>(d) = s;
+        } else {
+          d[0] = s.x;
+          if (c + 1 < w) d[1] = s.y;
+          if (c + 2 < w) d[2] = s.z;
+        }
+      }
+    } else {
+      s = raw;
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i) hist[i] = hist[i + 1];
+    hist[4] = pyr_row(s);
+    // level-1 row Y reads source rows 2Y-2 .. 2Y+2 (rows -2, -1 reflect to 2, 1)
+    if (sr >= 2 && !(sr & 1)) {
+      const int Y = (sr - 2) >> 1;
+      if (Y >= Ya && Y < Yb) {
+        if (Y == 0)
+          store1(Y, hist[4], hist[3], hist[2], hist[3], hist[4]);
+        else
+          store1(Y, hist[0], hist[1], hist[2], hist[3], hist[4]);
+      }
+    }
+  }
+  // the last level-1 row may read rows h and h+1, which reflect to h-2 and h-3
+  if (s_stop == h) {
+    const int Y = (h - 1) >> 1;
+    if (Y >= Ya && Y < Yb) {
+      if (h & 1)
+        store1(Y, hist[2], hist[3], hist[4], hist[3], hist[2]);
+      else
+        store1(Y, hist[1], hist[2], hist[3], hist[4], hist[3]);
+    }
+  }
+}
+
+namespace {
+StripArgs strip_grid(StripArgs a, int nframes, dim3& g) {
+  a.nstrips = (a.w + SW_COLS - 1) / SW_COLS;
+  a.nchunks = (a.h + SW_RB - 1) / SW_RB;
+  const long long nw = (long long)a.nstrips * a.nchunks * nframes, nb = (nw + 3) / 4, per = (nb + 7) / 8;
+  a.nwaves = (int)nw;
+  a.per_xcd = (int)per;
+  g = dim3((unsigned)(per * 8));
+  return a;
+}
+bool strip_grid_fits(int w, int h, int nframes) {
+  const long long nw = (long long)((w + SW_COLS - 1) / SW_COLS) * ((h + SW_RB - 1) / SW_RB) * nframes;
+  return nw + 32 < (long long)INT32_MAX;
+}
+}  // namespace
+
+bool strip_pyrdown_supported(const float* src, long long src_fstride, int src_pitch, int w, int h, const float* dst,
+                             long long dst_fstride, int dst_pitch, int dw, int dh, int nframes) {
+  return nframes > 0 && w >= 8 && h >= 8 && src_pitch % 4 == 0 && src_pitch >= w && src_fstride % 4 == 0 &&
+         ((uintptr_t)src & 15) == 0 && ((uintptr_t)dst & 7) == 0 && dst_fstride % 2 == 0 && dst_pitch % 2 == 0 &&
+         dst_pitch >= dw && dw >= 1 && dh >= 1 && 2 * dw <= w + 1 && 2 * dh <= h + 1 && strip_grid_fits(w, h, nframes);
+}
+
+hipError_t launch_strip_pyrdown(const float* src, long long src_fstride, int src_pitch, int w, int h, float* dst,
+                                long long dst_fstride, int dst_pitch, int dw, int dh, int nframes, hipStream_t s) {
+  if (!strip_pyrdown_supported(src, src_fstride, src_pitch, w, h, dst, dst_fstride, dst_pitch, dw, dh, nframes))
+    return hipErrorInvalidValue;
+  StripArgs a{};
+  a.src = src;
+  a.src_fstride = src_fstride;
+  a.src_pitch = src_pitch;
+  a.w = w;
+  a.h = h;
+  a.d1 = dst;
+  a.d1_fstride = dst_fstride;
+  a.pitch1 = dst_pitch;
+  a.dw = dw;
+  a.dh = dh;
+  dim3 g;
+  a = strip_grid(a, nframes, g);
+  hipLaunchKernelGGL((k_strip_pyrdown<0>), g, dim3(256), 0, s, a, SepKernel{});
+  return hipGetLastError();
+}
+
+bool smooth_pyrdown_supported(const SepKernel& k, int src_pitch, const float* pyr, long long fstride, int pitch0,
+                              int w, int h, long long off1, int pitch1, int dw, int dh, int nframes) {
+  return nframes > 0 && k.nx == k.ny && (k.nx & 1) && k.nx >= 3 && k.nx <= 9 && w >= 8 && h >= 8 &&
+         src_pitch % 4 == 0 && src_pitch >= w && pitch0 % 4 == 0 && fstride % 4 == 0 && ((uintptr_t)pyr & 15) == 0 &&
+         off1 % 2 == 0 && pitch1 % 2 == 0 && pitch1 >= dw && dw >= 1 && dh >= 1 && 2 * dw <= w + 1 &&
+         2 * dh <= h + 1 && strip_grid_fits(w, h, nframes);
+}
+
+hipError_t launch_smooth_pyrdown(const float* const* src_ptrs, int src_pitch, float* pyr, long long fstride,
+                                 int pitch0, int w, int h, long long off1, int pitch1, int dw, int dh, int nframes,
+                                 const SepKernel& k, hipStream_t s) {
+  if (!smooth_pyrdown_supported(k, src_pitch, pyr, fstride, pitch0, w, h, off1, pitch1, dw, dh, nframes))
+    return hipErrorInvalidValue;
+  StripArgs a{};
+  a.src_ptrs = src_ptrs;
+  a.src_pitch = src_pitch;
+  a.w = w;
+  a.h = h;
+  a.d0 = pyr;
+  a.d0_fstride = fstride;
+  a.pitch0 = pitch0;
+  a.d1 = pyr + off1;
+  a.d1_fstride = fstride;
+  a.pitch1 = pitch1;
+  a.dw = dw;
+  a.dh = dh;
+  dim3 g;
+  a = strip_grid(a, nframes, g);
+  switch (k.nx / 2) {
+    case 1: hipLaunchKernelGGL((k_strip_pyrdown<1>), g, dim3(256), 0, s, a, k); break;
+    case 2: hipLaunchKernelGGL((k_strip_pyrdown<2>), g, dim3(256), 0, s, a, k); break;
+    case 3: hipLaunchKernelGGL((k_strip_pyrdown<3>), g, dim3(256), 0, s, a, k); break;
+    default: hipLaunchKernelGGL((k_strip_pyrdown<4>), g, dim3(256), 0, s, a, k); break;
+  }
   return hipGetLastError();
 }
 
