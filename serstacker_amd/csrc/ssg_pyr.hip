@@ -282,8 +282,8 @@ hipError_t launch_pyrdown(const float* src, long long src_fstride, int src_pitch
 // same order as k_sepfilter_t and k_pyrdown, so the same values.
 // ---------------------------------------------------------------------------
 constexpr int SW_COLS = 240;  // source columns a strip stores (lanes 2..61)
-constexpr int SW_RB = 64;     // source rows a wave covers
-constexpr int SW_D = 8;       // source rows in flight per wave
+constexpr int SW_RB = 128;    // source rows a wave covers
+constexpr int SW_D = 4;       // source rows in flight per wave
 
 __device__ __forceinline__ float from_lane_below(float v) {  // lane i receives lane i-1 (wave_shr:1)
   return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x138, 0xF, 0xF, true));
