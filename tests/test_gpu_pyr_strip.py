@@ -3,8 +3,9 @@ level 0 and level 1 in one pass, the other levels by pyrDown strips) against the
 (knob 0: a smoothing pass, then pyrDown per level), through the C ABI. The reference frame's and
 every batch frame's pyramid feed the ECC iterations, so identical statuses and parameters bit for
 bit -- on sizes whose edge strips and row chunks are partial, at every smoothing width (3..9 taps
-take the fused pass, 15 and none the strip pyrDown alone) and with a frame whose rows are not
-16-byte aligned -- say that both builds give the same levels."""
+take the fused pass, 15 and none the strip pyrDown alone), at ecc.scale 1 (the strips read the
+caller's frames, one of them not 16-byte aligned) and 0.5 (they read the half-size copies) --
+say that both builds give the same levels."""
 import numpy as np
 import pytest
 
@@ -47,7 +48,8 @@ def misaligned(t):
 
 @pytest.mark.parametrize("w,h", [(240, 180), (260, 132), (196, 70), (128, 128), (68, 65), (250, 101)])
 @pytest.mark.parametrize("sigma", [1.0, 0.5, 0.7, 1.4, 2.4, 0.0])
-def test_strip_pyramid_same_registration(ssg, w, h, sigma):
+@pytest.mark.parametrize("scale", [1.0, 0.5])
+def test_strip_pyramid_same_registration(ssg, w, h, sigma, scale):
     import torch
 
     S = ssg
@@ -56,8 +58,8 @@ def test_strip_pyramid_same_registration(ssg, w, h, sigma):
     dev = [torch.from_numpy(np.ascontiguousarray(f)).cuda() for f in frames]
     dev.append(misaligned(dev[0]))
     for motion in (0, 3):
-        kw = dict(motion_type=motion, eps=0.02, ecch_max_level=3, max_iterations=20, input_smooth_sigma=sigma,
-                  reference_smooth_sigma=sigma)
+        kw = dict(motion_type=motion, scale=scale, eps=0.02, ecch_max_level=3, max_iterations=20,
+                  input_smooth_sigma=sigma, reference_smooth_sigma=sigma)
         st0 = register(S, kw, ref, dev, 0)
         for variant in (1,):
             st1 = register(S, kw, ref, dev, variant)
