@@ -329,6 +329,7 @@ struct ssg_registration {
   DevBuf cur_pyr, cur_mask_pyr, half_buf, norm_buf, gray_buf;
   DevBuf xf_t, xf_m, res, partials, active[2], active_count[2], lm;
   DevBuf frame_ptrs, mask_ptrs, zero_count, frame_list, tmp_in, tmp_mask, sum_tmp;
+  DevBuf half_ptrs;  // sources of the batched half-size pyrDown
   int* pinned_count = nullptr;
   bool have_ref = false;
   Profiler prof;
@@ -503,6 +504,15 @@ namespace {
 
 bool use_half(const ssg_image_registration_options& o) { return o.ecc.scale > 0 && o.ecc.scale != 1; }
 
+// scaleImage's pyrDown of a batch in one strip launch (SSG_TUNE_PYR_STRIP 1) when the geometry
+// allows it; else one tiled launch per frame (the same values)
+bool half_batch(const ssg_registration* r, int src_pitch, const float* dst, long long dst_fstride, int dst_pitch,
+                int n) {
+  return tune_value(SSG_TUNE_PYR_STRIP) != 0 &&
+         strip_pyrdown_supported(nullptr, 0, src_pitch, r->ref_w, r->ref_h, dst, dst_fstride, dst_pitch, r->ew, r->eh,
+                                 n);
+}
+
 // Builds one level-0 image (smoothed or copied) and the pyramid for n frames.
 // src_ptrs: device array of n frame pointers (f32, pitch src_pitch floats).
 int build_pyramids(ssg_registration* r, const float* const* src_ptrs, int src_pitch, int n, double sigma,
@@ -532,8 +542,8 @@ int build_pyramids(ssg_registration* r, const float* const* src_ptrs, int src_pi
     float* dp = pyr + r->loff[l];
     if (strip && strip_pyrdown_supported(sp, r->pyr_elems, r->lpitch[l - 1], r->lw[l - 1], r->lh[l - 1], dp,
                                          r->pyr_elems, r->lpitch[l], r->lw[l], r->lh[l], n))
-      HIPCHK(launch_strip_pyrdown(sp, r->pyr_elems, r->lpitch[l - 1], r->lw[l - 1], r->lh[l - 1], dp, r->pyr_elems,
-                                  r->lpitch[l], r->lw[l], r->lh[l], n, s));
+      HIPCHK(launch_strip_pyrdown(nullptr, sp, r->pyr_elems, r->lpitch[l - 1], r->lw[l - 1], r->lh[l - 1], dp,
+                                  r->pyr_elems, r->lpitch[l], r->lw[l], r->lh[l], n, s));
     else
       HIPCHK(launch_pyrdown(sp, r->pyr_elems, r->lpitch[l - 1], r->lw[l - 1], r->lh[l - 1], dp, r->pyr_elems,
                             r->lpitch[l], r->lw[l], r->lh[l], n, BORDER_REFLECT101, s));
@@ -828,7 +838,7 @@ bool use_normalize(const ssg_image_registration_options& o) {
 int prepare_batch(ssg_registration* r, int n, const ssg_image* frames, const ssg_image* masks, bool* has_cmask) {
   hipStream_t s = r->stream;
   CHK(ensure_workspace(r, n));
-  std::vector<const float*> ptrs(n);
+  std::vector<const float*> ptrs(n), src_list;
   int src_pitch = 0;
   const bool half = use_half(r->o);
   size_t in_pitch = 0;
@@ -861,12 +871,23 @@ int prepare_batch(ssg_registration* r, int n, const ssg_image* frames, const ssg
     // scaleImage: pyrDown to the default half size (c_frame_registration.cc:230-250)
     const size_t hp = ((size_t)r->ew * 4 + 255) / 256 * 256;
     CHK(r->half_buf.ensure(hp * r->eh * n));
-    // pyrdown takes base + stride: run frame by frame on the pointers
-    for (int f = 0; f < n; ++f) {
-      HIPCHK(launch_pyrdown(ptrs[f], 0, src_pitch, r->ref_w, r->ref_h, r->half_buf.as<float>() + (hp / 4) * r->eh * f, 0,
-                            (int)(hp / 4), r->ew, r->eh, 1, BORDER_REFLECT101, s));
-      ptrs[f] = r->half_buf.as<float>() + (hp / 4) * r->eh * f;
+    float* hb = r->half_buf.as<float>();
+    const long long hfs = (long long)(hp / 4) * r->eh;
+    if (half_batch(r, src_pitch, hb, hfs, (int)(hp / 4), n)) {
+      // the whole batch in one launch, through a table of the sources (kept unchanged until the
+      // stream has read it: the function synchronises before it returns)
+      src_list.assign(ptrs.begin(), ptrs.end());
+      CHK(r->half_ptrs.ensure(sizeof(void*) * n));
+      HIPCHK(hipMemcpyAsync(r->half_ptrs.p, src_list.data(), sizeof(void*) * n, hipMemcpyHostToDevice, s));
+      HIPCHK(launch_strip_pyrdown(r->half_ptrs.as<const float*>(), nullptr, 0, src_pitch, r->ref_w, r->ref_h, hb, hfs,
+                                  (int)(hp / 4), r->ew, r->eh, n, s));
+    } else {
+      // pyrdown takes base + stride: run frame by frame on the pointers
+      for (int f = 0; f < n; ++f)
+        HIPCHK(launch_pyrdown(ptrs[f], 0, src_pitch, r->ref_w, r->ref_h, hb + hfs * f, 0, (int)(hp / 4), r->ew, r->eh,
+                              1, BORDER_REFLECT101, s));
     }
+    for (int f = 0; f < n; ++f) ptrs[f] = hb + hfs * f;
     src_pitch = (int)(hp / 4);
   }
   DevBuf tmpnm;

@@ -297,11 +297,14 @@ bool smooth_pyrdown_supported(const SepKernel& k, int src_pitch, const float* py
 hipError_t launch_smooth_pyrdown(const float* const* src_ptrs, int src_pitch, float* pyr, long long fstride,
                                  int pitch0, int w, int h, long long off1, int pitch1, int dw, int dh, int nframes,
                                  const SepKernel& k, hipStream_t s);
-// pyrDown (BORDER_REFLECT101) by column strips, one wave each: the values of launch_pyrdown
+// pyrDown (BORDER_REFLECT101) by column strips, one wave each: the values of launch_pyrdown. The
+// frames come from src + f * src_fstride, or (src_ptrs, with src null) from a device pointer table;
+// strip_pyrdown_supported takes src null for the table form
 bool strip_pyrdown_supported(const float* src, long long src_fstride, int src_pitch, int w, int h, const float* dst,
                              long long dst_fstride, int dst_pitch, int dw, int dh, int nframes);
-hipError_t launch_strip_pyrdown(const float* src, long long src_fstride, int src_pitch, int w, int h, float* dst,
-                                long long dst_fstride, int dst_pitch, int dw, int dh, int nframes, hipStream_t s);
+hipError_t launch_strip_pyrdown(const float* const* src_ptrs, const float* src, long long src_fstride, int src_pitch,
+                                int w, int h, float* dst, long long dst_fstride, int dst_pitch, int dw, int dh,
+                                int nframes, hipStream_t s);
 hipError_t launch_copy_frames(const float* const* src_ptrs, int src_pitch, float* dst, long long dst_fstride,
                               int dst_pitch, int w, int h, int nframes, hipStream_t s);
 hipError_t launch_mask_prepare(const uint8_t* const* src_ptrs, int src_pitch, uint8_t* dst, long long dst_fstride,

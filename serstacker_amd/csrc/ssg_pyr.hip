@@ -293,8 +293,8 @@ __device__ __forceinline__ float from_lane_above(float v) {  // lane i receives 
 }
 
 struct StripArgs {
-  const float* const* src_ptrs;  // R > 0: frame f's source
-  const float* src;              // R == 0: frame f at src + f * src_fstride
+  const float* const* src_ptrs;  // frame f's source (always when R > 0), else
+  const float* src;              // frame f at src + f * src_fstride
   long long src_fstride;
   int src_pitch, w, h;
   float* d0;                     // R > 0: the smoothed level, frame f at d0 + f * d0_fstride
@@ -317,7 +317,7 @@ __global__ void __launch_bounds__(256) k_strip_pyrdown(StripArgs a, SepKernel K)
   if (gw >= a.nwaves) return;
   const int w = a.w, h = a.h, dw = a.dw, dh = a.dh;
   const int sx = gw % a.nstrips, t = gw / a.nstrips, cy = t % a.nchunks, f = t / a.nchunks;
-  const float* __restrict__ S = SMOOTH ? a.src_ptrs[f] : a.src + f * a.src_fstride;
+  const float* __restrict__ S = (SMOOTH || a.src_ptrs) ? a.src_ptrs[f] : a.src + f * a.src_fstride;
   float* __restrict__ D0 = SMOOTH ? a.d0 + f * a.d0_fstride : nullptr;
   float* __restrict__ D1 = a.d1 + f * a.d1_fstride;
   const int xb = sx * SW_COLS - 8, c = xb + 4 * lane;
@@ -484,16 +484,21 @@ bool strip_grid_fits(int w, int h, int nframes) {
 
 bool strip_pyrdown_supported(const float* src, long long src_fstride, int src_pitch, int w, int h, const float* dst,
                              long long dst_fstride, int dst_pitch, int dw, int dh, int nframes) {
-  return nframes > 0 && w >= 8 && h >= 8 && src_pitch % 4 == 0 && src_pitch >= w && src_fstride % 4 == 0 &&
-         ((uintptr_t)src & 15) == 0 && ((uintptr_t)dst & 7) == 0 && dst_fstride % 2 == 0 && dst_pitch % 2 == 0 &&
-         dst_pitch >= dw && dw >= 1 && dh >= 1 && 2 * dw <= w + 1 && 2 * dh <= h + 1 && strip_grid_fits(w, h, nframes);
+  // src null: the frames come through a pointer table (each frame's alignment is checked on the device)
+  return nframes > 0 && w >= 8 && h >= 8 && src_pitch % 4 == 0 && src_pitch >= w &&
+         (!src || (src_fstride % 4 == 0 && ((uintptr_t)src & 15) == 0)) && ((uintptr_t)dst & 7) == 0 &&
+         dst_fstride % 2 == 0 && dst_pitch % 2 == 0 && dst_pitch >= dw && dw >= 1 && dh >= 1 && 2 * dw <= w + 1 &&
+         2 * dh <= h + 1 && strip_grid_fits(w, h, nframes);
 }
 
-hipError_t launch_strip_pyrdown(const float* src, long long src_fstride, int src_pitch, int w, int h, float* dst,
-                                long long dst_fstride, int dst_pitch, int dw, int dh, int nframes, hipStream_t s) {
-  if (!strip_pyrdown_supported(src, src_fstride, src_pitch, w, h, dst, dst_fstride, dst_pitch, dw, dh, nframes))
+hipError_t launch_strip_pyrdown(const float* const* src_ptrs, const float* src, long long src_fstride, int src_pitch,
+                                int w, int h, float* dst, long long dst_fstride, int dst_pitch, int dw, int dh,
+                                int nframes, hipStream_t s) {
+  if (!src_ptrs == !src ||
+      !strip_pyrdown_supported(src, src_fstride, src_pitch, w, h, dst, dst_fstride, dst_pitch, dw, dh, nframes))
     return hipErrorInvalidValue;
   StripArgs a{};
+  a.src_ptrs = src_ptrs;
   a.src = src;
   a.src_fstride = src_fstride;
   a.src_pitch = src_pitch;
