@@ -275,14 +275,14 @@ hipError_t launch_pyrdown(const float* src, long long src_fstride, int src_pitch
 // smoothed level 0 (a symmetric gaussian of radius R with BORDER_REPLICATE)
 // and level 1 in one pass (R > 0), so that level 0 is stored once and never
 // read back. Each wave owns a strip of 240 source columns (4 per lane, 8 more
-// either side as halo) and slides down 64 source rows. A lane takes its
+// either side as halo) and slides down 128 source rows. A lane takes its
 // neighbours' columns with DPP lane shifts for the row filters; the smoothing
 // column window and pyrDown's five rows (BORDER_REFLECT101) stay in
 // registers; source rows are loaded SW_D rows ahead. Same expressions in the
 // same order as k_sepfilter_t and k_pyrdown, so the same values.
 // ---------------------------------------------------------------------------
 constexpr int SW_COLS = 240;  // source columns a strip stores (lanes 2..61)
-constexpr int SW_RB = 128;    // source rows a wave covers
+constexpr int SW_RB = 128;    // source rows a wave covers (fewer for small batches, down to 16)
 constexpr int SW_D = 4;       // source rows in flight per wave
 
 __device__ __forceinline__ float from_lane_below(float v) {  // lane i receives lane i-1 (wave_shr:1)
@@ -303,6 +303,7 @@ struct StripArgs {
   float* d1;                     // pyrDown output, frame f at d1 + f * d1_fstride
   long long d1_fstride;
   int pitch1, dw, dh;
+  int rb;                        // source rows per wave (even)
   int nstrips, nchunks, nwaves, per_xcd;
 };
 
@@ -386,8 +387,8 @@ __global__ void __launch_bounds__(256) k_strip_pyrdown(StripArgs a, SepKernel K)
         d[0] = v.x;
     }
   };
-  const int ys = cy * SW_RB, y_end0 = min(ys + SW_RB, h);
-  const int Ya = ys >> 1, Yb = min((ys + SW_RB) >> 1, dh);
+  const int ys = cy * a.rb, y_end0 = min(ys + a.rb, h);
+  const int Ya = ys >> 1, Yb = min((ys + a.rb) >> 1, dh);
   // source rows formed: s_first .. s_stop-1, those the stores and the level-1 rows read
   const int s_first = ys == 0 ? 0 : ys - 2, s_stop = min(max(SMOOTH ? y_end0 : 0, 2 * Yb + 1), h);
   constexpr int NW = SMOOTH ? 2 * R + 1 : 1;
@@ -469,7 +470,10 @@ __global__ void __launch_bounds__(256) k_strip_pyrdown(StripArgs a, SepKernel K)
 namespace {
 StripArgs strip_grid(StripArgs a, int nframes, dim3& g) {
   a.nstrips = (a.w + SW_COLS - 1) / SW_COLS;
-  a.nchunks = (a.h + SW_RB - 1) / SW_RB;
+  // shorter chunks while the launch would have fewer than 8 waves per CU (a single frame)
+  a.rb = SW_RB;
+  while (a.rb > 16 && (long long)a.nstrips * ((a.h + a.rb - 1) / a.rb) * nframes < 2048) a.rb >>= 1;
+  a.nchunks = (a.h + a.rb - 1) / a.rb;
   const long long nw = (long long)a.nstrips * a.nchunks * nframes, nb = (nw + 3) / 4, per = (nb + 7) / 8;
   a.nwaves = (int)nw;
   a.per_xcd = (int)per;
@@ -477,7 +481,7 @@ StripArgs strip_grid(StripArgs a, int nframes, dim3& g) {
   return a;
 }
 bool strip_grid_fits(int w, int h, int nframes) {
-  const long long nw = (long long)((w + SW_COLS - 1) / SW_COLS) * ((h + SW_RB - 1) / SW_RB) * nframes;
+  const long long nw = (long long)((w + SW_COLS - 1) / SW_COLS) * ((h + 15) / 16) * nframes;  // the most chunks
   return nw + 32 < (long long)INT32_MAX;
 }
 }  // namespace

@@ -70,3 +70,28 @@ def test_strip_pyramid_same_registration(ssg, w, h, sigma, scale):
                 assert list(a.params) == list(b.params), variant
             # the misaligned copy of frame 0 takes the gathered loads: the same result as frame 0
             assert list(st1[-1].params) == list(st1[0].params)
+
+
+def test_strip_pyramid_full_chunks(ssg):
+    """A batch large enough for the strips' full 128-row chunks (and the batched half-size step
+    over 1080p frames): 96 synthetic 1920x1080 frames, translation-first affine ECC at ecc.scale
+    1 and 0.5, strip build against tiled build, bit for bit."""
+    import torch
+
+    S = ssg
+    n, W, H = 96, 1920, 1080
+    p = S.synth_params(width=W, height=H, disk_radius=420.0)
+    fr = torch.empty((n, H, W), dtype=torch.float32, device="cuda")
+    S.synth_render(p, 0, [fr[i] for i in range(n)])
+    ref = torch.empty((H, W), dtype=torch.float32, device="cuda")
+    S.synth_render(p, -1, [ref])
+    frames = [fr[i] for i in range(n)]
+    for scale in (1.0, 0.5):
+        kw = dict(motion_type=S.MOTION_AFFINE, scale=scale, eps=0.02, ecch_max_level=4, max_iterations=50)
+        st0 = register(S, kw, ref, frames, 0)
+        st1 = register(S, kw, ref, frames, 1)
+        assert sum(s.ok for s in st1) == n
+        for a, b in zip(st0, st1):
+            assert (a.ok, a.code, a.iterations, a.iterations_translation, a.rho) == \
+                (b.ok, b.code, b.iterations, b.iterations_translation, b.rho)
+            assert list(a.params) == list(b.params)
