@@ -2002,12 +2002,67 @@ extern "C" int ssg_ecc_normalize(const ssg_image* src, const ssg_image* mask, ss
   return SSG_OK;
 }
 
+namespace {
+// local_variance_map's measure of one image, enqueued on s (core/proc/focus.cc: the gray image,
+// pdownscale :30-52, the morphological gradient, the block sums): a and b hold w * h floats,
+// the block sums go to part[0 .. 2 * nb).  The last level comes back in *cur (cw x ch) and the
+// gradient image in *grad.
+int lvm_enqueue(const void* sp, size_t spb, int type, int w, int h, int kradius, int dscale, float* a, float* b,
+                double* part, hipStream_t s, float** cur_out, float** grad, int* cw_out, int* ch_out, int* nb_out) {
+  const bool integral = type == SSG_8UC1 || type == SSG_16UC1;
+  // gray, dense
+  if (type == SSG_32FC3) {
+    HIPCHK(launch_bgr2gray((const float*)sp, (int)(spb / 4), w, h, a, s));
+  } else if (integral) {
+    HIPCHK(launch_int_to_f32(sp, (int)spb, w, h, type == SSG_16UC1, a, s));
+  } else {
+    HIPCHK(hipMemcpy2DAsync(a, (size_t)w * 4, sp, spb, (size_t)w * 4, h, hipMemcpyDeviceToDevice, s));
+  }
+  // pdownscale (:30-52): pyrDown while the image keeps a side >= 4
+  int cw = w, ch = h;
+  float* cur = a;
+  float* oth = b;
+  if (dscale > 0 && std::min(cw, ch) >= 4) {
+    for (int l = 0; l < dscale; ++l) {
+      const int nw = (cw + 1) / 2, nh = (ch + 1) / 2;
+      HIPCHK(launch_pyrdown(cur, 0, cw, cw, ch, oth, 0, nw, nw, nh, 1, SSG_BORDER_REFLECT101, s));
+      if (integral) HIPCHK(launch_round_half_up(oth, (size_t)nw * nh, s));
+      std::swap(cur, oth);
+      cw = nw;
+      ch = nh;
+      if (std::min(cw, ch) < 4) break;
+    }
+  }
+  const size_t nl = (size_t)cw * ch;
+  HIPCHK(launch_morph_grad(cur, cw, ch, std::max(1, kradius), oth, s));
+  HIPCHK(launch_lvm_sums(oth, nl, part, s));
+  *cur_out = cur;
+  *grad = oth;
+  *cw_out = cw;
+  *ch_out = ch;
+  *nb_out = lvm_sum_blocks(nl);
+  return SSG_OK;
+}
+
+// The measure from the block sums, folded in block order; *W receives the weight sum
+double lvm_fold(const double* hp, int nb, int type, double* W_out) {
+  double W = 0, S4 = 0;
+  for (int i = 0; i < nb; ++i) {
+    W += hp[2 * i];
+    S4 += hp[2 * i + 1];
+  }
+  // depthScale = 20 * maxval(CV_32F) / maxval(depth) (getMaxValForPixelDepth: 1, 255, 65535)
+  const double ds = 20.0 / (type == SSG_8UC1 ? 255.0 : (type == SSG_16UC1 ? 65535.0 : 1.0));
+  if (W_out) *W_out = W;
+  return W > 0 ? ds * ds * ds / W * S4 : 0.0;
+}
+}  // namespace
+
 extern "C" int ssg_local_variance_map(const ssg_image* image, int32_t kradius, int32_t dscale, ssg_image* map,
                                       double* q) {
   CHK(check_image(image, "image"));
   if (image->type != SSG_32FC1 && image->type != SSG_32FC3 && image->type != SSG_8UC1 && image->type != SSG_16UC1)
     return fail(SSG_E_INVALID_ARG, "image must be SSG_32FC1/3, SSG_8UC1 or SSG_16UC1");
-  const bool integral = image->type == SSG_8UC1 || image->type == SSG_16UC1;
   if (map) {
     CHK(check_image(map, "map"));
     if (map->type != SSG_32FC1 || map->width != image->width || map->height != image->height)
@@ -2031,46 +2086,18 @@ extern "C" int ssg_local_variance_map(const ssg_image* image, int32_t kradius, i
   const size_t n = (size_t)w * h;
   CHK(a.ensure(sizeof(float) * n));
   CHK(b.ensure(sizeof(float) * n));
-  // gray, dense
-  if (image->type == SSG_32FC3) {
-    HIPCHK(launch_bgr2gray((const float*)sp, (int)(spb / 4), w, h, a.as<float>(), s));
-  } else if (integral) {
-    HIPCHK(launch_int_to_f32(sp, (int)spb, w, h, image->type == SSG_16UC1, a.as<float>(), s));
-  } else {
-    HIPCHK(hipMemcpy2DAsync(a.p, (size_t)w * 4, sp, spb, (size_t)w * 4, h, hipMemcpyDeviceToDevice, s));
-  }
-  // pdownscale (:30-52): pyrDown while the image keeps a side >= 4
-  int cw = w, ch = h;
-  float* cur = a.as<float>();
-  float* oth = b.as<float>();
-  if (dscale > 0 && std::min(cw, ch) >= 4) {
-    for (int l = 0; l < dscale; ++l) {
-      const int nw = (cw + 1) / 2, nh = (ch + 1) / 2;
-      HIPCHK(launch_pyrdown(cur, 0, cw, cw, ch, oth, 0, nw, nw, nh, 1, SSG_BORDER_REFLECT101, s));
-      if (integral) HIPCHK(launch_round_half_up(oth, (size_t)nw * nh, s));
-      std::swap(cur, oth);
-      cw = nw;
-      ch = nh;
-      if (std::min(cw, ch) < 4) break;
-    }
-  }
+  CHK(part.ensure(sizeof(double) * 2 * lvm_sum_blocks(n)));
+  float *cur, *G;
+  int cw, ch, nb;
+  CHK(lvm_enqueue(sp, spb, image->type, w, h, kradius, dscale, a.as<float>(), b.as<float>(), part.as<double>(), s,
+                  &cur, &G, &cw, &ch, &nb));
   const size_t nl = (size_t)cw * ch;
-  HIPCHK(launch_morph_grad(cur, cw, ch, std::max(1, (int)kradius), oth, s));
-  float* G = oth;
-  const int nb = lvm_sum_blocks(nl);
-  CHK(part.ensure(sizeof(double) * 2 * nb));
-  HIPCHK(launch_lvm_sums(G, nl, part.as<double>(), s));
   std::vector<double> hp(2 * nb);
   HIPCHK(hipMemcpyAsync(hp.data(), part.p, sizeof(double) * 2 * nb, hipMemcpyDeviceToHost, s));
   HIPCHK(hipStreamSynchronize(s));
-  double W = 0, S4 = 0;
-  for (int i = 0; i < nb; ++i) {
-    W += hp[2 * i];
-    S4 += hp[2 * i + 1];
-  }
-  // depthScale = 20 * maxval(CV_32F) / maxval(depth) (getMaxValForPixelDepth: 1, 255, 65535)
+  double W = 0;
+  const double Q = lvm_fold(hp.data(), nb, image->type, &W);
   const double ds = 20.0 / (image->type == SSG_8UC1 ? 255.0 : (image->type == SSG_16UC1 ? 65535.0 : 1.0));
-  const double Q = W > 0 ? ds * ds * ds / W * S4 : 0.0;
   if (q) *q = Q;
   if (!map) return SSG_OK;
   if (!(W > 0)) {
