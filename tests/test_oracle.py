@@ -408,6 +408,28 @@ def test_ecc_fa_align_recovers_known_translation(oracle64, oracle32, bits):
     assert synth_np.max_displacement(ora.xform_matrix(t), synth_np.invert(T), w, h) < 0.02
 
 
+@pytest.mark.parametrize("scale", [1.0, 1.5])
+def test_ecc_fa_first_step_is_the_scaled_gauss_newton_step(oracle64, scale):
+    """One forward-additive iteration from the identity, on a smooth scene rendered under a
+    known small translation: to first order the Gauss-Newton step of a smooth image is the
+    shift itself, so the first update lands at update_step_scale x the true correction (the
+    inverse shift) -- a dropped, doubled or misplaced step scale, or a wrong sign, misses it by
+    far more than the linearisation error.  The count is the reference's post-incremented
+    `while (_num_iterations++ < _max_iterations)` (ecc2.cc:1295): a budget of one that does not
+    converge reports 2."""
+    w, h = 160, 128
+    render = synth_np.blob_scene(w, h, seed=41)
+    tx, ty = 0.23, -0.17
+    T = np.array([[1.0, 0.0, tx], [0.0, 1.0, ty]])
+    t = oracle64.xform(0)
+    ok, res = oracle64.ecc_fa_align(render([1, 0, 0, 0, 1, 0]), render(T), t, max_iterations=1, max_eps=1e-12,
+                                    update_step_scale=scale)
+    assert ok and res.num_iterations == 2
+    step = oracle64.xform_matrix(t).reshape(2, 3)[:, 2]
+    want = -scale * np.array([tx, ty])
+    assert np.linalg.norm(step - want) < 0.02 * np.linalg.norm(want), (step, want)
+
+
 def test_ecc_fa_align_identity_converges_at_once(oracle64):
     ref, _, _ = synth_np.frames(1, 128, 96, radius=40.0, seed=42)
     t = oracle64.xform(3)
