@@ -400,4 +400,56 @@ struct ReduceScatter {
   }
 };
 
+// The same recursive halving with the masks in wave_sum's order (32, 16, ..., 1) and any
+// associative-commutative operation: each step adds the two set-sums wave_sum's butterfly adds
+// at that level, so the total of a value is bit for bit wave_sum's (which is the same on every
+// lane).  Lane l then holds value slot_of(l) (or nothing, -1).
+template <int N>
+struct ReduceScatterDesc {
+  static constexpr int n1 = (N + 1) / 2, n2 = (n1 + 1) / 2, n3 = (n2 + 1) / 2, n4 = (n3 + 1) / 2,
+                       n5 = (n4 + 1) / 2, n6 = (n5 + 1) / 2;
+  static_assert(n6 == 1, "ReduceScatterDesc supports up to 64 values");
+
+  template <int M, int MASK, typename T, typename Op>
+  __device__ __forceinline__ static void step(const T* in, T* out, int lane, Op op) {
+    constexpr int H = (M + 1) / 2;
+    const bool hi = (lane & MASK) != 0;
+#pragma unroll
+    for (int i = 0; i < H; ++i) {
+      const T lo_v = in[i];
+      const T hi_v = (i + H < M) ? in[i + H] : lo_v;  // a padding slot is never reported
+      const T keep = hi ? hi_v : lo_v;
+      const T send = hi ? lo_v : hi_v;
+      out[i] = op(keep, __shfl_xor(send, MASK, WAVE));
+    }
+  }
+
+  __host__ __device__ static constexpr int slot_of(int lane) {
+    const int sizes[6] = {N, n1, n2, n3, n4, n5};
+    int idx = 0, r = N;
+    for (int k = 0; k < 6; ++k) {
+      const int h = (sizes[k] + 1) / 2;
+      if (lane & (32 >> k)) {
+        idx += h;
+        r = r - h > 0 ? r - h : 0;
+      } else {
+        r = r < h ? r : h;
+      }
+    }
+    return r >= 1 ? idx : -1;
+  }
+
+  template <typename T, typename Op>
+  __device__ __forceinline__ static T run(const T* v, int lane, Op op) {
+    T a[n1], b[n2], c[n3], d[n4], e[n5], f[n6];
+    step<N, 32>(v, a, lane, op);
+    step<n1, 16>(a, b, lane, op);
+    step<n2, 8>(b, c, lane, op);
+    step<n3, 4>(c, d, lane, op);
+    step<n4, 2>(d, e, lane, op);
+    step<n5, 1>(e, f, lane, op);
+    return f[0];
+  }
+};
+
 }  // namespace ssg

@@ -674,19 +674,17 @@ __global__ void __launch_bounds__(ECC_THREADS) k_corr_pass(CorrPassArgs a) {
       F2 = fmaf(fp, fp, F2); G2 = fmaf(gp, gp, G2); FG = fmaf(fp, gp, FG);
       fmn = fminf(fmn, fp); fmx = fmaxf(fmx, fp); gmn = fminf(gmn, gp); gmx = fmaxf(gmx, gp);
     }
-    double v[6] = {n, F1, G1, F2, G2, FG};
-#pragma unroll
-    for (int i = 0; i < 6; ++i) v[i] = wave_sum(v[i]);
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) {
-      fmn = fminf(fmn, __shfl_xor(fmn, o));
-      fmx = fmaxf(fmx, __shfl_xor(fmx, o));
-      gmn = fminf(gmn, __shfl_xor(gmn, o));
-      gmx = fmaxf(gmx, __shfl_xor(gmx, o));
-    }
-    if (lane == 0) {
-      for (int i = 0; i < 6; ++i) dred[wv][i] = v[i];
-      fred[wv][0] = fmn; fred[wv][1] = fmx; fred[wv][2] = gmn; fred[wv][3] = gmx;
+    {
+      // wave totals by recursive halving in the butterfly's mask order (the sums of wave_sum, bit
+      // for bit); the maxima as minima of the negated values (exact), all four in one halving
+      const double v[6] = {n, F1, G1, F2, G2, FG};
+      const double tot = ReduceScatterDesc<6>::run(v, lane, [](double p, double q) { return p + q; });
+      const int s6 = ReduceScatterDesc<6>::slot_of(lane);
+      if (s6 >= 0) dred[wv][s6] = tot;
+      const float e[4] = {fmn, -fmx, gmn, -gmx};
+      const float m = ReduceScatterDesc<4>::run(e, lane, [](float p, float q) { return fminf(p, q); });
+      const int s4 = ReduceScatterDesc<4>::slot_of(lane);
+      if (s4 >= 0) fred[wv][s4] = (s4 & 1) ? -m : m;
     }
     __syncthreads();
     double* part = reinterpret_cast<double*>(a.partials) + ((size_t)f * a.ntiles + tile) * NMOM_CORR;
